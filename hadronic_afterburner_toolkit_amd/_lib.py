@@ -1,0 +1,99 @@
+"""ctypes binding of the C ABI in include/hbt_hip.h (libhbt_hip.so).
+
+The library is built in-tree by ``build.py``.  Loading fails loudly when it
+is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhbt_hip.so")
+
+HBT_PREC_FP64 = 0
+HBT_PREC_FP32C = 1
+HBT_KERNEL_PLAIN = 0
+HBT_KERNEL_FAST = 1
+
+ERRORS = {0: "HBT_OK", -1: "HBT_ERR_ARG", -2: "HBT_ERR_HIP", -3: "HBT_ERR_NOMEM",
+          -4: "HBT_ERR_STATE", -5: "HBT_ERR_CAPACITY", -6: "HBT_ERR_NODEVICE"}
+
+
+class HbtConfig(C.Structure):
+    _fields_ = [
+        ("qnpts", C.c_int32), ("q_min", C.c_double), ("q_max", C.c_double),
+        ("n_KT", C.c_int32), ("KT_min", C.c_double), ("KT_max", C.c_double),
+        ("n_Kphi", C.c_int32), ("azimuthal_flag", C.c_int32),
+        ("invariant_radius_flag", C.c_int32), ("long_comoving_boost", C.c_int32),
+        ("HBTrap_min", C.c_double), ("HBTrap_max", C.c_double),
+        ("needed_number_of_pairs", C.c_double), ("number_of_mixed_events", C.c_int32),
+        ("precision", C.c_int32), ("kernel", C.c_int32), ("device", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+
+_U64P = C.POINTER(C.c_uint64)
+_F64P = C.POINTER(C.c_double)
+
+
+class HbtResult(C.Structure):
+    _fields_ = [
+        ("num_count", _U64P), ("den", _U64P), ("q_out_sum", _F64P), ("q_side_sum", _F64P),
+        ("q_long_sum", _F64P), ("num", _F64P), ("qinv_count", _U64P), ("qinv_den", _U64P),
+        ("qinv_sum", _F64P), ("qinv_num", _F64P), ("cnt_num_kt", _U64P),
+        ("cnt_den_kt", _U64P), ("cnt_num_qinv", _U64P), ("cnt_den_qinv", _U64P),
+        ("cnt_num_ktphi", _U64P), ("cnt_den_ktphi", _U64P),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("hbt_create", C.c_int, [C.POINTER(HbtConfig), C.POINTER(C.c_void_p)]),
+    ("hbt_destroy", C.c_int, [C.c_void_p]),
+    ("hbt_last_error", C.c_char_p, [C.c_void_p]),
+    ("hbt_reset", C.c_int, [C.c_void_p]),
+    ("hbt_upload", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64),
+                             C.POINTER(_F64P)]),
+    ("hbt_set_psi_ref", C.c_int, [C.c_void_p, C.c_double]),
+    ("hbt_run_same", C.c_int, [C.c_void_p]),
+    ("hbt_run_mixed", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), _F64P]),
+    ("hbt_read", C.c_int, [C.c_void_p, C.POINTER(HbtResult)]),
+    ("hbt_stage_counts", C.c_int, [C.c_void_p, _U64P, _U64P]),
+    ("hbt_rng_create", C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    ("hbt_rng_destroy", C.c_int, [C.c_void_p]),
+    ("hbt_rng_int", C.c_int32, [C.c_void_p]),
+    ("hbt_rng_uniform", C.c_double, [C.c_void_p]),
+    ("hbt_rng_draw_plan", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_int32), _F64P]),
+    ("hbt_calculate_buffer", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hbt_last_kernel_ms", C.c_int, [C.c_void_p, _F64P, _F64P]),
+    ("hbt_slot_info", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int64)]),
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libhbt_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            "g.build()'` (there is no fallback implementation)")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HbtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code

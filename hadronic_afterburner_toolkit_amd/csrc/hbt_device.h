@@ -1,0 +1,215 @@
+// hbt_device.h -- device-side parameters, histogram views and the exact
+// FP64 pair cascade shared by every HBT kernel.
+//
+// The cascade restates HBT_correlation.cpp:311-458 (same-event) and
+// :574-687 (mixed) operation for operation.  It must be compiled with
+// -ffp-contract=off: every decision (K_T cut, K_T index, window cuts, bin
+// indices) then uses the same correctly rounded + - * / sqrt sequence as
+// the reference's x86-64 build, so bin indices and counts are bit-exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hbt {
+
+struct DevParams {
+    int qnpts, n_KT, n_Kphi, nphi;
+    int azimuthal, qinv, boost, pad0;
+    double KT_min, dKT, dKphi, delta_q;
+    double q_base, q_lo_cut, q_hi_cut;
+    double KT_min_sq, KT_max_sq, hbarC_inv, psi_ref, two_pi;
+    // Conservative prefilter thresholds (a pair the exact cascade accepts
+    // always passes).  Per particle the record holds nrm = |pT|^2 - mid/2,
+    // so for a pair u = T - mid and v = Q - mid come from one dot product.
+    double pf_half;   // |T - mid| <= pf_half  (T = |p1T+p2T|^2 = 4 K_perp^2)
+    double pf_qthr;   // Q - mid <= pf_qthr    (Q = |p1T-p2T|^2 >= q_out^2+q_side^2)
+    double pf_mid;
+    float f_half, f_qthr, f_mid, f_pad;
+    // precision-form exact-stage constants
+    double inv_dKT, inv_dq, inv_two_pi, two_pi_hi, two_pi_lo;
+};
+
+struct DevHist {
+    unsigned long long *num_count, *den;
+    double *q_out_sum, *q_side_sum, *q_long_sum, *num;
+    unsigned long long *qinv_count, *qinv_den;
+    double *qinv_sum, *qinv_num;
+    unsigned long long *stage;  // [0..4] same, [5..9] mixed
+};
+
+// FP64 particle arrays in HBM (structure of arrays).
+struct DevSoA {
+    const double *__restrict__ px, *__restrict__ py, *__restrict__ pz, *__restrict__ E;
+    const double *__restrict__ x, *__restrict__ y, *__restrict__ z, *__restrict__ t;
+};
+
+// FP32 hi/lo split copies (v = hi + lo to ~2^-48 relative).
+struct DevSoA32 {
+    const float *__restrict__ pxh, *__restrict__ pyh, *__restrict__ pzh, *__restrict__ Eh;
+    const float *__restrict__ pxl, *__restrict__ pyl, *__restrict__ pzl, *__restrict__ El;
+    const float *__restrict__ xh, *__restrict__ yh, *__restrict__ zh, *__restrict__ th;
+    const float *__restrict__ xl, *__restrict__ yl, *__restrict__ zl, *__restrict__ tl;
+};
+
+__device__ __forceinline__ void atomic_add_u64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_add_f64(double *p, double v) {
+    unsafeAtomicAdd(p, v);
+}
+
+__device__ __forceinline__ long bin_index(const DevParams &P, int kt, int kphi, int io, int is,
+                                          int il) {
+    const long q = P.qnpts;
+    return ((((long)kt * P.nphi + kphi) * q + io) * q + is) * q + il;
+}
+
+// K_phi bin with the reference's wrap loops (HBT_correlation.cpp:408-423).
+// Returns -1 when the reference would skip the pair.
+__device__ __forceinline__ int kphi_index(const DevParams &P, double K_x, double K_y) {
+    double local_K_phi = atan2(K_y, K_x);
+    double delta_phi = local_K_phi - P.psi_ref;
+    while (delta_phi < 0.) delta_phi += P.two_pi;
+    while (delta_phi > P.two_pi) delta_phi -= P.two_pi;
+    int idx = (int)(delta_phi / P.dKphi);
+    if (idx < 0 || idx >= P.n_Kphi) return -1;
+    return idx;
+}
+
+// Result of the K/q cascade for one pair (shared by same and mixed).
+struct PairBins {
+    int kt, io, is, il, kphi;
+    double q_out, q_side, q_long;
+    double q_x, q_y, q_z, q_E;
+};
+
+// Stages reached, for the plain kernel's counters.
+enum { ST_KT = 1, ST_OUT = 2, ST_SIDE = 4, ST_BIN = 8 };
+
+// The K/q cascade (HBT_correlation.cpp:311-398 / :574-647).  MIXED selects
+// the mixed-event upper cuts (>=, :613, :622, :641).  Returns the stage
+// mask; ST_BIN set means all windows passed and b holds the bins.  The
+// q_inv branch is handled by the caller through qinv_* outputs.
+template <bool MIXED>
+__device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, double p1y,
+                                           double p1z, double p1E, double p2x, double p2y,
+                                           double p2z, double p2E, PairBins &b,
+                                           double &q_inv, int &qinv_ok) {
+    qinv_ok = 0;
+    double K_z = 0.5 * (p1z + p2z);
+    double K_E = 0.5 * (p1E + p2E);
+    double K_z_over_K_E = K_z / K_E;
+    double K_x = 0.5 * (p1x + p2x);
+    double K_y = 0.5 * (p1y + p2y);
+    double K_perp_sq = K_x * K_x + K_y * K_y;
+    // written so that NaN is rejected (the reference indexes with it: UB)
+    if (!(K_perp_sq >= P.KT_min_sq && K_perp_sq <= P.KT_max_sq)) return 0;
+    int st = ST_KT;
+    double K_perp = sqrt(K_perp_sq);
+    b.kt = (int)((K_perp - P.KT_min) / P.dKT);
+    if (b.kt < 0 || b.kt >= P.n_KT) return 0;  // cannot happen for KT_min <= K_perp <= KT_max
+    b.q_x = p1x - p2x;
+    b.q_y = p1y - p2y;
+    b.q_z = p1z - p2z;
+    b.q_E = p1E - p2E;
+    if (P.qinv) {
+        q_inv = sqrt(-(b.q_E * b.q_E - b.q_x * b.q_x - b.q_y * b.q_y - b.q_z * b.q_z));
+        if (q_inv > P.q_lo_cut && q_inv < P.q_hi_cut) {
+            int qi = (int)((q_inv - P.q_base) / P.delta_q);
+            if (qi < P.qnpts) qinv_ok = qi + 1;
+        }
+    }
+    double cos_K_phi = K_x / K_perp;
+    double sin_K_phi = K_y / K_perp;
+    double q_out = b.q_x * cos_K_phi + b.q_y * sin_K_phi;
+    if (MIXED) {
+        if (!(q_out >= P.q_lo_cut && q_out < P.q_hi_cut)) return st;
+    } else {
+        if (!(q_out >= P.q_lo_cut && q_out <= P.q_hi_cut)) return st;
+    }
+    b.io = (int)((q_out - P.q_base) / P.delta_q);
+    if (b.io >= P.qnpts || b.io < 0) return st;
+    st |= ST_OUT;
+    double q_side = b.q_y * cos_K_phi - b.q_x * sin_K_phi;
+    if (MIXED) {
+        if (!(q_side >= P.q_lo_cut && q_side < P.q_hi_cut)) return st;
+    } else {
+        if (!(q_side >= P.q_lo_cut && q_side <= P.q_hi_cut)) return st;
+    }
+    b.is = (int)((q_side - P.q_base) / P.delta_q);
+    if (b.is >= P.qnpts || b.is < 0) return st;
+    st |= ST_SIDE;
+    double q_long = b.q_z;
+    if (P.boost) {
+        double Mt = sqrt(K_E * K_E - K_z * K_z);
+        double boost_gamma = K_E / Mt;
+        double boost_beta = K_z_over_K_E;
+        q_long = boost_gamma * (b.q_z - boost_beta * b.q_E);
+    }
+    if (MIXED) {
+        if (!(q_long >= P.q_lo_cut && q_long < P.q_hi_cut)) return st;
+    } else {
+        if (!(q_long >= P.q_lo_cut && q_long <= P.q_hi_cut)) return st;
+    }
+    b.il = (int)((q_long - P.q_base) / P.delta_q);
+    if (b.il >= P.qnpts || b.il < 0) return st;
+    b.kphi = 0;
+    if (P.azimuthal) {
+        b.kphi = kphi_index(P, K_x, K_y);
+        if (b.kphi < 0) return st;
+    }
+    b.q_out = q_out;
+    b.q_side = q_side;
+    b.q_long = q_long;
+    return st | ST_BIN;
+}
+
+// Same-event pair: cascade + accumulation (HBT_correlation.cpp:339-356,
+// :431-446).  Returns the stage mask.
+__device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H, double p1x,
+                                            double p1y, double p1z, double p1E, double x1,
+                                            double y1, double z1, double t1, double p2x,
+                                            double p2y, double p2z, double p2E, double x2,
+                                            double y2, double z2, double t2) {
+    PairBins b;
+    double q_inv;
+    int qinv_ok;
+    int st = cascade_f64<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b, q_inv, qinv_ok);
+    if (!(st & (ST_BIN)) && !qinv_ok) return st;
+    double t_diff = t1 - t2;
+    double x_diff = x1 - x2;
+    double y_diff = y1 - y2;
+    double z_diff = z1 - z2;
+    double cos_qx = cos(P.hbarC_inv *
+                        (b.q_E * t_diff - b.q_x * x_diff - b.q_y * y_diff - b.q_z * z_diff));
+    if (qinv_ok) {
+        long k = (long)b.kt * P.qnpts + (qinv_ok - 1);
+        atomic_add_u64(&H.qinv_count[k], 1ull);
+        atomic_add_f64(&H.qinv_sum[k], q_inv);
+        atomic_add_f64(&H.qinv_num[k], cos_qx);
+    }
+    if (st & ST_BIN) {
+        long k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
+        atomic_add_u64(&H.num_count[k], 1ull);
+        atomic_add_f64(&H.q_out_sum[k], b.q_out);
+        atomic_add_f64(&H.q_side_sum[k], b.q_side);
+        atomic_add_f64(&H.q_long_sum[k], b.q_long);
+        atomic_add_f64(&H.num[k], cos_qx);
+    }
+    return st;
+}
+
+// Mixed pair: cascade + denominator (HBT_correlation.cpp:592-607, :681-683).
+__device__ __forceinline__ int bin_mixed_f64(const DevParams &P, const DevHist &H, double p1x,
+                                             double p1y, double p1z, double p1E, double p2x,
+                                             double p2y, double p2z, double p2E) {
+    PairBins b;
+    double q_inv;
+    int qinv_ok;
+    int st = cascade_f64<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b, q_inv, qinv_ok);
+    if (qinv_ok) atomic_add_u64(&H.qinv_den[(long)b.kt * P.qnpts + (qinv_ok - 1)], 1ull);
+    if (st & ST_BIN) atomic_add_u64(&H.den[bin_index(P, b.kt, b.kphi, b.io, b.is, b.il)], 1ull);
+    return st;
+}
+
+}  // namespace hbt

@@ -1,0 +1,969 @@
+// hbt_engine.hip -- the HBT pair-correlation engine for gfx950 and its C ABI
+// (include/hbt_hip.h).
+//
+// Data layout in HBM, per context:
+//   slot s (0 = same-event buffer, 1 = mixed-event buffer when it is read
+//   separately): the particles kept by the rapidity cut, in the reference's
+//   gather order, as eight FP64 arrays px,py,pz,E,x,y,z,t (64 B/particle),
+//   event offsets (int64, nev+1), and the prefilter records of the fast
+//   kernel (PfRec64 32 B + cw 8 B, PfRec32 16 B + cw 4 B per particle).
+//   Histograms: flat [kt][kphi][iqout][iqside][iqlong] (iqlong fastest),
+//   u64 counts (numerator, denominator) and FP64 sums (q_out, q_side,
+//   q_long, cos); q_inv histograms [kt][iq].
+//
+// Kernels:
+//   k_same_plain / k_mixed_plain  LDS-tiled direct cascade (plain mode)
+//   k_fast<double|float, R>       persistent: prefilter + compaction +
+//                                 exact cascade (hbt_fast.h)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/hbt_hip.h"
+#include "hbt_device.h"
+#include "hbt_fast.h"
+
+using namespace hbt;
+
+namespace {
+
+constexpr double kHbarC = 0.197327053;  // parameters.h:4
+constexpr int TILE = 256;
+
+// ------------------------------------------------------------------ kernels
+
+__device__ __forceinline__ void wave_add_stage(unsigned long long *dst, unsigned v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (__lane_id() == 0 && x) atomic_add_u64(dst, x);
+}
+
+// Same-event pairs i<j of one group [g0, g0+n): block (tj, ti) of 256x256
+// tiles, j tile staged in LDS, one thread per i (HBT_correlation.cpp:291-460).
+__global__ __launch_bounds__(TILE) void k_same_plain(DevParams P, DevHist H, DevSoA A,
+                                                     long long g0, int n) {
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj < ti) return;
+    __shared__ double s[8][TILE];
+    const int j0 = tj * TILE;
+    const int jn = min(TILE, n - j0);
+    const int tid = threadIdx.x;
+    if (tid < jn) {
+        const long long j = g0 + j0 + tid;
+        s[0][tid] = A.px[j];
+        s[1][tid] = A.py[j];
+        s[2][tid] = A.pz[j];
+        s[3][tid] = A.E[j];
+        s[4][tid] = A.x[j];
+        s[5][tid] = A.y[j];
+        s[6][tid] = A.z[j];
+        s[7][tid] = A.t[j];
+    }
+    __syncthreads();
+    const int i = ti * TILE + tid;
+    unsigned c_kt = 0, c_out = 0, c_side = 0, c_bin = 0;
+    if (i < n) {
+        const long long gi = g0 + i;
+        const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
+        const double x1 = A.x[gi], y1 = A.y[gi], z1 = A.z[gi], t1 = A.t[gi];
+        const int jstart = (ti == tj) ? tid + 1 : 0;
+        for (int jj = jstart; jj < jn; jj++) {
+            const int st = bin_same_f64(P, H, p1x, p1y, p1z, p1E, x1, y1, z1, t1, s[0][jj],
+                                        s[1][jj], s[2][jj], s[3][jj], s[4][jj], s[5][jj],
+                                        s[6][jj], s[7][jj]);
+            c_kt += st & 1;
+            c_out += (st >> 1) & 1;
+            c_side += (st >> 2) & 1;
+            c_bin += (st >> 3) & 1;
+        }
+    }
+    wave_add_stage(&H.stage[1], c_kt);
+    wave_add_stage(&H.stage[2], c_out);
+    wave_add_stage(&H.stage[3], c_side);
+    wave_add_stage(&H.stage[4], c_bin);
+}
+
+// Mixed pairs: block (tj, ti, row) with row = e*M + k; the partner's j tile
+// is rotated by (c_k, s_k) while staged (HBT_correlation.cpp:522-523).
+__global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, DevSoA A,
+                                                      DevSoA B, const long long *a_off,
+                                                      const long long *b_off, const int *ids,
+                                                      const double *cs, int M, int row0) {
+    const int row = row0 + blockIdx.z;
+    const int e = row / M;
+    const long long ib = a_off[e];
+    const int n1 = (int)(a_off[e + 1] - ib);
+    const int pid = ids[row];
+    const long long jb = b_off[pid];
+    const int n2 = (int)(b_off[pid + 1] - jb);
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (ti * TILE >= n1 || tj * TILE >= n2) return;
+    const double c = cs[2 * row], sn = cs[2 * row + 1];
+    __shared__ double s[4][TILE];
+    const int j0 = tj * TILE;
+    const int jn = min(TILE, n2 - j0);
+    const int tid = threadIdx.x;
+    if (tid < jn) {
+        const long long j = jb + j0 + tid;
+        const double px = B.px[j], py = B.py[j];
+        s[0][tid] = px * c - py * sn;
+        s[1][tid] = px * sn + py * c;
+        s[2][tid] = B.pz[j];
+        s[3][tid] = B.E[j];
+    }
+    __syncthreads();
+    const int i = ti * TILE + tid;
+    unsigned c_kt = 0, c_out = 0, c_side = 0, c_bin = 0;
+    if (i < n1) {
+        const long long gi = ib + i;
+        const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
+        for (int jj = 0; jj < jn; jj++) {
+            const int st =
+                bin_mixed_f64(P, H, p1x, p1y, p1z, p1E, s[0][jj], s[1][jj], s[2][jj], s[3][jj]);
+            c_kt += st & 1;
+            c_out += (st >> 1) & 1;
+            c_side += (st >> 2) & 1;
+            c_bin += (st >> 3) & 1;
+        }
+    }
+    wave_add_stage(&H.stage[6], c_kt);
+    wave_add_stage(&H.stage[7], c_out);
+    wave_add_stage(&H.stage[8], c_side);
+    wave_add_stage(&H.stage[9], c_bin);
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+// Sum of a u64 histogram over the q bins of each (kt, kphi): the
+// reference's number_of_pairs_*_KTdiff / _KTKphidiff counters.
+__global__ void k_sum_groups(const unsigned long long *h, long per, unsigned long long *out) {
+    const int g = blockIdx.x;
+    unsigned long long acc = 0;
+    for (long k = threadIdx.x; k < per; k += blockDim.x) acc += h[(long)g * per + k];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    __shared__ unsigned long long part[16];
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        out[g] = t;
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+struct Slot {
+    int nev = 0;
+    long long n = 0;        // kept particles
+    long long n_in = 0;     // particles passed in
+    std::vector<long long> off;
+    double *d_soa[8] = {};
+    long long *d_off = nullptr;
+    PfRec64 *d_pf64 = nullptr;
+    PfRec32 *d_pf32 = nullptr;
+    double *d_cw64 = nullptr;
+    float *d_cw32 = nullptr;
+    size_t cap = 0, cap_ev = 0;
+    double pmax2 = 0.0;     // max px^2+py^2 of kept particles
+    double bmax = 0.0;      // max |beta| of kept particles
+    double psi2 = 0.0;      // psi_ref (n=2) over all particles passed in
+    bool valid = false;
+};
+
+}  // namespace
+
+struct hbt_ctx {
+    hbt_config cfg;
+    DevParams P;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    int cu_count = 0;
+    long nbins = 0, nq = 0;
+    unsigned long long *d_num_count = nullptr, *d_den = nullptr;
+    double *d_q_out = nullptr, *d_q_side = nullptr, *d_q_long = nullptr, *d_num = nullptr;
+    unsigned long long *d_qinv_count = nullptr, *d_qinv_den = nullptr;
+    double *d_qinv_sum = nullptr, *d_qinv_num = nullptr;
+    unsigned long long *d_stage = nullptr;   // [10]
+    unsigned long long *d_binned = nullptr;  // [2]
+    unsigned long long *d_sums = nullptr;    // scratch for counters
+    Slot slot[2];
+    int *d_ids = nullptr;
+    double *d_cs = nullptr;
+    size_t plan_cap = 0;
+    WorkItem *d_items = nullptr;
+    size_t items_cap = 0;
+    int *d_counter = nullptr;
+    std::string err;
+    uint64_t needed = 0, needed_qinv = 0;
+    uint64_t enum_same_total = 0, enum_mixed_total = 0;  // since reset
+    float ms_same = 0.f, ms_mixed = 0.f;
+    double c_beta = 0.0, H = 0.0, mid = 0.0;
+    int R = 2, jchunk = 1024, kchunk = 8, blocks_per_cu = 8;
+};
+
+struct hbt_rng {
+    std::mt19937 gen;
+    std::uniform_real_distribution<double> uni{0.0, 1.0};
+    std::uniform_int_distribution<int> uint_{};
+};
+
+namespace {
+
+std::string g_create_err;
+
+int fail(hbt_ctx *c, int code, const std::string &msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_create_err = msg;
+    return code;
+}
+
+#define HIPCHECK(ctx, call)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(ctx, HBT_ERR_HIP,                                                \
+                        std::string(#call) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+template <typename T>
+int dev_alloc(hbt_ctx *c, T **p, size_t n) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, HBT_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return HBT_OK;
+}
+
+void free_slot(Slot &s) {
+    for (auto &p : s.d_soa) {
+        if (p) hipFree(p);
+        p = nullptr;
+    }
+    if (s.d_off) hipFree(s.d_off);
+    if (s.d_pf64) hipFree(s.d_pf64);
+    if (s.d_pf32) hipFree(s.d_pf32);
+    if (s.d_cw64) hipFree(s.d_cw64);
+    if (s.d_cw32) hipFree(s.d_cw32);
+    s = Slot();
+}
+
+DevHist hist_view(hbt_ctx *c) {
+    DevHist h;
+    h.num_count = c->d_num_count;
+    h.den = c->d_den;
+    h.q_out_sum = c->d_q_out;
+    h.q_side_sum = c->d_q_side;
+    h.q_long_sum = c->d_q_long;
+    h.num = c->d_num;
+    h.qinv_count = c->d_qinv_count;
+    h.qinv_den = c->d_qinv_den;
+    h.qinv_sum = c->d_qinv_sum;
+    h.qinv_num = c->d_qinv_num;
+    h.stage = c->d_stage;
+    return h;
+}
+
+DevSoA soa_view(const Slot &s) {
+    DevSoA a;
+    a.px = s.d_soa[0];
+    a.py = s.d_soa[1];
+    a.pz = s.d_soa[2];
+    a.E = s.d_soa[3];
+    a.x = s.d_soa[4];
+    a.y = s.d_soa[5];
+    a.z = s.d_soa[6];
+    a.t = s.d_soa[7];
+    return a;
+}
+
+const Slot &mixed_slot(const hbt_ctx *c) { return c->slot[1].valid ? c->slot[1] : c->slot[0]; }
+
+int zero_hist(hbt_ctx *c) {
+    const size_t b8 = (size_t)c->nbins * 8, q8 = (size_t)c->nq * 8;
+    HIPCHECK(c, hipMemsetAsync(c->d_num_count, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_den, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_q_out, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_q_side, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_q_long, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_num, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_qinv_count, 0, q8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_qinv_den, 0, q8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_qinv_sum, 0, q8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_qinv_num, 0, q8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_stage, 0, 10 * 8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_binned, 0, 2 * 8, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->enum_same_total = c->enum_mixed_total = 0;
+    return HBT_OK;
+}
+
+// Per-(kt,kphi) sums of a u64 histogram into host memory.
+int group_sums(hbt_ctx *c, const unsigned long long *h, long per, int groups,
+               std::vector<unsigned long long> &out) {
+    out.assign(groups, 0);
+    if (groups <= 0) return HBT_OK;
+    hipLaunchKernelGGL(k_sum_groups, dim3(groups), dim3(1024), 0, c->stream, h, per, c->d_sums);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(out.data(), c->d_sums, groups * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return HBT_OK;
+}
+
+// The pair cap (HBT_correlation.cpp:402-406, :651-655, q_inv :340-341,
+// :600-603) admits pairs in enumeration order while a counter is at most
+// needed_number_of_pairs.  The kernels bin without ordering, which is exact
+// as long as no counter can pass the cap inside this call.
+int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
+    const uint64_t total = (mixed ? c->enum_mixed_total : c->enum_same_total) + enum_call;
+    const bool qinv = c->cfg.invariant_radius_flag == 1;
+    if (total <= c->needed + 1 && (!qinv || total <= c->needed_qinv)) return HBT_OK;
+    const int groups = c->cfg.n_KT * c->P.nphi;
+    std::vector<unsigned long long> cnt;
+    int rc = group_sums(c, mixed ? c->d_den : c->d_num_count, (long)c->P.qnpts * c->P.qnpts *
+                                                                  c->P.qnpts,
+                        groups, cnt);
+    if (rc) return rc;
+    for (auto v : cnt)
+        if (v + enum_call > c->needed + 1)
+            return fail(c, HBT_ERR_CAPACITY,
+                        "needed_number_of_pairs may bind inside this call (order-dependent "
+                        "pair cap); raise needed_number_of_pairs or split the buffer");
+    if (qinv) {
+        std::vector<unsigned long long> qc;
+        rc = group_sums(c, mixed ? c->d_qinv_den : c->d_qinv_count, c->P.qnpts, c->cfg.n_KT, qc);
+        if (rc) return rc;
+        for (auto v : qc)
+            if (v + enum_call > c->needed_qinv)
+                return fail(c, HBT_ERR_CAPACITY, "q_inv pair cap may bind inside this call");
+    }
+    return HBT_OK;
+}
+
+int ensure_plan(hbt_ctx *c, size_t rows) {
+    if (rows <= c->plan_cap) return HBT_OK;
+    int rc = dev_alloc(c, &c->d_ids, rows);
+    if (!rc) rc = dev_alloc(c, &c->d_cs, 2 * rows);
+    if (rc) return rc;
+    c->plan_cap = rows;
+    return HBT_OK;
+}
+
+int ensure_items(hbt_ctx *c, size_t n) {
+    if (n <= c->items_cap) return HBT_OK;
+    int rc = dev_alloc(c, &c->d_items, n);
+    if (rc) return rc;
+    c->items_cap = n;
+    return HBT_OK;
+}
+
+// Prefilter margins for the FP32 records depend on the data's momentum
+// scale (|pT|^2 up to pmax2): FP32 rounding of px, py and of the dot
+// product is below 8 ulp of that scale.
+void set_fp32_margins(hbt_ctx *c) {
+    const Slot &a = c->slot[0];
+    const Slot &b = mixed_slot(c);
+    const double pm = std::max(1.0, std::max(a.pmax2, b.pmax2));
+    const double eps32 = 5.9604644775390625e-8;
+    const double m = 64.0 * eps32 * (2.0 * pm + std::fabs(c->mid));
+    c->P.f_half = (float)(c->P.pf_half + m);
+    c->P.f_qthr = (float)(c->P.pf_qthr + m);
+    if (!std::isfinite(c->P.pf_qthr)) c->P.f_qthr = INFINITY;
+}
+
+FastArgs fast_args(hbt_ctx *c) {
+    FastArgs a;
+    a.P = c->P;
+    a.H = hist_view(c);
+    const Slot &s0 = c->slot[0];
+    const Slot &s1 = mixed_slot(c);
+    a.A = soa_view(s0);
+    a.B = soa_view(s1);
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
+    a.pfA = f32 ? (const void *)s0.d_pf32 : (const void *)s0.d_pf64;
+    a.pfB = f32 ? (const void *)s1.d_pf32 : (const void *)s1.d_pf64;
+    a.cwA = s0.d_cw64;
+    a.cwB = s1.d_cw64;
+    a.cwA32 = s0.d_cw32;
+    a.cwB32 = s1.d_cw32;
+    a.mix_off = s1.d_off;
+    a.plan_ids = c->d_ids;
+    a.plan_cs = c->d_cs;
+    a.M = 0;
+    a.items = c->d_items;
+    a.nitems = 0;
+    a.counter = c->d_counter;
+    a.binned = c->d_binned;
+    return a;
+}
+
+int launch_fast(hbt_ctx *c, FastArgs &a) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, c->d_counter);
+    int per_cu = c->blocks_per_cu;
+    long blocks = std::min<long>((long)c->cu_count * per_cu, ((long)a.nitems + 3) / 4);
+    if (blocks < 1) blocks = 1;
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
+    if (f32) {
+        if (c->R == 1)
+            hipLaunchKernelGGL((k_fast<float, 1>), dim3(blocks), dim3(256), 0, c->stream, a);
+        else if (c->R == 4)
+            hipLaunchKernelGGL((k_fast<float, 4>), dim3(blocks), dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_fast<float, 2>), dim3(blocks), dim3(256), 0, c->stream, a);
+    } else {
+        if (c->R == 1)
+            hipLaunchKernelGGL((k_fast<double, 1>), dim3(blocks), dim3(256), 0, c->stream, a);
+        else if (c->R == 4)
+            hipLaunchKernelGGL((k_fast<double, 4>), dim3(blocks), dim3(256), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_fast<double, 2>), dim3(blocks), dim3(256), 0, c->stream, a);
+    }
+    HIPCHECK(c, hipGetLastError());
+    return HBT_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char *hbt_last_error(const hbt_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
+    if (!cfg || !out) return fail(nullptr, HBT_ERR_ARG, "null argument");
+    *out = nullptr;
+    const hbt_config &k = *cfg;
+    if (k.qnpts < 2) return fail(nullptr, HBT_ERR_ARG, "qnpts must be >= 2");
+    if (k.n_KT < 2) return fail(nullptr, HBT_ERR_ARG, "n_KT (number of K_T edges) must be >= 2");
+    if (k.azimuthal_flag && k.n_Kphi < 1) return fail(nullptr, HBT_ERR_ARG, "n_Kphi must be >= 1");
+    if (!(k.q_max > k.q_min)) return fail(nullptr, HBT_ERR_ARG, "q_max must exceed q_min");
+    if (!(k.KT_max > k.KT_min)) return fail(nullptr, HBT_ERR_ARG, "KT_max must exceed KT_min");
+    if (k.precision != HBT_PREC_FP64 && k.precision != HBT_PREC_FP32C)
+        return fail(nullptr, HBT_ERR_ARG, "unknown precision");
+    if (k.kernel != HBT_KERNEL_PLAIN && k.kernel != HBT_KERNEL_FAST)
+        return fail(nullptr, HBT_ERR_ARG, "unknown kernel");
+    if (k.precision == HBT_PREC_FP32C && k.kernel != HBT_KERNEL_FAST)
+        return fail(nullptr, HBT_ERR_ARG, "the precision form runs on the fast kernel only");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, HBT_ERR_NODEVICE, "no HIP device visible");
+    if (k.device < 0 || k.device >= ndev) return fail(nullptr, HBT_ERR_NODEVICE, "bad device");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, k.device) != hipSuccess)
+        return fail(nullptr, HBT_ERR_NODEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, HBT_ERR_NODEVICE,
+                    std::string("built for gfx950, device is ") + prop.gcnArchName);
+    if (hipSetDevice(k.device) != hipSuccess)
+        return fail(nullptr, HBT_ERR_NODEVICE, "hipSetDevice failed");
+
+    std::unique_ptr<hbt_ctx> c(new hbt_ctx());
+    c->cfg = k;
+    c->cu_count = prop.multiProcessorCount;
+    DevParams &P = c->P;
+    std::memset(&P, 0, sizeof(P));
+    // HBT_correlation.cpp:25-61 and the loop constants of :253-256, :289-290,
+    // :363-369, each evaluated in the reference's expression order.
+    P.qnpts = k.qnpts;
+    P.n_KT = k.n_KT;
+    P.n_Kphi = k.n_Kphi;
+    P.nphi = k.azimuthal_flag ? k.n_Kphi : 1;
+    P.azimuthal = k.azimuthal_flag == 1;
+    P.qinv = k.invariant_radius_flag == 1;
+    P.boost = k.long_comoving_boost == 1;
+    P.delta_q = (k.q_max - k.q_min) / (k.qnpts - 1);
+    P.KT_min = k.KT_min;
+    P.dKT = (k.KT_max - k.KT_min) / (k.n_KT - 1);
+    P.dKphi = 2 * M_PI / (k.n_Kphi > 0 ? k.n_Kphi : 1);
+    P.q_base = k.q_min - P.delta_q / 2.;
+    P.q_lo_cut = k.q_min - P.delta_q / 2. + 1e-8;
+    P.q_hi_cut = k.q_max + P.delta_q / 2. - 1e-8;
+    P.KT_min_sq = k.KT_min * k.KT_min;
+    P.KT_max_sq = k.KT_max * k.KT_max;
+    P.hbarC_inv = 1. / kHbarC;
+    P.psi_ref = 0.0;
+    P.two_pi = 2. * M_PI;
+    P.inv_dKT = 1.0 / P.dKT;
+    P.inv_dq = 1.0 / P.delta_q;
+    P.inv_two_pi = 1.0 / (2.0 * M_PI);
+    P.two_pi_hi = 6.28318530717958623199592693709e+00;  // 2*pi rounded to double
+    P.two_pi_lo = 2.44929359829470635445e-16;            // 2*pi - two_pi_hi
+    // Prefilter (see hbt_fast.h).  T = 4 K_perp^2 must lie in
+    // [4 KT_min_sq, 4 KT_max_sq]; Q <= 2 H^2.  Margins of 1e-9 relative
+    // (+1e-12 absolute) dominate the ~1e-15 rounding of either side.
+    const double t_lo = 4.0 * P.KT_min_sq, t_hi = 4.0 * P.KT_max_sq;
+    c->mid = 0.5 * (t_lo + t_hi);
+    P.pf_mid = c->mid;
+    P.pf_half = 0.5 * (t_hi - t_lo) * (1.0 + 1e-9) + 1e-9 * t_hi + 1e-12;
+    c->H = std::max(std::fabs(P.q_lo_cut), std::fabs(P.q_hi_cut));
+    const double Hm = c->H * (1.0 + 1e-9) + 1e-12;
+    P.pf_qthr = P.qinv ? INFINITY : (2.0 * Hm * Hm * (1.0 + 1e-9) + 1e-12 - c->mid);
+    c->c_beta = 0.5 * Hm;
+    P.f_mid = (float)c->mid;
+    P.f_half = (float)P.pf_half;
+    P.f_qthr = (float)P.pf_qthr;
+    c->needed = (uint64_t)(unsigned long long)k.needed_number_of_pairs;
+    c->needed_qinv = (uint64_t)50 * c->needed;
+
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(nullptr, HBT_ERR_HIP, "hipStreamCreate failed");
+    for (auto &ev : c->ev)
+        if (hipEventCreate(&ev) != hipSuccess)
+            return fail(nullptr, HBT_ERR_HIP, "hipEventCreate failed");
+    const long q = k.qnpts;
+    c->nbins = (long)k.n_KT * P.nphi * q * q * q;
+    c->nq = (long)k.n_KT * q;
+    hbt_ctx *cp = c.get();
+    int rc = dev_alloc(cp, &c->d_num_count, c->nbins);
+    if (!rc) rc = dev_alloc(cp, &c->d_den, c->nbins);
+    if (!rc) rc = dev_alloc(cp, &c->d_q_out, c->nbins);
+    if (!rc) rc = dev_alloc(cp, &c->d_q_side, c->nbins);
+    if (!rc) rc = dev_alloc(cp, &c->d_q_long, c->nbins);
+    if (!rc) rc = dev_alloc(cp, &c->d_num, c->nbins);
+    if (!rc) rc = dev_alloc(cp, &c->d_qinv_count, c->nq);
+    if (!rc) rc = dev_alloc(cp, &c->d_qinv_den, c->nq);
+    if (!rc) rc = dev_alloc(cp, &c->d_qinv_sum, c->nq);
+    if (!rc) rc = dev_alloc(cp, &c->d_qinv_num, c->nq);
+    if (!rc) rc = dev_alloc(cp, &c->d_stage, 10);
+    if (!rc) rc = dev_alloc(cp, &c->d_binned, 2);
+    if (!rc) rc = dev_alloc(cp, &c->d_sums, (size_t)k.n_KT * std::max(1, P.nphi));
+    if (!rc) rc = dev_alloc(cp, &c->d_counter, 1);
+    if (!rc) rc = zero_hist(cp);
+    if (rc) {
+        g_create_err = c->err;
+        hbt_destroy(c.release());
+        return rc;
+    }
+    *out = c.release();
+    return HBT_OK;
+}
+
+int hbt_destroy(hbt_ctx *c) {
+    if (!c) return HBT_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->d_num_count, c->d_den,      c->d_q_out,    c->d_q_side,
+                    c->d_q_long,    c->d_num,      c->d_qinv_count, c->d_qinv_den,
+                    c->d_qinv_sum,  c->d_qinv_num, c->d_stage,    c->d_binned,
+                    c->d_sums,      c->d_ids,      c->d_cs,       c->d_items,
+                    c->d_counter};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    free_slot(c->slot[0]);
+    free_slot(c->slot[1]);
+    for (auto &ev : c->ev)
+        if (ev) hipEventDestroy(ev);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return HBT_OK;
+}
+
+int hbt_reset(hbt_ctx *c) {
+    if (!c) return HBT_ERR_ARG;
+    return zero_hist(c);
+}
+
+int hbt_set_psi_ref(hbt_ctx *c, double psi_ref) {
+    if (!c) return HBT_ERR_ARG;
+    c->P.psi_ref = psi_ref;
+    return HBT_OK;
+}
+
+int hbt_upload(hbt_ctx *c, int32_t slot, int32_t nev, const int64_t *ev_off,
+               const double *const soa[8]) {
+    if (!c) return HBT_ERR_ARG;
+    if (slot != 0 && slot != 1) return fail(c, HBT_ERR_ARG, "slot must be 0 or 1");
+    if (nev < 0 || (nev > 0 && (!ev_off || !soa))) return fail(c, HBT_ERR_ARG, "bad buffer");
+    for (int a = 0; nev > 0 && a < 8; a++)
+        if (!soa[a]) return fail(c, HBT_ERR_ARG, "null SoA array");
+    for (int e = 0; e < nev; e++)
+        if (ev_off[e + 1] < ev_off[e]) return fail(c, HBT_ERR_ARG, "event offsets decrease");
+    Slot &s = c->slot[slot];
+    const long long n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
+    // Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476).
+    const double cut_max = tanh(c->cfg.HBTrap_max);
+    const double cut_min = tanh(c->cfg.HBTrap_min);
+    std::vector<long long> keep;
+    keep.reserve((size_t)n_in);
+    std::vector<long long> off(nev + 1, 0);
+    double vr = 0.0, vi = 0.0;
+    for (int e = 0; e < nev; e++) {
+        for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
+            const double ratio = soa[2][i] / soa[3][i];
+            if (ratio > cut_min && ratio < cut_max) keep.push_back(i);
+            if (c->P.azimuthal) {  // psi_ref over all particles (:233-249)
+                const double phi = atan2(soa[1][i], soa[0][i]);
+                double sn, cn;
+                sincos(2 * phi, &sn, &cn);  // as the reference's build evaluates it
+                vr += cn;
+                vi += sn;
+            }
+        }
+        off[e + 1] = (long long)keep.size();
+    }
+    const long long n = (long long)keep.size();
+    for (int e = 0; e < nev; e++)
+        if (off[e + 1] - off[e] >= (1 << 18))
+            return fail(c, HBT_ERR_CAPACITY, "an event keeps more than 262143 particles");
+    if (n >= (1ll << 31)) return fail(c, HBT_ERR_CAPACITY, "buffer too large");
+    if ((size_t)n > s.cap) {
+        for (auto &p : s.d_soa) {
+            int rc = dev_alloc(c, &p, (size_t)n);
+            if (rc) return rc;
+        }
+        int rc = dev_alloc(c, &s.d_pf64, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_pf32, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_cw64, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_cw32, (size_t)n);
+        if (rc) return rc;
+        s.cap = (size_t)n;
+    }
+    if ((size_t)nev + 1 > s.cap_ev) {
+        int rc = dev_alloc(c, &s.d_off, (size_t)nev + 1);
+        if (rc) return rc;
+        s.cap_ev = (size_t)nev + 1;
+    }
+    std::vector<double> col((size_t)n);
+    for (int a = 0; a < 8; a++) {
+        for (long long k = 0; k < n; k++) col[k] = soa[a][keep[k]];
+        HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    // prefilter records
+    std::vector<PfRec64> pf((size_t)n);
+    std::vector<PfRec32> pf32((size_t)n);
+    std::vector<double> cw((size_t)n);
+    std::vector<float> cw32((size_t)n);
+    double pmax2 = 0.0, bmax = 0.0;
+    const bool boost = c->P.boost;
+    const bool qinv = c->P.qinv;
+    for (long long k = 0; k < n; k++) {
+        const long long i = keep[k];
+        const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
+        const double n2 = px * px + py * py;
+        pmax2 = std::max(pmax2, n2);
+        pf[k].px = px;
+        pf[k].py = py;
+        pf[k].nrm = n2 - 0.5 * c->mid;
+        pf[k].beta = boost ? pz / E : pz;
+        bmax = std::max(bmax, std::fabs(pf[k].beta));
+        cw[k] = qinv ? INFINITY : (boost ? c->c_beta / E : c->c_beta);
+    }
+    const double eps32 = 5.9604644775390625e-8;
+    for (long long k = 0; k < n; k++) {
+        pf32[k].px = (float)pf[k].px;
+        pf32[k].py = (float)pf[k].py;
+        pf32[k].nrm = (float)pf[k].nrm;
+        pf32[k].beta = (float)pf[k].beta;
+        // covers the FP32 rounding of beta (|beta| <= bmax) and of cw itself
+        cw32[k] = qinv ? INFINITY : (float)(cw[k] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
+    }
+    HIPCHECK(c, hipMemcpy(s.d_pf64, pf.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(s.d_pf32, pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(s.d_cw64, cw.data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(s.d_cw32, cw32.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(s.d_off, off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
+    s.nev = nev;
+    s.n = n;
+    s.n_in = n_in;
+    s.off = off;
+    s.pmax2 = pmax2;
+    s.bmax = bmax;
+    s.psi2 = c->P.azimuthal ? atan2(vi, vr) / 2 : 0.0;
+    s.valid = true;
+    return HBT_OK;
+}
+
+int hbt_slot_info(hbt_ctx *c, int32_t slot, int32_t *nev, int64_t *n_kept) {
+    if (!c || (slot != 0 && slot != 1)) return HBT_ERR_ARG;
+    if (nev) *nev = c->slot[slot].nev;
+    if (n_kept) *n_kept = c->slot[slot].n;
+    return HBT_OK;
+}
+
+int hbt_run_same(hbt_ctx *c) {
+    if (!c) return HBT_ERR_ARG;
+    Slot &s = c->slot[0];
+    if (!s.valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
+    const long long n = s.n;
+    const uint64_t enum_call = (uint64_t)(n * (n - 1) / 2);
+    int rc = check_cap(c, false, enum_call);
+    if (rc) return rc;
+    DevHist H = hist_view(c);
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    if (n >= 2) {
+        if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
+            const int T = (int)((n + TILE - 1) / TILE);
+            if (T > 65535) return fail(c, HBT_ERR_CAPACITY, "group too large for the plain kernel");
+            hipLaunchKernelGGL(k_same_plain, dim3(T, T), dim3(TILE), 0, c->stream, c->P, H,
+                               soa_view(s), 0ll, (int)n);
+            HIPCHECK(c, hipGetLastError());
+        } else {
+            const int TI = 64 * c->R;
+            std::vector<WorkItem> items;
+            for (long long a = 0; a + 1 < n; a += TI) {
+                const int ni = (int)std::min<long long>(TI, n - a);
+                for (long long jb = a + 1; jb < n; jb += c->jchunk) {
+                    WorkItem w{};
+                    w.i0 = a;
+                    w.j0 = jb;
+                    w.ni = ni;
+                    w.nj = (int)std::min<long long>(c->jchunk, n - jb);
+                    w.kind = 0;
+                    items.push_back(w);
+                }
+            }
+            rc = ensure_items(c, items.size());
+            if (rc) return rc;
+            HIPCHECK(c, hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(WorkItem),
+                                       hipMemcpyHostToDevice, c->stream));
+            set_fp32_margins(c);
+            FastArgs a = fast_args(c);
+            a.nitems = (int)items.size();
+            rc = launch_fast(c, a);
+            if (rc) return rc;
+            HIPCHECK(c, hipStreamSynchronize(c->stream));  // items vector lifetime
+        }
+    }
+    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev[1]));
+    HIPCHECK(c, hipEventElapsedTime(&c->ms_same, c->ev[0], c->ev[1]));
+    c->enum_same_total += enum_call;
+    if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
+        // enumerated count is a closed form
+    }
+    return HBT_OK;
+}
+
+int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const double *cos_sin) {
+    if (!c) return HBT_ERR_ARG;
+    Slot &s = c->slot[0];
+    const Slot &m = mixed_slot(c);
+    if (!s.valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
+    if (s.nev == 0) return HBT_OK;
+    if (M <= 0 || M > 64 * 1024 || !partner_ids || !cos_sin)
+        return fail(c, HBT_ERR_ARG, "bad mixing plan");
+    const long long rows = (long long)s.nev * M;
+    uint64_t enum_call = 0;
+    for (int e = 0; e < s.nev; e++) {
+        const long long n1 = s.off[e + 1] - s.off[e];
+        for (int k = 0; k < M; k++) {
+            const int p = partner_ids[(long long)e * M + k];
+            if (p < 0 || p >= m.nev) return fail(c, HBT_ERR_ARG, "partner id out of range");
+            enum_call += (uint64_t)(n1 * (m.off[p + 1] - m.off[p]));
+        }
+    }
+    int rc = check_cap(c, true, enum_call);
+    if (rc) return rc;
+    rc = ensure_plan(c, (size_t)rows);
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpyAsync(c->d_ids, partner_ids, rows * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->d_cs, cos_sin, rows * 16, hipMemcpyHostToDevice, c->stream));
+    DevHist H = hist_view(c);
+    HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+    if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
+        long long max1 = 0, max2 = 0;
+        for (int e = 0; e < s.nev; e++) max1 = std::max(max1, s.off[e + 1] - s.off[e]);
+        for (int e = 0; e < m.nev; e++) max2 = std::max(max2, m.off[e + 1] - m.off[e]);
+        const int Ti = (int)((max1 + TILE - 1) / TILE), Tj = (int)((max2 + TILE - 1) / TILE);
+        if (Ti > 0 && Tj > 0) {
+            for (long long r0 = 0; r0 < rows; r0 += 65535) {
+                const int nz = (int)std::min<long long>(65535, rows - r0);
+                hipLaunchKernelGGL(k_mixed_plain, dim3(Tj, Ti, nz), dim3(TILE), 0, c->stream, c->P,
+                                   H, soa_view(s), soa_view(m), s.d_off, m.d_off, c->d_ids,
+                                   c->d_cs, M, (int)r0);
+                HIPCHECK(c, hipGetLastError());
+            }
+        }
+    } else {
+        const int TI = 64 * c->R;
+        const int KC = std::min(c->kchunk, 64);
+        std::vector<WorkItem> items;
+        for (int e = 0; e < s.nev; e++) {
+            const long long n1 = s.off[e + 1] - s.off[e];
+            for (long long a = 0; a < n1; a += TI) {
+                for (int k0 = 0; k0 < M; k0 += KC) {
+                    WorkItem w{};
+                    w.i0 = s.off[e] + a;
+                    w.ni = (int)std::min<long long>(TI, n1 - a);
+                    w.kind = 1;
+                    w.e = e;
+                    w.k0 = k0;
+                    w.k1 = std::min(M, k0 + KC);
+                    items.push_back(w);
+                }
+            }
+        }
+        if (!items.empty()) {
+            rc = ensure_items(c, items.size());
+            if (rc) return rc;
+            HIPCHECK(c, hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(WorkItem),
+                                       hipMemcpyHostToDevice, c->stream));
+            set_fp32_margins(c);
+            FastArgs a = fast_args(c);
+            a.M = M;
+            a.nitems = (int)items.size();
+            rc = launch_fast(c, a);
+            if (rc) return rc;
+        }
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev[3]));
+    HIPCHECK(c, hipEventElapsedTime(&c->ms_mixed, c->ev[2], c->ev[3]));
+    c->enum_mixed_total += enum_call;
+    return HBT_OK;
+}
+
+int hbt_last_kernel_ms(hbt_ctx *c, double *same_ms, double *mixed_ms) {
+    if (!c) return HBT_ERR_ARG;
+    if (same_ms) *same_ms = c->ms_same;
+    if (mixed_ms) *mixed_ms = c->ms_mixed;
+    return HBT_OK;
+}
+
+int hbt_read(hbt_ctx *c, hbt_result *o) {
+    if (!c || !o) return HBT_ERR_ARG;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const size_t b8 = (size_t)c->nbins * 8, q8 = (size_t)c->nq * 8;
+    struct {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    } cp[] = {{o->num_count, c->d_num_count, b8}, {o->den, c->d_den, b8},
+              {o->q_out_sum, c->d_q_out, b8},     {o->q_side_sum, c->d_q_side, b8},
+              {o->q_long_sum, c->d_q_long, b8},   {o->num, c->d_num, b8},
+              {o->qinv_count, c->d_qinv_count, q8}, {o->qinv_den, c->d_qinv_den, q8},
+              {o->qinv_sum, c->d_qinv_sum, q8},   {o->qinv_num, c->d_qinv_num, q8}};
+    for (auto &x : cp)
+        if (x.dst) HIPCHECK(c, hipMemcpy(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost));
+    const int nKT = c->cfg.n_KT, nphi = c->P.nphi;
+    const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
+    std::vector<unsigned long long> sn, sd, qn, qd;
+    int rc = group_sums(c, c->d_num_count, per, nKT * nphi, sn);
+    if (!rc) rc = group_sums(c, c->d_den, per, nKT * nphi, sd);
+    if (!rc) rc = group_sums(c, c->d_qinv_count, c->P.qnpts, nKT, qn);
+    if (!rc) rc = group_sums(c, c->d_qinv_den, c->P.qnpts, nKT, qd);
+    if (rc) return rc;
+    for (int k = 0; k < nKT; k++) {
+        uint64_t a = 0, b = 0;
+        for (int p = 0; p < nphi; p++) {
+            a += sn[k * nphi + p];
+            b += sd[k * nphi + p];
+        }
+        // with azimuthal_flag=1 the reference leaves the K_T counters at 0
+        if (o->cnt_num_kt) o->cnt_num_kt[k] = c->P.azimuthal ? 0 : a;
+        if (o->cnt_den_kt) o->cnt_den_kt[k] = c->P.azimuthal ? 0 : b;
+        if (o->cnt_num_qinv) o->cnt_num_qinv[k] = qn[k];
+        if (o->cnt_den_qinv) o->cnt_den_qinv[k] = qd[k];
+    }
+    if (c->P.azimuthal) {
+        for (int k = 0; k < nKT * nphi; k++) {
+            if (o->cnt_num_ktphi) o->cnt_num_ktphi[k] = sn[k];
+            if (o->cnt_den_ktphi) o->cnt_den_ktphi[k] = sd[k];
+        }
+    }
+    return HBT_OK;
+}
+
+int hbt_stage_counts(hbt_ctx *c, uint64_t same[5], uint64_t mixed[5]) {
+    if (!c) return HBT_ERR_ARG;
+    unsigned long long st[10], nb[2];
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(st, c->d_stage, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(nb, c->d_binned, sizeof(nb), hipMemcpyDeviceToHost));
+    const bool fast = c->cfg.kernel == HBT_KERNEL_FAST;
+    if (same) {
+        same[0] = c->enum_same_total;
+        for (int k = 1; k < 5; k++) same[k] = fast ? 0 : st[k];
+        if (fast) same[4] = nb[0];
+    }
+    if (mixed) {
+        mixed[0] = c->enum_mixed_total;
+        for (int k = 1; k < 5; k++) mixed[k] = fast ? 0 : st[5 + k];
+        if (fast) mixed[4] = nb[1];
+    }
+    return HBT_OK;
+}
+
+// ---------------------------------------------------------------- RNG
+
+int hbt_rng_create(int32_t seed, hbt_rng **out) {
+    if (!out) return HBT_ERR_ARG;
+    hbt_rng *r = new hbt_rng();
+    int s = seed;
+    if (s == -1) {  // Random.cpp:9-11
+        std::random_device dev;
+        s = dev();
+    }
+    r->gen.seed(s);
+    *out = r;
+    return HBT_OK;
+}
+
+int hbt_rng_destroy(hbt_rng *r) {
+    delete r;
+    return HBT_OK;
+}
+
+int32_t hbt_rng_int(hbt_rng *r) { return r->uint_(r->gen); }
+double hbt_rng_uniform(hbt_rng *r) { return r->uni(r->gen); }
+
+int32_t hbt_rng_draw_plan(hbt_rng *r, int32_t nev, int32_t mixed_nev,
+                          int32_t number_of_mixed_events, int32_t *ids, double *cs) {
+    if (!r || nev < 0 || mixed_nev <= 0) return HBT_ERR_ARG;
+    const int M = number_of_mixed_events > 0 ? number_of_mixed_events : mixed_nev / 2 + 1;
+    for (int iev = 0; iev < nev; iev++) {
+        for (int count = 0; count < M; count++) {  // HBT_correlation.cpp:206-215
+            int id = r->uint_(r->gen) % mixed_nev;
+            while (iev == id && mixed_nev != 1) id = r->uint_(r->gen) % mixed_nev;
+            ids[(long long)iev * M + count] = id;
+        }
+        for (int k = 0; k < M; k++) {  // :495-497
+            const double random_rotation = r->uni(r->gen) * 2 * M_PI;
+            // The reference's g++ -O3 build evaluates this cos/sin pair with
+            // glibc sincos(), which can differ from sin() in the last bit.
+            double sn, cn;
+            sincos(random_rotation, &sn, &cn);
+            cs[2 * ((long long)iev * M + k)] = cn;
+            cs[2 * ((long long)iev * M + k) + 1] = sn;
+        }
+    }
+    return M;
+}
+
+int hbt_calculate_buffer(hbt_ctx *c, hbt_rng *rng) {
+    if (!c || !rng) return HBT_ERR_ARG;
+    Slot &s = c->slot[0];
+    if (!s.valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
+    if (c->P.azimuthal) c->P.psi_ref = s.psi2;  // :181-183
+    int rc = hbt_run_same(c);
+    if (rc) return rc;
+    const Slot &m = mixed_slot(c);
+    const int nev = s.nev, mixed_nev = m.nev;
+    if (nev == 0) return HBT_OK;
+    if (mixed_nev <= 0) return fail(c, HBT_ERR_STATE, "mixed-event buffer is empty");
+    const int M = c->cfg.number_of_mixed_events > 0 ? c->cfg.number_of_mixed_events
+                                                    : mixed_nev / 2 + 1;
+    std::vector<int32_t> ids((size_t)nev * M);
+    std::vector<double> cs((size_t)2 * nev * M);
+    hbt_rng_draw_plan(rng, nev, mixed_nev, c->cfg.number_of_mixed_events, ids.data(), cs.data());
+    return hbt_run_mixed(c, M, ids.data(), cs.data());
+}
+
+}  // extern "C"

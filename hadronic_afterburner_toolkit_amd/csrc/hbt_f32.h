@@ -1,0 +1,176 @@
+// hbt_f32.h -- exact stage of the precision form (HBT_PREC_FP32C).
+//
+// On CDNA4 an unpacked FP32 add/mul/fma issues at the same rate as its FP64
+// form, so FP32 only pays where FP64 needs a multi-instruction sequence:
+// division, square root and cos.  The precision form therefore keeps the
+// pair's momentum sums/differences and products in FP64 (one instruction
+// each, exact same rounding as the reference), takes reciprocals and
+// reciprocal square roots from the FP32 transcendental unit
+// (v_rcp_f32 / v_rsq_f32, ~1 ulp) refined by two Newton steps in FP64, and
+// evaluates cos on the FP32 unit after an FP64 Cody-Waite reduction to
+// [-pi, pi].  All survivors of the FP32 prefilter come here.
+//
+// Bit-identical counts: every decision value (K_T index, window cuts, bin
+// indices) is within a few ulp of the reference's correctly rounded value;
+// a pair whose value lies within a guard band of a cut or bin edge is
+// re-run through the literal FP64 cascade (bin_*_f64), so bins and counts
+// equal the reference's exactly.  The accumulated q values carry ~1e-15
+// relative error, cos ~1e-7 absolute; sums are accumulated in FP64 atomics.
+#pragma once
+#include "hbt_device.h"
+
+namespace hbt {
+
+// 1/sqrt(a) from v_rsq_f32, two Newton steps in FP64.
+__device__ __forceinline__ double rsqrt_refined(double a) {
+    double r = (double)__builtin_amdgcn_rsqf((float)a);
+    const double h = 0.5 * a;
+    r = r * fma(-h * r, r, 1.5);
+    r = r * fma(-h * r, r, 1.5);
+    return r;
+}
+// 1/a from v_rcp_f32, two Newton steps in FP64.
+__device__ __forceinline__ double rcp_refined(double a) {
+    double r = (double)__builtin_amdgcn_rcpf((float)a);
+    r = r * fma(-a, r, 2.0);
+    r = r * fma(-a, r, 2.0);
+    return r;
+}
+
+// cos(x) with the argument reduced in FP64 and the cosine on FP32.
+__device__ __forceinline__ double cos_reduced_f32(const DevParams &P, double x) {
+    const double k = rint(x * P.inv_two_pi);
+    const double r = fma(-k, P.two_pi_lo, fma(-k, P.two_pi_hi, x));
+    return (double)cosf((float)r);
+}
+
+// Index with a guard: returns -2 when (v - base)*inv lies within g of an
+// integer boundary (the caller then falls back to the literal cascade).
+__device__ __forceinline__ int guarded_index(double w, double g) {
+    const double n = rint(w);
+    if (fabs(w - n) < g && n != 0.0) return -2;
+    return (int)w;
+}
+
+// Window test with guard: 1 inside, 0 outside, -1 ambiguous.  upper_open
+// selects the mixed-event form (v < hi) instead of (v <= hi).
+__device__ __forceinline__ int guarded_window(double v, double lo, double hi, double g) {
+    if (fabs(v - lo) < g || fabs(v - hi) < g) return -1;
+    return (v >= lo && v <= hi) ? 1 : 0;
+}
+
+constexpr double GUARD_IDX = 1e-9;   // in bin-index units
+constexpr double GUARD_REL = 1e-12;  // relative to the q scale of the pair
+
+// The K/q cascade of the precision form.  Returns the stage mask as
+// cascade_f64 does, or -1 when a decision is ambiguous.
+template <bool MIXED>
+__device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, double p1y,
+                                            double p1z, double p1E, double p2x, double p2y,
+                                            double p2z, double p2E, PairBins &b) {
+    const double K_z = 0.5 * (p1z + p2z);
+    const double K_E = 0.5 * (p1E + p2E);
+    const double K_x = 0.5 * (p1x + p2x);
+    const double K_y = 0.5 * (p1y + p2y);
+    const double K_perp_sq = K_x * K_x + K_y * K_y;  // same ops as the reference
+    if (!(K_perp_sq >= P.KT_min_sq && K_perp_sq <= P.KT_max_sq)) return 0;
+    int st = ST_KT;
+    const double rK = rsqrt_refined(K_perp_sq);
+    const double K_perp = K_perp_sq * rK;
+    b.kt = guarded_index((K_perp - P.KT_min) * P.inv_dKT, GUARD_IDX);
+    if (b.kt < -1) return -1;
+    if (b.kt < 0 || b.kt >= P.n_KT) return -1;  // out of range: decide with the literal cascade
+    b.q_x = p1x - p2x;
+    b.q_y = p1y - p2y;
+    b.q_z = p1z - p2z;
+    b.q_E = p1E - p2E;
+    const double gq = GUARD_REL * (fabs(b.q_x) + fabs(b.q_y) + 1e-3);
+    const double cK = K_x * rK, sK = K_y * rK;
+    const double q_out = b.q_x * cK + b.q_y * sK;
+    int w = guarded_window(q_out, P.q_lo_cut, P.q_hi_cut, gq);
+    if (w < 0) return -1;
+    if (!w) return st;
+    b.io = guarded_index((q_out - P.q_base) * P.inv_dq, GUARD_IDX);
+    if (b.io < -1) return -1;
+    if (b.io >= P.qnpts) return st;
+    st |= ST_OUT;
+    const double q_side = b.q_y * cK - b.q_x * sK;
+    w = guarded_window(q_side, P.q_lo_cut, P.q_hi_cut, gq);
+    if (w < 0) return -1;
+    if (!w) return st;
+    b.is = guarded_index((q_side - P.q_base) * P.inv_dq, GUARD_IDX);
+    if (b.is < -1) return -1;
+    if (b.is >= P.qnpts) return st;
+    st |= ST_SIDE;
+    double q_long = b.q_z;
+    double gl = GUARD_REL * (fabs(b.q_z) + 1e-3);
+    if (P.boost) {
+        const double rM = rsqrt_refined(K_E * K_E - K_z * K_z);
+        const double gamma = K_E * rM;
+        const double beta = K_z * rcp_refined(K_E);
+        q_long = gamma * (b.q_z - beta * b.q_E);
+        gl = GUARD_REL * gamma * (fabs(b.q_z) + fabs(b.q_E) + 1e-3);
+    }
+    w = guarded_window(q_long, P.q_lo_cut, P.q_hi_cut, gl);
+    if (w < 0) return -1;
+    if (!w) return st;
+    b.il = guarded_index((q_long - P.q_base) * P.inv_dq, GUARD_IDX);
+    if (b.il < -1) return -1;
+    if (b.il >= P.qnpts) return st;
+    b.kphi = 0;
+    b.q_out = q_out;
+    b.q_side = q_side;
+    b.q_long = q_long;
+    return st | ST_BIN;
+}
+
+// Same-event pair in the precision form.  The q_inv and K_phi variants use
+// the literal FP64 cascade.
+__device__ __forceinline__ int bin_same_fast(const DevParams &P, const DevHist &H,
+                                             const DevSoA &S, long long i, long long j) {
+    const double p1x = S.px[i], p1y = S.py[i], p1z = S.pz[i], p1E = S.E[i];
+    const double p2x = S.px[j], p2y = S.py[j], p2z = S.pz[j], p2E = S.E[j];
+    if (P.qinv | P.azimuthal)
+        return bin_same_f64(P, H, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x, p2y,
+                            p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
+    PairBins b;
+    const int st = cascade_fast<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
+    if (st < 0)
+        return bin_same_f64(P, H, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x, p2y,
+                            p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
+    if (!(st & ST_BIN)) return st;
+    const double t_diff = S.t[i] - S.t[j];
+    const double x_diff = S.x[i] - S.x[j];
+    const double y_diff = S.y[i] - S.y[j];
+    const double z_diff = S.z[i] - S.z[j];
+    const double phase =
+        P.hbarC_inv * (b.q_E * t_diff - b.q_x * x_diff - b.q_y * y_diff - b.q_z * z_diff);
+    const double c = cos_reduced_f32(P, phase);
+    const long k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
+    atomic_add_u64(&H.num_count[k], 1ull);
+    atomic_add_f64(&H.q_out_sum[k], b.q_out);
+    atomic_add_f64(&H.q_side_sum[k], b.q_side);
+    atomic_add_f64(&H.q_long_sum[k], b.q_long);
+    atomic_add_f64(&H.num[k], c);
+    return st;
+}
+
+// Mixed pair in the precision form; j is rotated exactly as the reference
+// does (HBT_correlation.cpp:522-523) before the cascade.
+__device__ __forceinline__ int bin_mixed_fast(const DevParams &P, const DevHist &H,
+                                              const DevSoA &A, const DevSoA &B, long long i,
+                                              long long j, double c, double s) {
+    const double px = B.px[j], py = B.py[j];
+    const double p2x = px * c - py * s, p2y = px * s + py * c;
+    const double p1x = A.px[i], p1y = A.py[i], p1z = A.pz[i], p1E = A.E[i];
+    const double p2z = B.pz[j], p2E = B.E[j];
+    if (P.qinv | P.azimuthal) return bin_mixed_f64(P, H, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
+    PairBins b;
+    // mixed windows are half-open: v < hi (HBT_correlation.cpp:613)
+    const int st = cascade_fast<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
+    if (st < 0) return bin_mixed_f64(P, H, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
+    if (st & ST_BIN) atomic_add_u64(&H.den[bin_index(P, b.kt, 0, b.io, b.is, b.il)], 1ull);
+    return st;
+}
+
+}  // namespace hbt

@@ -1,0 +1,208 @@
+"""Python host side of the HBT engine: a thin mirror of the reference's
+HBT_correlation interface (src/HBT_correlation.h:63-85) over the C ABI.
+
+``HBTCorrelation(params)`` takes the reference's parameter keys;
+``calculate_HBT_correlation_function(buffer)`` runs one particle buffer
+(same-event pairs of the whole buffer, then mixed pairs with the mixing plan
+drawn from the shared random stream), histograms accumulating across calls;
+``results()`` returns the raw histograms and counters as numpy arrays.  The
+text writers live in the C++ drop-in class (csrc/host).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import HbtConfig, HbtError, HbtResult
+
+DEFAULTS = dict(qnpts=31, q_min=-0.15, q_max=0.15, n_KT=5, KT_min=0.15, KT_max=0.55,
+                n_Kphi=48, azimuthal_flag=0, invariant_radius_flag=0,
+                long_comoving_boost=1, HBTrap_min=-0.5, HBTrap_max=0.5,
+                needed_number_of_pairs=30000000, number_of_mixed_events=0)
+
+
+def _f64p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _u64p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+class Random:
+    """RandomUtil::Random (Random.h:11-25): std::mt19937 with the reference's
+    uniform_real_distribution<double>(0,1) and uniform_int_distribution<int>()."""
+
+    def __init__(self, seed: int):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        rc = self._lib.hbt_rng_create(int(seed), C.byref(h))
+        if rc:
+            raise HbtError(rc, "hbt_rng_create")
+        self.h = h
+
+    def rand_int_uniform(self) -> int:
+        return int(self._lib.hbt_rng_int(self.h))
+
+    def rand_uniform(self) -> float:
+        return float(self._lib.hbt_rng_uniform(self.h))
+
+    def draw_plan(self, nev: int, mixed_nev: int, number_of_mixed_events: int = 0):
+        M = number_of_mixed_events if number_of_mixed_events > 0 else mixed_nev // 2 + 1
+        ids = np.zeros(max(nev * M, 1), dtype=np.int32)
+        cs = np.zeros(max(2 * nev * M, 2), dtype=np.float64)
+        r = self._lib.hbt_rng_draw_plan(self.h, nev, mixed_nev, number_of_mixed_events,
+                                        ids.ctypes.data_as(C.POINTER(C.c_int32)), _f64p(cs))
+        if r < 0:
+            raise HbtError(r, "hbt_rng_draw_plan")
+        return r, ids[:nev * M], cs[:2 * nev * M]
+
+    def __del__(self):
+        try:
+            self._lib.hbt_rng_destroy(self.h)
+        except Exception:
+            pass
+
+
+@dataclass
+class Results:
+    num_count: np.ndarray
+    den: np.ndarray
+    q_out_sum: np.ndarray
+    q_side_sum: np.ndarray
+    q_long_sum: np.ndarray
+    num: np.ndarray
+    qinv_count: np.ndarray
+    qinv_den: np.ndarray
+    qinv_sum: np.ndarray
+    qinv_num: np.ndarray
+    cnt_num_kt: np.ndarray
+    cnt_den_kt: np.ndarray
+    cnt_num_qinv: np.ndarray
+    cnt_den_qinv: np.ndarray
+    cnt_num_ktphi: np.ndarray
+    cnt_den_ktphi: np.ndarray
+
+
+class HBTCorrelation:
+    """GPU HBT correlation accumulator with the reference's parameter keys."""
+
+    def __init__(self, params: dict | None = None, precision: int = _lib.HBT_PREC_FP64,
+                 kernel: int = _lib.HBT_KERNEL_FAST, device: int = 0):
+        p = dict(DEFAULTS)
+        if params:
+            p.update({k: v for k, v in params.items() if k in DEFAULTS})
+        self.params = p
+        self._lib = _lib.load()
+        cfg = HbtConfig()
+        cfg.qnpts = int(p["qnpts"])
+        cfg.q_min = float(p["q_min"])
+        cfg.q_max = float(p["q_max"])
+        cfg.n_KT = int(p["n_KT"])
+        cfg.KT_min = float(p["KT_min"])
+        cfg.KT_max = float(p["KT_max"])
+        cfg.n_Kphi = int(p["n_Kphi"])
+        cfg.azimuthal_flag = int(p["azimuthal_flag"])
+        cfg.invariant_radius_flag = int(p["invariant_radius_flag"])
+        cfg.long_comoving_boost = int(p["long_comoving_boost"])
+        cfg.HBTrap_min = float(p["HBTrap_min"])
+        cfg.HBTrap_max = float(p["HBTrap_max"])
+        cfg.needed_number_of_pairs = float(p["needed_number_of_pairs"])
+        cfg.number_of_mixed_events = int(p["number_of_mixed_events"])
+        cfg.precision = precision
+        cfg.kernel = kernel
+        cfg.device = device
+        h = C.c_void_p()
+        rc = self._lib.hbt_create(C.byref(cfg), C.byref(h))
+        if rc:
+            raise HbtError(rc, self._lib.hbt_last_error(None).decode())
+        self.h = h
+        self.nphi = cfg.n_Kphi if cfg.azimuthal_flag else 1
+        self._keep = []
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            raise HbtError(rc, f"{what}: {self._lib.hbt_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.hbt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        self._check(self._lib.hbt_reset(self.h), "hbt_reset")
+
+    def upload(self, buf, slot: int = 0):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in buf.soa()]
+        off = np.ascontiguousarray(buf.ev_off, dtype=np.int64)
+        ptrs = (C.POINTER(C.c_double) * 8)(*[_f64p(a) for a in arrs])
+        self._check(self._lib.hbt_upload(self.h, slot, buf.nev,
+                                         off.ctypes.data_as(C.POINTER(C.c_int64)), ptrs),
+                    "hbt_upload")
+
+    def set_psi_ref(self, psi: float):
+        self._check(self._lib.hbt_set_psi_ref(self.h, float(psi)), "hbt_set_psi_ref")
+
+    def run_same(self):
+        self._check(self._lib.hbt_run_same(self.h), "hbt_run_same")
+
+    def run_mixed(self, M: int, ids: np.ndarray, cs: np.ndarray):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        cs = np.ascontiguousarray(cs, dtype=np.float64)
+        self._check(self._lib.hbt_run_mixed(self.h, int(M),
+                                            ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            _f64p(cs)), "hbt_run_mixed")
+
+    def calculate_HBT_correlation_function(self, buf, rng: Random, mixed_buf=None):
+        """HBT_correlation::calculate_HBT_correlation_function (HBT_correlation.cpp:177-218)."""
+        self.upload(buf, 0)
+        if mixed_buf is not None:
+            self.upload(mixed_buf, 1)
+        self._check(self._lib.hbt_calculate_buffer(self.h, rng.h), "hbt_calculate_buffer")
+
+    def kernel_ms(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self._lib.hbt_last_kernel_ms(self.h, C.byref(a), C.byref(b)), "ms")
+        return a.value, b.value
+
+    def slot_info(self, slot: int = 0):
+        nev, n = C.c_int32(), C.c_int64()
+        self._check(self._lib.hbt_slot_info(self.h, slot, C.byref(nev), C.byref(n)), "slot")
+        return nev.value, n.value
+
+    def stage_counts(self):
+        s = np.zeros(5, np.uint64)
+        m = np.zeros(5, np.uint64)
+        self._check(self._lib.hbt_stage_counts(self.h, _u64p(s), _u64p(m)), "stage")
+        return s, m
+
+    def results(self) -> Results:
+        p = self.params
+        q = int(p["qnpts"])
+        nkt = int(p["n_KT"])
+        nb = nkt * self.nphi * q ** 3
+        nq = nkt * q
+        r = Results(
+            num_count=np.zeros(nb, np.uint64), den=np.zeros(nb, np.uint64),
+            q_out_sum=np.zeros(nb), q_side_sum=np.zeros(nb), q_long_sum=np.zeros(nb),
+            num=np.zeros(nb), qinv_count=np.zeros(nq, np.uint64),
+            qinv_den=np.zeros(nq, np.uint64), qinv_sum=np.zeros(nq), qinv_num=np.zeros(nq),
+            cnt_num_kt=np.zeros(nkt, np.uint64), cnt_den_kt=np.zeros(nkt, np.uint64),
+            cnt_num_qinv=np.zeros(nkt, np.uint64), cnt_den_qinv=np.zeros(nkt, np.uint64),
+            cnt_num_ktphi=np.zeros(nkt * self.nphi, np.uint64),
+            cnt_den_ktphi=np.zeros(nkt * self.nphi, np.uint64))
+        res = HbtResult()
+        for name, _ in HbtResult._fields_:
+            a = getattr(r, name)
+            setattr(res, name, _u64p(a) if a.dtype == np.uint64 else _f64p(a))
+        self._check(self._lib.hbt_read(self.h, C.byref(res)), "hbt_read")
+        return r

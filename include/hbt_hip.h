@@ -1,0 +1,179 @@
+/*
+ * hbt_hip.h -- C ABI of the MI355X (gfx950) HBT pair-correlation engine.
+ *
+ * The reference has no C ABI; its boundary for this path is the C++ class
+ * HBT_correlation (src/HBT_correlation.h:14-104), called per particle
+ * buffer by Analysis::HBTAnalysis (src/Analysis.cpp:817-835).  Every entry
+ * point below replaces one piece of that class; the comment on each names
+ * the reference member it stands in for.  The drop-in C++ class
+ * (hadronic_afterburner_toolkit_amd/csrc/host/HBT_correlation.h) keeps the
+ * reference's signatures and forwards to these functions.
+ *
+ * Conventions: plain pointers and sizes only; every function returns 0 on
+ * success or a negative HBT_ERR_* code, with a message in
+ * hbt_last_error(ctx).  A context is single-owner and not thread-safe; all
+ * its device work runs on its own HIP stream.  Host arrays passed in are
+ * only read during the call.
+ */
+#ifndef HBT_HIP_H
+#define HBT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HBT_ABI_VERSION 1
+
+enum {
+    HBT_OK = 0,
+    HBT_ERR_ARG = -1,        /* invalid argument or parameter */
+    HBT_ERR_HIP = -2,        /* a HIP runtime call failed */
+    HBT_ERR_NOMEM = -3,      /* device or host allocation failed */
+    HBT_ERR_STATE = -4,      /* call out of order (e.g. run before upload) */
+    HBT_ERR_CAPACITY = -5,   /* request exceeds a stated limit */
+    HBT_ERR_NODEVICE = -6    /* no usable gfx950 device */
+};
+
+/* Arithmetic of the pair cascade. */
+enum {
+    HBT_PREC_FP64 = 0,  /* plain mode: FP64, bit-exact bins/counts with the reference */
+    HBT_PREC_FP32C = 1  /* precision form: FP32 + compensation, FP64 final reduction */
+};
+
+/* Kernel family. */
+enum {
+    HBT_KERNEL_PLAIN = 0, /* LDS-tiled direct cascade, one thread per i */
+    HBT_KERNEL_FAST = 1   /* prefilter + per-wave LDS compaction, exact cascade on survivors */
+};
+
+/* Analysis parameters: the keys HBT_correlation reads in its constructor
+ * (HBT_correlation.cpp:22-46), same names and meaning, plus the engine's
+ * own choices at the end. */
+typedef struct {
+    int32_t qnpts;
+    double q_min, q_max;
+    int32_t n_KT;            /* number of K_T EDGES, as in the reference (:48) */
+    double KT_min, KT_max;
+    int32_t n_Kphi;
+    int32_t azimuthal_flag;
+    int32_t invariant_radius_flag;
+    int32_t long_comoving_boost;
+    double HBTrap_min, HBTrap_max;
+    double needed_number_of_pairs;   /* converted to unsigned long long like :36 */
+    int32_t number_of_mixed_events;  /* <=0: reference rule floor(mixed_nev/2)+1 (:200) */
+    /* engine choices */
+    int32_t precision;       /* HBT_PREC_* */
+    int32_t kernel;          /* HBT_KERNEL_* */
+    int32_t device;          /* HIP device ordinal */
+    int32_t reserved[5];
+} hbt_config;
+
+typedef struct hbt_ctx hbt_ctx;
+typedef struct hbt_rng hbt_rng;
+
+/* Replaces HBT_correlation::HBT_correlation (HBT_correlation.cpp:16-143):
+ * validates the parameters, derives the grids and allocates zeroed
+ * histograms on the device. */
+int hbt_create(const hbt_config *cfg, hbt_ctx **out);
+
+/* Replaces HBT_correlation::~HBT_correlation (HBT_correlation.cpp:145-175). */
+int hbt_destroy(hbt_ctx *ctx);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * hbt_create when ctx is NULL). */
+const char *hbt_last_error(const hbt_ctx *ctx);
+
+/* Zeroes histograms (no reference counterpart: a fresh HBT_correlation). */
+int hbt_reset(hbt_ctx *ctx);
+
+/* Replaces the particle gathers of combine_and_bin_particle_pairs and
+ * combine_and_bin_particle_pairs_mixed_events (HBT_correlation.cpp:253-281,
+ * :466-489, :498-551): takes one buffer of nev events (already filtered to
+ * one species, particleSamples.cpp:1329-1346) as host SoA arrays with
+ * nev+1 event offsets, applies the tanh(HBTrap) cut on pz/E, and makes the
+ * kept particles resident in HBM.  slot 0 = same-event list, slot 1 = the
+ * mixed-event list when read_in_real_mixed_events=1 (otherwise slot 1
+ * aliases slot 0, particleSamples.cpp:528-531).  soa = {px,py,pz,E,x,y,z,t}. */
+int hbt_upload(hbt_ctx *ctx, int32_t slot, int32_t nev, const int64_t *ev_off,
+               const double *const soa[8]);
+
+/* Replaces calculate_flow_event_plane_angle (HBT_correlation.cpp:233-249)
+ * as used at :181-183: the caller computes psi_ref on the host and sets it
+ * before running azimuthal (azimuthal_flag=1) pairs. */
+int hbt_set_psi_ref(hbt_ctx *ctx, double psi_ref);
+
+/* Replaces combine_and_bin_particle_pairs (HBT_correlation.cpp:251-462):
+ * all i<j pairs of the slot-0 buffer, binned into the resident histograms. */
+int hbt_run_same(hbt_ctx *ctx);
+
+/* Replaces the per-event loop of combine_and_bin_particle_pairs_mixed_events
+ * (HBT_correlation.cpp:464-692) for every event of the slot-0 buffer:
+ * partner_ids[nev*M] index events of the mixed list, cos_sin[2*nev*M] hold
+ * cos and sin of each partner's rotation angle, drawn on the host in the
+ * reference's order (hbt_rng_draw_plan). */
+int hbt_run_mixed(hbt_ctx *ctx, int32_t M, const int32_t *partner_ids,
+                  const double *cos_sin);
+
+/* Raw results, host arrays owned by the caller, sized for n_KT (edges),
+ * nphi = (azimuthal_flag ? n_Kphi : 1) and qnpts^3 bins per (K_T, K_phi).
+ * Layout [kt][kphi][iqout][iqside][iqlong], iqlong fastest.  Any pointer
+ * may be NULL to skip it.  The pair counters are the reference's
+ * number_of_pairs_*_KTdiff (HBT_correlation.h:42-48). */
+typedef struct {
+    uint64_t *num_count;     /* correl_3d_num_count */
+    uint64_t *den;           /* correl_3d_denorm */
+    double *q_out_sum, *q_side_sum, *q_long_sum; /* q_*_mean before division */
+    double *num;             /* correl_3d_num (sum of cos) */
+    uint64_t *qinv_count, *qinv_den; /* [kt][q] */
+    double *qinv_sum, *qinv_num;
+    uint64_t *cnt_num_kt, *cnt_den_kt, *cnt_num_qinv, *cnt_den_qinv; /* [kt] */
+    uint64_t *cnt_num_ktphi, *cnt_den_ktphi;                         /* [kt][kphi] */
+} hbt_result;
+
+/* Replaces reading the member arrays in the output_* writers
+ * (HBT_correlation.cpp:694-855).  Synchronises the context's stream. */
+int hbt_read(hbt_ctx *ctx, hbt_result *out);
+
+/* Stage reach counters of the last runs since reset (enumerated, K_T cut
+ * passed, q_out passed, q_side passed, binned), same and mixed; the inputs
+ * of the FLOP model.  Only the plain kernel counts every stage; the fast
+ * kernel reports enumerated and binned. */
+int hbt_stage_counts(hbt_ctx *ctx, uint64_t same[5], uint64_t mixed[5]);
+
+/* Replaces RandomUtil::Random (Random.h:11-25, Random.cpp:7-14):
+ * std::mt19937(seed) with the reference's two distributions.  seed -1
+ * draws from std::random_device like the reference. */
+int hbt_rng_create(int32_t seed, hbt_rng **out);
+int hbt_rng_destroy(hbt_rng *rng);
+int32_t hbt_rng_int(hbt_rng *rng);    /* rand_int_uniform() */
+double hbt_rng_uniform(hbt_rng *rng); /* rand_uniform() */
+
+/* Draws the mixing plan of one buffer in the reference's order
+ * (HBT_correlation.cpp:200-216 for ids, :495-497 for angles): per event,
+ * M partner ids (redrawn while equal to the event unless mixed_nev==1),
+ * then M angles theta = U*2*pi, returned as cos, sin.  Returns M (>0) or an
+ * error code.  number_of_mixed_events <= 0 selects floor(mixed_nev/2)+1. */
+int32_t hbt_rng_draw_plan(hbt_rng *rng, int32_t nev, int32_t mixed_nev,
+                          int32_t number_of_mixed_events, int32_t *partner_ids,
+                          double *cos_sin);
+
+/* Replaces calculate_HBT_correlation_function (HBT_correlation.cpp:177-218)
+ * on the buffers already uploaded: psi_ref (when azimuthal) computed by the
+ * caller beforehand, same-event pairs, then the mixing plan drawn from rng
+ * and the mixed pairs.  Histograms accumulate across calls like the
+ * reference's. */
+int hbt_calculate_buffer(hbt_ctx *ctx, hbt_rng *rng);
+
+/* Wall time (ms, HIP events on the context's stream) of the last
+ * hbt_run_same / hbt_run_mixed kernel launches. */
+int hbt_last_kernel_ms(hbt_ctx *ctx, double *same_ms, double *mixed_ms);
+
+/* Number of particles kept by the rapidity cut in a slot, and of events. */
+int hbt_slot_info(hbt_ctx *ctx, int32_t slot, int32_t *nev, int64_t *n_kept);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,702 @@
+/*
+ * hbt_oracle.c -- CPU oracle (test infrastructure only) for the HBT
+ * pair-correlation path.  Plain C restatement of the reference; every
+ * function cites the reference lines it follows.  Only tests/, the smoke
+ * check and bench.py's CPU-baseline leg may load it, and only as the checker.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math: strict
+ * IEEE double, no contraction, like the reference's g++ -O3 on x86-64).
+ *
+ * Pinning (what checks each part against something other than itself):
+ *  - orc_combine_same / orc_combine_mixed / orc_calculate_buffer:
+ *      bit-exact against the reference's own HBT_correlation compiled from
+ *      its sources (oracle/_ref, tests/test_oracle_vs_ref.py) on the
+ *      reference fixture and on seeded synthetic buffers (counts, sums as
+ *      hex floats); committed golden dumps (tests/golden/); hand-built
+ *      edge pairs (window edges +-1 ulp, K_T edges, > vs >=); closed-form
+ *      pair totals.
+ *  - orc_flow_event_plane_angle: HBT_unittest.cc:39,43,47 golden psi_ref.
+ *  - orc_rng_* / orc_draw_mixing_plan: first draws from seed 0 against the
+ *      values the reference's libstdc++ produces (1178568022,
+ *      0.71518936510962539) and against libstdc++ itself
+ *      (tests/test_oracle_rng.py compares with std::mt19937 +
+ *      std::uniform_*_distribution compiled by g++); whole plans pinned
+ *      indirectly through the mixed-event histograms of the reference runs.
+ *  - orc_reader_*: particleSamples_unittest.cc:199-206 counts (619/635 pi+)
+ *      on the reference fixture; golden comparison with the reference reader.
+ *  - orc_write_outputs: byte comparison with files written by the reference
+ *      binary on the same inputs.
+ */
+#define _GNU_SOURCE
+#include "hbt_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <zlib.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static const double ORC_HBARC = 0.197327053; /* parameters.h:4 */
+
+/* HBT_correlation.cpp:25-61, :253-256, :289-290; the constant expressions
+ * are evaluated exactly as the reference writes them. */
+void orc_derive(const orc_params *p, orc_derived *d) {
+    d->delta_q = (p->q_max - p->q_min) / (p->qnpts - 1);
+    d->dKT = (p->KT_max - p->KT_min) / (p->n_KT - 1);
+    d->dKphi = 2 * M_PI / p->n_Kphi;
+    d->q_base = p->q_min - d->delta_q / 2.;
+    d->q_lo_cut = p->q_min - d->delta_q / 2. + 1e-8;
+    d->q_hi_cut = p->q_max + d->delta_q / 2. - 1e-8;
+    d->KT_min_sq = p->KT_min * p->KT_min;
+    d->KT_max_sq = p->KT_max * p->KT_max;
+    d->hbarC_inv = 1. / ORC_HBARC;
+    d->rap_cut_max = tanh(p->HBTrap_max);
+    d->rap_cut_min = tanh(p->HBTrap_min);
+    d->needed = (uint64_t)(unsigned long long)p->needed_number_of_pairs;
+    d->needed_qinv = (uint64_t)50 * d->needed; /* wraps like ull in :341 */
+}
+
+int orc_init(orc_state *s, const orc_params *p) {
+    memset(s, 0, sizeof(*s));
+    s->p = *p;
+    orc_derive(p, &s->d);
+    s->nphi = p->azimuthal_flag ? p->n_Kphi : 1;
+    long q = p->qnpts;
+    s->nbins3d = (long)p->n_KT * s->nphi * q * q * q;
+    double **arrs[] = {&s->num_count, &s->q_out_sum, &s->q_side_sum,
+                       &s->q_long_sum, &s->num, &s->den, &s->abs_num,
+                       &s->abs_q_out, &s->abs_q_side, &s->abs_q_long};
+    for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) {
+        *arrs[k] = (double *)calloc((size_t)s->nbins3d, sizeof(double));
+        if (!*arrs[k]) return -1;
+    }
+    long nq = (long)p->n_KT * q;
+    double **qa[] = {&s->qinv_sum, &s->qinv_num, &s->qinv_count, &s->qinv_den};
+    for (size_t k = 0; k < 4; k++) {
+        *qa[k] = (double *)calloc((size_t)nq, sizeof(double));
+        if (!*qa[k]) return -1;
+    }
+    s->cnt_num_kt = (uint64_t *)calloc(p->n_KT, 8);
+    s->cnt_den_kt = (uint64_t *)calloc(p->n_KT, 8);
+    s->cnt_num_qinv = (uint64_t *)calloc(p->n_KT, 8);
+    s->cnt_den_qinv = (uint64_t *)calloc(p->n_KT, 8);
+    s->cnt_num_ktphi = (uint64_t *)calloc((size_t)p->n_KT * p->n_Kphi, 8);
+    s->cnt_den_ktphi = (uint64_t *)calloc((size_t)p->n_KT * p->n_Kphi, 8);
+    return 0;
+}
+
+void orc_free(orc_state *s) {
+    double *arrs[] = {s->num_count, s->q_out_sum, s->q_side_sum,
+                      s->q_long_sum, s->num, s->den, s->abs_num,
+                      s->abs_q_out, s->abs_q_side, s->abs_q_long,
+                      s->qinv_sum, s->qinv_num, s->qinv_count, s->qinv_den};
+    for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
+    free(s->cnt_num_kt); free(s->cnt_den_kt);
+    free(s->cnt_num_qinv); free(s->cnt_den_qinv);
+    free(s->cnt_num_ktphi); free(s->cnt_den_ktphi);
+    memset(s, 0, sizeof(*s));
+}
+
+/* HBT_correlation.cpp:233-249 */
+double orc_flow_event_plane_angle(const orc_buffer *b, int n_order) {
+    double vn_real = 0.0, vn_imag = 0.0;
+    for (int iev = 0; iev < b->nev; iev++) {
+        for (int64_t i = b->ev_off[iev]; i < b->ev_off[iev + 1]; i++) {
+            double temp_phi = atan2(b->py[i], b->px[i]);
+            /* g++ -O3 merges the reference's cos()/sin() of one argument
+             * into glibc sincos(); spelled out here so the result does not
+             * depend on the compiler doing the same */
+            double sn, cn;
+            sincos(n_order * temp_phi, &sn, &cn);
+            vn_real += cn;
+            vn_imag += sn;
+        }
+    }
+    return atan2(vn_imag, vn_real) / n_order;
+}
+
+typedef struct {
+    long n, cap;
+    double *px, *py, *pz, *E, *x, *y, *z, *t;
+} plist;
+
+static void pl_reserve(plist *l, long n) {
+    if (n <= l->cap) return;
+    long c = l->cap ? l->cap : 256;
+    while (c < n) c *= 2;
+    double **a[] = {&l->px, &l->py, &l->pz, &l->E, &l->x, &l->y, &l->z, &l->t};
+    for (int k = 0; k < 8; k++) *a[k] = (double *)realloc(*a[k], c * sizeof(double));
+    l->cap = c;
+}
+static void pl_push(plist *l, double px, double py, double pz, double E,
+                    double x, double y, double z, double t) {
+    pl_reserve(l, l->n + 1);
+    long k = l->n++;
+    l->px[k] = px; l->py[k] = py; l->pz[k] = pz; l->E[k] = E;
+    l->x[k] = x; l->y[k] = y; l->z[k] = z; l->t[k] = t;
+}
+static void pl_free(plist *l) {
+    free(l->px); free(l->py); free(l->pz); free(l->E);
+    free(l->x); free(l->y); free(l->z); free(l->t);
+    memset(l, 0, sizeof(*l));
+}
+
+static inline long bin3(const orc_state *s, int kt, int kphi, int io, int is, int il) {
+    long q = s->p.qnpts;
+    return ((((long)kt * s->nphi + kphi) * q + io) * q + is) * q + il;
+}
+
+/* K_phi index with the reference's wrap loops (:408-423 / :657-672).
+ * Returns -1 when the pair is skipped. */
+static int kphi_index(const orc_state *s, double K_x, double K_y) {
+    double local_K_phi = atan2(K_y, K_x);
+    double delta_phi = local_K_phi - s->psi_ref;
+    while (delta_phi < 0.) delta_phi += 2. * M_PI;
+    while (delta_phi > 2. * M_PI) delta_phi -= 2. * M_PI;
+    int idx = (int)(delta_phi / s->d.dKphi);
+    if (idx < 0 || idx >= s->p.n_Kphi) return -1;
+    return idx;
+}
+
+/* Same-event pairs: gather with rapidity cut (HBT_correlation.cpp:253-281),
+ * then the i<j loop (:289-460). */
+void orc_combine_same(orc_state *s, const orc_buffer *b) {
+    const orc_derived *d = &s->d;
+    const orc_params *p = &s->p;
+    plist L = {0};
+    for (int ev = 0; ev < b->nev; ev++) {
+        for (int64_t i = b->ev_off[ev]; i < b->ev_off[ev + 1]; i++) {
+            double temp_ratio = b->pz[i] / b->E[i];
+            if (temp_ratio > d->rap_cut_min && temp_ratio < d->rap_cut_max)
+                pl_push(&L, b->px[i], b->py[i], b->pz[i], b->E[i], b->x[i],
+                        b->y[i], b->z[i], b->t[i]);
+        }
+    }
+    const long n = L.n;
+    const int qnpts = p->qnpts;
+    for (long i = 0; i < n; i++) {
+        const double p1x = L.px[i], p1y = L.py[i], p1z = L.pz[i], p1E = L.E[i];
+        const double x1 = L.x[i], y1 = L.y[i], z1 = L.z[i], t1 = L.t[i];
+        for (long j = i + 1; j < n; j++) {
+            const double p2x = L.px[j], p2y = L.py[j], p2z = L.pz[j], p2E = L.E[j];
+            s->same_st.enumerated++;
+            double K_z = 0.5 * (p1z + p2z);
+            double K_E = 0.5 * (p1E + p2E);
+            double K_z_over_K_E = K_z / K_E;
+            double K_x = 0.5 * (p1x + p2x);
+            double K_y = 0.5 * (p1y + p2y);
+            double K_perp_sq = K_x * K_x + K_y * K_y;
+            if (K_perp_sq < d->KT_min_sq || K_perp_sq > d->KT_max_sq) continue;
+            s->same_st.kt++;
+            double K_perp = sqrt(K_perp_sq);
+            int Kperp_idx = (int)((K_perp - p->KT_min) / d->dKT);
+            double q_x = p1x - p2x;
+            double q_y = p1y - p2y;
+            double q_z = p1z - p2z;
+            double q_E = p1E - p2E;
+            double t_diff = t1 - L.t[j];
+            double x_diff = x1 - L.x[j];
+            double y_diff = y1 - L.y[j];
+            double z_diff = z1 - L.z[j];
+            if (p->invariant_radius_flag == 1) { /* :339-356 */
+                double local_q_inv = sqrt(-(q_E * q_E - q_x * q_x - q_y * q_y - q_z * q_z));
+                if (s->cnt_num_qinv[Kperp_idx] < d->needed_qinv) {
+                    if (local_q_inv > d->q_lo_cut && local_q_inv < d->q_hi_cut) {
+                        int qinv_idx = (int)((local_q_inv - d->q_base) / d->delta_q);
+                        /* the reference has no upper check here; an index
+                         * of qnpts cannot occur for dq >> 1e-8 */
+                        if (qinv_idx < qnpts) {
+                            s->cnt_num_qinv[Kperp_idx]++;
+                            double cos_qx = cos(d->hbarC_inv *
+                                (q_E * t_diff - q_x * x_diff - q_y * y_diff - q_z * z_diff));
+                            long k = (long)Kperp_idx * qnpts + qinv_idx;
+                            s->qinv_count[k]++;
+                            s->qinv_sum[k] += local_q_inv;
+                            s->qinv_num[k] += cos_qx;
+                        }
+                    }
+                }
+            }
+            double cos_K_phi = K_x / K_perp;
+            double sin_K_phi = K_y / K_perp;
+            double local_q_out = q_x * cos_K_phi + q_y * sin_K_phi;
+            if (local_q_out < d->q_lo_cut || local_q_out > d->q_hi_cut) continue;
+            int qout_idx = (int)((local_q_out - d->q_base) / d->delta_q);
+            if (qout_idx >= qnpts) continue;
+            s->same_st.out++;
+            double local_q_side = q_y * cos_K_phi - q_x * sin_K_phi;
+            if (local_q_side < d->q_lo_cut || local_q_side > d->q_hi_cut) continue;
+            int qside_idx = (int)((local_q_side - d->q_base) / d->delta_q);
+            if (qside_idx >= qnpts) continue;
+            s->same_st.side++;
+            double local_q_long = q_z;
+            if (p->long_comoving_boost) {
+                double Mt = sqrt(K_E * K_E - K_z * K_z);
+                double boost_gamma = K_E / Mt;
+                double boost_beta = K_z_over_K_E;
+                local_q_long = boost_gamma * (q_z - boost_beta * q_E);
+            }
+            if (local_q_long < d->q_lo_cut || local_q_long > d->q_hi_cut) continue;
+            int qlong_idx = (int)((local_q_long - d->q_base) / d->delta_q);
+            if (qlong_idx >= qnpts) continue;
+            int Kphi_idx = 0;
+            if (p->azimuthal_flag == 0) {
+                if (s->cnt_num_kt[Kperp_idx] > d->needed) continue;
+                s->cnt_num_kt[Kperp_idx]++;
+            } else {
+                Kphi_idx = kphi_index(s, K_x, K_y);
+                if (Kphi_idx < 0) { s->kphi_skipped++; continue; }
+                uint64_t *c = &s->cnt_num_ktphi[(long)Kperp_idx * p->n_Kphi + Kphi_idx];
+                if (*c > d->needed) continue;
+                (*c)++;
+            }
+            s->same_st.binned++;
+            double cos_qx = cos(d->hbarC_inv *
+                (q_E * t_diff - q_x * x_diff - q_y * y_diff - q_z * z_diff));
+            long k = bin3(s, Kperp_idx, Kphi_idx, qout_idx, qside_idx, qlong_idx);
+            s->num_count[k]++;
+            s->q_out_sum[k] += local_q_out;
+            s->q_side_sum[k] += local_q_side;
+            s->q_long_sum[k] += local_q_long;
+            s->num[k] += cos_qx;
+            s->abs_num[k] += fabs(cos_qx);
+            s->abs_q_out[k] += fabs(local_q_out);
+            s->abs_q_side[k] += fabs(local_q_side);
+            s->abs_q_long[k] += fabs(local_q_long);
+        }
+    }
+    pl_free(&L);
+}
+
+/* Mixed pairs of event iev (HBT_correlation.cpp:464-692). */
+void orc_combine_mixed(orc_state *s, const orc_buffer *b, int iev,
+                       const orc_buffer *mix, int M, const int *partner_ids,
+                       const double *cs) {
+    const orc_derived *d = &s->d;
+    const orc_params *p = &s->p;
+    plist L1 = {0}, L2 = {0};
+    for (int64_t i = b->ev_off[iev]; i < b->ev_off[iev + 1]; i++) { /* :471-489 */
+        double temp_ratio = b->pz[i] / b->E[i];
+        if (temp_ratio > d->rap_cut_min && temp_ratio < d->rap_cut_max)
+            pl_push(&L1, b->px[i], b->py[i], b->pz[i], b->E[i], 0, 0, 0, 0);
+    }
+    for (int k = 0; k < M; k++) { /* :493-552 */
+        const double cos_phi = cs[2 * k], sin_phi = cs[2 * k + 1];
+        int id = partner_ids[k];
+        for (int64_t i = mix->ev_off[id]; i < mix->ev_off[id + 1]; i++) {
+            double temp_ratio = mix->pz[i] / mix->E[i];
+            if (temp_ratio > d->rap_cut_min && temp_ratio < d->rap_cut_max) {
+                double px_temp = mix->px[i], py_temp = mix->py[i];
+                pl_push(&L2, px_temp * cos_phi - py_temp * sin_phi,
+                        px_temp * sin_phi + py_temp * cos_phi, mix->pz[i],
+                        mix->E[i], 0, 0, 0, 0);
+            }
+        }
+    }
+    const int qnpts = p->qnpts;
+    for (long i = 0; i < L1.n; i++) {
+        const double p1z = L1.pz[i], p1x = L1.px[i], p1y = L1.py[i], p1E = L1.E[i];
+        for (long j = 0; j < L2.n; j++) {
+            const double p2z = L2.pz[j], p2x = L2.px[j], p2y = L2.py[j], p2E = L2.E[j];
+            s->mixed_st.enumerated++;
+            double K_z = 0.5 * (p1z + p2z);
+            double K_E = 0.5 * (p1E + p2E);
+            double K_z_over_K_E = K_z / K_E;
+            double K_x = 0.5 * (p1x + p2x);
+            double K_y = 0.5 * (p1y + p2y);
+            double K_perp_sq = K_x * K_x + K_y * K_y;
+            if (K_perp_sq < d->KT_min_sq || K_perp_sq > d->KT_max_sq) continue;
+            s->mixed_st.kt++;
+            double K_perp = sqrt(K_perp_sq);
+            int Kperp_idx = (int)((K_perp - p->KT_min) / d->dKT);
+            double q_z = p1z - p2z;
+            double q_E = p1E - p2E;
+            double q_x = p1x - p2x;
+            double q_y = p1y - p2y;
+            if (p->invariant_radius_flag == 1) { /* :592-607 */
+                double local_q_inv = sqrt(-(q_E * q_E - q_x * q_x - q_y * q_y - q_z * q_z));
+                if (local_q_inv > d->q_lo_cut && local_q_inv < d->q_hi_cut) {
+                    int qinv_idx = (int)((local_q_inv - d->q_base) / d->delta_q);
+                    if (qinv_idx < qnpts && s->cnt_den_qinv[Kperp_idx] < d->needed_qinv) {
+                        s->cnt_den_qinv[Kperp_idx]++;
+                        s->qinv_den[(long)Kperp_idx * qnpts + qinv_idx] += 1.0;
+                    }
+                }
+            }
+            double cos_K_phi = K_x / K_perp;
+            double sin_K_phi = K_y / K_perp;
+            double local_q_out = q_x * cos_K_phi + q_y * sin_K_phi;
+            if (local_q_out < d->q_lo_cut || local_q_out >= d->q_hi_cut) continue;
+            int qout_idx = (int)((local_q_out - d->q_base) / d->delta_q);
+            if (qout_idx >= qnpts) continue;
+            s->mixed_st.out++;
+            double local_q_side = q_y * cos_K_phi - q_x * sin_K_phi;
+            if (local_q_side < d->q_lo_cut || local_q_side >= d->q_hi_cut) continue;
+            int qside_idx = (int)((local_q_side - d->q_base) / d->delta_q);
+            if (qside_idx >= qnpts) continue;
+            s->mixed_st.side++;
+            double local_q_long = q_z;
+            if (p->long_comoving_boost) {
+                double Mt = sqrt(K_E * K_E - K_z * K_z);
+                double boost_gamma = K_E / Mt;
+                double boost_beta = K_z_over_K_E;
+                local_q_long = boost_gamma * (q_z - boost_beta * q_E);
+            }
+            if (local_q_long < d->q_lo_cut || local_q_long >= d->q_hi_cut) continue;
+            int qlong_idx = (int)((local_q_long - d->q_base) / d->delta_q);
+            if (qlong_idx >= qnpts) continue;
+            int Kphi_idx = 0;
+            if (p->azimuthal_flag == 0) {
+                if (s->cnt_den_kt[Kperp_idx] > d->needed) continue;
+                s->cnt_den_kt[Kperp_idx]++;
+            } else {
+                Kphi_idx = kphi_index(s, K_x, K_y);
+                if (Kphi_idx < 0) { s->kphi_skipped++; continue; }
+                uint64_t *c = &s->cnt_den_ktphi[(long)Kperp_idx * p->n_Kphi + Kphi_idx];
+                if (*c > d->needed) continue;
+                (*c)++;
+            }
+            s->mixed_st.binned++;
+            s->den[bin3(s, Kperp_idx, Kphi_idx, qout_idx, qside_idx, qlong_idx)] += 1.0;
+        }
+    }
+    pl_free(&L1);
+    pl_free(&L2);
+}
+
+/* ---------------- random stream ----------------
+ * std::mt19937 (Matsumoto & Nishimura 1998, the parameters fixed by
+ * [rand.predef]) and libstdc++ 11.4's distributions:
+ *  - uniform_int_distribution<int>() over [0, INT_MAX]: bits/uniform_int_dist.h
+ *    takes Lemire's nearly-divisionless path for a 32-bit engine
+ *    (64-bit product, threshold = -range % range).
+ *  - uniform_real_distribution<double>(0,1): generate_canonical<double,53>
+ *    with two engine calls: (u1 + u2*2^32)/2^64, clamped below 1
+ *    (bits/random.tcc), then *(b-a)+a.
+ * Random.h:11-25, Random.cpp:7-14. */
+void orc_rng_seed(orc_rng *r, uint32_t seed) {
+    r->mt[0] = seed;
+    for (int i = 1; i < 624; i++)
+        r->mt[i] = 1812433253u * (r->mt[i - 1] ^ (r->mt[i - 1] >> 30)) + (uint32_t)i;
+    r->idx = 624;
+}
+
+uint32_t orc_rng_next(orc_rng *r) {
+    if (r->idx >= 624) {
+        for (int i = 0; i < 624; i++) {
+            uint32_t y = (r->mt[i] & 0x80000000u) | (r->mt[(i + 1) % 624] & 0x7fffffffu);
+            r->mt[i] = r->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        r->idx = 0;
+    }
+    uint32_t y = r->mt[r->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+int orc_rand_int_uniform(orc_rng *r) {
+    const uint32_t range = 0x80000000u; /* INT_MAX - 0 + 1 */
+    uint64_t product = (uint64_t)orc_rng_next(r) * (uint64_t)range;
+    uint32_t low = (uint32_t)product;
+    if (low < range) {
+        uint32_t threshold = (uint32_t)(-range) % range;
+        while (low < threshold) {
+            product = (uint64_t)orc_rng_next(r) * (uint64_t)range;
+            low = (uint32_t)product;
+        }
+    }
+    return (int)(product >> 32);
+}
+
+double orc_rand_uniform(orc_rng *r) {
+    double sum = 0.0, tmp = 1.0;
+    for (int k = 0; k < 2; k++) {
+        sum += (double)orc_rng_next(r) * tmp;
+        tmp *= 4294967296.0;
+    }
+    double ret = sum / tmp;
+    if (ret >= 1.0) ret = nextafter(1.0, 0.0);
+    return ret * (1.0 - 0.0) + 0.0;
+}
+
+/* HBT_correlation.cpp:200-216 (ids) interleaved with :495-497 (angles). */
+int orc_draw_mixing_plan(orc_rng *r, int nev, int mixed_nev,
+                         int number_of_mixed_events, int *ids, double *cs) {
+    int M = number_of_mixed_events > 0 ? number_of_mixed_events : mixed_nev / 2 + 1;
+    for (int iev = 0; iev < nev; iev++) {
+        for (int count = 0; count < M; count++) {
+            int id = orc_rand_int_uniform(r) % mixed_nev;
+            while (iev == id && mixed_nev != 1) id = orc_rand_int_uniform(r) % mixed_nev;
+            ids[(long)iev * M + count] = id;
+        }
+        for (int k = 0; k < M; k++) {
+            const double random_rotation = orc_rand_uniform(r) * 2 * M_PI;
+            double sn, cn;
+            sincos(random_rotation, &sn, &cn); /* see orc_flow_event_plane_angle */
+            cs[2 * ((long)iev * M + k)] = cn;
+            cs[2 * ((long)iev * M + k) + 1] = sn;
+        }
+    }
+    return M;
+}
+
+/* HBT_correlation.cpp:177-218 */
+void orc_calculate_buffer(orc_state *s, orc_rng *r, const orc_buffer *b,
+                          const orc_buffer *mix) {
+    if (s->p.azimuthal_flag == 1) s->psi_ref = orc_flow_event_plane_angle(b, 2);
+    orc_combine_same(s, b);
+    int nev = b->nev, mixed_nev = mix->nev;
+    if (nev <= 0) return;
+    int M = s->p.number_of_mixed_events > 0 ? s->p.number_of_mixed_events
+                                            : mixed_nev / 2 + 1;
+    int *ids = (int *)malloc(sizeof(int) * (size_t)M);
+    double *cs = (double *)malloc(sizeof(double) * 2 * (size_t)M);
+    for (int iev = 0; iev < nev; iev++) {
+        /* per event: M ids, then M angles inside the mixed call */
+        for (int count = 0; count < M; count++) {
+            int id = orc_rand_int_uniform(r) % mixed_nev;
+            while (iev == id && mixed_nev != 1) id = orc_rand_int_uniform(r) % mixed_nev;
+            ids[count] = id;
+        }
+        for (int k = 0; k < M; k++) {
+            const double random_rotation = orc_rand_uniform(r) * 2 * M_PI;
+            double sn, cn;
+            sincos(random_rotation, &sn, &cn);
+            cs[2 * k] = cn;
+            cs[2 * k + 1] = sn;
+        }
+        orc_combine_mixed(s, b, iev, mix, M, ids, cs);
+    }
+    free(ids);
+    free(cs);
+}
+
+/* ---------------- writers ----------------
+ * Formatting of the reference's iostreams: `std::scientific <<
+ * std::setw(18) << std::setprecision(8)` on the first field only, then
+ * "    " separators (printf "%18.8e" / "%.8e"); file names print KT and
+ * K_phi values with the default ostream format ("%g"). */
+static int write_3d(const orc_state *s, const char *path, int eco) { /* :726-783 */
+    const orc_params *p = &s->p;
+    const int q = p->qnpts;
+    const double dq = s->d.delta_q;
+    for (int iK = 0; iK < p->n_KT - 1; iK++) {
+        double npair_ratio = (double)s->cnt_num_kt[iK] / (double)s->cnt_den_kt[iK];
+        char fn[4096];
+        snprintf(fn, sizeof(fn), "%s/HBT_correlation_function_KT_%g_%g.dat", path,
+                 p->KT_min + iK * s->d.dKT, p->KT_min + (iK + 1) * s->d.dKT);
+        FILE *f = fopen(fn, "w");
+        if (!f) return -1;
+        for (int il = 0; il < q; il++)
+            for (int io = 0; io < q; io++)
+                for (int is = 0; is < q; is++) {
+                    long k = bin3(s, iK, 0, io, is, il);
+                    int npart_num = (int)s->num_count[k];
+                    int npart_denorm = (int)s->den[k];
+                    double qo, qs, ql, cnum, cden;
+                    if (npart_num < 2 || npart_denorm < 2) {
+                        qo = p->q_min + io * dq;
+                        qs = p->q_min + is * dq;
+                        ql = p->q_min + il * dq;
+                        cnum = 0.0;
+                        cden = npart_denorm;
+                    } else {
+                        qo = s->q_out_sum[k] / npart_num;
+                        qs = s->q_side_sum[k] / npart_num;
+                        ql = s->q_long_sum[k] / npart_num;
+                        cnum = s->num[k];
+                        cden = npair_ratio * s->den[k];
+                    }
+                    if (eco == 1)
+                        fprintf(f, "%18.8e    %.8e\n", cnum, cden);
+                    else
+                        fprintf(f, "%18.8e    %.8e    %.8e    %.8e    %.8e\n", qo, qs,
+                                ql, cnum, cden);
+                }
+        fclose(f);
+    }
+    return 0;
+}
+
+static int write_3d_kphi(const orc_state *s, const char *path, int eco) { /* :785-855 */
+    const orc_params *p = &s->p;
+    const int q = p->qnpts;
+    const double dq = s->d.delta_q;
+    for (int iK = 0; iK < p->n_KT - 1; iK++) {
+        for (int iphi = 0; iphi < p->n_Kphi; iphi++) {
+            double npair_ratio =
+                (double)s->cnt_num_ktphi[(long)iK * p->n_Kphi + iphi] /
+                (double)s->cnt_den_ktphi[(long)iK * p->n_Kphi + iphi];
+            char fn[4096];
+            snprintf(fn, sizeof(fn),
+                     "%s/HBT_correlation_function_KT_%g_%g_Kphi_%g.dat", path,
+                     p->KT_min + iK * s->d.dKT, p->KT_min + (iK + 1) * s->d.dKT,
+                     iphi * s->d.dKphi);
+            FILE *f = fopen(fn, "w");
+            if (!f) return -1;
+            for (int il = 0; il < q; il++)
+                for (int io = 0; io < q; io++)
+                    for (int is = 0; is < q; is++) {
+                        long k = bin3(s, iK, iphi, io, is, il);
+                        int npart_num = (int)s->num_count[k];
+                        int npart_denorm = (int)s->den[k];
+                        double qo, qs, ql, cnum, cden;
+                        if (npart_num < 2 || npart_denorm < 2) {
+                            qo = p->q_min + io * dq;
+                            qs = p->q_min + is * dq;
+                            ql = p->q_min + il * dq;
+                            cnum = 0.0;
+                            cden = npart_denorm;
+                        } else {
+                            qo = s->q_out_sum[k] / npart_num;
+                            qs = s->q_side_sum[k] / npart_num;
+                            ql = s->q_long_sum[k] / npart_num;
+                            cnum = s->num[k];
+                            cden = npair_ratio * s->den[k];
+                        }
+                        if (eco == 1)
+                            fprintf(f, "%18.8e    %.8e\n", cnum, cden);
+                        else
+                            fprintf(f, "%18.8e    %.8e    %.8e    %.8e    %.8e\n",
+                                    qo, qs, ql, cnum, cden);
+                    }
+            fclose(f);
+        }
+    }
+    return 0;
+}
+
+static int write_inv(const orc_state *s, const char *path, int eco) { /* :694-724 */
+    const orc_params *p = &s->p;
+    for (int iK = 0; iK < p->n_KT - 1; iK++) {
+        double npair_ratio = (double)s->cnt_num_qinv[iK] / (double)s->cnt_den_qinv[iK];
+        char fn[4096];
+        snprintf(fn, sizeof(fn), "%s/HBT_correlation_function_inv_KT_%g_%g.dat", path,
+                 p->KT_min + iK * s->d.dKT, p->KT_min + (iK + 1) * s->d.dKT);
+        FILE *f = fopen(fn, "w");
+        if (!f) return -1;
+        for (int iq = 0; iq < p->qnpts; iq++) {
+            long k = (long)iK * p->qnpts + iq;
+            double q_inv_local = s->qinv_sum[k] / s->qinv_count[k];
+            double cnum = s->qinv_num[k];
+            double cden = s->qinv_den[k] * npair_ratio;
+            if (eco == 1)
+                fprintf(f, "%18.8e    %.8e\n", cnum, cden);
+            else
+                fprintf(f, "%18.8e    %.8e    %.8e\n", q_inv_local, cnum, cden);
+        }
+        fclose(f);
+    }
+    return 0;
+}
+
+/* HBT_correlation.cpp:220-228 */
+int orc_write_outputs(const orc_state *s, const char *path, int eco_output) {
+    int rc = 0;
+    if (s->p.invariant_radius_flag == 1) rc |= write_inv(s, path, eco_output);
+    if (s->p.azimuthal_flag == 0)
+        rc |= write_3d(s, path, eco_output);
+    else
+        rc |= write_3d_kphi(s, path, eco_output);
+    return rc;
+}
+
+/* ---------------- mode-10 reader ---------------- */
+struct orc_reader {
+    gzFile gz;
+    int event_buffer_size;
+    int monval;
+    int eof;
+    long total;
+    plist L;
+    int64_t *off;
+    int off_cap;
+    orc_buffer buf;
+};
+
+orc_reader *orc_reader_open(const char *filename, int event_buffer_size,
+                            int particle_monval) {
+    gzFile gz = gzopen(filename, "rb");
+    if (!gz) return NULL;
+    orc_reader *rd = (orc_reader *)calloc(1, sizeof(orc_reader));
+    rd->gz = gz;
+    rd->event_buffer_size = event_buffer_size;
+    rd->monval = particle_monval;
+    return rd;
+}
+
+/* One line without its '\n' (particleSamples.cpp:2209-2218).  Sets *eof
+ * like gzeof(): only when a read found no more data. */
+static int read_line(orc_reader *rd, char *buf, int cap) {
+    int n = 0;
+    for (;;) {
+        int c = gzgetc(rd->gz);
+        if (c < 0) { rd->eof = 1; break; }
+        if (c == '\n') break;
+        if (n < cap - 1) buf[n++] = (char)c;
+    }
+    buf[n] = 0;
+    return n;
+}
+
+/* particleSamples.cpp:1247-1286 (read whole events while the running
+ * particle total is below event_buffer_size), :1329-1346 (species filter by
+ * exact monval; 99999 keeps all). */
+int orc_reader_next(orc_reader *rd) {
+    rd->L.n = 0;
+    rd->total = 0;
+    int nev = 0;
+    char line[4096];
+    long num_particles = 0;
+    if (rd->off_cap < 1) { rd->off_cap = 64; rd->off = (int64_t *)malloc(8 * 64); }
+    rd->off[0] = 0;
+    while (num_particles < rd->event_buffer_size) {
+        read_line(rd, line, sizeof(line));
+        if (rd->eof) break;
+        int n_particle = 0;
+        sscanf(line, "%d", &n_particle);
+        for (int ip = 0; ip < n_particle; ip++) {
+            read_line(rd, line, sizeof(line));
+            int mon = 0;
+            double mass, t, x, y, z, E, px, py, pz;
+            mass = t = x = y = z = E = px = py = pz = 0.0;
+            sscanf(line, "%d %lf %lf %lf %lf %lf %lf %lf %lf %lf", &mon, &mass, &t, &x,
+                   &y, &z, &E, &px, &py, &pz);
+            if (rd->monval == 99999 || mon == rd->monval)
+                pl_push(&rd->L, px, py, pz, E, x, y, z, t);
+        }
+        num_particles += n_particle;
+        nev++;
+        if (nev + 1 > rd->off_cap) {
+            rd->off_cap *= 2;
+            rd->off = (int64_t *)realloc(rd->off, 8 * (size_t)rd->off_cap);
+        }
+        rd->off[nev] = rd->L.n;
+    }
+    rd->total = num_particles;
+    rd->buf.nev = nev;
+    rd->buf.ev_off = rd->off;
+    rd->buf.px = rd->L.px; rd->buf.py = rd->L.py; rd->buf.pz = rd->L.pz;
+    rd->buf.E = rd->L.E; rd->buf.x = rd->L.x; rd->buf.y = rd->L.y;
+    rd->buf.z = rd->L.z; rd->buf.t = rd->L.t;
+    return nev;
+}
+
+int orc_reader_eof(const orc_reader *rd) { return rd->eof; }
+const orc_buffer *orc_reader_buffer(const orc_reader *rd) { return &rd->buf; }
+long orc_reader_total_particles(const orc_reader *rd) { return rd->total; }
+
+void orc_reader_close(orc_reader *rd) {
+    if (!rd) return;
+    gzclose(rd->gz);
+    pl_free(&rd->L);
+    free(rd->off);
+    free(rd);
+}

@@ -1,0 +1,269 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this module, and only as the checker.  See hbt_oracle.c for what each
+function restates and how it is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("qnpts", C.c_int), ("q_min", C.c_double), ("q_max", C.c_double),
+        ("n_KT", C.c_int), ("n_Kphi", C.c_int), ("KT_min", C.c_double),
+        ("KT_max", C.c_double), ("azimuthal_flag", C.c_int),
+        ("invariant_radius_flag", C.c_int), ("long_comoving_boost", C.c_int),
+        ("HBTrap_min", C.c_double), ("HBTrap_max", C.c_double),
+        ("needed_number_of_pairs", C.c_double), ("number_of_mixed_events", C.c_int),
+    ]
+
+
+class Derived(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "delta_q", "dKT", "dKphi", "q_base", "q_lo_cut", "q_hi_cut", "KT_min_sq",
+        "KT_max_sq", "hbarC_inv", "rap_cut_min", "rap_cut_max")] + [
+        ("needed", C.c_uint64), ("needed_qinv", C.c_uint64)]
+
+
+class Stages(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("enumerated", "kt", "out", "side", "binned")]
+
+
+_D = C.POINTER(C.c_double)
+_U = C.POINTER(C.c_uint64)
+
+
+class State(C.Structure):
+    _fields_ = [("p", Params), ("d", Derived), ("psi_ref", C.c_double), ("nphi", C.c_int),
+                ("nbins3d", C.c_long)] + [
+        (n, _D) for n in ("num_count", "q_out_sum", "q_side_sum", "q_long_sum", "num", "den",
+                          "abs_num", "abs_q_out", "abs_q_side", "abs_q_long", "qinv_sum",
+                          "qinv_num", "qinv_count", "qinv_den")] + [
+        (n, _U) for n in ("cnt_num_kt", "cnt_den_kt", "cnt_num_qinv", "cnt_den_qinv",
+                          "cnt_num_ktphi", "cnt_den_ktphi")] + [
+        ("same_st", Stages), ("mixed_st", Stages), ("kphi_skipped", C.c_uint64)]
+
+
+class BufferC(C.Structure):
+    _fields_ = [("nev", C.c_int), ("ev_off", C.POINTER(C.c_int64))] + [
+        (n, _D) for n in ("px", "py", "pz", "E", "x", "y", "z", "t")]
+
+
+class Rng(C.Structure):
+    _fields_ = [("mt", C.c_uint32 * 624), ("idx", C.c_int)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_init.argtypes = [C.POINTER(State), C.POINTER(Params)]
+        L.orc_free.argtypes = [C.POINTER(State)]
+        L.orc_flow_event_plane_angle.argtypes = [C.POINTER(BufferC), C.c_int]
+        L.orc_flow_event_plane_angle.restype = C.c_double
+        L.orc_combine_same.argtypes = [C.POINTER(State), C.POINTER(BufferC)]
+        L.orc_combine_mixed.argtypes = [C.POINTER(State), C.POINTER(BufferC), C.c_int,
+                                        C.POINTER(BufferC), C.c_int, C.POINTER(C.c_int), _D]
+        L.orc_rng_seed.argtypes = [C.POINTER(Rng), C.c_uint32]
+        L.orc_rng_next.argtypes = [C.POINTER(Rng)]
+        L.orc_rng_next.restype = C.c_uint32
+        L.orc_rand_int_uniform.argtypes = [C.POINTER(Rng)]
+        L.orc_rand_int_uniform.restype = C.c_int
+        L.orc_rand_uniform.argtypes = [C.POINTER(Rng)]
+        L.orc_rand_uniform.restype = C.c_double
+        L.orc_draw_mixing_plan.argtypes = [C.POINTER(Rng), C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_int), _D]
+        L.orc_draw_mixing_plan.restype = C.c_int
+        L.orc_calculate_buffer.argtypes = [C.POINTER(State), C.POINTER(Rng),
+                                           C.POINTER(BufferC), C.POINTER(BufferC)]
+        L.orc_write_outputs.argtypes = [C.POINTER(State), C.c_char_p, C.c_int]
+        L.orc_write_outputs.restype = C.c_int
+        L.orc_reader_open.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        L.orc_reader_open.restype = C.c_void_p
+        L.orc_reader_next.argtypes = [C.c_void_p]
+        L.orc_reader_eof.argtypes = [C.c_void_p]
+        L.orc_reader_buffer.argtypes = [C.c_void_p]
+        L.orc_reader_buffer.restype = C.POINTER(BufferC)
+        L.orc_reader_total_particles.argtypes = [C.c_void_p]
+        L.orc_reader_total_particles.restype = C.c_long
+        L.orc_reader_close.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+DEFAULTS = dict(qnpts=31, q_min=-0.15, q_max=0.15, n_KT=5, KT_min=0.15, KT_max=0.55,
+                n_Kphi=48, azimuthal_flag=0, invariant_radius_flag=0,
+                long_comoving_boost=1, HBTrap_min=-0.5, HBTrap_max=0.5,
+                needed_number_of_pairs=30000000, number_of_mixed_events=0)
+
+
+class CBuffer:
+    """Keeps numpy arrays alive behind an orc_buffer."""
+
+    def __init__(self, buf):
+        self.arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in buf.soa()]
+        self.off = np.ascontiguousarray(buf.ev_off, dtype=np.int64)
+        self.c = BufferC()
+        self.c.nev = int(buf.nev)
+        self.c.ev_off = self.off.ctypes.data_as(C.POINTER(C.c_int64))
+        for name, a in zip(("px", "py", "pz", "E", "x", "y", "z", "t"), self.arrs):
+            setattr(self.c, name, a.ctypes.data_as(_D))
+
+
+class Oracle:
+    """CPU restatement of HBT_correlation: same accumulation semantics."""
+
+    def __init__(self, params: dict | None = None):
+        p = dict(DEFAULTS)
+        if params:
+            p.update({k: v for k, v in params.items() if k in DEFAULTS})
+        self.params = p
+        self.L = lib()
+        cp = Params()
+        for name, _ in Params._fields_:
+            setattr(cp, name, p[name])
+        self.s = State()
+        if self.L.orc_init(C.byref(self.s), C.byref(cp)):
+            raise MemoryError("orc_init")
+        self.nphi = self.s.nphi
+
+    def __del__(self):
+        try:
+            self.L.orc_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    @property
+    def derived(self):
+        return self.s.d
+
+    def combine_same(self, buf):
+        b = CBuffer(buf)
+        self.L.orc_combine_same(C.byref(self.s), C.byref(b.c))
+
+    def combine_mixed(self, buf, iev, mix, M, ids, cs):
+        b, m = CBuffer(buf), (CBuffer(mix) if mix is not buf else None)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        cs = np.ascontiguousarray(cs, dtype=np.float64)
+        self.L.orc_combine_mixed(C.byref(self.s), C.byref(b.c), int(iev),
+                                 C.byref((m or b).c), int(M),
+                                 ids.ctypes.data_as(C.POINTER(C.c_int)), cs.ctypes.data_as(_D))
+
+    def run_mixed_plan(self, buf, M, ids, cs, mix=None):
+        """All events of buf with a pre-drawn plan (ids[nev*M], cs[2*nev*M])."""
+        b = CBuffer(buf)
+        m = CBuffer(mix) if mix is not None else b
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        cs = np.ascontiguousarray(cs, dtype=np.float64)
+        for e in range(buf.nev):
+            self.L.orc_combine_mixed(
+                C.byref(self.s), C.byref(b.c), e, C.byref(m.c), int(M),
+                ids[e * M:].ctypes.data_as(C.POINTER(C.c_int)),
+                cs[2 * e * M:].ctypes.data_as(_D))
+
+    def calculate_buffer(self, buf, rng: "OracleRng", mix=None):
+        b = CBuffer(buf)
+        m = CBuffer(mix) if mix is not None else b
+        self.L.orc_calculate_buffer(C.byref(self.s), C.byref(rng.r), C.byref(b.c),
+                                    C.byref(m.c))
+
+    def set_psi_ref(self, psi):
+        self.s.psi_ref = psi
+
+    def write_outputs(self, path: str, eco: int = 0) -> int:
+        return self.L.orc_write_outputs(C.byref(self.s), path.encode(), eco)
+
+    def arrays(self) -> dict:
+        s = self.s
+        nb = int(s.nbins3d)
+        q = int(s.p.qnpts)
+        nkt = int(s.p.n_KT)
+        out = {}
+        for name in ("num_count", "q_out_sum", "q_side_sum", "q_long_sum", "num", "den",
+                     "abs_num", "abs_q_out", "abs_q_side", "abs_q_long"):
+            out[name] = np.ctypeslib.as_array(getattr(s, name), shape=(nb,)).copy()
+        for name in ("qinv_sum", "qinv_num", "qinv_count", "qinv_den"):
+            out[name] = np.ctypeslib.as_array(getattr(s, name), shape=(nkt * q,)).copy()
+        for name in ("cnt_num_kt", "cnt_den_kt", "cnt_num_qinv", "cnt_den_qinv"):
+            out[name] = np.ctypeslib.as_array(getattr(s, name), shape=(nkt,)).copy()
+        for name in ("cnt_num_ktphi", "cnt_den_ktphi"):
+            out[name] = np.ctypeslib.as_array(getattr(s, name),
+                                              shape=(nkt * int(s.p.n_Kphi),)).copy()
+        out["same_st"] = [getattr(s.same_st, f) for f, _ in Stages._fields_]
+        out["mixed_st"] = [getattr(s.mixed_st, f) for f, _ in Stages._fields_]
+        return out
+
+
+class OracleRng:
+    """C restatement of std::mt19937 + the reference's two distributions."""
+
+    def __init__(self, seed: int):
+        self.L = lib()
+        self.r = Rng()
+        self.L.orc_rng_seed(C.byref(self.r), C.c_uint32(seed & 0xFFFFFFFF))
+
+    def rand_int_uniform(self):
+        return self.L.orc_rand_int_uniform(C.byref(self.r))
+
+    def rand_uniform(self):
+        return self.L.orc_rand_uniform(C.byref(self.r))
+
+    def next_u32(self):
+        return self.L.orc_rng_next(C.byref(self.r))
+
+    def draw_plan(self, nev, mixed_nev, number_of_mixed_events=0):
+        M = number_of_mixed_events if number_of_mixed_events > 0 else mixed_nev // 2 + 1
+        ids = np.zeros(max(nev * M, 1), np.int32)
+        cs = np.zeros(max(2 * nev * M, 2))
+        self.L.orc_draw_mixing_plan(C.byref(self.r), nev, mixed_nev, number_of_mixed_events,
+                                    ids.ctypes.data_as(C.POINTER(C.c_int)), cs.ctypes.data_as(_D))
+        return M, ids[:nev * M], cs[:2 * nev * M]
+
+
+def psi_ref(buf, n_order: int) -> float:
+    b = CBuffer(buf)
+    return lib().orc_flow_event_plane_angle(C.byref(b.c), n_order)
+
+
+def read_mode10(filename: str, event_buffer_size: int, monval: int):
+    """All buffers of a mode-10 file as the reference's reader forms them."""
+    from hadronic_afterburner_toolkit_amd.samples import Buffer
+    L = lib()
+    rd = L.orc_reader_open(filename.encode(), int(event_buffer_size), int(monval))
+    if not rd:
+        raise FileNotFoundError(filename)
+    bufs = []
+    try:
+        while not L.orc_reader_eof(rd):
+            nev = L.orc_reader_next(rd)
+            b = L.orc_reader_buffer(rd).contents
+            off = np.ctypeslib.as_array(b.ev_off, shape=(nev + 1,)).copy() if nev else \
+                np.zeros(1, np.int64)
+            n = int(off[-1])
+            cols = {}
+            for name in ("px", "py", "pz", "E", "x", "y", "z", "t"):
+                cols[name] = np.ctypeslib.as_array(getattr(b, name), shape=(n,)).copy() \
+                    if n else np.zeros(0)
+            bufs.append((Buffer(ev_off=off, **cols), L.orc_reader_total_particles(rd)))
+    finally:
+        L.orc_reader_close(rd)
+    return bufs

@@ -1,0 +1,93 @@
+"""Loads the golden cases written by make_golden.py and rebuilds their inputs.
+
+Inputs of synthetic cases are regenerated from their seeds and checked
+against the committed digest of the mode-10 text the reference read.
+"""
+from __future__ import annotations
+
+import gzip
+import hashlib
+import io
+import json
+import os
+
+import numpy as np
+
+from hadronic_afterburner_toolkit_amd import samples
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE_GZ = os.path.join(HERE, "ref_fixture", "particle_samples.gz")
+
+CASE_NAMES = ["fixture_3d", "fixture_q11", "fixture_kphi_qinv", "fixture_budget700",
+              "fixture_qinv", "synth_cfg0",
+              "synth_noboost", "synth_cap", "synth_mixedfile", "synth_budget_exact"]
+
+HBT_KEYS = ("qnpts", "q_min", "q_max", "n_KT", "KT_min", "KT_max", "n_Kphi",
+            "azimuthal_flag", "invariant_radius_flag", "long_comoving_boost", "HBTrap_min",
+            "HBTrap_max", "needed_number_of_pairs", "number_of_mixed_events")
+# the reference's parameters.dat values of the HBT keys (parameters.dat:83-119)
+REF_DEFAULTS = dict(qnpts=31, q_min=-0.15, q_max=0.15, n_KT=5, KT_min=0.15, KT_max=0.55,
+                    n_Kphi=48, azimuthal_flag=0, invariant_radius_flag=0,
+                    long_comoving_boost=1, HBTrap_min=-0.5, HBTrap_max=0.5,
+                    needed_number_of_pairs=30000000, number_of_mixed_events=0)
+
+
+def load(name):
+    d = os.path.join(HERE, name)
+    with open(os.path.join(d, "params.json")) as f:
+        spec = json.load(f)
+    ref = dict(np.load(os.path.join(d, "ref.npz")))
+    with open(os.path.join(d, "outputs.json")) as f:
+        outputs = json.load(f)
+    with open(os.path.join(d, "input.sha256")) as f:
+        digests = json.load(f)
+    return spec, ref, outputs, digests
+
+
+def hbt_params(spec):
+    p = dict(REF_DEFAULTS)
+    for k, v in spec["params"].items():
+        if k in HBT_KEYS:
+            p[k] = v
+    return p
+
+
+def write_inputs(spec, digests, dirpath):
+    """Writes the case's mode-10 input(s) into dirpath; returns file paths."""
+    inp = spec["input"]
+    paths = {}
+    if inp == "fixture":
+        p = os.path.join(dirpath, "particle_samples.gz")
+        with open(FIXTURE_GZ, "rb") as f, open(p, "wb") as g:
+            g.write(f.read())
+        paths["same"] = p
+    elif inp[0] == "synth":
+        _, seed, nev, mult = inp
+        p = os.path.join(dirpath, "particle_samples.gz")
+        samples.write_mode10(p, samples.generate(seed, nev, mult))
+        paths["same"] = p
+    else:
+        _, seed, seed2, nev, mult = inp
+        p = os.path.join(dirpath, "particle_samples.gz")
+        samples.write_mode10(p, samples.generate(seed, nev, mult))
+        q = os.path.join(dirpath, "particle_samples_mixed_event.gz")
+        samples.write_mode10(q, samples.generate(seed2, nev + 3, mult))
+        paths["same"], paths["mixed"] = p, q
+    for key, fn in (("same", "particle_samples.gz"), ("mixed", "particle_samples_mixed_event.gz")):
+        if key in paths and fn in digests:
+            with gzip.open(paths[key], "rb") as f:
+                got = hashlib.sha256(f.read()).hexdigest()
+            if got != digests[fn]:
+                raise AssertionError(f"regenerated input {fn} differs from the golden digest")
+    return paths
+
+
+def ref_arrays_as_u64(ref):
+    """The reference keeps counts in doubles; give integer views for comparison."""
+    out = dict(ref)
+    for k in ("num_count", "den", "qinv_count", "qinv_den"):
+        if k in out:
+            v = out[k]
+            assert np.all(v == np.floor(v))
+            out[k] = v.astype(np.uint64)
+    return out

@@ -1,0 +1,56 @@
+"""The C ABI library: it loads, exports every entry point include/hbt_hip.h
+declares, and reports errors cleanly (CPU only; no compute calls)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from hadronic_afterburner_toolkit_amd import _lib, hbt
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(REPO, "include", "hbt_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hbt_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "hbt_create" in names and "hbt_run_same" in names and "hbt_run_mixed" in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_binding_covers_header():
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(declared_functions()) <= bound
+
+
+def test_create_without_gpu_or_bad_args_fails_cleanly():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.hbt_create(None, C.byref(h)) == -1
+    assert lib.hbt_last_error(None)
+    cfg = _lib.HbtConfig()
+    cfg.qnpts = 1  # invalid
+    assert lib.hbt_create(C.byref(cfg), C.byref(h)) == -1
+    assert b"qnpts" in lib.hbt_last_error(None)
+
+
+def test_rng_works_without_gpu():
+    r = hbt.Random(0)
+    assert r.rand_int_uniform() == 1178568022
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(RuntimeError, match="missing"):
+        _lib.load()
