@@ -1,0 +1,241 @@
+"""HBT pair-correlation throughput on one MI355X.
+
+Workload (the precision form's single large workload): 2000 pi+ events,
+multiplicity Poisson(600), source model of hadronic_afterburner_toolkit_amd.
+samples, grouped by 50 consecutive events (oversample 50: 40 groups of
+~30k particles, ~4.5e8 same-event pairs each), 10 rotated mixed partners
+per event, q grid 41^3 on [-0.15, 0.15] GeV, K_T edges 0.2..0.7 GeV (step
+0.1; the [0.5,0.7] bin of the quoted edges is the sum of the last two),
+|y| < 1, pair cap off.
+
+One step = one pass of the hot path over the whole batch: every same-event
+pair of the 40 groups and every mixed pair of the 2000 events, one
+persistent launch replayed from a hipGraph, inputs resident in HBM.
+
+Printed: one JSON line; ``value`` is the precision form's pairs/s (same +
+mixed enumerated pairs), with the FP64 kernels, the per-bin error of the
+precision form against the FP64 oracle, the roofline of the dominant kernel
+and the CPU oracle timed on a bounded sample beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector peak
+
+PARAMS = dict(qnpts=41, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
+              HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18,
+              number_of_mixed_events=10, long_comoving_boost=1)
+
+
+def make_workload(nev, mult, per_group, seed):
+    from hadronic_afterburner_toolkit_amd import samples
+    full = samples.generate(seed, nev, mult)
+    return [full.events(g * per_group, per_group) for g in range(nev // per_group)]
+
+
+def draw_plans(groups, seed, M):
+    from hadronic_afterburner_toolkit_amd import hbt
+    rng = hbt.Random(seed)
+    Mg, ids, cs = [], [], []
+    for b in groups:
+        m, i, c = rng.draw_plan(b.nev, b.nev, M)
+        Mg.append(m)
+        ids.append(i)
+        cs.append(c)
+    return np.array(Mg, np.int32), np.concatenate(ids), np.concatenate(cs)
+
+
+def flops(stages_same, stages_mixed):
+    """Algorithmic FLOPs of the cascade per launch (+ - * / sqrt count 1 each,
+    attributed to the first stage that needs them; cos not counted):
+    same  = 7 N_enum + 12 N_KT + 5 N_out + 17 N_side + 17 N_bin
+    mixed = 7 N_enum + 12 N_KT + 5 N_out + 17 N_side +  1 N_bin"""
+    s, m = [float(v) for v in stages_same], [float(v) for v in stages_mixed]
+    return (7 * s[0] + 12 * s[1] + 5 * s[2] + 17 * s[3] + 17 * s[4]
+            + 7 * m[0] + 12 * m[1] + 5 * m[2] + 17 * m[3] + 1 * m[4])
+
+
+def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log):
+    from hadronic_afterburner_toolkit_amd import hbt
+    g = hbt.HBTCorrelation(PARAMS, precision=prec, kernel=kern)
+    g.upload_groups(groups)
+    g.prepare_batch(*plan)
+    tune = g.autotune() if autotune else g.tuning()
+    es, em, nitems = g.batch_info()
+    for _ in range(warmup):
+        g.launch_batch()
+    g.sync()
+    g.reset()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch_batch()
+        dev_ms.append(g.sync())
+    wall = time.perf_counter() - t0
+    log(f"  prec={prec} kernel={kern} steps={steps} wall={wall:.3f}s "
+        f"dev_ms/step={np.mean(dev_ms):.2f} tune={tune}")
+    return g, dict(wall=wall, dev_ms=dev_ms, enum_same=es, enum_mixed=em, nitems=nitems,
+                   tune=tune)
+
+
+def cpu_baseline(group, plan_g, log):
+    """The CPU oracle on group 0 (its same-event pairs and its events' mixed
+    pairs with the same plan), one thread."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    oracle.build()
+    M, ids, cs = plan_g
+    o = oracle.Oracle(PARAMS)
+    t0 = time.perf_counter()
+    o.combine_same(group)
+    o.run_mixed_plan(group, M, ids, cs)
+    dt = time.perf_counter() - t0
+    a = o.arrays()
+    pairs = a["same_st"][0] + a["mixed_st"][0]
+    log(f"  cpu oracle: {pairs:.3e} pairs in {dt:.2f}s")
+    return o, a, pairs / dt, dt, pairs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--events", type=int, default=2000)
+    ap.add_argument("--mult", type=float, default=600.0)
+    ap.add_argument("--per-group", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--only", choices=["fp32c", "fp64", "plain"], default=None)
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the HBT engine runs on a single MI355X here "
+              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        if not args.quiet:
+            print(msg, file=sys.stderr, flush=True)
+
+    from hadronic_afterburner_toolkit_amd import _lib
+    log(f"generating {args.events} events, Poisson({args.mult}), groups of {args.per_group}")
+    groups = make_workload(args.events, args.mult, args.per_group, args.seed)
+    plan = draw_plans(groups, args.seed + 1, PARAMS["number_of_mixed_events"])
+    res = {}
+    modes = {"fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps),
+             "fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps),
+             "plain": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, max(1, min(2, args.steps)))}
+    ctx = {}
+    for name, (prec, kern, k) in modes.items():
+        if args.only and name != args.only:
+            continue
+        log(f"mode {name}")
+        ctx[name], res[name] = run_mode(groups, plan, prec, kern, k,
+                                        args.warmup if name != "plain" else 1,
+                                        not args.no_autotune, log)
+
+    def pairs_per_s(r):
+        return (r["enum_same"] + r["enum_mixed"]) * len(r["dev_ms"]) / r["wall"]
+
+    head = "fp32c" if "fp32c" in res else next(iter(res))
+    r = res[head]
+    out = {
+        "metric": "pairs/sec (same+mixed) FP32-comp vs FP64 at <=1e-6 bin err; p50/p99 latency",
+        "value": pairs_per_s(r),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": len(r["dev_ms"]),
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["wall"] / len(r["dev_ms"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32c" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "precision form: 2000 pi+ events Poisson(600), oversample 50 "
+                               "(40 groups ~30k particles), 10 mixed rotations, q 41^3, 4(5) K_T bins",
+                   "events": args.events, "mult": args.mult, "per_group": args.per_group,
+                   "pairs_per_step": r["enum_same"] + r["enum_mixed"],
+                   "same_pairs_per_step": r["enum_same"], "mixed_pairs_per_step": r["enum_mixed"],
+                   "tuning": r["tune"]},
+    }
+    for name in res:
+        out[f"{name}_pairs_per_s"] = pairs_per_s(res[name])
+        out[f"{name}_dev_ms_per_step"] = float(np.mean(res[name]["dev_ms"]))
+    if "fp32c" in res and "plain" in res:
+        out["speedup_vs_plain_fp64"] = out["fp32c_pairs_per_s"] / out["plain_pairs_per_s"]
+    if "fp32c" in res and "fp64" in res:
+        out["speedup_vs_fast_fp64"] = out["fp32c_pairs_per_s"] / out["fp64_pairs_per_s"]
+        a, b = ctx["fp32c"].results(), ctx["fp64"].results()
+        out["counts_identical_fp32c_vs_fp64"] = bool(
+            np.array_equal(a.num_count, b.num_count) and np.array_equal(a.den, b.den))
+        cnt = b.num_count.astype(float)
+        sel = cnt > 0
+        per_pair = np.abs(a.num - b.num)[sel] / cnt[sel]
+        out["num_err_per_pair_max"] = float(per_pair.max()) if per_pair.size else 0.0
+
+    # roofline of the dominant kernel: stage counts from the plain kernel
+    # (it counts every stage of the cascade), FLOP model per launch
+    if "plain" in res:
+        s, m = ctx["plain"].stage_counts()
+        nsteps = len(res["plain"]["dev_ms"])
+        s = [int(v) // nsteps for v in s]
+        m = [int(v) // nsteps for v in m]
+        fl = flops(s, m)
+        dev_s = float(np.mean(r["dev_ms"])) * 1e-3
+        ach = fl / dev_s / 1e12
+        peak = FP32_PEAK_TFLOPS if head == "fp32c" else FP64_PEAK_TFLOPS
+        out["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                           "frac": ach / peak, "traffic": None,
+                           "kernel": "k_fast<float,R>" if head == "fp32c" else "k_fast<double,R>",
+                           "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
+    if not args.no_cpu:
+        g0 = groups[0]
+        M = int(plan[0][0])
+        plan_g = (M, plan[1][:g0.nev * M], plan[2][:2 * g0.nev * M])
+        o, a, rate, dt, pairs = cpu_baseline(g0, plan_g, log)
+        out["cpu_baseline"] = {"value": rate, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"group 0 of {len(groups)} ({g0.nev} events, "
+                                         f"{g0.n} particles): {pairs:.3e} same+mixed pairs in "
+                                         f"{dt:.1f}s, oracle/hbt_oracle.c -O2"}
+        # precision form vs the FP64 oracle on that sample, per bin
+        if "fp32c" in res:
+            from hadronic_afterburner_toolkit_amd import hbt
+            gs = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP32C,
+                                    kernel=_lib.HBT_KERNEL_FAST)
+            gs.upload_groups([g0])
+            gs.prepare_batch(np.array([M], np.int32), plan_g[1], plan_g[2])
+            gs.launch_batch()
+            gs.sync()
+            rr = gs.results()
+            errs = []
+            for key, fl_key in (("num", "abs_num"), ("q_out_sum", "abs_q_out"),
+                                ("q_side_sum", "abs_q_side"), ("q_long_sum", "abs_q_long")):
+                ref = a[key]
+                den = np.maximum(np.abs(ref), a[fl_key])
+                sel = den > 0
+                errs.append(np.abs(getattr(rr, key) - ref)[sel] / den[sel])
+            e = np.concatenate(errs)
+            out["fp32c_vs_fp64_oracle"] = {
+                "max_rel_bin_err": float(e.max()), "rms_rel_bin_err": float(np.sqrt(np.mean(e * e))),
+                "counts_identical": bool(np.array_equal(rr.num_count, a["num_count"].astype(np.uint64))
+                                         and np.array_equal(rr.den, a["den"].astype(np.uint64))),
+                "metric": "|S-S_ref|/max(|S_ref|,sum|.|_bin), bins with pairs, num and q sums"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

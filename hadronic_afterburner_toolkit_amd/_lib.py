@@ -33,6 +33,11 @@ class HbtConfig(C.Structure):
     ]
 
 
+class HbtTuning(C.Structure):
+    _fields_ = [("R", C.c_int32), ("jchunk", C.c_int32), ("kchunk", C.c_int32),
+                ("blocks_per_cu", C.c_int32), ("ms", C.c_double)]
+
+
 _U64P = C.POINTER(C.c_uint64)
 _F64P = C.POINTER(C.c_double)
 
@@ -70,6 +75,16 @@ SIGNATURES = [
     ("hbt_last_kernel_ms", C.c_int, [C.c_void_p, _F64P, _F64P]),
     ("hbt_slot_info", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int64)]),
+    ("hbt_upload_groups", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                    C.c_int32, C.POINTER(C.c_int64), C.POINTER(_F64P)]),
+    ("hbt_prepare_batch", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    _F64P]),
+    ("hbt_launch_batch", C.c_int, [C.c_void_p]),
+    ("hbt_sync", C.c_int, [C.c_void_p, _F64P]),
+    ("hbt_batch_info", C.c_int, [C.c_void_p, _U64P, _U64P, C.POINTER(C.c_int32)]),
+    ("hbt_get_tuning", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
+    ("hbt_set_tuning", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
+    ("hbt_autotune", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
 ]
 
 _lib = None

@@ -17,7 +17,7 @@ struct DevParams {
     int azimuthal, qinv, boost, pad0;
     double KT_min, dKT, dKphi, delta_q;
     double q_base, q_lo_cut, q_hi_cut;
-    double KT_min_sq, KT_max_sq, hbarC_inv, psi_ref, two_pi;
+    double KT_min_sq, KT_max_sq, hbarC_inv, two_pi;
     // Conservative prefilter thresholds (a pair the exact cascade accepts
     // always passes).  Per particle the record holds nrm = |pT|^2 - mid/2,
     // so for a pair u = T - mid and v = Q - mid come from one dot product.
@@ -66,9 +66,10 @@ __device__ __forceinline__ long bin_index(const DevParams &P, int kt, int kphi, 
 
 // K_phi bin with the reference's wrap loops (HBT_correlation.cpp:408-423).
 // Returns -1 when the reference would skip the pair.
-__device__ __forceinline__ int kphi_index(const DevParams &P, double K_x, double K_y) {
+__device__ __forceinline__ int kphi_index(const DevParams &P, double K_x, double K_y,
+                                          double psi_ref) {
     double local_K_phi = atan2(K_y, K_x);
-    double delta_phi = local_K_phi - P.psi_ref;
+    double delta_phi = local_K_phi - psi_ref;
     while (delta_phi < 0.) delta_phi += P.two_pi;
     while (delta_phi > P.two_pi) delta_phi -= P.two_pi;
     int idx = (int)(delta_phi / P.dKphi);
@@ -93,8 +94,8 @@ enum { ST_KT = 1, ST_OUT = 2, ST_SIDE = 4, ST_BIN = 8 };
 template <bool MIXED>
 __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, double p1y,
                                            double p1z, double p1E, double p2x, double p2y,
-                                           double p2z, double p2E, PairBins &b,
-                                           double &q_inv, int &qinv_ok) {
+                                           double p2z, double p2E, double psi_ref,
+                                           PairBins &b, double &q_inv, int &qinv_ok) {
     qinv_ok = 0;
     double K_z = 0.5 * (p1z + p2z);
     double K_E = 0.5 * (p1E + p2E);
@@ -155,7 +156,7 @@ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, doubl
     if (b.il >= P.qnpts || b.il < 0) return st;
     b.kphi = 0;
     if (P.azimuthal) {
-        b.kphi = kphi_index(P, K_x, K_y);
+        b.kphi = kphi_index(P, K_x, K_y, psi_ref);
         if (b.kphi < 0) return st;
     }
     b.q_out = q_out;
@@ -166,7 +167,8 @@ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, doubl
 
 // Same-event pair: cascade + accumulation (HBT_correlation.cpp:339-356,
 // :431-446).  Returns the stage mask.
-__device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H, double p1x,
+__device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H, double psi_ref,
+                                            double p1x,
                                             double p1y, double p1z, double p1E, double x1,
                                             double y1, double z1, double t1, double p2x,
                                             double p2y, double p2z, double p2E, double x2,
@@ -174,7 +176,8 @@ __device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H
     PairBins b;
     double q_inv;
     int qinv_ok;
-    int st = cascade_f64<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b, q_inv, qinv_ok);
+    int st = cascade_f64<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b, q_inv,
+                                qinv_ok);
     if (!(st & (ST_BIN)) && !qinv_ok) return st;
     double t_diff = t1 - t2;
     double x_diff = x1 - x2;
@@ -200,13 +203,15 @@ __device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H
 }
 
 // Mixed pair: cascade + denominator (HBT_correlation.cpp:592-607, :681-683).
-__device__ __forceinline__ int bin_mixed_f64(const DevParams &P, const DevHist &H, double p1x,
+__device__ __forceinline__ int bin_mixed_f64(const DevParams &P, const DevHist &H, double psi_ref,
+                                             double p1x,
                                              double p1y, double p1z, double p1E, double p2x,
                                              double p2y, double p2z, double p2E) {
     PairBins b;
     double q_inv;
     int qinv_ok;
-    int st = cascade_f64<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b, q_inv, qinv_ok);
+    int st = cascade_f64<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b, q_inv,
+                               qinv_ok);
     if (qinv_ok) atomic_add_u64(&H.qinv_den[(long)b.kt * P.qnpts + (qinv_ok - 1)], 1ull);
     if (st & ST_BIN) atomic_add_u64(&H.den[bin_index(P, b.kt, b.kphi, b.io, b.is, b.il)], 1ull);
     return st;

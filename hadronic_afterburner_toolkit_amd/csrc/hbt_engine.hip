@@ -2,19 +2,25 @@
 // (include/hbt_hip.h).
 //
 // Data layout in HBM, per context:
-//   slot s (0 = same-event buffer, 1 = mixed-event buffer when it is read
-//   separately): the particles kept by the rapidity cut, in the reference's
-//   gather order, as eight FP64 arrays px,py,pz,E,x,y,z,t (64 B/particle),
-//   event offsets (int64, nev+1), and the prefilter records of the fast
-//   kernel (PfRec64 32 B + cw 8 B, PfRec32 16 B + cw 4 B per particle).
-//   Histograms: flat [kt][kphi][iqout][iqside][iqlong] (iqlong fastest),
-//   u64 counts (numerator, denominator) and FP64 sums (q_out, q_side,
-//   q_long, cos); q_inv histograms [kt][iq].
+//   slot s (0 = same-event list, 1 = mixed-event list when it is read
+//   separately): G groups (the reference's particle buffers) resident at
+//   once.  Per slot: the particles kept by the rapidity cut in the
+//   reference's gather order, as eight FP64 arrays px,py,pz,E,x,y,z,t
+//   (64 B/particle); event offsets (int64, nev+1); group -> first event
+//   (int32, G+1); the fast kernel's prefilter records (PfRec64 32 B + cw
+//   8 B, PfRec32 16 B + cw 4 B per particle).
+//   Mixing plan: per event e, rows [rowoff[e], rowoff[e+1]) each holding the
+//   partner event (global index in the mixed list) and cos, sin of its
+//   rotation.  Histograms: flat [kt][kphi][iqout][iqside][iqlong] (iqlong
+//   fastest), u64 counts (numerator, denominator) and FP64 sums (q_out,
+//   q_side, q_long, cos); q_inv histograms [kt][iq].
 //
 // Kernels:
 //   k_same_plain / k_mixed_plain  LDS-tiled direct cascade (plain mode)
-//   k_fast<double|float, R>       persistent: prefilter + compaction +
-//                                 exact cascade (hbt_fast.h)
+//   k_fast<double|float, R>       persistent work-queue kernel: prefilter +
+//                                 compaction + exact cascade (hbt_fast.h);
+//                                 one launch covers every same-event and
+//                                 mixed pair of all resident groups
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,7 +54,7 @@ __device__ __forceinline__ void wave_add_stage(unsigned long long *dst, unsigned
 // Same-event pairs i<j of one group [g0, g0+n): block (tj, ti) of 256x256
 // tiles, j tile staged in LDS, one thread per i (HBT_correlation.cpp:291-460).
 __global__ __launch_bounds__(TILE) void k_same_plain(DevParams P, DevHist H, DevSoA A,
-                                                     long long g0, int n) {
+                                                     long long g0, int n, double psi) {
     const int ti = blockIdx.y, tj = blockIdx.x;
     if (tj < ti) return;
     __shared__ double s[8][TILE];
@@ -75,7 +81,7 @@ __global__ __launch_bounds__(TILE) void k_same_plain(DevParams P, DevHist H, Dev
         const double x1 = A.x[gi], y1 = A.y[gi], z1 = A.z[gi], t1 = A.t[gi];
         const int jstart = (ti == tj) ? tid + 1 : 0;
         for (int jj = jstart; jj < jn; jj++) {
-            const int st = bin_same_f64(P, H, p1x, p1y, p1z, p1E, x1, y1, z1, t1, s[0][jj],
+            const int st = bin_same_f64(P, H, psi, p1x, p1y, p1z, p1E, x1, y1, z1, t1, s[0][jj],
                                         s[1][jj], s[2][jj], s[3][jj], s[4][jj], s[5][jj],
                                         s[6][jj], s[7][jj]);
             c_kt += st & 1;
@@ -90,14 +96,17 @@ __global__ __launch_bounds__(TILE) void k_same_plain(DevParams P, DevHist H, Dev
     wave_add_stage(&H.stage[4], c_bin);
 }
 
-// Mixed pairs: block (tj, ti, row) with row = e*M + k; the partner's j tile
-// is rotated by (c_k, s_k) while staged (HBT_correlation.cpp:522-523).
+// Mixed pairs of the events [e0, e0 + nz/M) of one group whose plan has M
+// rows per event: block (tj, ti, z) with event e0 + z/M, row row0 + z.  The
+// partner's j tile is rotated by (c, s) while staged
+// (HBT_correlation.cpp:522-523).
 __global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, DevSoA A,
                                                       DevSoA B, const long long *a_off,
                                                       const long long *b_off, const int *ids,
-                                                      const double *cs, int M, int row0) {
-    const int row = row0 + blockIdx.z;
-    const int e = row / M;
+                                                      const double *cs, int M, int e0,
+                                                      long long row0, double psi) {
+    const long long row = row0 + blockIdx.z;
+    const int e = e0 + (int)(blockIdx.z / M);
     const long long ib = a_off[e];
     const int n1 = (int)(a_off[e + 1] - ib);
     const int pid = ids[row];
@@ -125,8 +134,8 @@ __global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, De
         const long long gi = ib + i;
         const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
         for (int jj = 0; jj < jn; jj++) {
-            const int st =
-                bin_mixed_f64(P, H, p1x, p1y, p1z, p1E, s[0][jj], s[1][jj], s[2][jj], s[3][jj]);
+            const int st = bin_mixed_f64(P, H, psi, p1x, p1y, p1z, p1E, s[0][jj], s[1][jj],
+                                         s[2][jj], s[3][jj]);
             c_kt += st & 1;
             c_out += (st >> 1) & 1;
             c_side += (st >> 2) & 1;
@@ -161,10 +170,12 @@ __global__ void k_sum_groups(const unsigned long long *h, long per, unsigned lon
 // ------------------------------------------------------------------ host
 
 struct Slot {
-    int nev = 0;
+    int ngroups = 0, nev = 0;
     long long n = 0;        // kept particles
     long long n_in = 0;     // particles passed in
-    std::vector<long long> off;
+    std::vector<int> grp_ev;       // ngroups+1
+    std::vector<long long> off;    // nev+1, kept-particle offsets
+    std::vector<double> psi2;      // psi_ref (n=2) per group, over all particles passed in
     double *d_soa[8] = {};
     long long *d_off = nullptr;
     PfRec64 *d_pf64 = nullptr;
@@ -173,8 +184,6 @@ struct Slot {
     float *d_cw32 = nullptr;
     size_t cap = 0, cap_ev = 0;
     double pmax2 = 0.0;     // max px^2+py^2 of kept particles
-    double bmax = 0.0;      // max |beta| of kept particles
-    double psi2 = 0.0;      // psi_ref (n=2) over all particles passed in
     bool valid = false;
 };
 
@@ -195,18 +204,31 @@ struct hbt_ctx {
     unsigned long long *d_binned = nullptr;  // [2]
     unsigned long long *d_sums = nullptr;    // scratch for counters
     Slot slot[2];
+    // mixing plan (device), and its host copy for the plain kernels
     int *d_ids = nullptr;
     double *d_cs = nullptr;
-    size_t plan_cap = 0;
+    long long *d_rowoff = nullptr;
+    size_t plan_cap = 0, rowoff_cap = 0;
+    std::vector<long long> rowoff;
+    std::vector<int> M_grp;
+    bool plan_valid = false;
+    double *d_psi = nullptr;
+    size_t psi_cap = 0;
+    std::vector<double> psi;  // per group (set from the upload or by the caller)
     WorkItem *d_items = nullptr;
     size_t items_cap = 0;
+    int nitems = 0, nitems_same = 0;
     int *d_counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool batch_ready = false;
     std::string err;
     uint64_t needed = 0, needed_qinv = 0;
     uint64_t enum_same_total = 0, enum_mixed_total = 0;  // since reset
-    float ms_same = 0.f, ms_mixed = 0.f;
+    uint64_t batch_enum_same = 0, batch_enum_mixed = 0, plan_enum_mixed = 0;
+    float ms_same = 0.f, ms_mixed = 0.f, ms_batch = 0.f;
     double c_beta = 0.0, H = 0.0, mid = 0.0;
-    int R = 2, jchunk = 1024, kchunk = 8, blocks_per_cu = 8;
+    hbt_tuning tune{2, 1024, 8, 8, 0.0};
 };
 
 struct hbt_rng {
@@ -259,6 +281,14 @@ void free_slot(Slot &s) {
     if (s.d_cw64) hipFree(s.d_cw64);
     if (s.d_cw32) hipFree(s.d_cw32);
     s = Slot();
+}
+
+void drop_graph(hbt_ctx *c) {
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->batch_ready = false;
 }
 
 DevHist hist_view(hbt_ctx *c) {
@@ -334,9 +364,8 @@ int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
     if (total <= c->needed + 1 && (!qinv || total <= c->needed_qinv)) return HBT_OK;
     const int groups = c->cfg.n_KT * c->P.nphi;
     std::vector<unsigned long long> cnt;
-    int rc = group_sums(c, mixed ? c->d_den : c->d_num_count, (long)c->P.qnpts * c->P.qnpts *
-                                                                  c->P.qnpts,
-                        groups, cnt);
+    int rc = group_sums(c, mixed ? c->d_den : c->d_num_count,
+                        (long)c->P.qnpts * c->P.qnpts * c->P.qnpts, groups, cnt);
     if (rc) return rc;
     for (auto v : cnt)
         if (v + enum_call > c->needed + 1)
@@ -354,26 +383,18 @@ int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
     return HBT_OK;
 }
 
-int ensure_plan(hbt_ctx *c, size_t rows) {
-    if (rows <= c->plan_cap) return HBT_OK;
-    int rc = dev_alloc(c, &c->d_ids, rows);
-    if (!rc) rc = dev_alloc(c, &c->d_cs, 2 * rows);
+template <typename T>
+int ensure(hbt_ctx *c, T **p, size_t &cap, size_t n) {
+    if (n <= cap && *p) return HBT_OK;
+    int rc = dev_alloc(c, p, n);
     if (rc) return rc;
-    c->plan_cap = rows;
-    return HBT_OK;
-}
-
-int ensure_items(hbt_ctx *c, size_t n) {
-    if (n <= c->items_cap) return HBT_OK;
-    int rc = dev_alloc(c, &c->d_items, n);
-    if (rc) return rc;
-    c->items_cap = n;
+    cap = n;
     return HBT_OK;
 }
 
 // Prefilter margins for the FP32 records depend on the data's momentum
 // scale (|pT|^2 up to pmax2): FP32 rounding of px, py and of the dot
-// product is below 8 ulp of that scale.
+// product stays below 8 ulp of that scale; 64 ulp are allowed.
 void set_fp32_margins(hbt_ctx *c) {
     const Slot &a = c->slot[0];
     const Slot &b = mixed_slot(c);
@@ -381,12 +402,22 @@ void set_fp32_margins(hbt_ctx *c) {
     const double eps32 = 5.9604644775390625e-8;
     const double m = 64.0 * eps32 * (2.0 * pm + std::fabs(c->mid));
     c->P.f_half = (float)(c->P.pf_half + m);
-    c->P.f_qthr = (float)(c->P.pf_qthr + m);
-    if (!std::isfinite(c->P.pf_qthr)) c->P.f_qthr = INFINITY;
+    c->P.f_qthr = std::isfinite(c->P.pf_qthr) ? (float)(c->P.pf_qthr + m) : INFINITY;
+}
+
+int upload_psi(hbt_ctx *c) {
+    const size_t G = std::max<size_t>(1, c->psi.size());
+    int rc = ensure(c, &c->d_psi, c->psi_cap, G);
+    if (rc) return rc;
+    std::vector<double> p(c->psi);
+    if (p.empty()) p.push_back(0.0);
+    HIPCHECK(c, hipMemcpy(c->d_psi, p.data(), p.size() * 8, hipMemcpyHostToDevice));
+    return HBT_OK;
 }
 
 FastArgs fast_args(hbt_ctx *c) {
     FastArgs a;
+    set_fp32_margins(c);
     a.P = c->P;
     a.H = hist_view(c);
     const Slot &s0 = c->slot[0];
@@ -401,38 +432,349 @@ FastArgs fast_args(hbt_ctx *c) {
     a.cwA32 = s0.d_cw32;
     a.cwB32 = s1.d_cw32;
     a.mix_off = s1.d_off;
+    a.rowoff = c->d_rowoff;
     a.plan_ids = c->d_ids;
     a.plan_cs = c->d_cs;
-    a.M = 0;
+    a.psi = c->d_psi;
     a.items = c->d_items;
-    a.nitems = 0;
+    a.nitems = c->nitems;
     a.counter = c->d_counter;
     a.binned = c->d_binned;
     return a;
 }
 
-int launch_fast(hbt_ctx *c, FastArgs &a) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, c->d_counter);
-    int per_cu = c->blocks_per_cu;
-    long blocks = std::min<long>((long)c->cu_count * per_cu, ((long)a.nitems + 3) / 4);
+int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, c->d_counter);
+    long blocks = std::min<long>((long)c->cu_count * c->tune.blocks_per_cu,
+                                 ((long)a.nitems + 3) / 4);
     if (blocks < 1) blocks = 1;
     const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
+    const int R = c->tune.R;
+#define HBT_LAUNCH(T, RR) hipLaunchKernelGGL((k_fast<T, RR>), dim3(blocks), dim3(256), 0, st, a)
     if (f32) {
-        if (c->R == 1)
-            hipLaunchKernelGGL((k_fast<float, 1>), dim3(blocks), dim3(256), 0, c->stream, a);
-        else if (c->R == 4)
-            hipLaunchKernelGGL((k_fast<float, 4>), dim3(blocks), dim3(256), 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((k_fast<float, 2>), dim3(blocks), dim3(256), 0, c->stream, a);
+        if (R == 1) HBT_LAUNCH(float, 1);
+        else if (R == 4) HBT_LAUNCH(float, 4);
+        else HBT_LAUNCH(float, 2);
     } else {
-        if (c->R == 1)
-            hipLaunchKernelGGL((k_fast<double, 1>), dim3(blocks), dim3(256), 0, c->stream, a);
-        else if (c->R == 4)
-            hipLaunchKernelGGL((k_fast<double, 4>), dim3(blocks), dim3(256), 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((k_fast<double, 2>), dim3(blocks), dim3(256), 0, c->stream, a);
+        if (R == 1) HBT_LAUNCH(double, 1);
+        else if (R == 4) HBT_LAUNCH(double, 4);
+        else HBT_LAUNCH(double, 2);
     }
+#undef HBT_LAUNCH
     HIPCHECK(c, hipGetLastError());
+    return HBT_OK;
+}
+
+// Work items: same-event tiles of every group, then mixed tiles of every
+// event.  Same: i tile of 64R particles x j chunk (j > i inside the tile).
+// Mixed: i tile x up to kchunk partners of the event's plan rows.
+// enum_same receives the same-event pairs the items enumerate; the mixed
+// count is known from the plan (upload_plan).
+void build_items(hbt_ctx *c, bool same, bool mixed, std::vector<WorkItem> &items,
+                 uint64_t &enum_same) {
+    const Slot &s = c->slot[0];
+    const int TI = 64 * c->tune.R;
+    const int JC = std::max(64, c->tune.jchunk);
+    const int KC = std::max(1, std::min(c->tune.kchunk, 64));
+    items.clear();
+    enum_same = 0;
+    if (same) {
+        for (int g = 0; g < s.ngroups; g++) {
+            const long long g0 = s.off[s.grp_ev[g]], n = s.off[s.grp_ev[g + 1]] - g0;
+            enum_same += (uint64_t)(n * (n - 1) / 2);
+            for (long long a = 0; a + 1 < n; a += TI) {
+                const int ni = (int)std::min<long long>(TI, n - a);
+                for (long long jb = a + 1; jb < n; jb += JC) {
+                    WorkItem w{};
+                    w.i0 = g0 + a;
+                    w.j0 = g0 + jb;
+                    w.ni = ni;
+                    w.nj = (int)std::min<long long>(JC, n - jb);
+                    w.kind = 0;
+                    w.grp = g;
+                    items.push_back(w);
+                }
+            }
+        }
+    }
+    c->nitems_same = (int)items.size();
+    if (mixed && c->plan_valid) {
+        for (int g = 0; g < s.ngroups; g++) {
+            for (int e = s.grp_ev[g]; e < s.grp_ev[g + 1]; e++) {
+                const long long n1 = s.off[e + 1] - s.off[e];
+                const int M = (int)(c->rowoff[e + 1] - c->rowoff[e]);
+                for (long long a = 0; a < n1; a += TI) {
+                    for (int k0 = 0; k0 < M; k0 += KC) {
+                        WorkItem w{};
+                        w.i0 = s.off[e] + a;
+                        w.ni = (int)std::min<long long>(TI, n1 - a);
+                        w.kind = 1;
+                        w.e = e;
+                        w.k0 = k0;
+                        w.k1 = std::min(M, k0 + KC);
+                        w.grp = g;
+                        items.push_back(w);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Uploads the plan: ids are relative to the event's own group and are
+// turned into global event indices of the mixed list.
+int upload_plan(hbt_ctx *c, const std::vector<int> &M_per_event, const int32_t *ids,
+                const double *cs, uint64_t &enum_mixed) {
+    Slot &s = c->slot[0];
+    const Slot &m = mixed_slot(c);
+    c->rowoff.assign(s.nev + 1, 0);
+    for (int e = 0; e < s.nev; e++) c->rowoff[e + 1] = c->rowoff[e] + M_per_event[e];
+    const long long rows = c->rowoff[s.nev];
+    std::vector<int> gids((size_t)std::max<long long>(rows, 1));
+    enum_mixed = 0;
+    for (int g = 0; g < s.ngroups; g++) {
+        const int mb = m.grp_ev[g], mn = m.grp_ev[g + 1] - m.grp_ev[g];
+        for (int e = s.grp_ev[g]; e < s.grp_ev[g + 1]; e++) {
+            const long long n1 = s.off[e + 1] - s.off[e];
+            for (long long r = c->rowoff[e]; r < c->rowoff[e + 1]; r++) {
+                const int p = ids[r];
+                if (p < 0 || p >= mn) return fail(c, HBT_ERR_ARG, "partner id out of range");
+                gids[r] = mb + p;
+                enum_mixed += (uint64_t)(n1 * (m.off[mb + p + 1] - m.off[mb + p]));
+            }
+        }
+    }
+    const size_t need = (size_t)std::max<long long>(rows, 1);
+    if (need > c->plan_cap || !c->d_ids || !c->d_cs) {
+        int rc = dev_alloc(c, &c->d_ids, need);
+        if (!rc) rc = dev_alloc(c, &c->d_cs, 2 * need);
+        if (rc) return rc;
+        c->plan_cap = need;
+    }
+    int rc = ensure(c, &c->d_rowoff, c->rowoff_cap, (size_t)s.nev + 1);
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpy(c->d_ids, gids.data(), rows * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_cs, cs, rows * 16, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_rowoff, c->rowoff.data(), (s.nev + 1) * 8, hipMemcpyHostToDevice));
+    c->plan_valid = true;
+    c->plan_enum_mixed = enum_mixed;
+    return HBT_OK;
+}
+
+int upload_items(hbt_ctx *c, const std::vector<WorkItem> &items) {
+    int rc = ensure(c, &c->d_items, c->items_cap, std::max<size_t>(1, items.size()));
+    if (rc) return rc;
+    if (!items.empty())
+        HIPCHECK(c, hipMemcpy(c->d_items, items.data(), items.size() * sizeof(WorkItem),
+                              hipMemcpyHostToDevice));
+    c->nitems = (int)items.size();
+    return HBT_OK;
+}
+
+int timed_fast(hbt_ctx *c, float *ms) {
+    FastArgs a = fast_args(c);
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    if (a.nitems > 0) {
+        int rc = launch_fast(c, a, c->stream);
+        if (rc) return rc;
+    }
+    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHECK(c, hipEventSynchronize(c->ev[1]));
+    HIPCHECK(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+    return HBT_OK;
+}
+
+// Plain kernels: one launch per group (same) and per group (mixed).
+int run_plain(hbt_ctx *c, bool same, bool mixed) {
+    Slot &s = c->slot[0];
+    const Slot &m = mixed_slot(c);
+    DevHist H = hist_view(c);
+    if (same) {
+        for (int g = 0; g < s.ngroups; g++) {
+            const long long g0 = s.off[s.grp_ev[g]], n = s.off[s.grp_ev[g + 1]] - g0;
+            if (n < 2) continue;
+            const int T = (int)((n + TILE - 1) / TILE);
+            if (T > 65535) return fail(c, HBT_ERR_CAPACITY, "group too large for the plain kernel");
+            hipLaunchKernelGGL(k_same_plain, dim3(T, T), dim3(TILE), 0, c->stream, c->P, H,
+                               soa_view(s), g0, (int)n, c->psi[g]);
+            HIPCHECK(c, hipGetLastError());
+        }
+    }
+    if (mixed) {
+        long long max2 = 0;
+        for (int e = 0; e < m.nev; e++) max2 = std::max(max2, m.off[e + 1] - m.off[e]);
+        const int Tj = (int)((max2 + TILE - 1) / TILE);
+        for (int g = 0; g < s.ngroups && Tj > 0; g++) {
+            const int e0 = s.grp_ev[g], e1 = s.grp_ev[g + 1];
+            if (e1 == e0) continue;
+            const int M = (int)(c->rowoff[e0 + 1] - c->rowoff[e0]);
+            long long max1 = 0;
+            for (int e = e0; e < e1; e++) max1 = std::max(max1, s.off[e + 1] - s.off[e]);
+            const int Ti = (int)((max1 + TILE - 1) / TILE);
+            if (Ti == 0 || M == 0) continue;
+            // events per launch so that gridDim.z stays below 65536
+            const int per = std::max(1, 65535 / M);
+            for (int eb = e0; eb < e1; eb += per) {
+                const int ne = std::min(per, e1 - eb);
+                hipLaunchKernelGGL(k_mixed_plain, dim3(Tj, Ti, ne * M), dim3(TILE), 0, c->stream,
+                                   c->P, H, soa_view(s), soa_view(m), s.d_off, m.d_off, c->d_ids,
+                                   c->d_cs, M, eb, c->rowoff[eb], c->psi[g]);
+                HIPCHECK(c, hipGetLastError());
+            }
+        }
+    }
+    return HBT_OK;
+}
+
+int prepare_and_capture(hbt_ctx *c) {
+    drop_graph(c);
+    std::vector<WorkItem> items;
+    build_items(c, true, true, items, c->batch_enum_same);
+    c->batch_enum_mixed = c->plan_valid ? c->plan_enum_mixed : 0;
+    int rc = upload_items(c, items);
+    if (rc) return rc;
+    rc = upload_psi(c);
+    if (rc) return rc;
+    if (c->cfg.kernel != HBT_KERNEL_FAST) {  // the plain batch runs launch by launch
+        c->batch_ready = true;
+        return HBT_OK;
+    }
+    FastArgs a = fast_args(c);
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_fast(c, a, c->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(c, HBT_ERR_HIP, "hipStreamEndCapture failed");
+    c->graph = g;
+    HIPCHECK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    c->batch_ready = true;
+    return HBT_OK;
+}
+
+int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev, int32_t nev,
+                const int64_t *ev_off, const double *const soa[8]) {
+    if (slot != 0 && slot != 1) return fail(c, HBT_ERR_ARG, "slot must be 0 or 1");
+    if (nev < 0 || ngroups < 1 || (nev > 0 && (!ev_off || !soa)))
+        return fail(c, HBT_ERR_ARG, "bad buffer");
+    for (int a = 0; nev > 0 && a < 8; a++)
+        if (!soa[a]) return fail(c, HBT_ERR_ARG, "null SoA array");
+    for (int e = 0; e < nev; e++)
+        if (ev_off[e + 1] < ev_off[e]) return fail(c, HBT_ERR_ARG, "event offsets decrease");
+    if (grp_ev[0] != 0 || grp_ev[ngroups] != nev)
+        return fail(c, HBT_ERR_ARG, "group offsets must run from 0 to nev");
+    for (int g = 0; g < ngroups; g++)
+        if (grp_ev[g + 1] < grp_ev[g]) return fail(c, HBT_ERR_ARG, "group offsets decrease");
+    drop_graph(c);
+    c->plan_valid = false;
+    Slot &s = c->slot[slot];
+    const long long n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
+    // Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476).
+    const double cut_max = tanh(c->cfg.HBTrap_max);
+    const double cut_min = tanh(c->cfg.HBTrap_min);
+    std::vector<long long> keep;
+    keep.reserve((size_t)n_in);
+    std::vector<long long> off(nev + 1, 0);
+    std::vector<double> psi2(ngroups, 0.0);
+    for (int g = 0; g < ngroups; g++) {
+        double vr = 0.0, vi = 0.0;
+        for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
+            for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
+                const double ratio = soa[2][i] / soa[3][i];
+                if (ratio > cut_min && ratio < cut_max) keep.push_back(i);
+                if (c->P.azimuthal) {  // psi_ref over all particles (:233-249)
+                    const double phi = atan2(soa[1][i], soa[0][i]);
+                    double sn, cn;
+                    sincos(2 * phi, &sn, &cn);  // as the reference's build evaluates it
+                    vr += cn;
+                    vi += sn;
+                }
+            }
+            off[e + 1] = (long long)keep.size();
+        }
+        psi2[g] = c->P.azimuthal ? atan2(vi, vr) / 2 : 0.0;
+    }
+    const long long n = (long long)keep.size();
+    for (int e = 0; e < nev; e++)
+        if (off[e + 1] - off[e] >= (1 << 18))
+            return fail(c, HBT_ERR_CAPACITY, "an event keeps more than 262143 particles");
+    if (n >= (1ll << 31)) return fail(c, HBT_ERR_CAPACITY, "buffer too large");
+    if ((size_t)n > s.cap || !s.d_soa[0]) {
+        for (auto &p : s.d_soa) {
+            int rc = dev_alloc(c, &p, (size_t)n);
+            if (rc) return rc;
+        }
+        int rc = dev_alloc(c, &s.d_pf64, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_pf32, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_cw64, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_cw32, (size_t)n);
+        if (rc) return rc;
+        s.cap = (size_t)n;
+    }
+    if ((size_t)nev + 1 > s.cap_ev || !s.d_off) {
+        int rc = dev_alloc(c, &s.d_off, (size_t)nev + 1);
+        if (rc) return rc;
+        s.cap_ev = (size_t)nev + 1;
+    }
+    std::vector<double> col((size_t)n);
+    for (int a = 0; a < 8; a++) {
+        for (long long k = 0; k < n; k++) col[k] = soa[a][keep[k]];
+        HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    // prefilter records
+    std::vector<PfRec64> pf((size_t)n);
+    std::vector<PfRec32> pf32((size_t)n);
+    std::vector<double> cw((size_t)n);
+    std::vector<float> cw32((size_t)n);
+    double pmax2 = 0.0, bmax = 0.0;
+    const bool boost = c->P.boost;
+    const bool qinv = c->P.qinv;
+    for (long long k = 0; k < n; k++) {
+        const long long i = keep[k];
+        const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
+        const double n2 = px * px + py * py;
+        pmax2 = std::max(pmax2, n2);
+        pf[k].px = px;
+        pf[k].py = py;
+        pf[k].nrm = n2 - 0.5 * c->mid;
+        pf[k].beta = boost ? pz / E : pz;
+        bmax = std::max(bmax, std::fabs(pf[k].beta));
+        cw[k] = qinv ? INFINITY : (boost ? c->c_beta / E : c->c_beta);
+    }
+    const double eps32 = 5.9604644775390625e-8;
+    for (long long k = 0; k < n; k++) {
+        pf32[k].px = (float)pf[k].px;
+        pf32[k].py = (float)pf[k].py;
+        pf32[k].nrm = (float)pf[k].nrm;
+        pf32[k].beta = (float)pf[k].beta;
+        // covers the FP32 rounding of beta (|beta| <= bmax) and of cw itself
+        cw32[k] = qinv ? INFINITY
+                       : (float)(cw[k] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
+    }
+    if (n > 0) {
+        HIPCHECK(c, hipMemcpy(s.d_pf64, pf.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_pf32, pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_cw64, cw.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_cw32, cw32.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHECK(c, hipMemcpy(s.d_off, off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
+    s.ngroups = ngroups;
+    s.grp_ev.assign(grp_ev, grp_ev + ngroups + 1);
+    s.nev = nev;
+    s.n = n;
+    s.n_in = n_in;
+    s.off = off;
+    s.pmax2 = pmax2;
+    s.psi2 = psi2;
+    s.valid = true;
+    if (slot == 0) c->psi = psi2;
+    return HBT_OK;
+}
+
+int check_slots(hbt_ctx *c) {
+    if (!c->slot[0].valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
+    if (c->slot[1].valid && c->slot[1].ngroups != c->slot[0].ngroups)
+        return fail(c, HBT_ERR_STATE, "the mixed-event list has a different number of groups");
     return HBT_OK;
 }
 
@@ -498,14 +840,13 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     P.KT_min_sq = k.KT_min * k.KT_min;
     P.KT_max_sq = k.KT_max * k.KT_max;
     P.hbarC_inv = 1. / kHbarC;
-    P.psi_ref = 0.0;
     P.two_pi = 2. * M_PI;
     P.inv_dKT = 1.0 / P.dKT;
     P.inv_dq = 1.0 / P.delta_q;
     P.inv_two_pi = 1.0 / (2.0 * M_PI);
     P.two_pi_hi = 6.28318530717958623199592693709e+00;  // 2*pi rounded to double
     P.two_pi_lo = 2.44929359829470635445e-16;            // 2*pi - two_pi_hi
-    // Prefilter (see hbt_fast.h).  T = 4 K_perp^2 must lie in
+    // Prefilter (hbt_fast.h).  T = 4 K_perp^2 must lie in
     // [4 KT_min_sq, 4 KT_max_sq]; Q <= 2 H^2.  Margins of 1e-9 relative
     // (+1e-12 absolute) dominate the ~1e-15 rounding of either side.
     const double t_lo = 4.0 * P.KT_min_sq, t_hi = 4.0 * P.KT_max_sq;
@@ -518,7 +859,7 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     c->c_beta = 0.5 * Hm;
     P.f_mid = (float)c->mid;
     P.f_half = (float)P.pf_half;
-    P.f_qthr = (float)P.pf_qthr;
+    P.f_qthr = P.qinv ? INFINITY : (float)P.pf_qthr;
     c->needed = (uint64_t)(unsigned long long)k.needed_number_of_pairs;
     c->needed_qinv = (uint64_t)50 * c->needed;
 
@@ -558,11 +899,12 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
 int hbt_destroy(hbt_ctx *c) {
     if (!c) return HBT_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->d_num_count, c->d_den,      c->d_q_out,    c->d_q_side,
-                    c->d_q_long,    c->d_num,      c->d_qinv_count, c->d_qinv_den,
-                    c->d_qinv_sum,  c->d_qinv_num, c->d_stage,    c->d_binned,
-                    c->d_sums,      c->d_ids,      c->d_cs,       c->d_items,
-                    c->d_counter};
+    drop_graph(c);
+    void *ptrs[] = {c->d_num_count, c->d_den,       c->d_q_out,      c->d_q_side,
+                    c->d_q_long,    c->d_num,       c->d_qinv_count, c->d_qinv_den,
+                    c->d_qinv_sum,  c->d_qinv_num,  c->d_stage,      c->d_binned,
+                    c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
+                    c->d_items,     c->d_counter,   c->d_psi};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -581,112 +923,23 @@ int hbt_reset(hbt_ctx *c) {
 
 int hbt_set_psi_ref(hbt_ctx *c, double psi_ref) {
     if (!c) return HBT_ERR_ARG;
-    c->P.psi_ref = psi_ref;
-    return HBT_OK;
+    for (auto &p : c->psi) p = psi_ref;
+    if (c->psi.empty()) c->psi.push_back(psi_ref);
+    c->batch_ready = false;
+    return upload_psi(c);
 }
 
 int hbt_upload(hbt_ctx *c, int32_t slot, int32_t nev, const int64_t *ev_off,
                const double *const soa[8]) {
     if (!c) return HBT_ERR_ARG;
-    if (slot != 0 && slot != 1) return fail(c, HBT_ERR_ARG, "slot must be 0 or 1");
-    if (nev < 0 || (nev > 0 && (!ev_off || !soa))) return fail(c, HBT_ERR_ARG, "bad buffer");
-    for (int a = 0; nev > 0 && a < 8; a++)
-        if (!soa[a]) return fail(c, HBT_ERR_ARG, "null SoA array");
-    for (int e = 0; e < nev; e++)
-        if (ev_off[e + 1] < ev_off[e]) return fail(c, HBT_ERR_ARG, "event offsets decrease");
-    Slot &s = c->slot[slot];
-    const long long n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
-    // Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476).
-    const double cut_max = tanh(c->cfg.HBTrap_max);
-    const double cut_min = tanh(c->cfg.HBTrap_min);
-    std::vector<long long> keep;
-    keep.reserve((size_t)n_in);
-    std::vector<long long> off(nev + 1, 0);
-    double vr = 0.0, vi = 0.0;
-    for (int e = 0; e < nev; e++) {
-        for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
-            const double ratio = soa[2][i] / soa[3][i];
-            if (ratio > cut_min && ratio < cut_max) keep.push_back(i);
-            if (c->P.azimuthal) {  // psi_ref over all particles (:233-249)
-                const double phi = atan2(soa[1][i], soa[0][i]);
-                double sn, cn;
-                sincos(2 * phi, &sn, &cn);  // as the reference's build evaluates it
-                vr += cn;
-                vi += sn;
-            }
-        }
-        off[e + 1] = (long long)keep.size();
-    }
-    const long long n = (long long)keep.size();
-    for (int e = 0; e < nev; e++)
-        if (off[e + 1] - off[e] >= (1 << 18))
-            return fail(c, HBT_ERR_CAPACITY, "an event keeps more than 262143 particles");
-    if (n >= (1ll << 31)) return fail(c, HBT_ERR_CAPACITY, "buffer too large");
-    if ((size_t)n > s.cap) {
-        for (auto &p : s.d_soa) {
-            int rc = dev_alloc(c, &p, (size_t)n);
-            if (rc) return rc;
-        }
-        int rc = dev_alloc(c, &s.d_pf64, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_pf32, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_cw64, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_cw32, (size_t)n);
-        if (rc) return rc;
-        s.cap = (size_t)n;
-    }
-    if ((size_t)nev + 1 > s.cap_ev) {
-        int rc = dev_alloc(c, &s.d_off, (size_t)nev + 1);
-        if (rc) return rc;
-        s.cap_ev = (size_t)nev + 1;
-    }
-    std::vector<double> col((size_t)n);
-    for (int a = 0; a < 8; a++) {
-        for (long long k = 0; k < n; k++) col[k] = soa[a][keep[k]];
-        HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
-    }
-    // prefilter records
-    std::vector<PfRec64> pf((size_t)n);
-    std::vector<PfRec32> pf32((size_t)n);
-    std::vector<double> cw((size_t)n);
-    std::vector<float> cw32((size_t)n);
-    double pmax2 = 0.0, bmax = 0.0;
-    const bool boost = c->P.boost;
-    const bool qinv = c->P.qinv;
-    for (long long k = 0; k < n; k++) {
-        const long long i = keep[k];
-        const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
-        const double n2 = px * px + py * py;
-        pmax2 = std::max(pmax2, n2);
-        pf[k].px = px;
-        pf[k].py = py;
-        pf[k].nrm = n2 - 0.5 * c->mid;
-        pf[k].beta = boost ? pz / E : pz;
-        bmax = std::max(bmax, std::fabs(pf[k].beta));
-        cw[k] = qinv ? INFINITY : (boost ? c->c_beta / E : c->c_beta);
-    }
-    const double eps32 = 5.9604644775390625e-8;
-    for (long long k = 0; k < n; k++) {
-        pf32[k].px = (float)pf[k].px;
-        pf32[k].py = (float)pf[k].py;
-        pf32[k].nrm = (float)pf[k].nrm;
-        pf32[k].beta = (float)pf[k].beta;
-        // covers the FP32 rounding of beta (|beta| <= bmax) and of cw itself
-        cw32[k] = qinv ? INFINITY : (float)(cw[k] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
-    }
-    HIPCHECK(c, hipMemcpy(s.d_pf64, pf.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(s.d_pf32, pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(s.d_cw64, cw.data(), n * 8, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(s.d_cw32, cw32.data(), n * 4, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(s.d_off, off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
-    s.nev = nev;
-    s.n = n;
-    s.n_in = n_in;
-    s.off = off;
-    s.pmax2 = pmax2;
-    s.bmax = bmax;
-    s.psi2 = c->P.azimuthal ? atan2(vi, vr) / 2 : 0.0;
-    s.valid = true;
-    return HBT_OK;
+    const int32_t grp[2] = {0, nev};
+    return upload_slot(c, slot, 1, grp, nev, ev_off, soa);
+}
+
+int hbt_upload_groups(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev,
+                      int32_t nev, const int64_t *ev_off, const double *const soa[8]) {
+    if (!c || !grp_ev) return HBT_ERR_ARG;
+    return upload_slot(c, slot, ngroups, grp_ev, nev, ev_off, soa);
 }
 
 int hbt_slot_info(hbt_ctx *c, int32_t slot, int32_t *nev, int64_t *n_kept) {
@@ -698,135 +951,182 @@ int hbt_slot_info(hbt_ctx *c, int32_t slot, int32_t *nev, int64_t *n_kept) {
 
 int hbt_run_same(hbt_ctx *c) {
     if (!c) return HBT_ERR_ARG;
-    Slot &s = c->slot[0];
-    if (!s.valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
-    const long long n = s.n;
-    const uint64_t enum_call = (uint64_t)(n * (n - 1) / 2);
-    int rc = check_cap(c, false, enum_call);
+    int rc = check_slots(c);
     if (rc) return rc;
-    DevHist H = hist_view(c);
-    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-    if (n >= 2) {
-        if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
-            const int T = (int)((n + TILE - 1) / TILE);
-            if (T > 65535) return fail(c, HBT_ERR_CAPACITY, "group too large for the plain kernel");
-            hipLaunchKernelGGL(k_same_plain, dim3(T, T), dim3(TILE), 0, c->stream, c->P, H,
-                               soa_view(s), 0ll, (int)n);
-            HIPCHECK(c, hipGetLastError());
-        } else {
-            const int TI = 64 * c->R;
-            std::vector<WorkItem> items;
-            for (long long a = 0; a + 1 < n; a += TI) {
-                const int ni = (int)std::min<long long>(TI, n - a);
-                for (long long jb = a + 1; jb < n; jb += c->jchunk) {
-                    WorkItem w{};
-                    w.i0 = a;
-                    w.j0 = jb;
-                    w.ni = ni;
-                    w.nj = (int)std::min<long long>(c->jchunk, n - jb);
-                    w.kind = 0;
-                    items.push_back(w);
-                }
-            }
-            rc = ensure_items(c, items.size());
-            if (rc) return rc;
-            HIPCHECK(c, hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(WorkItem),
-                                       hipMemcpyHostToDevice, c->stream));
-            set_fp32_margins(c);
-            FastArgs a = fast_args(c);
-            a.nitems = (int)items.size();
-            rc = launch_fast(c, a);
-            if (rc) return rc;
-            HIPCHECK(c, hipStreamSynchronize(c->stream));  // items vector lifetime
-        }
-    }
-    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-    HIPCHECK(c, hipEventSynchronize(c->ev[1]));
-    HIPCHECK(c, hipEventElapsedTime(&c->ms_same, c->ev[0], c->ev[1]));
-    c->enum_same_total += enum_call;
+    std::vector<WorkItem> items;
+    uint64_t es = 0;
+    build_items(c, true, false, items, es);
+    rc = check_cap(c, false, es);
+    if (rc) return rc;
+    rc = upload_psi(c);
+    if (rc) return rc;
     if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
-        // enumerated count is a closed form
+        HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+        rc = run_plain(c, true, false);
+        if (rc) return rc;
+        HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHECK(c, hipEventSynchronize(c->ev[1]));
+        HIPCHECK(c, hipEventElapsedTime(&c->ms_same, c->ev[0], c->ev[1]));
+    } else {
+        c->batch_ready = false;
+        rc = upload_items(c, items);
+        if (!rc) rc = timed_fast(c, &c->ms_same);
+        if (rc) return rc;
     }
+    c->enum_same_total += es;
     return HBT_OK;
 }
 
 int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const double *cos_sin) {
     if (!c) return HBT_ERR_ARG;
+    int rc = check_slots(c);
+    if (rc) return rc;
     Slot &s = c->slot[0];
-    const Slot &m = mixed_slot(c);
-    if (!s.valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
     if (s.nev == 0) return HBT_OK;
     if (M <= 0 || M > 64 * 1024 || !partner_ids || !cos_sin)
         return fail(c, HBT_ERR_ARG, "bad mixing plan");
-    const long long rows = (long long)s.nev * M;
-    uint64_t enum_call = 0;
-    for (int e = 0; e < s.nev; e++) {
-        const long long n1 = s.off[e + 1] - s.off[e];
-        for (int k = 0; k < M; k++) {
-            const int p = partner_ids[(long long)e * M + k];
-            if (p < 0 || p >= m.nev) return fail(c, HBT_ERR_ARG, "partner id out of range");
-            enum_call += (uint64_t)(n1 * (m.off[p + 1] - m.off[p]));
-        }
-    }
-    int rc = check_cap(c, true, enum_call);
+    std::vector<int> Me(s.nev, M);
+    uint64_t em = 0, es = 0;
+    c->batch_ready = false;
+    rc = upload_plan(c, Me, partner_ids, cos_sin, em);
     if (rc) return rc;
-    rc = ensure_plan(c, (size_t)rows);
+    rc = check_cap(c, true, em);
     if (rc) return rc;
-    HIPCHECK(c, hipMemcpyAsync(c->d_ids, partner_ids, rows * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(c->d_cs, cos_sin, rows * 16, hipMemcpyHostToDevice, c->stream));
-    DevHist H = hist_view(c);
-    HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+    rc = upload_psi(c);
+    if (rc) return rc;
     if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
-        long long max1 = 0, max2 = 0;
-        for (int e = 0; e < s.nev; e++) max1 = std::max(max1, s.off[e + 1] - s.off[e]);
-        for (int e = 0; e < m.nev; e++) max2 = std::max(max2, m.off[e + 1] - m.off[e]);
-        const int Ti = (int)((max1 + TILE - 1) / TILE), Tj = (int)((max2 + TILE - 1) / TILE);
-        if (Ti > 0 && Tj > 0) {
-            for (long long r0 = 0; r0 < rows; r0 += 65535) {
-                const int nz = (int)std::min<long long>(65535, rows - r0);
-                hipLaunchKernelGGL(k_mixed_plain, dim3(Tj, Ti, nz), dim3(TILE), 0, c->stream, c->P,
-                                   H, soa_view(s), soa_view(m), s.d_off, m.d_off, c->d_ids,
-                                   c->d_cs, M, (int)r0);
-                HIPCHECK(c, hipGetLastError());
-            }
-        }
+        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+        rc = run_plain(c, false, true);
+        if (rc) return rc;
+        HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+        HIPCHECK(c, hipEventSynchronize(c->ev[3]));
+        HIPCHECK(c, hipEventElapsedTime(&c->ms_mixed, c->ev[2], c->ev[3]));
     } else {
-        const int TI = 64 * c->R;
-        const int KC = std::min(c->kchunk, 64);
         std::vector<WorkItem> items;
-        for (int e = 0; e < s.nev; e++) {
-            const long long n1 = s.off[e + 1] - s.off[e];
-            for (long long a = 0; a < n1; a += TI) {
-                for (int k0 = 0; k0 < M; k0 += KC) {
-                    WorkItem w{};
-                    w.i0 = s.off[e] + a;
-                    w.ni = (int)std::min<long long>(TI, n1 - a);
-                    w.kind = 1;
-                    w.e = e;
-                    w.k0 = k0;
-                    w.k1 = std::min(M, k0 + KC);
-                    items.push_back(w);
-                }
-            }
-        }
-        if (!items.empty()) {
-            rc = ensure_items(c, items.size());
-            if (rc) return rc;
-            HIPCHECK(c, hipMemcpyAsync(c->d_items, items.data(), items.size() * sizeof(WorkItem),
-                                       hipMemcpyHostToDevice, c->stream));
-            set_fp32_margins(c);
-            FastArgs a = fast_args(c);
-            a.M = M;
-            a.nitems = (int)items.size();
-            rc = launch_fast(c, a);
-            if (rc) return rc;
-        }
-        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        build_items(c, false, true, items, es);
+        rc = upload_items(c, items);
+        if (!rc) rc = timed_fast(c, &c->ms_mixed);
+        if (rc) return rc;
     }
-    HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-    HIPCHECK(c, hipEventSynchronize(c->ev[3]));
-    HIPCHECK(c, hipEventElapsedTime(&c->ms_mixed, c->ev[2], c->ev[3]));
-    c->enum_mixed_total += enum_call;
+    c->enum_mixed_total += em;
+    return HBT_OK;
+}
+
+int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *partner_ids,
+                      const double *cos_sin) {
+    if (!c || !M_per_group) return HBT_ERR_ARG;
+    int rc = check_slots(c);
+    if (rc) return rc;
+    Slot &s = c->slot[0];
+    std::vector<int> Me(s.nev, 0);
+    long long rows = 0;
+    for (int g = 0; g < s.ngroups; g++) {
+        if (M_per_group[g] < 0 || M_per_group[g] > 64 * 1024)
+            return fail(c, HBT_ERR_ARG, "bad number of mixed events");
+        for (int e = s.grp_ev[g]; e < s.grp_ev[g + 1]; e++) {
+            Me[e] = M_per_group[g];
+            rows += Me[e];
+        }
+    }
+    if (rows > 0 && (!partner_ids || !cos_sin)) return fail(c, HBT_ERR_ARG, "bad mixing plan");
+    uint64_t em = 0;
+    rc = upload_plan(c, Me, partner_ids, cos_sin, em);
+    if (rc) return rc;
+    return prepare_and_capture(c);
+}
+
+int hbt_launch_batch(hbt_ctx *c) {
+    if (!c) return HBT_ERR_ARG;
+    if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
+    int rc = check_cap(c, false, c->batch_enum_same);
+    if (!rc) rc = check_cap(c, true, c->batch_enum_mixed);
+    if (rc) return rc;
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    if (c->cfg.kernel == HBT_KERNEL_FAST) {
+        if (c->nitems > 0) HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    } else {
+        rc = run_plain(c, true, true);
+        if (rc) return rc;
+    }
+    HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+    c->enum_same_total += c->batch_enum_same;
+    c->enum_mixed_total += c->batch_enum_mixed;
+    return HBT_OK;
+}
+
+int hbt_sync(hbt_ctx *c, double *batch_ms) {
+    if (!c) return HBT_ERR_ARG;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->ms_batch = ms;
+    if (batch_ms) *batch_ms = c->ms_batch;
+    return HBT_OK;
+}
+
+int hbt_batch_info(hbt_ctx *c, uint64_t *enum_same, uint64_t *enum_mixed, int32_t *nitems) {
+    if (!c) return HBT_ERR_ARG;
+    if (enum_same) *enum_same = c->batch_enum_same;
+    if (enum_mixed) *enum_mixed = c->batch_enum_mixed;
+    if (nitems) *nitems = c->nitems;
+    return HBT_OK;
+}
+
+int hbt_get_tuning(hbt_ctx *c, hbt_tuning *t) {
+    if (!c || !t) return HBT_ERR_ARG;
+    *t = c->tune;
+    return HBT_OK;
+}
+
+int hbt_set_tuning(hbt_ctx *c, const hbt_tuning *t) {
+    if (!c || !t) return HBT_ERR_ARG;
+    if ((t->R != 1 && t->R != 2 && t->R != 4) || t->jchunk < 64 || t->jchunk > (1 << 24) ||
+        t->kchunk < 1 || t->kchunk > 64 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64)
+        return fail(c, HBT_ERR_ARG, "tuning out of range");
+    c->tune = *t;
+    if (c->batch_ready) return prepare_and_capture(c);
+    return HBT_OK;
+}
+
+// Times each candidate (one untimed and one timed launch of the prepared
+// batch) and keeps the fastest; histograms are cleared afterwards.
+int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
+    if (!c) return HBT_ERR_ARG;
+    if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
+    if (c->cfg.kernel != HBT_KERNEL_FAST) {
+        if (best_out) *best_out = c->tune;
+        return HBT_OK;
+    }
+    const int Rs[] = {1, 2, 4};
+    const int JCs[] = {1024, 4096};
+    const int KCs[] = {4, 16};
+    const int BPCs[] = {4, 8, 16};
+    hbt_tuning best = c->tune;
+    best.ms = 1e30;
+    for (int R : Rs)
+        for (int jc : JCs)
+            for (int kc : KCs)
+                for (int bpc : BPCs) {
+                    hbt_tuning t{R, jc, kc, bpc, 0.0};
+                    c->tune = t;
+                    std::vector<WorkItem> items;
+                    uint64_t es;
+                    build_items(c, true, true, items, es);
+                    int rc = upload_items(c, items);
+                    if (rc) return rc;
+                    float ms = 0.f;
+                    rc = timed_fast(c, &ms);
+                    if (!rc) rc = timed_fast(c, &ms);
+                    if (rc) return rc;
+                    if (ms < best.ms) {
+                        best = t;
+                        best.ms = ms;
+                    }
+                }
+    c->tune = best;
+    int rc = prepare_and_capture(c);
+    if (!rc) rc = zero_hist(c);
+    if (rc) return rc;
+    if (best_out) *best_out = best;
     return HBT_OK;
 }
 
@@ -845,11 +1145,11 @@ int hbt_read(hbt_ctx *c, hbt_result *o) {
         void *dst;
         const void *src;
         size_t bytes;
-    } cp[] = {{o->num_count, c->d_num_count, b8}, {o->den, c->d_den, b8},
-              {o->q_out_sum, c->d_q_out, b8},     {o->q_side_sum, c->d_q_side, b8},
-              {o->q_long_sum, c->d_q_long, b8},   {o->num, c->d_num, b8},
+    } cp[] = {{o->num_count, c->d_num_count, b8},   {o->den, c->d_den, b8},
+              {o->q_out_sum, c->d_q_out, b8},       {o->q_side_sum, c->d_q_side, b8},
+              {o->q_long_sum, c->d_q_long, b8},     {o->num, c->d_num, b8},
               {o->qinv_count, c->d_qinv_count, q8}, {o->qinv_den, c->d_qinv_den, q8},
-              {o->qinv_sum, c->d_qinv_sum, q8},   {o->qinv_num, c->d_qinv_num, q8}};
+              {o->qinv_sum, c->d_qinv_sum, q8},     {o->qinv_num, c->d_qinv_num, q8}};
     for (auto &x : cp)
         if (x.dst) HIPCHECK(c, hipMemcpy(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost));
     const int nKT = c->cfg.n_KT, nphi = c->P.nphi;
@@ -947,23 +1247,37 @@ int32_t hbt_rng_draw_plan(hbt_rng *r, int32_t nev, int32_t mixed_nev,
     return M;
 }
 
+// Every group of slot 0 is one of the reference's buffers; they are
+// processed as consecutive calls of calculate_HBT_correlation_function
+// (HBT_correlation.cpp:177-218): the plans are drawn group by group from the
+// one stream, then all pairs of all groups run in one batch.
 int hbt_calculate_buffer(hbt_ctx *c, hbt_rng *rng) {
     if (!c || !rng) return HBT_ERR_ARG;
-    Slot &s = c->slot[0];
-    if (!s.valid) return fail(c, HBT_ERR_STATE, "no buffer uploaded in slot 0");
-    if (c->P.azimuthal) c->P.psi_ref = s.psi2;  // :181-183
-    int rc = hbt_run_same(c);
+    int rc = check_slots(c);
     if (rc) return rc;
+    Slot &s = c->slot[0];
     const Slot &m = mixed_slot(c);
-    const int nev = s.nev, mixed_nev = m.nev;
-    if (nev == 0) return HBT_OK;
-    if (mixed_nev <= 0) return fail(c, HBT_ERR_STATE, "mixed-event buffer is empty");
-    const int M = c->cfg.number_of_mixed_events > 0 ? c->cfg.number_of_mixed_events
-                                                    : mixed_nev / 2 + 1;
-    std::vector<int32_t> ids((size_t)nev * M);
-    std::vector<double> cs((size_t)2 * nev * M);
-    hbt_rng_draw_plan(rng, nev, mixed_nev, c->cfg.number_of_mixed_events, ids.data(), cs.data());
-    return hbt_run_mixed(c, M, ids.data(), cs.data());
+    std::vector<int32_t> Mg(s.ngroups, 0);
+    std::vector<int32_t> ids;
+    std::vector<double> cs;
+    for (int g = 0; g < s.ngroups; g++) {
+        const int nev = s.grp_ev[g + 1] - s.grp_ev[g];
+        const int mixed_nev = m.grp_ev[g + 1] - m.grp_ev[g];
+        if (nev == 0) continue;  // no draws (the extra empty buffer at end of file)
+        if (mixed_nev <= 0) return fail(c, HBT_ERR_STATE, "mixed-event buffer is empty");
+        const int M = c->cfg.number_of_mixed_events > 0 ? c->cfg.number_of_mixed_events
+                                                        : mixed_nev / 2 + 1;
+        Mg[g] = M;
+        const size_t base = ids.size();
+        ids.resize(base + (size_t)nev * M);
+        cs.resize(2 * (base + (size_t)nev * M));
+        hbt_rng_draw_plan(rng, nev, mixed_nev, c->cfg.number_of_mixed_events, ids.data() + base,
+                          cs.data() + 2 * base);
+    }
+    rc = hbt_prepare_batch(c, Mg.data(), ids.data(), cs.data());
+    if (!rc) rc = hbt_launch_batch(c);
+    if (!rc) rc = hbt_sync(c, nullptr);
+    return rc;
 }
 
 }  // extern "C"

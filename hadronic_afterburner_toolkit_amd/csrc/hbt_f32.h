@@ -37,11 +37,18 @@ __device__ __forceinline__ double rcp_refined(double a) {
     return r;
 }
 
-// cos(x) with the argument reduced in FP64 and the cosine on FP32.
+// cos(x): argument reduced to [-pi, pi] in FP64, then split r = rh + rl
+// with rh in FP32; cos and sin of rh on FP32 and the first-order
+// compensation cos(rh + rl) = cos(rh) - rl sin(rh) (|rl| < 2^-22, so the
+// dropped term rl^2/2 is below 1e-13).  Error ~1 FP32 ulp of the result.
 __device__ __forceinline__ double cos_reduced_f32(const DevParams &P, double x) {
     const double k = rint(x * P.inv_two_pi);
     const double r = fma(-k, P.two_pi_lo, fma(-k, P.two_pi_hi, x));
-    return (double)cosf((float)r);
+    const float rh = (float)r;
+    const float rl = (float)(r - (double)rh);
+    float s, c;
+    sincosf(rh, &s, &c);
+    return (double)c - (double)rl * (double)s;
 }
 
 // Index with a guard: returns -2 when (v - base)*inv lies within g of an
@@ -126,18 +133,18 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
 
 // Same-event pair in the precision form.  The q_inv and K_phi variants use
 // the literal FP64 cascade.
-__device__ __forceinline__ int bin_same_fast(const DevParams &P, const DevHist &H,
+__device__ __forceinline__ int bin_same_fast(const DevParams &P, const DevHist &H, double psi,
                                              const DevSoA &S, long long i, long long j) {
     const double p1x = S.px[i], p1y = S.py[i], p1z = S.pz[i], p1E = S.E[i];
     const double p2x = S.px[j], p2y = S.py[j], p2z = S.pz[j], p2E = S.E[j];
     if (P.qinv | P.azimuthal)
-        return bin_same_f64(P, H, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x, p2y,
-                            p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
+        return bin_same_f64(P, H, psi, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x,
+                            p2y, p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
     PairBins b;
     const int st = cascade_fast<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
     if (st < 0)
-        return bin_same_f64(P, H, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x, p2y,
-                            p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
+        return bin_same_f64(P, H, psi, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x,
+                            p2y, p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
     if (!(st & ST_BIN)) return st;
     const double t_diff = S.t[i] - S.t[j];
     const double x_diff = S.x[i] - S.x[j];
@@ -157,18 +164,19 @@ __device__ __forceinline__ int bin_same_fast(const DevParams &P, const DevHist &
 
 // Mixed pair in the precision form; j is rotated exactly as the reference
 // does (HBT_correlation.cpp:522-523) before the cascade.
-__device__ __forceinline__ int bin_mixed_fast(const DevParams &P, const DevHist &H,
+__device__ __forceinline__ int bin_mixed_fast(const DevParams &P, const DevHist &H, double psi,
                                               const DevSoA &A, const DevSoA &B, long long i,
                                               long long j, double c, double s) {
     const double px = B.px[j], py = B.py[j];
     const double p2x = px * c - py * s, p2y = px * s + py * c;
     const double p1x = A.px[i], p1y = A.py[i], p1z = A.pz[i], p1E = A.E[i];
     const double p2z = B.pz[j], p2E = B.E[j];
-    if (P.qinv | P.azimuthal) return bin_mixed_f64(P, H, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
+    if (P.qinv | P.azimuthal)
+        return bin_mixed_f64(P, H, psi, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
     PairBins b;
     // mixed windows are half-open: v < hi (HBT_correlation.cpp:613)
     const int st = cascade_fast<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
-    if (st < 0) return bin_mixed_f64(P, H, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
+    if (st < 0) return bin_mixed_f64(P, H, psi, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
     if (st & ST_BIN) atomic_add_u64(&H.den[bin_index(P, b.kt, 0, b.io, b.is, b.il)], 1ull);
     return st;
 }

@@ -42,8 +42,10 @@ struct WorkItem {
     int ni;         // number of i particles (<= 64*R)
     int nj;         // same-event: number of j particles
     int kind;       // 0 same-event, 1 mixed
-    int e;          // mixed: event index (row of the mixing plan)
-    int k0, k1;     // mixed: partner range [k0,k1) of the plan row
+    int e;          // mixed: event index (its plan rows start at rowoff[e])
+    int k0, k1;     // mixed: partner range [k0,k1) of the event's plan rows
+    int grp;        // group (buffer) of the pair, selects psi_ref
+    int pad;
 };
 
 struct FastArgs {
@@ -56,9 +58,10 @@ struct FastArgs {
     const double *cwA, *cwB;  // per-particle c/E bound (FP64)
     const float *cwA32, *cwB32;
     const long long *mix_off;  // event offsets of the mixed list (cut)
-    const int *plan_ids;       // [nev*M]
-    const double *plan_cs;     // [2*nev*M]
-    int M;
+    const long long *rowoff;   // plan rows of event e: [rowoff[e], rowoff[e+1])
+    const int *plan_ids;       // partner event (global index in the mixed list)
+    const double *plan_cs;     // cos, sin of the partner's rotation
+    const double *psi;         // psi_ref per group
     const WorkItem *items;
     int nitems;
     int *counter;              // work counter, zeroed before launch
@@ -84,58 +87,63 @@ struct Exact;
 
 template <>
 struct Exact<double> {
-    static __device__ __forceinline__ int same(const FastArgs &a, long long i, long long j) {
+    static __device__ __forceinline__ int same(const FastArgs &a, double psi, long long i,
+                                               long long j) {
         const DevSoA &S = a.A;
-        return bin_same_f64(a.P, a.H, S.px[i], S.py[i], S.pz[i], S.E[i], S.x[i], S.y[i],
+        return bin_same_f64(a.P, a.H, psi, S.px[i], S.py[i], S.pz[i], S.E[i], S.x[i], S.y[i],
                             S.z[i], S.t[i], S.px[j], S.py[j], S.pz[j], S.E[j], S.x[j],
                             S.y[j], S.z[j], S.t[j]);
     }
-    static __device__ __forceinline__ int mixed(const FastArgs &a, long long i, long long j,
-                                                double c, double s) {
+    static __device__ __forceinline__ int mixed(const FastArgs &a, double psi, long long i,
+                                                long long j, double c, double s) {
         const double px = a.B.px[j], py = a.B.py[j];
-        return bin_mixed_f64(a.P, a.H, a.A.px[i], a.A.py[i], a.A.pz[i], a.A.E[i],
+        return bin_mixed_f64(a.P, a.H, psi, a.A.px[i], a.A.py[i], a.A.pz[i], a.A.E[i],
                              px * c - py * s, px * s + py * c, a.B.pz[j], a.B.E[j]);
     }
 };
 
 template <>
 struct Exact<float> {
-    static __device__ __forceinline__ int same(const FastArgs &a, long long i, long long j) {
-        return bin_same_fast(a.P, a.H, a.A, i, j);
+    static __device__ __forceinline__ int same(const FastArgs &a, double psi, long long i,
+                                               long long j) {
+        return bin_same_fast(a.P, a.H, psi, a.A, i, j);
     }
-    static __device__ __forceinline__ int mixed(const FastArgs &a, long long i, long long j,
-                                                double c, double s) {
-        return bin_mixed_fast(a.P, a.H, a.A, a.B, i, j, c, s);
+    static __device__ __forceinline__ int mixed(const FastArgs &a, double psi, long long i,
+                                                long long j, double c, double s) {
+        return bin_mixed_fast(a.P, a.H, psi, a.A, a.B, i, j, c, s);
     }
 };
 
 template <typename Prec, int QCAP>
 __device__ __forceinline__ int drain_same(const FastArgs &a, const unsigned *q, unsigned head,
-                                          int cnt, long long i0, long long j0) {
+                                          int cnt, long long i0, int ni, long long j0,
+                                          double psi) {
     const int lane = __lane_id();
     int binned = 0;
     if (lane < cnt) {
         const unsigned e = q[(head + lane) & (QCAP - 1)];
-        const long long i = i0 + (e & 255u);
-        const long long j = j0 + (e >> 8);
-        binned = (Exact<Prec>::same(a, i, j) & ST_BIN) ? 1 : 0;
+        if ((int)(e & 255u) < ni) {  // always true: only lanes inside the tile enqueue
+            const long long i = i0 + (e & 255u);
+            const long long j = j0 + (e >> 8);
+            binned = (Exact<Prec>::same(a, psi, i, j) & ST_BIN) ? 1 : 0;
+        }
     }
     return binned;
 }
 
 template <typename Prec, int QCAP>
 __device__ __forceinline__ int drain_mixed(const FastArgs &a, const unsigned *q, unsigned head,
-                                           int cnt, long long i0, int e, int k0) {
+                                           int cnt, long long i0, int ni, long long row0,
+                                           double psi) {
     const int lane = __lane_id();
     int binned = 0;
-    if (lane < cnt) {
+    if (lane < cnt && (int)(q[(head + lane) & (QCAP - 1)] & 255u) < ni) {
         const unsigned ent = q[(head + lane) & (QCAP - 1)];
-        const int k = k0 + (int)(ent >> 26);
-        const long long row = (long long)e * a.M + k;
+        const long long row = row0 + (long long)(ent >> 26);
         const int pid = a.plan_ids[row];
         const long long j = a.mix_off[pid] + ((ent >> 8) & 0x3ffffu);
         const double c = a.plan_cs[2 * row], s = a.plan_cs[2 * row + 1];
-        binned = (Exact<Prec>::mixed(a, i0 + (ent & 255u), j, c, s) & ST_BIN) ? 1 : 0;
+        binned = (Exact<Prec>::mixed(a, psi, i0 + (ent & 255u), j, c, s) & ST_BIN) ? 1 : 0;
     }
     return binned;
 }
@@ -172,6 +180,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
         if (it >= a.nitems) break;
         const WorkItem w = a.items[it];
+        const double psi = a.psi[w.grp];
         // i particles held in registers, R per lane
         Prec ix[R], iy[R], in[R], ib[R], icw[R];
         long long ii[R];
@@ -185,7 +194,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
             ix[r] = p.px;
             iy[r] = p.py;
             in[r] = p.nrm;
-            ib[r] = iv[r] ? p.beta : (Prec)1e30;  // invalid lanes fail the beta test
+            ib[r] = p.beta;  // lanes past the tile (iv false) never pass
             icw[r] = cwA[ii[r]];
         }
         unsigned head = 0, tail = 0;
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
                     const Prec v = fma(-two, d, sn);
                     const Prec db = ib[r] - pj.beta;
                     const Prec rw = icw[r] + jcw;
-                    bool pass = (fabs(u) <= half_t) & (v <= q_thr) & (fabs(db) <= rw);
+                    bool pass = iv[r] & (fabs(u) <= half_t) & (v <= q_thr) & (fabs(db) <= rw);
                     if (tri) pass = pass & (j > ii[r]);
                     const unsigned long long m = __ballot(pass);
                     if (m) {
@@ -215,19 +224,20 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
                 }
                 while (tail - head >= 64) {
                     __builtin_amdgcn_wave_barrier();
-                    nb_same += drain_same<Prec, QCAP>(a, q, head, 64, w.i0, w.j0);
+                    nb_same += drain_same<Prec, QCAP>(a, q, head, 64, w.i0, w.ni, w.j0, psi);
                     head += 64;
                 }
             }
             __builtin_amdgcn_wave_barrier();
             while (tail != head) {
                 const int cnt = (int)min(64u, tail - head);
-                nb_same += drain_same<Prec, QCAP>(a, q, head, cnt, w.i0, w.j0);
+                nb_same += drain_same<Prec, QCAP>(a, q, head, cnt, w.i0, w.ni, w.j0, psi);
                 head += cnt;
             }
         } else {
+            const long long row0 = a.rowoff[w.e] + w.k0;
             for (int k = w.k0; k < w.k1; k++) {
-                const long long row = (long long)w.e * a.M + k;
+                const long long row = a.rowoff[w.e] + k;
                 const int pid = a.plan_ids[row];
                 const double c = a.plan_cs[2 * row], s = a.plan_cs[2 * row + 1];
                 const long long jb = a.mix_off[pid], je = a.mix_off[pid + 1];
@@ -251,7 +261,8 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
                         const Prec v = fma(-two, d, sn);
                         const Prec db = ib[r] - pj.beta;
                         const Prec rw = icw[r] + jcw;
-                        const bool pass = (fabs(u) <= half_t) & (v <= q_thr) & (fabs(db) <= rw);
+                        const bool pass =
+                            iv[r] & (fabs(u) <= half_t) & (v <= q_thr) & (fabs(db) <= rw);
                         const unsigned long long m = __ballot(pass);
                         if (m) {
                             const unsigned pos = tail + (unsigned)__popcll(m & lanemask_lt());
@@ -261,7 +272,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
                     }
                     while (tail - head >= 64) {
                         __builtin_amdgcn_wave_barrier();
-                        nb_mixed += drain_mixed<Prec, QCAP>(a, q, head, 64, w.i0, w.e, w.k0);
+                        nb_mixed += drain_mixed<Prec, QCAP>(a, q, head, 64, w.i0, w.ni, row0, psi);
                         head += 64;
                     }
                 }
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
             __builtin_amdgcn_wave_barrier();
             while (tail != head) {
                 const int cnt = (int)min(64u, tail - head);
-                nb_mixed += drain_mixed<Prec, QCAP>(a, q, head, cnt, w.i0, w.e, w.k0);
+                nb_mixed += drain_mixed<Prec, QCAP>(a, q, head, cnt, w.i0, w.ni, row0, psi);
                 head += cnt;
             }
         }

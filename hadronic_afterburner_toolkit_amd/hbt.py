@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import HbtConfig, HbtError, HbtResult
+from ._lib import HbtConfig, HbtError, HbtResult, HbtTuning
 
 DEFAULTS = dict(qnpts=31, q_min=-0.15, q_max=0.15, n_KT=5, KT_min=0.15, KT_max=0.55,
                 n_Kphi=48, azimuthal_flag=0, invariant_radius_flag=0,
@@ -148,6 +148,72 @@ class HBTCorrelation:
         self._check(self._lib.hbt_upload(self.h, slot, buf.nev,
                                          off.ctypes.data_as(C.POINTER(C.c_int64)), ptrs),
                     "hbt_upload")
+
+    def upload_groups(self, bufs, slot: int = 0):
+        """Makes several buffers (groups) resident at once (batched form)."""
+        grp = np.zeros(len(bufs) + 1, np.int32)
+        offs, cols = [np.zeros(1, np.int64)], [[] for _ in range(8)]
+        base_p = 0
+        for g, b in enumerate(bufs):
+            grp[g + 1] = grp[g] + b.nev
+            offs.append(np.asarray(b.ev_off[1:], np.int64) - int(b.ev_off[0]) + base_p)
+            base_p += b.n
+            a0 = int(b.ev_off[0])
+            for k, a in enumerate(b.soa()):
+                cols[k].append(np.asarray(a[a0:a0 + b.n], np.float64))
+        off = np.ascontiguousarray(np.concatenate(offs))
+        arrs = [np.ascontiguousarray(np.concatenate(c)) if c else np.zeros(0) for c in cols]
+        ptrs = (C.POINTER(C.c_double) * 8)(*[_f64p(a) for a in arrs])
+        self._check(self._lib.hbt_upload_groups(
+            self.h, slot, len(bufs), grp.ctypes.data_as(C.POINTER(C.c_int32)), int(grp[-1]),
+            off.ctypes.data_as(C.POINTER(C.c_int64)), ptrs), "hbt_upload_groups")
+
+    def prepare_batch(self, M_per_group, ids, cs):
+        Mg = np.ascontiguousarray(M_per_group, dtype=np.int32)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        cs = np.ascontiguousarray(cs, dtype=np.float64)
+        if ids.size == 0:
+            ids = np.zeros(1, np.int32)
+            cs = np.zeros(2)
+        self._check(self._lib.hbt_prepare_batch(self.h, Mg.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                _f64p(cs)), "hbt_prepare_batch")
+
+    def launch_batch(self):
+        self._check(self._lib.hbt_launch_batch(self.h), "hbt_launch_batch")
+
+    def sync(self) -> float:
+        ms = C.c_double()
+        self._check(self._lib.hbt_sync(self.h, C.byref(ms)), "hbt_sync")
+        return ms.value
+
+    def batch_info(self):
+        a, b, n = C.c_uint64(), C.c_uint64(), C.c_int32()
+        self._check(self._lib.hbt_batch_info(self.h, C.byref(a), C.byref(b), C.byref(n)),
+                    "hbt_batch_info")
+        return a.value, b.value, n.value
+
+    def tuning(self) -> dict:
+        t = HbtTuning()
+        self._check(self._lib.hbt_get_tuning(self.h, C.byref(t)), "hbt_get_tuning")
+        return {k: getattr(t, k) for k, _ in HbtTuning._fields_}
+
+    def set_tuning(self, R=2, jchunk=1024, kchunk=8, blocks_per_cu=8):
+        t = HbtTuning(R, jchunk, kchunk, blocks_per_cu, 0.0)
+        self._check(self._lib.hbt_set_tuning(self.h, C.byref(t)), "hbt_set_tuning")
+
+    def autotune(self) -> dict:
+        t = HbtTuning()
+        self._check(self._lib.hbt_autotune(self.h, C.byref(t)), "hbt_autotune")
+        return {k: getattr(t, k) for k, _ in HbtTuning._fields_}
+
+    def calculate_groups(self, bufs, rng: "Random", mixed_bufs=None):
+        """Several buffers in one batch: same result as calling
+        calculate_HBT_correlation_function on each in turn."""
+        self.upload_groups(bufs, 0)
+        if mixed_bufs is not None:
+            self.upload_groups(mixed_bufs, 1)
+        self._check(self._lib.hbt_calculate_buffer(self.h, rng.h), "hbt_calculate_buffer")
 
     def set_psi_ref(self, psi: float):
         self._check(self._lib.hbt_set_psi_ref(self.h, float(psi)), "hbt_set_psi_ref")

@@ -99,9 +99,10 @@ int hbt_reset(hbt_ctx *ctx);
 int hbt_upload(hbt_ctx *ctx, int32_t slot, int32_t nev, const int64_t *ev_off,
                const double *const soa[8]);
 
-/* Replaces calculate_flow_event_plane_angle (HBT_correlation.cpp:233-249)
- * as used at :181-183: the caller computes psi_ref on the host and sets it
- * before running azimuthal (azimuthal_flag=1) pairs. */
+/* Overrides psi_ref for azimuthal (azimuthal_flag=1) pairs of all groups.
+ * By default each group's psi_ref is computed at upload exactly as
+ * calculate_flow_event_plane_angle(2) does (HBT_correlation.cpp:181-183,
+ * :233-249), over all particles passed in, before the rapidity cut. */
 int hbt_set_psi_ref(hbt_ctx *ctx, double psi_ref);
 
 /* Replaces combine_and_bin_particle_pairs (HBT_correlation.cpp:251-462):
@@ -160,11 +161,63 @@ int32_t hbt_rng_draw_plan(hbt_rng *rng, int32_t nev, int32_t mixed_nev,
                           double *cos_sin);
 
 /* Replaces calculate_HBT_correlation_function (HBT_correlation.cpp:177-218)
- * on the buffers already uploaded: psi_ref (when azimuthal) computed by the
- * caller beforehand, same-event pairs, then the mixing plan drawn from rng
- * and the mixed pairs.  Histograms accumulate across calls like the
- * reference's. */
+ * on the buffer(s) already uploaded: psi_ref (when azimuthal) of each group
+ * computed at upload over all its particles (:181-183), the mixing plan of
+ * each group drawn from rng in group order (:200-216, :495-497), then all
+ * same-event and mixed pairs as one batch.  Histograms accumulate across
+ * calls like the reference's. */
 int hbt_calculate_buffer(hbt_ctx *ctx, hbt_rng *rng);
+
+/* ---------------- batched form ----------------
+ * A slot may hold G groups at once, each one of the reference's particle
+ * buffers (Analysis.cpp:821-833 processes them one after another).  The
+ * batch runs every same-event pair of every group and every mixed pair of
+ * every event in ONE persistent launch of the fast kernel (a work queue of
+ * tiles of uneven size), captured once in a hipGraph and replayed by
+ * hbt_launch_batch.  Results equal G consecutive
+ * calculate_HBT_correlation_function calls. */
+
+/* hbt_upload for G groups: grp_ev[G+1] are event offsets of the groups
+ * (grp_ev[0] = 0, grp_ev[G] = nev).  The mixed list (slot 1), when used,
+ * must have the same number of groups. */
+int hbt_upload_groups(hbt_ctx *ctx, int32_t slot, int32_t ngroups, const int32_t *grp_ev,
+                      int32_t nev, const int64_t *ev_off, const double *const soa[8]);
+
+/* Uploads the mixing plans of all groups and prepares the batch: M_per_group[G]
+ * partners per event, partner_ids and cos_sin laid out event after event
+ * (each event's M rows), ids relative to the event's own group, as
+ * hbt_rng_draw_plan returns them. */
+int hbt_prepare_batch(hbt_ctx *ctx, const int32_t *M_per_group, const int32_t *partner_ids,
+                      const double *cos_sin);
+
+/* Enqueues the prepared batch on the context's stream (asynchronous). */
+int hbt_launch_batch(hbt_ctx *ctx);
+
+/* Waits for the context's stream; *batch_ms (may be NULL) receives the
+ * device time of the last hbt_launch_batch. */
+int hbt_sync(hbt_ctx *ctx, double *batch_ms);
+
+/* Pairs enumerated by one launch of the prepared batch (same-event
+ * sum N_g(N_g-1)/2, mixed sum N_e * sum N_partner) and its work items. */
+int hbt_batch_info(hbt_ctx *ctx, uint64_t *enum_same, uint64_t *enum_mixed, int32_t *nitems);
+
+/* Launch shape of the fast kernel: R i-particles per lane (1, 2, 4), j
+ * chunk of same-event tiles, partners per mixed tile, persistent blocks per
+ * CU; ms is the measured batch time when set by hbt_autotune. */
+typedef struct {
+    int32_t R;
+    int32_t jchunk;
+    int32_t kchunk;
+    int32_t blocks_per_cu;
+    double ms;
+} hbt_tuning;
+
+int hbt_get_tuning(hbt_ctx *ctx, hbt_tuning *t);
+int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *t);
+
+/* Autotuner: times the prepared batch over a grid of launch shapes and keeps
+ * the fastest (written to *best when not NULL).  Clears the histograms. */
+int hbt_autotune(hbt_ctx *ctx, hbt_tuning *best);
 
 /* Wall time (ms, HIP events on the context's stream) of the last
  * hbt_run_same / hbt_run_mixed kernel launches. */

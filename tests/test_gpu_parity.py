@@ -136,3 +136,68 @@ def test_modes_agree_on_uneven_events():
     for mode in ("fast_fp64", "fast_fp32c"):
         assert np.array_equal(res[mode].num_count, res["plain_fp64"].num_count)
         assert np.array_equal(res[mode].den, res["plain_fp64"].den)
+
+
+@pytest.mark.parametrize("mode", ["fast_fp64", "fast_fp32c", "plain_fp64"])
+def test_batch_equals_sequential_buffers(mode):
+    """The batched form (all groups resident, one persistent launch) gives the
+    same result as the groups processed one after another, and the hipGraph
+    replays add up exactly."""
+    prec, kern, _ = MODES[mode]
+    params = dict(qnpts=21, n_KT=4, KT_min=0.2, KT_max=0.8, HBTrap_min=-1.0, HBTrap_max=1.0,
+                  needed_number_of_pairs=1e18)
+    full = samples.generate(555, 24, samples.log_uniform_mult(20, 400))
+    groups = [full.events(0, 5), full.events(5, 9), full.events(14, 1), full.events(15, 9)]
+    seq = hbt.HBTCorrelation(params, precision=prec, kernel=kern)
+    rs = hbt.Random(3)
+    for b in groups:
+        seq.calculate_HBT_correlation_function(b, rs)
+    bat = hbt.HBTCorrelation(params, precision=prec, kernel=kern)
+    bat.calculate_groups(groups, hbt.Random(3))
+    a, b = seq.results(), bat.results()
+    assert np.array_equal(a.num_count, b.num_count)
+    assert np.array_equal(a.den, b.den)
+    assert np.allclose(a.num, b.num, rtol=1e-12, atol=1e-12)
+    # replay the prepared batch twice more: exactly 3x the counts
+    bat.launch_batch()
+    bat.sync()
+    bat.launch_batch()
+    bat.sync()
+    c = bat.results()
+    assert np.array_equal(c.num_count, 3 * b.num_count)
+    assert np.array_equal(c.den, 3 * b.den)
+    # and the oracle agrees with the sequential run
+    o = oracle.Oracle(params)
+    ro = oracle.OracleRng(3)
+    for g in groups:
+        o.calculate_buffer(g, ro)
+    assert np.array_equal(a.num_count, o.arrays()["num_count"].astype(np.uint64))
+    assert np.array_equal(a.den, o.arrays()["den"].astype(np.uint64))
+
+
+def test_autotune_keeps_results():
+    params = dict(qnpts=21, n_KT=4, KT_min=0.2, KT_max=0.8, HBTrap_min=-1.0, HBTrap_max=1.0,
+                  needed_number_of_pairs=1e18, number_of_mixed_events=4)
+    full = samples.generate(556, 16, 300.0)
+    groups = [full.events(0, 8), full.events(8, 8)]
+    ref = None
+    for tune in (False, True):
+        g = hbt.HBTCorrelation(params, precision=_lib.HBT_PREC_FP32C, kernel=_lib.HBT_KERNEL_FAST)
+        g.upload_groups(groups)
+        rng = hbt.Random(9)
+        plans = [rng.draw_plan(b.nev, b.nev, 4) for b in groups]
+        g.prepare_batch([p[0] for p in plans], np.concatenate([p[1] for p in plans]),
+                        np.concatenate([p[2] for p in plans]))
+        if tune:
+            t = g.autotune()
+            assert t["R"] in (1, 2, 4) and t["ms"] > 0
+            z = g.results()
+            assert int(z.num_count.sum()) == 0  # autotune clears the histograms
+        g.launch_batch()
+        g.sync()
+        r = g.results()
+        if ref is None:
+            ref = r
+        else:
+            assert np.array_equal(r.num_count, ref.num_count)
+            assert np.array_equal(r.den, ref.den)
