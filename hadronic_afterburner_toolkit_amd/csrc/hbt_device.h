@@ -29,9 +29,13 @@ struct DevParams {
     double inv_dKT, inv_dq, inv_two_pi, two_pi_hi, two_pi_lo;
 };
 
+// Histogram bins, array of structures: 8 doubles (64 bytes, one aligned
+// segment) per bin, so one bin's accumulations share a cache line.
+enum { BIN_COUNT = 0, BIN_QO = 1, BIN_QS = 2, BIN_QL = 3, BIN_NUM = 4, BIN_DEN = 5,
+       BIN_STRIDE = 8 };
+
 struct DevHist {
-    unsigned long long *num_count, *den;
-    double *q_out_sum, *q_side_sum, *q_long_sum, *num;
+    double *bins;  // [nbins][BIN_STRIDE]; COUNT and DEN hold u64 bit patterns
     unsigned long long *qinv_count, *qinv_den;
     double *qinv_sum, *qinv_num;
     unsigned long long *stage;  // [0..4] same, [5..9] mixed
@@ -91,7 +95,9 @@ enum { ST_KT = 1, ST_OUT = 2, ST_SIDE = 4, ST_BIN = 8 };
 // the mixed-event upper cuts (>=, :613, :622, :641).  Returns the stage
 // mask; ST_BIN set means all windows passed and b holds the bins.  The
 // q_inv branch is handled by the caller through qinv_* outputs.
-template <bool MIXED>
+// EXT = false compiles the q_inv and K_phi branches out (the caller
+// guarantees invariant_radius_flag = azimuthal_flag = 0).
+template <bool MIXED, bool EXT = true>
 __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, double p1y,
                                            double p1z, double p1E, double p2x, double p2y,
                                            double p2z, double p2E, double psi_ref,
@@ -113,7 +119,7 @@ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, doubl
     b.q_y = p1y - p2y;
     b.q_z = p1z - p2z;
     b.q_E = p1E - p2E;
-    if (P.qinv) {
+    if (EXT && P.qinv) {
         q_inv = sqrt(-(b.q_E * b.q_E - b.q_x * b.q_x - b.q_y * b.q_y - b.q_z * b.q_z));
         if (q_inv > P.q_lo_cut && q_inv < P.q_hi_cut) {
             int qi = (int)((q_inv - P.q_base) / P.delta_q);
@@ -155,7 +161,7 @@ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, doubl
     b.il = (int)((q_long - P.q_base) / P.delta_q);
     if (b.il >= P.qnpts || b.il < 0) return st;
     b.kphi = 0;
-    if (P.azimuthal) {
+    if (EXT && P.azimuthal) {
         b.kphi = kphi_index(P, K_x, K_y, psi_ref);
         if (b.kphi < 0) return st;
     }
@@ -165,19 +171,26 @@ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, doubl
     return st | ST_BIN;
 }
 
-// Same-event pair: cascade + accumulation (HBT_correlation.cpp:339-356,
-// :431-446).  Returns the stage mask.
-__device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H, double psi_ref,
-                                            double p1x,
-                                            double p1y, double p1z, double p1E, double x1,
-                                            double y1, double z1, double t1, double p2x,
-                                            double p2y, double p2z, double p2E, double x2,
-                                            double y2, double z2, double t2) {
+// One binned same-event pair: bin and the four FP64 sums it adds to.
+struct SameRec {
+    long k;
+    double qo, qs, ql, c;
+};
+
+// Same-event pair: cascade and cos (HBT_correlation.cpp:339-356, :431-446).
+// q_inv contributions are accumulated here; for ST_BIN the caller adds rec.
+template <bool EXT = true>
+__device__ __forceinline__ int eval_same_f64(const DevParams &P, const DevHist &H, double psi_ref,
+                                             double p1x, double p1y, double p1z, double p1E,
+                                             double x1, double y1, double z1, double t1,
+                                             double p2x, double p2y, double p2z, double p2E,
+                                             double x2, double y2, double z2, double t2,
+                                             SameRec &rec) {
     PairBins b;
     double q_inv;
     int qinv_ok;
-    int st = cascade_f64<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b, q_inv,
-                                qinv_ok);
+    int st = cascade_f64<false, EXT>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b,
+                                     q_inv, qinv_ok);
     if (!(st & (ST_BIN)) && !qinv_ok) return st;
     double t_diff = t1 - t2;
     double x_diff = x1 - x2;
@@ -192,28 +205,66 @@ __device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H
         atomic_add_f64(&H.qinv_num[k], cos_qx);
     }
     if (st & ST_BIN) {
-        long k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
-        atomic_add_u64(&H.num_count[k], 1ull);
-        atomic_add_f64(&H.q_out_sum[k], b.q_out);
-        atomic_add_f64(&H.q_side_sum[k], b.q_side);
-        atomic_add_f64(&H.q_long_sum[k], b.q_long);
-        atomic_add_f64(&H.num[k], cos_qx);
+        rec.k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
+        rec.qo = b.q_out;
+        rec.qs = b.q_side;
+        rec.ql = b.q_long;
+        rec.c = cos_qx;
     }
     return st;
 }
 
-// Mixed pair: cascade + denominator (HBT_correlation.cpp:592-607, :681-683).
-__device__ __forceinline__ int bin_mixed_f64(const DevParams &P, const DevHist &H, double psi_ref,
-                                             double p1x,
-                                             double p1y, double p1z, double p1E, double p2x,
-                                             double p2y, double p2z, double p2E) {
+// Mixed pair: cascade (HBT_correlation.cpp:592-607); q_inv denominators are
+// accumulated here, for ST_BIN the caller adds 1 to the den of bin k.
+template <bool EXT = true>
+__device__ __forceinline__ int eval_mixed_f64(const DevParams &P, const DevHist &H, double psi_ref,
+                                              double p1x, double p1y, double p1z, double p1E,
+                                              double p2x, double p2y, double p2z, double p2E,
+                                              long &k) {
     PairBins b;
     double q_inv;
     int qinv_ok;
-    int st = cascade_f64<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b, q_inv,
-                               qinv_ok);
+    int st = cascade_f64<true, EXT>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b, q_inv,
+                                    qinv_ok);
     if (qinv_ok) atomic_add_u64(&H.qinv_den[(long)b.kt * P.qnpts + (qinv_ok - 1)], 1ull);
-    if (st & ST_BIN) atomic_add_u64(&H.den[bin_index(P, b.kt, b.kphi, b.io, b.is, b.il)], 1ull);
+    if (st & ST_BIN) k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
+    return st;
+}
+
+// Direct accumulation of one pair (HBT_correlation.cpp:437-446, :681-683).
+__device__ __forceinline__ void add_same_direct(const DevHist &H, const SameRec &r) {
+    double *b = H.bins + r.k * BIN_STRIDE;
+    atomic_add_u64((unsigned long long *)(b + BIN_COUNT), 1ull);
+    atomic_add_f64(b + BIN_QO, r.qo);
+    atomic_add_f64(b + BIN_QS, r.qs);
+    atomic_add_f64(b + BIN_QL, r.ql);
+    atomic_add_f64(b + BIN_NUM, r.c);
+}
+__device__ __forceinline__ void add_mixed_direct(const DevHist &H, long k) {
+    atomic_add_u64((unsigned long long *)(H.bins + k * BIN_STRIDE + BIN_DEN), 1ull);
+}
+
+template <bool EXT = true>
+__device__ __forceinline__ int bin_same_f64(const DevParams &P, const DevHist &H, double psi_ref,
+                                            double p1x, double p1y, double p1z, double p1E,
+                                            double x1, double y1, double z1, double t1,
+                                            double p2x, double p2y, double p2z, double p2E,
+                                            double x2, double y2, double z2, double t2) {
+    SameRec r;
+    const int st = eval_same_f64<EXT>(P, H, psi_ref, p1x, p1y, p1z, p1E, x1, y1, z1, t1, p2x,
+                                      p2y, p2z, p2E, x2, y2, z2, t2, r);
+    if (st & ST_BIN) add_same_direct(H, r);
+    return st;
+}
+
+template <bool EXT = true>
+__device__ __forceinline__ int bin_mixed_f64(const DevParams &P, const DevHist &H, double psi_ref,
+                                             double p1x, double p1y, double p1z, double p1E,
+                                             double p2x, double p2y, double p2z, double p2E) {
+    long k = 0;
+    const int st =
+        eval_mixed_f64<EXT>(P, H, psi_ref, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, k);
+    if (st & ST_BIN) add_mixed_direct(H, k);
     return st;
 }
 

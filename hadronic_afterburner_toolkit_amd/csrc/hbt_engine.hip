@@ -150,12 +150,15 @@ __global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, De
 
 __global__ void k_zero_int(int *p) { *p = 0; }
 
-// Sum of a u64 histogram over the q bins of each (kt, kphi): the
-// reference's number_of_pairs_*_KTdiff / _KTKphidiff counters.
-__global__ void k_sum_groups(const unsigned long long *h, long per, unsigned long long *out) {
+// Sum of a u64 field over the q bins of each (kt, kphi): the reference's
+// number_of_pairs_*_KTdiff / _KTKphidiff counters.  Element k of group g
+// is h[(g*per + k)*stride].
+__global__ void k_sum_groups(const unsigned long long *h, long per, int stride,
+                             unsigned long long *out) {
     const int g = blockIdx.x;
     unsigned long long acc = 0;
-    for (long k = threadIdx.x; k < per; k += blockDim.x) acc += h[(long)g * per + k];
+    for (long k = threadIdx.x; k < per; k += blockDim.x)
+        acc += h[((long)g * per + k) * stride];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
     __shared__ unsigned long long part[16];
     if (__lane_id() == 0) part[threadIdx.x >> 6] = acc;
@@ -196,8 +199,7 @@ struct hbt_ctx {
     hipEvent_t ev[4] = {};
     int cu_count = 0;
     long nbins = 0, nq = 0;
-    unsigned long long *d_num_count = nullptr, *d_den = nullptr;
-    double *d_q_out = nullptr, *d_q_side = nullptr, *d_q_long = nullptr, *d_num = nullptr;
+    double *d_bins = nullptr;  // [nbins][BIN_STRIDE]
     unsigned long long *d_qinv_count = nullptr, *d_qinv_den = nullptr;
     double *d_qinv_sum = nullptr, *d_qinv_num = nullptr;
     unsigned long long *d_stage = nullptr;   // [10]
@@ -293,12 +295,7 @@ void drop_graph(hbt_ctx *c) {
 
 DevHist hist_view(hbt_ctx *c) {
     DevHist h;
-    h.num_count = c->d_num_count;
-    h.den = c->d_den;
-    h.q_out_sum = c->d_q_out;
-    h.q_side_sum = c->d_q_side;
-    h.q_long_sum = c->d_q_long;
-    h.num = c->d_num;
+    h.bins = c->d_bins;
     h.qinv_count = c->d_qinv_count;
     h.qinv_den = c->d_qinv_den;
     h.qinv_sum = c->d_qinv_sum;
@@ -324,12 +321,7 @@ const Slot &mixed_slot(const hbt_ctx *c) { return c->slot[1].valid ? c->slot[1] 
 
 int zero_hist(hbt_ctx *c) {
     const size_t b8 = (size_t)c->nbins * 8, q8 = (size_t)c->nq * 8;
-    HIPCHECK(c, hipMemsetAsync(c->d_num_count, 0, b8, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_den, 0, b8, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_q_out, 0, b8, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_q_side, 0, b8, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_q_long, 0, b8, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_num, 0, b8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_bins, 0, b8 * BIN_STRIDE, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_qinv_count, 0, q8, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_qinv_den, 0, q8, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_qinv_sum, 0, q8, c->stream));
@@ -341,12 +333,13 @@ int zero_hist(hbt_ctx *c) {
     return HBT_OK;
 }
 
-// Per-(kt,kphi) sums of a u64 histogram into host memory.
-int group_sums(hbt_ctx *c, const unsigned long long *h, long per, int groups,
+// Per-(kt,kphi) sums of a u64 histogram field into host memory.
+int group_sums(hbt_ctx *c, const unsigned long long *h, long per, int groups, int stride,
                std::vector<unsigned long long> &out) {
     out.assign(groups, 0);
     if (groups <= 0) return HBT_OK;
-    hipLaunchKernelGGL(k_sum_groups, dim3(groups), dim3(1024), 0, c->stream, h, per, c->d_sums);
+    hipLaunchKernelGGL(k_sum_groups, dim3(groups), dim3(1024), 0, c->stream, h, per, stride,
+                       c->d_sums);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipMemcpyAsync(out.data(), c->d_sums, groups * 8, hipMemcpyDeviceToHost,
                                c->stream));
@@ -364,8 +357,10 @@ int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
     if (total <= c->needed + 1 && (!qinv || total <= c->needed_qinv)) return HBT_OK;
     const int groups = c->cfg.n_KT * c->P.nphi;
     std::vector<unsigned long long> cnt;
-    int rc = group_sums(c, mixed ? c->d_den : c->d_num_count,
-                        (long)c->P.qnpts * c->P.qnpts * c->P.qnpts, groups, cnt);
+    const unsigned long long *field =
+        (const unsigned long long *)(c->d_bins + (mixed ? BIN_DEN : BIN_COUNT));
+    int rc = group_sums(c, field, (long)c->P.qnpts * c->P.qnpts * c->P.qnpts, groups, BIN_STRIDE,
+                        cnt);
     if (rc) return rc;
     for (auto v : cnt)
         if (v + enum_call > c->needed + 1)
@@ -374,7 +369,8 @@ int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
                         "pair cap); raise needed_number_of_pairs or split the buffer");
     if (qinv) {
         std::vector<unsigned long long> qc;
-        rc = group_sums(c, mixed ? c->d_qinv_den : c->d_qinv_count, c->P.qnpts, c->cfg.n_KT, qc);
+        rc = group_sums(c, mixed ? c->d_qinv_den : c->d_qinv_count, c->P.qnpts, c->cfg.n_KT, 1,
+                        qc);
         if (rc) return rc;
         for (auto v : qc)
             if (v + enum_call > c->needed_qinv)
@@ -450,16 +446,23 @@ int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
     if (blocks < 1) blocks = 1;
     const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
     const int R = c->tune.R;
-#define HBT_LAUNCH(T, RR) hipLaunchKernelGGL((k_fast<T, RR>), dim3(blocks), dim3(256), 0, st, a)
+#define HBT_LAUNCH(T, RR, X) \
+    hipLaunchKernelGGL((k_fast<T, RR, X>), dim3(blocks), dim3(256), 0, st, a)
+#define HBT_LAUNCH_R(T, X)                 \
+    do {                                   \
+        if (R == 1) HBT_LAUNCH(T, 1, X);   \
+        else if (R == 4) HBT_LAUNCH(T, 4, X); \
+        else HBT_LAUNCH(T, 2, X);          \
+    } while (0)
+    const bool ext = c->P.qinv || c->P.azimuthal;
     if (f32) {
-        if (R == 1) HBT_LAUNCH(float, 1);
-        else if (R == 4) HBT_LAUNCH(float, 4);
-        else HBT_LAUNCH(float, 2);
+        if (ext) HBT_LAUNCH_R(float, true);
+        else HBT_LAUNCH_R(float, false);
     } else {
-        if (R == 1) HBT_LAUNCH(double, 1);
-        else if (R == 4) HBT_LAUNCH(double, 4);
-        else HBT_LAUNCH(double, 2);
+        if (ext) HBT_LAUNCH_R(double, true);
+        else HBT_LAUNCH_R(double, false);
     }
+#undef HBT_LAUNCH_R
 #undef HBT_LAUNCH
     HIPCHECK(c, hipGetLastError());
     return HBT_OK;
@@ -872,12 +875,7 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     c->nbins = (long)k.n_KT * P.nphi * q * q * q;
     c->nq = (long)k.n_KT * q;
     hbt_ctx *cp = c.get();
-    int rc = dev_alloc(cp, &c->d_num_count, c->nbins);
-    if (!rc) rc = dev_alloc(cp, &c->d_den, c->nbins);
-    if (!rc) rc = dev_alloc(cp, &c->d_q_out, c->nbins);
-    if (!rc) rc = dev_alloc(cp, &c->d_q_side, c->nbins);
-    if (!rc) rc = dev_alloc(cp, &c->d_q_long, c->nbins);
-    if (!rc) rc = dev_alloc(cp, &c->d_num, c->nbins);
+    int rc = dev_alloc(cp, &c->d_bins, (size_t)c->nbins * BIN_STRIDE);
     if (!rc) rc = dev_alloc(cp, &c->d_qinv_count, c->nq);
     if (!rc) rc = dev_alloc(cp, &c->d_qinv_den, c->nq);
     if (!rc) rc = dev_alloc(cp, &c->d_qinv_sum, c->nq);
@@ -900,8 +898,7 @@ int hbt_destroy(hbt_ctx *c) {
     if (!c) return HBT_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_graph(c);
-    void *ptrs[] = {c->d_num_count, c->d_den,       c->d_q_out,      c->d_q_side,
-                    c->d_q_long,    c->d_num,       c->d_qinv_count, c->d_qinv_den,
+    void *ptrs[] = {c->d_bins,      c->d_qinv_count, c->d_qinv_den,
                     c->d_qinv_sum,  c->d_qinv_num,  c->d_stage,      c->d_binned,
                     c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
                     c->d_items,     c->d_counter,   c->d_psi};
@@ -1140,25 +1137,37 @@ int hbt_last_kernel_ms(hbt_ctx *c, double *same_ms, double *mixed_ms) {
 int hbt_read(hbt_ctx *c, hbt_result *o) {
     if (!c || !o) return HBT_ERR_ARG;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    const size_t b8 = (size_t)c->nbins * 8, q8 = (size_t)c->nq * 8;
+    const size_t q8 = (size_t)c->nq * 8;
+    if (o->num_count || o->den || o->q_out_sum || o->q_side_sum || o->q_long_sum || o->num) {
+        std::vector<double> aos((size_t)c->nbins * BIN_STRIDE);
+        HIPCHECK(c, hipMemcpy(aos.data(), c->d_bins, aos.size() * 8, hipMemcpyDeviceToHost));
+        for (long k = 0; k < c->nbins; k++) {
+            const double *b = aos.data() + k * BIN_STRIDE;
+            if (o->num_count) std::memcpy(&o->num_count[k], b + BIN_COUNT, 8);
+            if (o->den) std::memcpy(&o->den[k], b + BIN_DEN, 8);
+            if (o->q_out_sum) o->q_out_sum[k] = b[BIN_QO];
+            if (o->q_side_sum) o->q_side_sum[k] = b[BIN_QS];
+            if (o->q_long_sum) o->q_long_sum[k] = b[BIN_QL];
+            if (o->num) o->num[k] = b[BIN_NUM];
+        }
+    }
     struct {
         void *dst;
         const void *src;
-        size_t bytes;
-    } cp[] = {{o->num_count, c->d_num_count, b8},   {o->den, c->d_den, b8},
-              {o->q_out_sum, c->d_q_out, b8},       {o->q_side_sum, c->d_q_side, b8},
-              {o->q_long_sum, c->d_q_long, b8},     {o->num, c->d_num, b8},
-              {o->qinv_count, c->d_qinv_count, q8}, {o->qinv_den, c->d_qinv_den, q8},
-              {o->qinv_sum, c->d_qinv_sum, q8},     {o->qinv_num, c->d_qinv_num, q8}};
+    } cp[] = {{o->qinv_count, c->d_qinv_count}, {o->qinv_den, c->d_qinv_den},
+              {o->qinv_sum, c->d_qinv_sum},     {o->qinv_num, c->d_qinv_num}};
     for (auto &x : cp)
-        if (x.dst) HIPCHECK(c, hipMemcpy(x.dst, x.src, x.bytes, hipMemcpyDeviceToHost));
+        if (x.dst) HIPCHECK(c, hipMemcpy(x.dst, x.src, q8, hipMemcpyDeviceToHost));
     const int nKT = c->cfg.n_KT, nphi = c->P.nphi;
     const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
     std::vector<unsigned long long> sn, sd, qn, qd;
-    int rc = group_sums(c, c->d_num_count, per, nKT * nphi, sn);
-    if (!rc) rc = group_sums(c, c->d_den, per, nKT * nphi, sd);
-    if (!rc) rc = group_sums(c, c->d_qinv_count, c->P.qnpts, nKT, qn);
-    if (!rc) rc = group_sums(c, c->d_qinv_den, c->P.qnpts, nKT, qd);
+    int rc = group_sums(c, (const unsigned long long *)(c->d_bins + BIN_COUNT), per, nKT * nphi,
+                        BIN_STRIDE, sn);
+    if (!rc)
+        rc = group_sums(c, (const unsigned long long *)(c->d_bins + BIN_DEN), per, nKT * nphi,
+                        BIN_STRIDE, sd);
+    if (!rc) rc = group_sums(c, c->d_qinv_count, c->P.qnpts, nKT, 1, qn);
+    if (!rc) rc = group_sums(c, c->d_qinv_den, c->P.qnpts, nKT, 1, qd);
     if (rc) return rc;
     for (int k = 0; k < nKT; k++) {
         uint64_t a = 0, b = 0;

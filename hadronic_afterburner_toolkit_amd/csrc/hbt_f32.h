@@ -131,20 +131,22 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
     return st | ST_BIN;
 }
 
-// Same-event pair in the precision form.  The q_inv and K_phi variants use
-// the literal FP64 cascade.
-__device__ __forceinline__ int bin_same_fast(const DevParams &P, const DevHist &H, double psi,
-                                             const DevSoA &S, long long i, long long j) {
+// Same-event pair in the precision form (see eval_same_f64 for the
+// contract).  The q_inv and K_phi variants, and ambiguous pairs, use the
+// literal FP64 cascade.
+template <bool EXT = true>
+__device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist &H, double psi,
+                                              const DevSoA &S, long long i, long long j,
+                                              SameRec &rec) {
     const double p1x = S.px[i], p1y = S.py[i], p1z = S.pz[i], p1E = S.E[i];
     const double p2x = S.px[j], p2y = S.py[j], p2z = S.pz[j], p2E = S.E[j];
-    if (P.qinv | P.azimuthal)
-        return bin_same_f64(P, H, psi, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x,
-                            p2y, p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
     PairBins b;
-    const int st = cascade_fast<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
+    const int st = (EXT && (P.qinv | P.azimuthal))
+                       ? -1
+                       : cascade_fast<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
     if (st < 0)
-        return bin_same_f64(P, H, psi, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i], p2x,
-                            p2y, p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j]);
+        return eval_same_f64<EXT>(P, H, psi, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i],
+                                  p2x, p2y, p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j], rec);
     if (!(st & ST_BIN)) return st;
     const double t_diff = S.t[i] - S.t[j];
     const double x_diff = S.x[i] - S.x[j];
@@ -152,32 +154,33 @@ __device__ __forceinline__ int bin_same_fast(const DevParams &P, const DevHist &
     const double z_diff = S.z[i] - S.z[j];
     const double phase =
         P.hbarC_inv * (b.q_E * t_diff - b.q_x * x_diff - b.q_y * y_diff - b.q_z * z_diff);
-    const double c = cos_reduced_f32(P, phase);
-    const long k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
-    atomic_add_u64(&H.num_count[k], 1ull);
-    atomic_add_f64(&H.q_out_sum[k], b.q_out);
-    atomic_add_f64(&H.q_side_sum[k], b.q_side);
-    atomic_add_f64(&H.q_long_sum[k], b.q_long);
-    atomic_add_f64(&H.num[k], c);
+    rec.k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
+    rec.qo = b.q_out;
+    rec.qs = b.q_side;
+    rec.ql = b.q_long;
+    rec.c = cos_reduced_f32(P, phase);
     return st;
 }
 
 // Mixed pair in the precision form; j is rotated exactly as the reference
 // does (HBT_correlation.cpp:522-523) before the cascade.
-__device__ __forceinline__ int bin_mixed_fast(const DevParams &P, const DevHist &H, double psi,
-                                              const DevSoA &A, const DevSoA &B, long long i,
-                                              long long j, double c, double s) {
+template <bool EXT = true>
+__device__ __forceinline__ int eval_mixed_fast(const DevParams &P, const DevHist &H, double psi,
+                                               const DevSoA &A, const DevSoA &B, long long i,
+                                               long long j, double c, double s, long &k) {
     const double px = B.px[j], py = B.py[j];
     const double p2x = px * c - py * s, p2y = px * s + py * c;
     const double p1x = A.px[i], p1y = A.py[i], p1z = A.pz[i], p1E = A.E[i];
     const double p2z = B.pz[j], p2E = B.E[j];
-    if (P.qinv | P.azimuthal)
-        return bin_mixed_f64(P, H, psi, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
     PairBins b;
-    // mixed windows are half-open: v < hi (HBT_correlation.cpp:613)
-    const int st = cascade_fast<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
-    if (st < 0) return bin_mixed_f64(P, H, psi, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E);
-    if (st & ST_BIN) atomic_add_u64(&H.den[bin_index(P, b.kt, 0, b.io, b.is, b.il)], 1ull);
+    // mixed windows are half-open: v < hi (HBT_correlation.cpp:613); the
+    // guard band around the cut sends v == hi to the literal cascade
+    const int st = (EXT && (P.qinv | P.azimuthal))
+                       ? -1
+                       : cascade_fast<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
+    if (st < 0)
+        return eval_mixed_f64<EXT>(P, H, psi, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, k);
+    if (st & ST_BIN) k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
     return st;
 }
 

@@ -75,89 +75,175 @@ struct Ring {
     static constexpr int CAP = (64 * (R + 1) <= 256) ? 256 : 512;
 };
 
-__device__ __forceinline__ unsigned long long lanemask_lt() {
-    const unsigned lane = __lane_id();
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+// ------------------------------------------------------- wave helpers
+
+// Mask of lanes where a <= b (ordered: NaN compares false), as one SGPR pair.
+__device__ __forceinline__ unsigned long long le_mask(float a, float b) {
+    return __builtin_amdgcn_fcmpf(a, b, 5 /* FCMP_OLE */);
+}
+__device__ __forceinline__ unsigned long long le_mask(double a, double b) {
+    return __builtin_amdgcn_fcmp(a, b, 5 /* FCMP_OLE */);
+}
+// Value of lane t (t uniform) in every lane.
+__device__ __forceinline__ float bcast(float v, int t) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), t));
+}
+__device__ __forceinline__ double bcast(double v, int t) {
+    const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, t);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), t);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Lanes 0..n-1 (n clamped to [0, 64]); 32-bit count.
+__device__ __forceinline__ unsigned long long low_lanes32(int n) {
+    return n <= 0 ? 0ull : (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
+}
+// Lanes 0..n-1 (n clamped to [0, 64]).
+__device__ __forceinline__ unsigned long long low_lanes(long long n) {
+    return n <= 0 ? 0ull : (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
 }
 
 // ---------------------------------------------------------------- drain
 // Each lane takes one queued survivor (lane < cnt), runs the exact cascade.
-template <typename Prec>
+template <typename Prec, bool EXT>
 struct Exact;
 
-template <>
-struct Exact<double> {
+template <bool EXT>
+struct Exact<double, EXT> {
     static __device__ __forceinline__ int same(const FastArgs &a, double psi, long long i,
-                                               long long j) {
+                                               long long j, SameRec &rec) {
         const DevSoA &S = a.A;
-        return bin_same_f64(a.P, a.H, psi, S.px[i], S.py[i], S.pz[i], S.E[i], S.x[i], S.y[i],
-                            S.z[i], S.t[i], S.px[j], S.py[j], S.pz[j], S.E[j], S.x[j],
-                            S.y[j], S.z[j], S.t[j]);
+        return eval_same_f64<EXT>(a.P, a.H, psi, S.px[i], S.py[i], S.pz[i], S.E[i], S.x[i],
+                                  S.y[i], S.z[i], S.t[i], S.px[j], S.py[j], S.pz[j], S.E[j],
+                                  S.x[j], S.y[j], S.z[j], S.t[j], rec);
     }
     static __device__ __forceinline__ int mixed(const FastArgs &a, double psi, long long i,
-                                                long long j, double c, double s) {
+                                                long long j, double c, double s, long &k) {
         const double px = a.B.px[j], py = a.B.py[j];
-        return bin_mixed_f64(a.P, a.H, psi, a.A.px[i], a.A.py[i], a.A.pz[i], a.A.E[i],
-                             px * c - py * s, px * s + py * c, a.B.pz[j], a.B.E[j]);
+        return eval_mixed_f64<EXT>(a.P, a.H, psi, a.A.px[i], a.A.py[i], a.A.pz[i], a.A.E[i],
+                                   px * c - py * s, px * s + py * c, a.B.pz[j], a.B.E[j], k);
     }
 };
 
-template <>
-struct Exact<float> {
+template <bool EXT>
+struct Exact<float, EXT> {
     static __device__ __forceinline__ int same(const FastArgs &a, double psi, long long i,
-                                               long long j) {
-        return bin_same_fast(a.P, a.H, psi, a.A, i, j);
+                                               long long j, SameRec &rec) {
+        return eval_same_fast<EXT>(a.P, a.H, psi, a.A, i, j, rec);
     }
     static __device__ __forceinline__ int mixed(const FastArgs &a, double psi, long long i,
-                                                long long j, double c, double s) {
-        return bin_mixed_fast(a.P, a.H, psi, a.A, a.B, i, j, c, s);
+                                                long long j, double c, double s, long &k) {
+        return eval_mixed_fast<EXT>(a.P, a.H, psi, a.A, a.B, i, j, c, s, k);
     }
 };
 
-template <typename Prec, int QCAP>
-__device__ __forceinline__ int drain_same(const FastArgs &a, const unsigned *q, unsigned head,
-                                          int cnt, long long i0, int ni, long long j0,
-                                          double psi) {
+// Per-wave LDS staging of binned same-event pairs for the cooperative
+// scatter: bin index and the four sums of up to 64 pairs.
+struct Stage {
+    long k[64];
+    double v[64][4];
+};
+
+// Each lane with lane < cnt takes one queued survivor and runs the exact
+// cascade.  Same-event entries: (j - j0) << 8 | slot; mixed entries:
+// (k - k0) << 26 | (j - partner start) << 8 | slot, slot = r*64 + lane of
+// the i particle in the tile.
+// Binned same-event pairs are added cooperatively: they are staged in LDS,
+// then groups of 5 lanes add one pair's count and four sums, so one atomic
+// instruction touches 12 bins (12 cache lines) instead of 64, and the five
+// values of a bin travel together (atomics on this part execute at the
+// memory side, where the cost is per line request).
+template <typename Prec, bool EXT, int QCAP>
+__device__ __forceinline__ int drain(const FastArgs &a, const unsigned *q, Stage &stg,
+                                     unsigned head, int cnt, const WorkItem &w, long long row0,
+                                     double psi) {
     const int lane = __lane_id();
-    int binned = 0;
+    int st = 0;
+    SameRec rec;
+    long kmix = 0;
     if (lane < cnt) {
-        const unsigned e = q[(head + lane) & (QCAP - 1)];
-        if ((int)(e & 255u) < ni) {  // always true: only lanes inside the tile enqueue
-            const long long i = i0 + (e & 255u);
-            const long long j = j0 + (e >> 8);
-            binned = (Exact<Prec>::same(a, psi, i, j) & ST_BIN) ? 1 : 0;
+        const unsigned ent = q[(head + lane) & (QCAP - 1)];
+        const int slot = (int)(ent & 255u);
+        if (slot < w.ni) {  // always true: only lanes inside the tile enqueue
+            const long long i = w.i0 + slot;
+            if (w.kind == 0) {
+                st = Exact<Prec, EXT>::same(a, psi, i, w.j0 + (ent >> 8), rec);
+            } else {
+                const long long row = row0 + (long long)(ent >> 26);
+                const long long j = a.mix_off[a.plan_ids[row]] + ((ent >> 8) & 0x3ffffu);
+                st = Exact<Prec, EXT>::mixed(a, psi, i, j, a.plan_cs[2 * row],
+                                             a.plan_cs[2 * row + 1], kmix);
+            }
         }
     }
-    return binned;
+    const bool binned = (st & ST_BIN) != 0;
+    if (w.kind != 0) {
+        if (binned) add_mixed_direct(a.H, kmix);
+        return binned ? 1 : 0;
+    }
+    const unsigned long long m = __ballot(binned);
+    const int nb = __popcll(m);
+    if (nb == 0) return 0;
+    if (binned) {
+        const unsigned idx = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        stg.k[idx] = rec.k;
+        stg.v[idx][0] = rec.qo;
+        stg.v[idx][1] = rec.qs;
+        stg.v[idx][2] = rec.ql;
+        stg.v[idx][3] = rec.c;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int grp = lane / 5, sub = lane - 5 * grp;
+    for (int base = 0; base < nb; base += 12) {
+        const int p = base + grp;
+        if (lane < 60 && p < nb) {
+            double *b = a.H.bins + stg.k[p] * BIN_STRIDE;
+            if (sub == 0)
+                atomic_add_u64((unsigned long long *)(b + BIN_COUNT), 1ull);
+            else
+                atomic_add_f64(b + sub, stg.v[p][sub - 1]);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return binned ? 1 : 0;
 }
 
-template <typename Prec, int QCAP>
-__device__ __forceinline__ int drain_mixed(const FastArgs &a, const unsigned *q, unsigned head,
-                                           int cnt, long long i0, int ni, long long row0,
-                                           double psi) {
-    const int lane = __lane_id();
-    int binned = 0;
-    if (lane < cnt && (int)(q[(head + lane) & (QCAP - 1)] & 255u) < ni) {
-        const unsigned ent = q[(head + lane) & (QCAP - 1)];
-        const long long row = row0 + (long long)(ent >> 26);
-        const int pid = a.plan_ids[row];
-        const long long j = a.mix_off[pid] + ((ent >> 8) & 0x3ffffu);
-        const double c = a.plan_cs[2 * row], s = a.plan_cs[2 * row + 1];
-        binned = (Exact<Prec>::mixed(a, psi, i0 + (ent & 255u), j, c, s) & ST_BIN) ? 1 : 0;
+// 64 j records of one wave, one per lane (coalesced load), broadcast with
+// readlane while the next 64 are in flight.
+template <typename Prec, typename Rec>
+struct JChunk {
+    Prec px, py, nrm, beta, cw;
+    __device__ __forceinline__ void load(const Rec *__restrict__ pf, const Prec *__restrict__ cw_,
+                                         long long jb, long long je) {
+        const long long j = min(jb + (long long)__lane_id(), je - 1);
+        const Rec r = pf[j];
+        px = r.px;
+        py = r.py;
+        nrm = r.nrm;
+        beta = r.beta;
+        cw = cw_[j];
     }
-    return binned;
-}
+};
 
 // ---------------------------------------------------------------- kernel
-template <typename Prec, int R>
+// Persistent: each wave pulls work items from a global counter until none
+// are left.  Per item the wave holds 64R i particles (R per lane) and walks
+// its j segments (same-event: one range with j > i; mixed: one segment per
+// partner with the i particles rotated by -theta).  Survivors of the
+// prefilter go to the wave's LDS ring; whenever 64 are queued they are
+// binned, one per lane.  The loop is a flat state machine so the exact
+// cascade has a single call site.
+template <typename Prec, int R, bool EXT>
 __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
     using Rec = typename std::conditional<std::is_same<Prec, double>::value, PfRec64,
                                           PfRec32>::type;
     constexpr int QCAP = Ring<R>::CAP;
     __shared__ unsigned ring[4][QCAP];
+    __shared__ Stage stage[4];
     const int lane = __lane_id();
-    const int wid = threadIdx.x >> 6;
-    unsigned *q = ring[wid];
+    unsigned *q = ring[threadIdx.x >> 6];
+    Stage &stg = stage[threadIdx.x >> 6];
     const Rec *__restrict__ pfA = (const Rec *)a.pfA;
     const Rec *__restrict__ pfB = (const Rec *)a.pfB;
     const Prec *__restrict__ cwA;
@@ -181,107 +267,111 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         if (it >= a.nitems) break;
         const WorkItem w = a.items[it];
         const double psi = a.psi[w.grp];
-        // i particles held in registers, R per lane
-        Prec ix[R], iy[R], in[R], ib[R], icw[R];
-        long long ii[R];
-        bool iv[R];
+        const bool mixed = w.kind != 0;
+        const Rec *__restrict__ pf = mixed ? pfB : pfA;
+        const Prec *__restrict__ cwj = mixed ? cwB : cwA;
+        // i particles, R per lane
+        Prec ix[R], iy[R], in[R], ib[R], icw[R], rx[R], ry[R];
+        unsigned long long vmask[R];
+        unsigned slot[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int s = r * 64 + lane;
-            iv[r] = s < w.ni;
-            ii[r] = w.i0 + (iv[r] ? s : 0);
-            const Rec p = pfA[ii[r]];
-            ix[r] = p.px;
-            iy[r] = p.py;
+            slot[r] = (unsigned)(r * 64 + lane);
+            vmask[r] = low_lanes((long long)w.ni - r * 64);
+            const long long i = w.i0 + min((int)slot[r], w.ni - 1);
+            const Rec p = pfA[i];
+            ix[r] = rx[r] = p.px;
+            iy[r] = ry[r] = p.py;
             in[r] = p.nrm;
-            ib[r] = p.beta;  // lanes past the tile (iv false) never pass
-            icw[r] = cwA[ii[r]];
+            ib[r] = p.beta;
+            icw[r] = cwA[i];
         }
+        // segment state
+        const int nseg = mixed ? (w.k1 - w.k0) : 1;
+        const long long row0 = mixed ? a.rowoff[w.e] + w.k0 : 0;
+        int seg = -1;
+        long long jb = 0, je = 0, jc = 0;  // segment range, current chunk start
+        int t = 0, cnt = 0;
+        bool finished = false;
+        JChunk<Prec, Rec> cur, nxt;
         unsigned head = 0, tail = 0;
-        if (w.kind == 0) {
-            const long long i_last = w.i0 + w.ni - 1;
-            for (int jj = 0; jj < w.nj; jj++) {
-                const long long j = w.j0 + jj;
-                const Rec pj = pfA[j];
-                const Prec jcw = cwA[j];
-                const bool tri = j <= i_last;  // uniform
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const Prec d = fma(ix[r], pj.px, iy[r] * pj.py);
-                    const Prec sn = in[r] + pj.nrm;
-                    const Prec u = fma(two, d, sn);
-                    const Prec v = fma(-two, d, sn);
-                    const Prec db = ib[r] - pj.beta;
-                    const Prec rw = icw[r] + jcw;
-                    bool pass = iv[r] & (fabs(u) <= half_t) & (v <= q_thr) & (fabs(db) <= rw);
-                    if (tri) pass = pass & (j > ii[r]);
-                    const unsigned long long m = __ballot(pass);
-                    if (m) {
-                        const unsigned pos = tail + (unsigned)__popcll(m & lanemask_lt());
-                        if (pass) q[pos & (QCAP - 1)] = ((unsigned)jj << 8) | (unsigned)(r * 64 + lane);
-                        tail += (unsigned)__popcll(m);
+        for (;;) {
+            if (tail - head >= 64 || (finished && tail != head)) {
+                const int n = (int)min(64u, tail - head);
+                __builtin_amdgcn_wave_barrier();
+                const int nb = drain<Prec, EXT, QCAP>(a, q, stg, head, n, w, row0, psi);
+                if (mixed)
+                    nb_mixed += nb;
+                else
+                    nb_same += nb;
+                head += n;
+                continue;
+            }
+            if (finished) break;
+            if (t == cnt) {  // next chunk, or next segment
+                if (seg >= 0 && jc + 64 < je) {
+                    jc += 64;
+                    cur = nxt;
+                } else {
+                    if (++seg == nseg) {
+                        finished = true;
+                        continue;
                     }
-                }
-                while (tail - head >= 64) {
-                    __builtin_amdgcn_wave_barrier();
-                    nb_same += drain_same<Prec, QCAP>(a, q, head, 64, w.i0, w.ni, w.j0, psi);
-                    head += 64;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            while (tail != head) {
-                const int cnt = (int)min(64u, tail - head);
-                nb_same += drain_same<Prec, QCAP>(a, q, head, cnt, w.i0, w.ni, w.j0, psi);
-                head += cnt;
-            }
-        } else {
-            const long long row0 = a.rowoff[w.e] + w.k0;
-            for (int k = w.k0; k < w.k1; k++) {
-                const long long row = a.rowoff[w.e] + k;
-                const int pid = a.plan_ids[row];
-                const double c = a.plan_cs[2 * row], s = a.plan_cs[2 * row + 1];
-                const long long jb = a.mix_off[pid], je = a.mix_off[pid + 1];
-                // rotate the lane's i particles by -theta
-                Prec rx[R], ry[R];
+                    if (mixed) {
+                        const long long row = row0 + seg;
+                        const int pid = a.plan_ids[row];
+                        jb = a.mix_off[pid];
+                        je = a.mix_off[pid + 1];
+                        const double c = a.plan_cs[2 * row], s = a.plan_cs[2 * row + 1];
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    rx[r] = (Prec)((double)ix[r] * c + (double)iy[r] * s);
-                    ry[r] = (Prec)((double)iy[r] * c - (double)ix[r] * s);
-                }
-                const unsigned kk = (unsigned)(k - w.k0) << 26;
-                for (long long j = jb; j < je; j++) {
-                    const Rec pj = pfB[j];
-                    const Prec jcw = cwB[j];
-                    const unsigned jo = (unsigned)(j - jb) << 8;
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const Prec d = fma(rx[r], pj.px, ry[r] * pj.py);
-                        const Prec sn = in[r] + pj.nrm;
-                        const Prec u = fma(two, d, sn);
-                        const Prec v = fma(-two, d, sn);
-                        const Prec db = ib[r] - pj.beta;
-                        const Prec rw = icw[r] + jcw;
-                        const bool pass =
-                            iv[r] & (fabs(u) <= half_t) & (v <= q_thr) & (fabs(db) <= rw);
-                        const unsigned long long m = __ballot(pass);
-                        if (m) {
-                            const unsigned pos = tail + (unsigned)__popcll(m & lanemask_lt());
-                            if (pass) q[pos & (QCAP - 1)] = kk | jo | (unsigned)(r * 64 + lane);
-                            tail += (unsigned)__popcll(m);
+                        for (int r = 0; r < R; r++) {  // rotate i by -theta
+                            rx[r] = (Prec)((double)ix[r] * c + (double)iy[r] * s);
+                            ry[r] = (Prec)((double)iy[r] * c - (double)ix[r] * s);
                         }
+                    } else {
+                        jb = w.j0;
+                        je = w.j0 + w.nj;
                     }
-                    while (tail - head >= 64) {
-                        __builtin_amdgcn_wave_barrier();
-                        nb_mixed += drain_mixed<Prec, QCAP>(a, q, head, 64, w.i0, w.ni, row0, psi);
-                        head += 64;
+                    if (jb >= je) {
+                        cnt = t = 0;
+                        continue;
                     }
+                    jc = jb;
+                    cur.load(pf, cwj, jc, je);
                 }
+                if (jc + 64 < je) nxt.load(pf, cwj, jc + 64, je);
+                cnt = (int)min(64ll, je - jc);
+                t = 0;
             }
-            __builtin_amdgcn_wave_barrier();
-            while (tail != head) {
-                const int cnt = (int)min(64u, tail - head);
-                nb_mixed += drain_mixed<Prec, QCAP>(a, q, head, cnt, w.i0, w.ni, row0, psi);
-                head += cnt;
+            // one j against the wave's 64R i particles
+            const long long j = jc + t;
+            const Prec jx = bcast(cur.px, t), jy = bcast(cur.py, t);
+            const Prec jn = bcast(cur.nrm, t), jbeta = bcast(cur.beta, t);
+            const Prec jcw = bcast(cur.cw, t);
+            const unsigned ebase = mixed ? ((unsigned)seg << 26) | ((unsigned)(j - jb) << 8)
+                                         : ((unsigned)(j - w.j0) << 8);
+            // same-event: lanes with i < j (only the first chunks of a tile
+            // overlap it; clamp so the 32-bit shift below stays defined)
+            const int tri = (int)min(max(j - w.i0, -1ll), 1024ll);
+            t++;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const Prec d = fma(rx[r], jx, ry[r] * jy);
+                const Prec sn = in[r] + jn;
+                const Prec u = fma(two, d, sn);
+                const Prec v = fma(-two, d, sn);
+                const Prec db = ib[r] - jbeta;
+                const Prec rw = icw[r] + jcw;
+                unsigned long long m = le_mask(fabs(u), half_t) & le_mask(v, q_thr) &
+                                       le_mask(fabs(db), rw) & vmask[r];
+                if (!mixed) m &= low_lanes32(tri - r * 64);
+                if (m) {
+                    const unsigned pos =
+                        tail + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                    if (__builtin_amdgcn_inverse_ballot_w64(m)) q[pos & (QCAP - 1)] = ebase | slot[r];
+                    tail += (unsigned)__popcll(m);
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
