@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contract=
           "-Wno-unused-result", "-Wno-unused-value", "-Wno-bitwise-instead-of-logical"]
 
 HOST_SRCS = ["host/ParameterReader.cpp", "host/Random.cpp", "host/particleSamples.cpp",
-             "host/HBT_correlation.cpp"]
+             "host/HBT_correlation.cpp"]  # the drop-in classes of the CLI
 
 
 def _run(cmd):
@@ -40,7 +40,8 @@ def _run(cmd):
 def build_engine() -> str:
     out = os.path.join(HERE, "libhbt_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + COMMON + ["-shared", "-o", tmp, os.path.join(CSRC, "hbt_engine.hip")])
+    _run([HIPCC] + COMMON + ["-shared", "-o", tmp, os.path.join(CSRC, "hbt_engine.hip"),
+                             os.path.join(CSRC, "host", "hbt_host.cpp"), "-lz"])
     os.replace(tmp, out)
     return out
 

@@ -1,0 +1,60 @@
+// HBT_correlation -- drop-in replacement of the reference's class
+// (src/HBT_correlation.h:14-104): same constructor, methods and output
+// files; the pair loops run on the MI355X through the C ABI
+// (include/hbt_hip.h).  Engine keys read from the parameters (optional):
+//   hbt_precision  0 = FP64 (bit-exact counts, default), 1 = precision form
+//   hbt_kernel     0 = plain LDS-tiled kernel, 1 = fast kernel (default)
+//   hbt_device     HIP device ordinal (default 0)
+//   number_of_mixed_events  partners per event (default: the reference's
+//                           floor(nev/2)+1)
+#ifndef HBT_AMD_HBT_CORRELATION_H
+#define HBT_AMD_HBT_CORRELATION_H
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ParameterReader.h"
+#include "Random.h"
+#include "hbt_hip.h"
+#include "particleSamples.h"
+
+class HBT_correlation {
+  public:
+    HBT_correlation(ParameterReader &paraRdr, std::string path,
+                    std::shared_ptr<RandomUtil::Random> ran_gen);
+    ~HBT_correlation();
+
+    double get_psi_ref() { return psi_ref; }
+    void set_particle_list(std::shared_ptr<particleSamples> particle_list_in) {
+        particle_list = particle_list_in;
+    }
+
+    void calculate_flow_event_plane_angle(int n_order);
+    void calculate_HBT_correlation_function(std::shared_ptr<particleSamples> particle_list_in);
+    void combine_and_bin_particle_pairs(std::vector<int> event_list);
+    void combine_and_bin_particle_pairs_mixed_events(int event_id,
+                                                     std::vector<int> mixed_event_list);
+
+    void output_HBTcorrelation();
+    void output_correlation_function_inv();
+    void output_correlation_function();
+    void output_correlation_function_Kphi_differential();
+
+  private:
+    const ParameterReader paraRdr_;
+    const std::string path_;
+    std::shared_ptr<particleSamples> particle_list;
+    std::shared_ptr<RandomUtil::Random> ran_gen_ptr_;
+    hbt_config cfg_;
+    hbt_ctx *ctx_ = nullptr;
+    double psi_ref = 0.0;
+    int number_of_oversample_events_ = 0;
+
+    void check(int rc, const char *what) const;
+    void upload_events(int slot, const particleSamples::Events &ev,
+                       const std::vector<int> &event_list);
+    void write(int which);
+};
+
+#endif

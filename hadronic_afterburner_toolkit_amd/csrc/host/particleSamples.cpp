@@ -1,0 +1,90 @@
+// particleSamples.cpp -- see particleSamples.h.  Reader mode 10 of the
+// reference (particleSamples.cpp:20-221 for the keys it reads, :480-552 for
+// the buffer calls, :1247-1346 for the format and the species filter).
+#include "particleSamples.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <iostream>
+#include <sstream>
+
+particleSamples::particleSamples(ParameterReader &paraRdr, std::string path,
+                                 std::shared_ptr<RandomUtil::Random> /*ran_gen*/)
+    : paraRdr_(paraRdr), path_(path) {
+    event_buffer_size = (int)paraRdr_.getVal("event_buffer_size");
+    const int read_in_mode = (int)paraRdr_.getVal("read_in_mode");
+    particle_monval = (int)paraRdr_.getVal("particle_monval");
+    read_mixed_events = paraRdr_.getVal("read_in_real_mixed_events") == 1;
+    if (read_in_mode != 10) {
+        std::cout << "particleSamples:: Error: read_in_mode " << read_in_mode
+                  << " is not supported by this build (mode 10, gzip text, only)" << std::endl;
+        exit(1);
+    }
+    const std::string fn = path_ + "/particle_samples.gz";
+    const std::string fn_mixed = path_ + "/particle_samples_mixed_event.gz";
+    if (hbt_reader_open(fn.c_str(), event_buffer_size, particle_monval, &reader) != HBT_OK) {
+        std::cout << "particleSamples:: Error: input file: " << fn << " can not open!"
+                  << std::endl;
+        exit(1);
+    }
+    if (read_mixed_events &&
+        hbt_reader_open(fn_mixed.c_str(), event_buffer_size, particle_monval, &reader_mixed) !=
+            HBT_OK) {
+        std::cout << "particleSamples:: Error: input file: " << fn_mixed << " can not open!"
+                  << std::endl;
+        exit(1);
+    }
+}
+
+particleSamples::~particleSamples() {
+    hbt_reader_close(reader);
+    hbt_reader_close(reader_mixed);
+}
+
+void particleSamples::read(hbt_reader *rd, Events &ev) {
+    const int nev = hbt_reader_next(rd);
+    const int64_t *off = nullptr;
+    const double *soa[8] = {};
+    const double *mass = nullptr;
+    hbt_reader_buffer(rd, &off, soa, &mass, nullptr);
+    ev.off.assign(off, off + nev + 1);
+    const int64_t n = off[nev];
+    for (int a = 0; a < 8; a++) ev.cols[a].assign(soa[a], soa[a] + n);
+    ev.mass.assign(mass, mass + n);
+}
+
+void particleSamples::read_in_particle_samples_and_filter() { read(reader, same_); }
+
+void particleSamples::read_in_particle_samples_mixed_event_and_filter() {
+    // without read_in_real_mixed_events the mixed list is the same-event
+    // list (particleSamples.cpp:528-531)
+    if (read_mixed_events) read(reader_mixed, mixed_);
+}
+
+bool particleSamples::end_of_file() const { return hbt_reader_eof(reader) != 0; }
+bool particleSamples::end_of_file_mixed_event() const {
+    return read_mixed_events ? hbt_reader_eof(reader_mixed) != 0 : end_of_file();
+}
+
+particle_info particleSamples::Events::get(int ev, int i, int monval) const {
+    const int64_t k = off[ev] + i;
+    particle_info p;
+    p.monval = monval == 99999 ? 0 : monval;
+    p.mass = mass[k];
+    p.baryon = 0;
+    p.px = cols[0][k];
+    p.py = cols[1][k];
+    p.pz = cols[2][k];
+    p.E = cols[3][k];
+    p.x = cols[4][k];
+    p.y = cols[5][k];
+    p.z = cols[6][k];
+    p.t = cols[7][k];
+    // as particleSamples::boostParticles fills them (rapidity shift 0)
+    p.pT = sqrt(p.px * p.px + p.py * p.py);
+    const double mT = sqrt(p.pT * p.pT + p.mass * p.mass);
+    p.phi_p = atan2(p.py, p.px);
+    p.rap_y = asinh(p.pz / mT);
+    p.rap_eta = asinh(p.pz / p.pT);
+    return p;
+}

@@ -32,6 +32,77 @@ def _u64p(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_uint64))
 
 
+def config_from_params(params: dict, precision: int = _lib.HBT_PREC_FP64,
+                       kernel: int = _lib.HBT_KERNEL_FAST, device: int = 0) -> HbtConfig:
+    p = dict(DEFAULTS)
+    p.update({k: v for k, v in (params or {}).items() if k in DEFAULTS})
+    cfg = HbtConfig()
+    for name in ("qnpts", "n_KT", "n_Kphi", "azimuthal_flag", "invariant_radius_flag",
+                 "long_comoving_boost", "number_of_mixed_events"):
+        setattr(cfg, name, int(p[name]))
+    for name in ("q_min", "q_max", "KT_min", "KT_max", "HBTrap_min", "HBTrap_max",
+                 "needed_number_of_pairs"):
+        setattr(cfg, name, float(p[name]))
+    cfg.precision, cfg.kernel, cfg.device = precision, kernel, device
+    return cfg
+
+
+def write_outputs(params: dict, arrays: dict, path: str, eco_output: int = 0, what: int = 0):
+    """The reference's text writers (HBT_correlation.cpp:694-855) on raw
+    results: arrays holds the hbt_result fields (numpy)."""
+    lib = _lib.load()
+    cfg = config_from_params(params)
+    keep = {}
+    res = HbtResult()
+    for name, typ in HbtResult._fields_:
+        a = arrays.get(name)
+        if a is None:
+            continue
+        dt = np.uint64 if typ is _lib._U64P else np.float64
+        keep[name] = np.ascontiguousarray(a, dtype=dt)
+        setattr(res, name, _u64p(keep[name]) if dt == np.uint64 else _f64p(keep[name]))
+    rc = lib.hbt_write_outputs(C.byref(cfg), C.byref(res), path.encode(), int(eco_output),
+                               int(what))
+    if rc:
+        raise HbtError(rc, "hbt_write_outputs")
+
+
+def flow_event_plane_angle(px, py, n_order: int) -> float:
+    lib = _lib.load()
+    px = np.ascontiguousarray(px, np.float64)
+    py = np.ascontiguousarray(py, np.float64)
+    return float(lib.hbt_flow_event_plane_angle(len(px), _f64p(px), _f64p(py), int(n_order)))
+
+
+def read_buffers(filename: str, event_buffer_size: int, particle_monval: int):
+    """The mode-10 reader of the engine library, buffer by buffer (the
+    reference's read loop, Analysis.cpp:821-833)."""
+    from .samples import Buffer
+    lib = _lib.load()
+    rd = C.c_void_p()
+    rc = lib.hbt_reader_open(filename.encode(), int(event_buffer_size), int(particle_monval),
+                             C.byref(rd))
+    if rc:
+        raise HbtError(rc, f"hbt_reader_open {filename}")
+    out = []
+    try:
+        while not lib.hbt_reader_eof(rd):
+            nev = lib.hbt_reader_next(rd)
+            off_p = C.POINTER(C.c_int64)()
+            soa = (_lib._F64P * 8)()
+            mass = _lib._F64P()
+            total = C.c_int64()
+            lib.hbt_reader_buffer(rd, C.byref(off_p), soa, C.byref(mass), C.byref(total))
+            off = np.ctypeslib.as_array(off_p, shape=(nev + 1,)).copy()
+            n = int(off[-1])
+            cols = [np.ctypeslib.as_array(soa[a], shape=(n,)).copy() if n else np.zeros(0)
+                    for a in range(8)]
+            out.append((Buffer(off, *cols), int(total.value)))
+    finally:
+        lib.hbt_reader_close(rd)
+    return out
+
+
 class Random:
     """RandomUtil::Random (Random.h:11-25): std::mt19937 with the reference's
     uniform_real_distribution<double>(0,1) and uniform_int_distribution<int>()."""

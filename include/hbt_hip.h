@@ -219,6 +219,42 @@ int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *t);
  * the fastest (written to *best when not NULL).  Clears the histograms. */
 int hbt_autotune(hbt_ctx *ctx, hbt_tuning *best);
 
+/* ---------------- host-side pieces (no GPU needed) ---------------- */
+
+/* Replaces output_HBTcorrelation and the three writers it calls
+ * (HBT_correlation.cpp:220-228, 694-855): writes
+ * HBT_correlation_function[_inv]_KT_<a>_<b>[_Kphi_<c>].dat into path from
+ * raw results (hbt_read), byte-identical to the reference's files for the
+ * same values.  eco_output = the reference's ecoOutput key.  what selects
+ * the writers: HBT_WRITE_INV (output_correlation_function_inv),
+ * HBT_WRITE_3D (output_correlation_function), HBT_WRITE_KPHI
+ * (output_correlation_function_Kphi_differential), or 0 for what
+ * output_HBTcorrelation writes under the configuration's flags. */
+enum { HBT_WRITE_INV = 1, HBT_WRITE_3D = 2, HBT_WRITE_KPHI = 4 };
+int hbt_write_outputs(const hbt_config *cfg, const hbt_result *r, const char *path,
+                      int32_t eco_output, int32_t what);
+
+/* Replaces HBT_correlation::calculate_flow_event_plane_angle
+ * (HBT_correlation.cpp:233-249) on n particles (px, py). */
+double hbt_flow_event_plane_angle(int32_t n, const double *px, const double *py,
+                                  int32_t n_order);
+
+/* Replaces particleSamples' mode-10 reader (particleSamples.cpp:1247-1286,
+ * gz_readline :2209-2218) with the species filter (:1329-1346): each
+ * hbt_reader_next reads whole events while the running particle total (all
+ * species) is below event_buffer_size and returns the number of events
+ * (possibly 0 at end of file); hbt_reader_buffer exposes the kept
+ * particles (SoA {px,py,pz,E,x,y,z,t}, event offsets) until the next call.
+ * particle_monval selects one Monte-Carlo id (99999 keeps all). */
+typedef struct hbt_reader hbt_reader;
+int hbt_reader_open(const char *filename, int32_t event_buffer_size, int32_t particle_monval,
+                    hbt_reader **out);
+int32_t hbt_reader_next(hbt_reader *rd);
+int32_t hbt_reader_eof(const hbt_reader *rd);
+int hbt_reader_buffer(const hbt_reader *rd, const int64_t **ev_off, const double *soa[8],
+                      const double **mass, int64_t *total_read);
+int hbt_reader_close(hbt_reader *rd);
+
 /* Wall time (ms, HIP events on the context's stream) of the last
  * hbt_run_same / hbt_run_mixed kernel launches. */
 int hbt_last_kernel_ms(hbt_ctx *ctx, double *same_ms, double *mixed_ms);
