@@ -66,11 +66,15 @@ def flops(stages_same, stages_mixed):
             + 7 * m[0] + 12 * m[1] + 5 * m[2] + 17 * m[3] + 1 * m[4])
 
 
-def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log):
+def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None):
     from hadronic_afterburner_toolkit_amd import hbt
     g = hbt.HBTCorrelation(PARAMS, precision=prec, kernel=kern)
+    t0 = time.perf_counter()
     g.upload_groups(groups)
     g.prepare_batch(*plan)
+    upload_s = time.perf_counter() - t0
+    if tuning:
+        g.set_tuning(*tuning)
     tune = g.autotune() if autotune else g.tuning()
     es, em, nitems = g.batch_info()
     for _ in range(warmup):
@@ -86,7 +90,7 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log):
     log(f"  prec={prec} kernel={kern} steps={steps} wall={wall:.3f}s "
         f"dev_ms/step={np.mean(dev_ms):.2f} tune={tune}")
     return g, dict(wall=wall, dev_ms=dev_ms, enum_same=es, enum_mixed=em, nitems=nitems,
-                   tune=tune)
+                   tune=tune, upload_s=upload_s)
 
 
 def cpu_baseline(group, plan_g, log):
@@ -107,6 +111,36 @@ def cpu_baseline(group, plan_g, log):
     return o, a, pairs / dt, dt, pairs
 
 
+def copy_calibration(nbytes, reps):
+    """A plain device copy of known size (torch's copy kernel), for scaling
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": reps}), flush=True)
+
+
+TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "k_fast_traffic.md")
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (lines 'traffic_bytes_per_launch <kernel> N')."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" \
+                        and parts[1] == kernel_prefix:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +154,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--only", choices=["fp32c", "fp64", "plain"], default=None)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--tuning", default=None, metavar="R,JCHUNK,KCHUNK,BLOCKS_PER_CU",
+                    help="fixed launch shape of the fast kernel (implies --no-autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the HBT engine runs on a single MI355X here "
@@ -130,6 +168,11 @@ def main():
         if not args.quiet:
             print(msg, file=sys.stderr, flush=True)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(1, args.steps))
+        return
+
+    tuning = [int(v) for v in args.tuning.split(",")] if args.tuning else None
     from hadronic_afterburner_toolkit_amd import _lib
     log(f"generating {args.events} events, Poisson({args.mult}), groups of {args.per_group}")
     groups = make_workload(args.events, args.mult, args.per_group, args.seed)
@@ -145,7 +188,7 @@ def main():
         log(f"mode {name}")
         ctx[name], res[name] = run_mode(groups, plan, prec, kern, k,
                                         args.warmup if name != "plain" else 1,
-                                        not args.no_autotune, log)
+                                        not (args.no_autotune or tuning), log, tuning)
 
     def pairs_per_s(r):
         return (r["enum_same"] + r["enum_mixed"]) * len(r["dev_ms"]) / r["wall"]
@@ -175,6 +218,7 @@ def main():
     for name in res:
         out[f"{name}_pairs_per_s"] = pairs_per_s(res[name])
         out[f"{name}_dev_ms_per_step"] = float(np.mean(res[name]["dev_ms"]))
+        out[f"{name}_upload_s"] = res[name]["upload_s"]
     if "fp32c" in res and "plain" in res:
         out["speedup_vs_plain_fp64"] = out["fp32c_pairs_per_s"] / out["plain_pairs_per_s"]
     if "fp32c" in res and "fp64" in res:
@@ -198,9 +242,10 @@ def main():
         dev_s = float(np.mean(r["dev_ms"])) * 1e-3
         ach = fl / dev_s / 1e12
         peak = FP32_PEAK_TFLOPS if head == "fp32c" else FP64_PEAK_TFLOPS
+        kname = "k_fast<float>" if head == "fp32c" else "k_fast<double>"
         out["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                           "frac": ach / peak, "traffic": None,
-                           "kernel": "k_fast<float,R>" if head == "fp32c" else "k_fast<double,R>",
+                           "frac": ach / peak, "traffic": measured_traffic(kname),
+                           "kernel": kname,
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
     if not args.no_cpu:
         g0 = groups[0]

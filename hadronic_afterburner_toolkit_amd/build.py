@@ -5,6 +5,8 @@
 Products:
   hadronic_afterburner_toolkit_amd/libhbt_hip.so   C ABI of include/hbt_hip.h
   hadronic_afterburner_toolkit_amd/hbt_afterburner.e  CLI (drop-in driver)
+  hadronic_afterburner_toolkit_amd/hist_atomics_bench.e  histogram-atomics
+      micro-benchmark (csrc/microbench/hist_atomics.hip)
 """
 from __future__ import annotations
 
@@ -58,9 +60,16 @@ def build_cli() -> str | None:
     return out
 
 
+def build_microbench() -> str:
+    out = os.path.join(HERE, "hist_atomics_bench.e")
+    _run([HIPCC] + COMMON + ["-o", out, os.path.join(CSRC, "microbench", "hist_atomics.hip")])
+    return out
+
+
 def build_all() -> None:
     build_engine()
     build_cli()
+    build_microbench()
 
 
 if __name__ == "__main__":
