@@ -47,6 +47,13 @@ struct DevSoA {
     const double *__restrict__ x, *__restrict__ y, *__restrict__ z, *__restrict__ t;
 };
 
+// One particle for the fast kernel's exact stage, array of structures:
+// momentum then space-time in one 64-byte line, so a gathered particle
+// costs one line request instead of eight.
+struct __align__(64) PartRec {
+    double px, py, pz, E, x, y, z, t;
+};
+
 // FP32 hi/lo split copies (v = hi + lo to ~2^-48 relative).
 struct DevSoA32 {
     const float *__restrict__ pxh, *__restrict__ pyh, *__restrict__ pzh, *__restrict__ Eh;

@@ -179,7 +179,8 @@ struct Slot {
     std::vector<int> grp_ev;       // ngroups+1
     std::vector<long long> off;    // nev+1, kept-particle offsets
     std::vector<double> psi2;      // psi_ref (n=2) per group, over all particles passed in
-    double *d_soa[8] = {};
+    double *d_soa[8] = {};   // plain kernels (structure of arrays)
+    PartRec *d_rec = nullptr;  // fast kernel (64-byte records)
     long long *d_off = nullptr;
     PfRec64 *d_pf64 = nullptr;
     PfRec32 *d_pf32 = nullptr;
@@ -221,6 +222,7 @@ struct hbt_ctx {
     size_t items_cap = 0;
     int nitems = 0, nitems_same = 0;
     int *d_counter = nullptr;
+    View *d_view = nullptr;  // the context's one analysis view
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool batch_ready = false;
@@ -278,6 +280,7 @@ void free_slot(Slot &s) {
         p = nullptr;
     }
     if (s.d_off) hipFree(s.d_off);
+    if (s.d_rec) hipFree(s.d_rec);
     if (s.d_pf64) hipFree(s.d_pf64);
     if (s.d_pf32) hipFree(s.d_pf32);
     if (s.d_cw64) hipFree(s.d_cw64);
@@ -411,32 +414,39 @@ int upload_psi(hbt_ctx *c) {
     return HBT_OK;
 }
 
-FastArgs fast_args(hbt_ctx *c) {
-    FastArgs a;
-    set_fp32_margins(c);
-    a.P = c->P;
-    a.H = hist_view(c);
+View make_view(hbt_ctx *c) {
+    View v{};
+    v.H = hist_view(c);
     const Slot &s0 = c->slot[0];
     const Slot &s1 = mixed_slot(c);
-    a.A = soa_view(s0);
-    a.B = soa_view(s1);
+    v.A = s0.d_rec;
+    v.B = s1.d_rec;
     const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
-    a.pfA = f32 ? (const void *)s0.d_pf32 : (const void *)s0.d_pf64;
-    a.pfB = f32 ? (const void *)s1.d_pf32 : (const void *)s1.d_pf64;
-    a.cwA = s0.d_cw64;
-    a.cwB = s1.d_cw64;
-    a.cwA32 = s0.d_cw32;
-    a.cwB32 = s1.d_cw32;
-    a.mix_off = s1.d_off;
-    a.rowoff = c->d_rowoff;
-    a.plan_ids = c->d_ids;
-    a.plan_cs = c->d_cs;
-    a.psi = c->d_psi;
+    v.pfA = f32 ? (const void *)s0.d_pf32 : (const void *)s0.d_pf64;
+    v.pfB = f32 ? (const void *)s1.d_pf32 : (const void *)s1.d_pf64;
+    v.cwA = f32 ? (const void *)s0.d_cw32 : (const void *)s0.d_cw64;
+    v.cwB = f32 ? (const void *)s1.d_cw32 : (const void *)s1.d_cw64;
+    v.mix_off = s1.d_off;
+    v.rowoff = c->d_rowoff;
+    v.plan_ids = c->d_ids;
+    v.plan_cs = c->d_cs;
+    v.psi = c->d_psi;
+    return v;
+}
+
+// Kernel arguments of the context's own launches: one view (index 0),
+// copied to the device before the launch (and before a graph capture).
+int fast_args(hbt_ctx *c, FastArgs &a) {
+    set_fp32_margins(c);
+    const View v = make_view(c);
+    HIPCHECK(c, hipMemcpy(c->d_view, &v, sizeof(View), hipMemcpyHostToDevice));
+    a.P = c->P;
+    a.views = c->d_view;
     a.items = c->d_items;
     a.nitems = c->nitems;
     a.counter = c->d_counter;
     a.binned = c->d_binned;
-    return a;
+    return HBT_OK;
 }
 
 int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
@@ -575,7 +585,9 @@ int upload_items(hbt_ctx *c, const std::vector<WorkItem> &items) {
 }
 
 int timed_fast(hbt_ctx *c, float *ms) {
-    FastArgs a = fast_args(c);
+    FastArgs a;
+    int rc0 = fast_args(c, a);
+    if (rc0) return rc0;
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     if (a.nitems > 0) {
         int rc = launch_fast(c, a, c->stream);
@@ -642,7 +654,9 @@ int prepare_and_capture(hbt_ctx *c) {
         c->batch_ready = true;
         return HBT_OK;
     }
-    FastArgs a = fast_args(c);
+    FastArgs a;
+    rc = fast_args(c, a);
+    if (rc) return rc;
     HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     rc = launch_fast(c, a, c->stream);
     hipGraph_t g = nullptr;
@@ -708,6 +722,7 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
             if (rc) return rc;
         }
         int rc = dev_alloc(c, &s.d_pf64, (size_t)n);
+        if (!rc) rc = dev_alloc(c, &s.d_rec, (size_t)n);
         if (!rc) rc = dev_alloc(c, &s.d_pf32, (size_t)n);
         if (!rc) rc = dev_alloc(c, &s.d_cw64, (size_t)n);
         if (!rc) rc = dev_alloc(c, &s.d_cw32, (size_t)n);
@@ -723,6 +738,16 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     for (int a = 0; a < 8; a++) {
         for (long long k = 0; k < n; k++) col[k] = soa[a][keep[k]];
         HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    {
+        std::vector<PartRec> rec((size_t)n);
+        for (long long k = 0; k < n; k++) {
+            const long long i = keep[k];
+            rec[k] = PartRec{soa[0][i], soa[1][i], soa[2][i], soa[3][i],
+                             soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
+        }
+        if (n > 0)
+            HIPCHECK(c, hipMemcpy(s.d_rec, rec.data(), n * sizeof(PartRec), hipMemcpyHostToDevice));
     }
     // prefilter records
     std::vector<PfRec64> pf((size_t)n);
@@ -884,6 +909,7 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     if (!rc) rc = dev_alloc(cp, &c->d_binned, 2);
     if (!rc) rc = dev_alloc(cp, &c->d_sums, (size_t)k.n_KT * std::max(1, P.nphi));
     if (!rc) rc = dev_alloc(cp, &c->d_counter, 1);
+    if (!rc) rc = dev_alloc(cp, &c->d_view, 1);
     if (!rc) rc = zero_hist(cp);
     if (rc) {
         g_create_err = c->err;
@@ -901,7 +927,7 @@ int hbt_destroy(hbt_ctx *c) {
     void *ptrs[] = {c->d_bins,      c->d_qinv_count, c->d_qinv_den,
                     c->d_qinv_sum,  c->d_qinv_num,  c->d_stage,      c->d_binned,
                     c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
-                    c->d_items,     c->d_counter,   c->d_psi};
+                    c->d_items,     c->d_counter,   c->d_psi,        c->d_view};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);

@@ -136,22 +136,21 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
 // literal FP64 cascade.
 template <bool EXT = true>
 __device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist &H, double psi,
-                                              const DevSoA &S, long long i, long long j,
+                                              const PartRec &p1, const PartRec &p2,
                                               SameRec &rec) {
-    const double p1x = S.px[i], p1y = S.py[i], p1z = S.pz[i], p1E = S.E[i];
-    const double p2x = S.px[j], p2y = S.py[j], p2z = S.pz[j], p2E = S.E[j];
     PairBins b;
     const int st = (EXT && (P.qinv | P.azimuthal))
                        ? -1
-                       : cascade_fast<false>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
+                       : cascade_fast<false>(P, p1.px, p1.py, p1.pz, p1.E, p2.px, p2.py, p2.pz,
+                                             p2.E, b);
     if (st < 0)
-        return eval_same_f64<EXT>(P, H, psi, p1x, p1y, p1z, p1E, S.x[i], S.y[i], S.z[i], S.t[i],
-                                  p2x, p2y, p2z, p2E, S.x[j], S.y[j], S.z[j], S.t[j], rec);
+        return eval_same_f64<EXT>(P, H, psi, p1.px, p1.py, p1.pz, p1.E, p1.x, p1.y, p1.z, p1.t,
+                                  p2.px, p2.py, p2.pz, p2.E, p2.x, p2.y, p2.z, p2.t, rec);
     if (!(st & ST_BIN)) return st;
-    const double t_diff = S.t[i] - S.t[j];
-    const double x_diff = S.x[i] - S.x[j];
-    const double y_diff = S.y[i] - S.y[j];
-    const double z_diff = S.z[i] - S.z[j];
+    const double t_diff = p1.t - p2.t;
+    const double x_diff = p1.x - p2.x;
+    const double y_diff = p1.y - p2.y;
+    const double z_diff = p1.z - p2.z;
     const double phase =
         P.hbarC_inv * (b.q_E * t_diff - b.q_x * x_diff - b.q_y * y_diff - b.q_z * z_diff);
     rec.k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
@@ -162,16 +161,13 @@ __device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist 
     return st;
 }
 
-// Mixed pair in the precision form; j is rotated exactly as the reference
-// does (HBT_correlation.cpp:522-523) before the cascade.
+// Mixed pair in the precision form; p2x, p2y are the partner's momentum
+// already rotated as the reference does (HBT_correlation.cpp:522-523).
 template <bool EXT = true>
 __device__ __forceinline__ int eval_mixed_fast(const DevParams &P, const DevHist &H, double psi,
-                                               const DevSoA &A, const DevSoA &B, long long i,
-                                               long long j, double c, double s, long &k) {
-    const double px = B.px[j], py = B.py[j];
-    const double p2x = px * c - py * s, p2y = px * s + py * c;
-    const double p1x = A.px[i], p1y = A.py[i], p1z = A.pz[i], p1E = A.E[i];
-    const double p2z = B.pz[j], p2E = B.E[j];
+                                               double p1x, double p1y, double p1z, double p1E,
+                                               double p2x, double p2y, double p2z, double p2E,
+                                               long &k) {
     PairBins b;
     // mixed windows are half-open: v < hi (HBT_correlation.cpp:613); the
     // guard band around the cut sends v == hi to the literal cascade

@@ -45,28 +45,45 @@ struct WorkItem {
     int e;          // mixed: event index (its plan rows start at rowoff[e])
     int k0, k1;     // mixed: partner range [k0,k1) of the event's plan rows
     int grp;        // group (buffer) of the pair, selects psi_ref
-    int pad;
+    int view;       // analysis the item belongs to (index into FastArgs::views)
+};
+
+// One analysis resident in HBM: its particles, mixing plan and histograms.
+// A launch may carry items of many analyses (the request server batches
+// them); every index in a WorkItem is relative to its own View.
+struct View {
+    DevHist H;
+    const PartRec *A;    // same-event list (cut), 64 B per particle
+    const PartRec *B;    // mixed list (separate, or an alias of A)
+    const void *pfA;     // PfRec64 (FP64 kernel) or PfRec32 (precision form) of A
+    const void *pfB;     // of B
+    const void *cwA;     // per-particle c/E bound of A, double or float like pfA
+    const void *cwB;
+    const long long *mix_off;  // event offsets of the mixed list (cut)
+    const long long *rowoff;   // plan rows of event e: [rowoff[e], rowoff[e+1])
+    const int *plan_ids;       // partner event (index in the mixed list)
+    const double *plan_cs;     // cos, sin of the partner's rotation
+    const double *psi;         // psi_ref per group
 };
 
 struct FastArgs {
     DevParams P;
-    DevHist H;
-    DevSoA A;            // slot 0 FP64
-    DevSoA B;            // mixed list FP64 (slot 1 or alias of slot 0)
-    const void *pfA;     // PfRec64/PfRec32 of slot 0
-    const void *pfB;     // of the mixed list
-    const double *cwA, *cwB;  // per-particle c/E bound (FP64)
-    const float *cwA32, *cwB32;
-    const long long *mix_off;  // event offsets of the mixed list (cut)
-    const long long *rowoff;   // plan rows of event e: [rowoff[e], rowoff[e+1])
-    const int *plan_ids;       // partner event (global index in the mixed list)
-    const double *plan_cs;     // cos, sin of the partner's rotation
-    const double *psi;         // psi_ref per group
+    const View *views;
     const WorkItem *items;
     int nitems;
     int *counter;              // work counter, zeroed before launch
     unsigned long long *binned;  // [2]: binned same, mixed (for stats)
 };
+
+// The view table does not change during a launch.  Reading it through the
+// constant address space marks its pointers as invariant scalar loads, which
+// the compiler re-issues where they are used instead of holding ~40 SGPRs
+// for the whole work item.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HBT_LOAD_VIEW(tab, k) (((const __attribute__((address_space(4))) View *)(tab))[k])
+#else
+#define HBT_LOAD_VIEW(tab, k) ((tab)[k])
+#endif
 
 // LDS ring entries per wave (power of two).  Between drains the ring holds
 // fewer than 64 entries, and one j step appends at most 64*R.
@@ -110,30 +127,37 @@ struct Exact;
 
 template <bool EXT>
 struct Exact<double, EXT> {
-    static __device__ __forceinline__ int same(const FastArgs &a, double psi, long long i,
-                                               long long j, SameRec &rec) {
-        const DevSoA &S = a.A;
-        return eval_same_f64<EXT>(a.P, a.H, psi, S.px[i], S.py[i], S.pz[i], S.E[i], S.x[i],
-                                  S.y[i], S.z[i], S.t[i], S.px[j], S.py[j], S.pz[j], S.E[j],
-                                  S.x[j], S.y[j], S.z[j], S.t[j], rec);
+    static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
+                                               long long i, long long j, SameRec &rec) {
+        const PartRec p = v.A[i], r = v.A[j];
+        return eval_same_f64<EXT>(P, v.H, psi, p.px, p.py, p.pz, p.E, p.x, p.y, p.z, p.t, r.px,
+                                  r.py, r.pz, r.E, r.x, r.y, r.z, r.t, rec);
     }
-    static __device__ __forceinline__ int mixed(const FastArgs &a, double psi, long long i,
-                                                long long j, double c, double s, long &k) {
-        const double px = a.B.px[j], py = a.B.py[j];
-        return eval_mixed_f64<EXT>(a.P, a.H, psi, a.A.px[i], a.A.py[i], a.A.pz[i], a.A.E[i],
-                                   px * c - py * s, px * s + py * c, a.B.pz[j], a.B.E[j], k);
+    static __device__ __forceinline__ int mixed(const DevParams &P, const View &v, double psi,
+                                                long long i, long long j, double c, double s,
+                                                long &k) {
+        const PartRec &p = v.A[i];
+        const PartRec &r = v.B[j];
+        const double px = r.px, py = r.py;
+        return eval_mixed_f64<EXT>(P, v.H, psi, p.px, p.py, p.pz, p.E, px * c - py * s,
+                                   px * s + py * c, r.pz, r.E, k);
     }
 };
 
 template <bool EXT>
 struct Exact<float, EXT> {
-    static __device__ __forceinline__ int same(const FastArgs &a, double psi, long long i,
-                                               long long j, SameRec &rec) {
-        return eval_same_fast<EXT>(a.P, a.H, psi, a.A, i, j, rec);
+    static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
+                                               long long i, long long j, SameRec &rec) {
+        return eval_same_fast<EXT>(P, v.H, psi, v.A[i], v.A[j], rec);
     }
-    static __device__ __forceinline__ int mixed(const FastArgs &a, double psi, long long i,
-                                                long long j, double c, double s, long &k) {
-        return eval_mixed_fast<EXT>(a.P, a.H, psi, a.A, a.B, i, j, c, s, k);
+    static __device__ __forceinline__ int mixed(const DevParams &P, const View &v, double psi,
+                                                long long i, long long j, double c, double s,
+                                                long &k) {
+        const PartRec &p = v.A[i];
+        const PartRec &r = v.B[j];
+        const double px = r.px, py = r.py;
+        return eval_mixed_fast<EXT>(P, v.H, psi, p.px, p.py, p.pz, p.E, px * c - py * s,
+                                    px * s + py * c, r.pz, r.E, k);
     }
 };
 
@@ -154,7 +178,8 @@ struct Stage {
 // values of a bin travel together (atomics on this part execute at the
 // memory side, where the cost is per line request).
 template <typename Prec, bool EXT, int QCAP>
-__device__ __forceinline__ int drain(const FastArgs &a, const unsigned *q, Stage &stg,
+__device__ __forceinline__ int drain(const DevParams &P, const View &v, const unsigned *q,
+                                     Stage &stg,
                                      unsigned head, int cnt, const WorkItem &w, long long row0,
                                      double psi) {
     const int lane = __lane_id();
@@ -167,18 +192,18 @@ __device__ __forceinline__ int drain(const FastArgs &a, const unsigned *q, Stage
         if (slot < w.ni) {  // always true: only lanes inside the tile enqueue
             const long long i = w.i0 + slot;
             if (w.kind == 0) {
-                st = Exact<Prec, EXT>::same(a, psi, i, w.j0 + (ent >> 8), rec);
+                st = Exact<Prec, EXT>::same(P, v, psi, i, w.j0 + (ent >> 8), rec);
             } else {
                 const long long row = row0 + (long long)(ent >> 26);
-                const long long j = a.mix_off[a.plan_ids[row]] + ((ent >> 8) & 0x3ffffu);
-                st = Exact<Prec, EXT>::mixed(a, psi, i, j, a.plan_cs[2 * row],
-                                             a.plan_cs[2 * row + 1], kmix);
+                const long long j = v.mix_off[v.plan_ids[row]] + ((ent >> 8) & 0x3ffffu);
+                st = Exact<Prec, EXT>::mixed(P, v, psi, i, j, v.plan_cs[2 * row],
+                                             v.plan_cs[2 * row + 1], kmix);
             }
         }
     }
     const bool binned = (st & ST_BIN) != 0;
     if (w.kind != 0) {
-        if (binned) add_mixed_direct(a.H, kmix);
+        if (binned) add_mixed_direct(v.H, kmix);
         return binned ? 1 : 0;
     }
     const unsigned long long m = __ballot(binned);
@@ -198,7 +223,7 @@ __device__ __forceinline__ int drain(const FastArgs &a, const unsigned *q, Stage
     for (int base = 0; base < nb; base += 12) {
         const int p = base + grp;
         if (lane < 60 && p < nb) {
-            double *b = a.H.bins + stg.k[p] * BIN_STRIDE;
+            double *b = v.H.bins + stg.k[p] * BIN_STRIDE;
             if (sub == 0)
                 atomic_add_u64((unsigned long long *)(b + BIN_COUNT), 1ull);
             else
@@ -244,17 +269,6 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
     const int lane = __lane_id();
     unsigned *q = ring[threadIdx.x >> 6];
     Stage &stg = stage[threadIdx.x >> 6];
-    const Rec *__restrict__ pfA = (const Rec *)a.pfA;
-    const Rec *__restrict__ pfB = (const Rec *)a.pfB;
-    const Prec *__restrict__ cwA;
-    const Prec *__restrict__ cwB;
-    if constexpr (std::is_same<Prec, double>::value) {
-        cwA = a.cwA;
-        cwB = a.cwB;
-    } else {
-        cwA = a.cwA32;
-        cwB = a.cwB32;
-    }
     const Prec two = (Prec)2;
     const Prec half_t = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_half : a.P.f_half);
     const Prec q_thr = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_qthr : a.P.f_qthr);
@@ -266,7 +280,12 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
         if (it >= a.nitems) break;
         const WorkItem w = a.items[it];
-        const double psi = a.psi[w.grp];
+        const View v = HBT_LOAD_VIEW(a.views, w.view);
+        const Rec *__restrict__ pfA = (const Rec *)v.pfA;
+        const Rec *__restrict__ pfB = (const Rec *)v.pfB;
+        const Prec *__restrict__ cwA = (const Prec *)v.cwA;
+        const Prec *__restrict__ cwB = (const Prec *)v.cwB;
+        const double psi = v.psi[w.grp];
         const bool mixed = w.kind != 0;
         const Rec *__restrict__ pf = mixed ? pfB : pfA;
         const Prec *__restrict__ cwj = mixed ? cwB : cwA;
@@ -288,7 +307,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         }
         // segment state
         const int nseg = mixed ? (w.k1 - w.k0) : 1;
-        const long long row0 = mixed ? a.rowoff[w.e] + w.k0 : 0;
+        const long long row0 = mixed ? v.rowoff[w.e] + w.k0 : 0;
         int seg = -1;
         long long jb = 0, je = 0, jc = 0;  // segment range, current chunk start
         int t = 0, cnt = 0;
@@ -299,7 +318,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
             if (tail - head >= 64 || (finished && tail != head)) {
                 const int n = (int)min(64u, tail - head);
                 __builtin_amdgcn_wave_barrier();
-                const int nb = drain<Prec, EXT, QCAP>(a, q, stg, head, n, w, row0, psi);
+                const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, row0, psi);
                 if (mixed)
                     nb_mixed += nb;
                 else
@@ -319,10 +338,10 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
                     }
                     if (mixed) {
                         const long long row = row0 + seg;
-                        const int pid = a.plan_ids[row];
-                        jb = a.mix_off[pid];
-                        je = a.mix_off[pid + 1];
-                        const double c = a.plan_cs[2 * row], s = a.plan_cs[2 * row + 1];
+                        const int pid = v.plan_ids[row];
+                        jb = v.mix_off[pid];
+                        je = v.mix_off[pid + 1];
+                        const double c = v.plan_cs[2 * row], s = v.plan_cs[2 * row + 1];
 #pragma unroll
                         for (int r = 0; r < R; r++) {  // rotate i by -theta
                             rx[r] = (Prec)((double)ix[r] * c + (double)iy[r] * s);
