@@ -52,6 +52,32 @@ class HbtResult(C.Structure):
     ]
 
 
+class HbtServerConfig(C.Structure):
+    _fields_ = [("hbm_bytes", C.c_int64), ("max_resident", C.c_int32),
+                ("max_step_items", C.c_int32), ("step_pairs", C.c_double),
+                ("tune", HbtTuning), ("reserved", C.c_int32 * 8)]
+
+
+HBT_REQ_STATES = {0: "queued", 1: "loading", 2: "active", 3: "draining", 4: "done",
+                  5: "failed"}
+
+
+class HbtRequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("nev", C.c_int32), ("n_kept", C.c_int64),
+                ("enum_same", C.c_uint64), ("enum_mixed", C.c_uint64),
+                ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_active", C.c_double), ("t_done", C.c_double)]
+
+
+class HbtServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("launches", C.c_int64), ("items", C.c_int64), ("pairs", C.c_double),
+                ("arena_bytes", C.c_int64), ("arena_used", C.c_int64),
+                ("arena_peak", C.c_int64), ("resident", C.c_int32),
+                ("resident_peak", C.c_int32), ("queued", C.c_int32), ("reserved", C.c_int32),
+                ("busy_s", C.c_double)]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("hbt_create", C.c_int, [C.POINTER(HbtConfig), C.POINTER(C.c_void_p)]),
@@ -95,6 +121,18 @@ SIGNATURES = [
                                     C.POINTER(_F64P), C.POINTER(_F64P),
                                     C.POINTER(C.c_int64)]),
     ("hbt_reader_close", C.c_int, [C.c_void_p]),
+    ("hbt_server_create", C.c_int, [C.POINTER(HbtConfig), C.POINTER(HbtServerConfig),
+                                    C.POINTER(C.c_void_p)]),
+    ("hbt_server_destroy", C.c_int, [C.c_void_p]),
+    ("hbt_server_last_error", C.c_char_p, [C.c_void_p]),
+    ("hbt_server_submit", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
+                                    C.POINTER(_F64P), C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_int64)]),
+    ("hbt_server_info", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HbtRequestInfo)]),
+    ("hbt_server_wait", C.c_int, [C.c_void_p, C.c_int64, C.c_double]),
+    ("hbt_server_result", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HbtResult)]),
+    ("hbt_server_release", C.c_int, [C.c_void_p, C.c_int64]),
+    ("hbt_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(HbtServerStats)]),
 ]
 
 _lib = None

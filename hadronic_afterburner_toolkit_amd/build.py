@@ -43,7 +43,8 @@ def build_engine() -> str:
     out = os.path.join(HERE, "libhbt_hip.so")
     tmp = out + ".tmp"
     _run([HIPCC] + COMMON + ["-shared", "-o", tmp, os.path.join(CSRC, "hbt_engine.hip"),
-                             os.path.join(CSRC, "host", "hbt_host.cpp"), "-lz"])
+                             os.path.join(CSRC, "hbt_server.hip"),
+                             os.path.join(CSRC, "host", "hbt_host.cpp"), "-lz", "-lpthread"])
     os.replace(tmp, out)
     return out
 

@@ -35,12 +35,12 @@
 #include "../../include/hbt_hip.h"
 #include "hbt_device.h"
 #include "hbt_fast.h"
+#include "hbt_internal.h"
 
 using namespace hbt;
 
 namespace {
 
-constexpr double kHbarC = 0.197327053;  // parameters.h:4
 constexpr int TILE = 256;
 
 // ------------------------------------------------------------------ kernels
@@ -150,6 +150,12 @@ __global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, De
 
 __global__ void k_zero_int(int *p) { *p = 0; }
 
+__global__ void k_zero_words(unsigned long long *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 // Sum of a u64 field over the q bins of each (kt, kphi): the reference's
 // number_of_pairs_*_KTdiff / _KTKphidiff counters.  Element k of group g
 // is h[(g*per + k)*stride].
@@ -195,6 +201,7 @@ struct Slot {
 
 struct hbt_ctx {
     hbt_config cfg;
+    Derived D;  // constants derived from cfg
     DevParams P;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
@@ -213,6 +220,7 @@ struct hbt_ctx {
     long long *d_rowoff = nullptr;
     size_t plan_cap = 0, rowoff_cap = 0;
     std::vector<long long> rowoff;
+    std::vector<int> plan_gids;  // partner of each plan row (index in the mixed list)
     std::vector<int> M_grp;
     bool plan_valid = false;
     double *d_psi = nullptr;
@@ -391,17 +399,8 @@ int ensure(hbt_ctx *c, T **p, size_t &cap, size_t n) {
     return HBT_OK;
 }
 
-// Prefilter margins for the FP32 records depend on the data's momentum
-// scale (|pT|^2 up to pmax2): FP32 rounding of px, py and of the dot
-// product stays below 8 ulp of that scale; 64 ulp are allowed.
 void set_fp32_margins(hbt_ctx *c) {
-    const Slot &a = c->slot[0];
-    const Slot &b = mixed_slot(c);
-    const double pm = std::max(1.0, std::max(a.pmax2, b.pmax2));
-    const double eps32 = 5.9604644775390625e-8;
-    const double m = 64.0 * eps32 * (2.0 * pm + std::fabs(c->mid));
-    c->P.f_half = (float)(c->P.pf_half + m);
-    c->P.f_qthr = std::isfinite(c->P.pf_qthr) ? (float)(c->P.pf_qthr + m) : INFINITY;
+    fp32_margins(c->D, std::max(c->slot[0].pmax2, mixed_slot(c).pmax2), c->P);
 }
 
 int upload_psi(hbt_ctx *c) {
@@ -450,31 +449,10 @@ int fast_args(hbt_ctx *c, FastArgs &a) {
 }
 
 int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, c->d_counter);
     long blocks = std::min<long>((long)c->cu_count * c->tune.blocks_per_cu,
                                  ((long)a.nitems + 3) / 4);
-    if (blocks < 1) blocks = 1;
-    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
-    const int R = c->tune.R;
-#define HBT_LAUNCH(T, RR, X) \
-    hipLaunchKernelGGL((k_fast<T, RR, X>), dim3(blocks), dim3(256), 0, st, a)
-#define HBT_LAUNCH_R(T, X)                 \
-    do {                                   \
-        if (R == 1) HBT_LAUNCH(T, 1, X);   \
-        else if (R == 4) HBT_LAUNCH(T, 4, X); \
-        else HBT_LAUNCH(T, 2, X);          \
-    } while (0)
-    const bool ext = c->P.qinv || c->P.azimuthal;
-    if (f32) {
-        if (ext) HBT_LAUNCH_R(float, true);
-        else HBT_LAUNCH_R(float, false);
-    } else {
-        if (ext) HBT_LAUNCH_R(double, true);
-        else HBT_LAUNCH_R(double, false);
-    }
-#undef HBT_LAUNCH_R
-#undef HBT_LAUNCH
-    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, launch_fast_kernel(a, c->cfg.precision == HBT_PREC_FP32C, c->tune.R,
+                                   c->P.qinv || c->P.azimuthal, std::max(1l, blocks), st));
     return HBT_OK;
 }
 
@@ -486,52 +464,14 @@ int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
 void build_items(hbt_ctx *c, bool same, bool mixed, std::vector<WorkItem> &items,
                  uint64_t &enum_same) {
     const Slot &s = c->slot[0];
-    const int TI = 64 * c->tune.R;
-    const int JC = std::max(64, c->tune.jchunk);
-    const int KC = std::max(1, std::min(c->tune.kchunk, 64));
+    const ItemShape sh = item_shape(c->tune);
     items.clear();
     enum_same = 0;
-    if (same) {
-        for (int g = 0; g < s.ngroups; g++) {
-            const long long g0 = s.off[s.grp_ev[g]], n = s.off[s.grp_ev[g + 1]] - g0;
-            enum_same += (uint64_t)(n * (n - 1) / 2);
-            for (long long a = 0; a + 1 < n; a += TI) {
-                const int ni = (int)std::min<long long>(TI, n - a);
-                for (long long jb = a + 1; jb < n; jb += JC) {
-                    WorkItem w{};
-                    w.i0 = g0 + a;
-                    w.j0 = g0 + jb;
-                    w.ni = ni;
-                    w.nj = (int)std::min<long long>(JC, n - jb);
-                    w.kind = 0;
-                    w.grp = g;
-                    items.push_back(w);
-                }
-            }
-        }
-    }
+    if (same) items_same(sh, s.off, s.grp_ev, 0, items, nullptr, enum_same);
     c->nitems_same = (int)items.size();
-    if (mixed && c->plan_valid) {
-        for (int g = 0; g < s.ngroups; g++) {
-            for (int e = s.grp_ev[g]; e < s.grp_ev[g + 1]; e++) {
-                const long long n1 = s.off[e + 1] - s.off[e];
-                const int M = (int)(c->rowoff[e + 1] - c->rowoff[e]);
-                for (long long a = 0; a < n1; a += TI) {
-                    for (int k0 = 0; k0 < M; k0 += KC) {
-                        WorkItem w{};
-                        w.i0 = s.off[e] + a;
-                        w.ni = (int)std::min<long long>(TI, n1 - a);
-                        w.kind = 1;
-                        w.e = e;
-                        w.k0 = k0;
-                        w.k1 = std::min(M, k0 + KC);
-                        w.grp = g;
-                        items.push_back(w);
-                    }
-                }
-            }
-        }
-    }
+    if (mixed && c->plan_valid)
+        items_mixed(sh, s.off, s.grp_ev, c->rowoff, c->plan_gids, mixed_slot(c).off, 0, items,
+                    nullptr);
 }
 
 // Uploads the plan: ids are relative to the event's own group and are
@@ -567,6 +507,8 @@ int upload_plan(hbt_ctx *c, const std::vector<int> &M_per_event, const int32_t *
     int rc = ensure(c, &c->d_rowoff, c->rowoff_cap, (size_t)s.nev + 1);
     if (rc) return rc;
     HIPCHECK(c, hipMemcpy(c->d_ids, gids.data(), rows * 4, hipMemcpyHostToDevice));
+    gids.resize((size_t)rows);
+    c->plan_gids.swap(gids);
     HIPCHECK(c, hipMemcpy(c->d_cs, cs, rows * 16, hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(c->d_rowoff, c->rowoff.data(), (s.nev + 1) * 8, hipMemcpyHostToDevice));
     c->plan_valid = true;
@@ -672,56 +614,21 @@ int prepare_and_capture(hbt_ctx *c) {
 int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev, int32_t nev,
                 const int64_t *ev_off, const double *const soa[8]) {
     if (slot != 0 && slot != 1) return fail(c, HBT_ERR_ARG, "slot must be 0 or 1");
-    if (nev < 0 || ngroups < 1 || (nev > 0 && (!ev_off || !soa)))
-        return fail(c, HBT_ERR_ARG, "bad buffer");
-    for (int a = 0; nev > 0 && a < 8; a++)
-        if (!soa[a]) return fail(c, HBT_ERR_ARG, "null SoA array");
-    for (int e = 0; e < nev; e++)
-        if (ev_off[e + 1] < ev_off[e]) return fail(c, HBT_ERR_ARG, "event offsets decrease");
-    if (grp_ev[0] != 0 || grp_ev[ngroups] != nev)
-        return fail(c, HBT_ERR_ARG, "group offsets must run from 0 to nev");
-    for (int g = 0; g < ngroups; g++)
-        if (grp_ev[g + 1] < grp_ev[g]) return fail(c, HBT_ERR_ARG, "group offsets decrease");
+    if (ngroups < 1 || !grp_ev) return fail(c, HBT_ERR_ARG, "bad buffer");
+    HostParticles h;
+    std::string err;
+    int rc = gather_particles(c->D, c->cfg, ngroups, grp_ev, nev, ev_off, soa, h, err);
+    if (rc) return fail(c, rc, err);
     drop_graph(c);
     c->plan_valid = false;
     Slot &s = c->slot[slot];
-    const long long n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
-    // Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476).
-    const double cut_max = tanh(c->cfg.HBTrap_max);
-    const double cut_min = tanh(c->cfg.HBTrap_min);
-    std::vector<long long> keep;
-    keep.reserve((size_t)n_in);
-    std::vector<long long> off(nev + 1, 0);
-    std::vector<double> psi2(ngroups, 0.0);
-    for (int g = 0; g < ngroups; g++) {
-        double vr = 0.0, vi = 0.0;
-        for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
-            for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
-                const double ratio = soa[2][i] / soa[3][i];
-                if (ratio > cut_min && ratio < cut_max) keep.push_back(i);
-                if (c->P.azimuthal) {  // psi_ref over all particles (:233-249)
-                    const double phi = atan2(soa[1][i], soa[0][i]);
-                    double sn, cn;
-                    sincos(2 * phi, &sn, &cn);  // as the reference's build evaluates it
-                    vr += cn;
-                    vi += sn;
-                }
-            }
-            off[e + 1] = (long long)keep.size();
-        }
-        psi2[g] = c->P.azimuthal ? atan2(vi, vr) / 2 : 0.0;
-    }
-    const long long n = (long long)keep.size();
-    for (int e = 0; e < nev; e++)
-        if (off[e + 1] - off[e] >= (1 << 18))
-            return fail(c, HBT_ERR_CAPACITY, "an event keeps more than 262143 particles");
-    if (n >= (1ll << 31)) return fail(c, HBT_ERR_CAPACITY, "buffer too large");
+    const long long n = h.n;
     if ((size_t)n > s.cap || !s.d_soa[0]) {
         for (auto &p : s.d_soa) {
-            int rc = dev_alloc(c, &p, (size_t)n);
+            rc = dev_alloc(c, &p, (size_t)n);
             if (rc) return rc;
         }
-        int rc = dev_alloc(c, &s.d_pf64, (size_t)n);
+        rc = dev_alloc(c, &s.d_pf64, (size_t)n);
         if (!rc) rc = dev_alloc(c, &s.d_rec, (size_t)n);
         if (!rc) rc = dev_alloc(c, &s.d_pf32, (size_t)n);
         if (!rc) rc = dev_alloc(c, &s.d_cw64, (size_t)n);
@@ -730,72 +637,33 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         s.cap = (size_t)n;
     }
     if ((size_t)nev + 1 > s.cap_ev || !s.d_off) {
-        int rc = dev_alloc(c, &s.d_off, (size_t)nev + 1);
+        rc = dev_alloc(c, &s.d_off, (size_t)nev + 1);
         if (rc) return rc;
         s.cap_ev = (size_t)nev + 1;
     }
     std::vector<double> col((size_t)n);
     for (int a = 0; a < 8; a++) {
-        for (long long k = 0; k < n; k++) col[k] = soa[a][keep[k]];
+        for (long long k = 0; k < n; k++) col[k] = soa[a][h.keep[k]];
         HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
     }
-    {
-        std::vector<PartRec> rec((size_t)n);
-        for (long long k = 0; k < n; k++) {
-            const long long i = keep[k];
-            rec[k] = PartRec{soa[0][i], soa[1][i], soa[2][i], soa[3][i],
-                             soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
-        }
-        if (n > 0)
-            HIPCHECK(c, hipMemcpy(s.d_rec, rec.data(), n * sizeof(PartRec), hipMemcpyHostToDevice));
-    }
-    // prefilter records
-    std::vector<PfRec64> pf((size_t)n);
-    std::vector<PfRec32> pf32((size_t)n);
-    std::vector<double> cw((size_t)n);
-    std::vector<float> cw32((size_t)n);
-    double pmax2 = 0.0, bmax = 0.0;
-    const bool boost = c->P.boost;
-    const bool qinv = c->P.qinv;
-    for (long long k = 0; k < n; k++) {
-        const long long i = keep[k];
-        const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
-        const double n2 = px * px + py * py;
-        pmax2 = std::max(pmax2, n2);
-        pf[k].px = px;
-        pf[k].py = py;
-        pf[k].nrm = n2 - 0.5 * c->mid;
-        pf[k].beta = boost ? pz / E : pz;
-        bmax = std::max(bmax, std::fabs(pf[k].beta));
-        cw[k] = qinv ? INFINITY : (boost ? c->c_beta / E : c->c_beta);
-    }
-    const double eps32 = 5.9604644775390625e-8;
-    for (long long k = 0; k < n; k++) {
-        pf32[k].px = (float)pf[k].px;
-        pf32[k].py = (float)pf[k].py;
-        pf32[k].nrm = (float)pf[k].nrm;
-        pf32[k].beta = (float)pf[k].beta;
-        // covers the FP32 rounding of beta (|beta| <= bmax) and of cw itself
-        cw32[k] = qinv ? INFINITY
-                       : (float)(cw[k] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
-    }
     if (n > 0) {
-        HIPCHECK(c, hipMemcpy(s.d_pf64, pf.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_pf32, pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_cw64, cw.data(), n * 8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_cw32, cw32.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_rec, h.rec.data(), n * sizeof(PartRec), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_pf64, h.pf64.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_pf32, h.pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_cw64, h.cw64.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_cw32, h.cw32.data(), n * 4, hipMemcpyHostToDevice));
     }
-    HIPCHECK(c, hipMemcpy(s.d_off, off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(s.d_off, h.off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
     s.ngroups = ngroups;
     s.grp_ev.assign(grp_ev, grp_ev + ngroups + 1);
     s.nev = nev;
     s.n = n;
-    s.n_in = n_in;
-    s.off = off;
-    s.pmax2 = pmax2;
-    s.psi2 = psi2;
+    s.n_in = h.n_in;
+    s.off = h.off;
+    s.pmax2 = h.pmax2;
+    s.psi2 = h.psi2;
     s.valid = true;
-    if (slot == 0) c->psi = psi2;
+    if (slot == 0) c->psi = h.psi2;
     return HBT_OK;
 }
 
@@ -807,6 +675,41 @@ int check_slots(hbt_ctx *c) {
 }
 
 }  // namespace
+
+namespace hbt {
+
+hipError_t launch_fast_kernel(const FastArgs &a, bool f32, int R, bool ext, long blocks,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, a.counter);
+#define HBT_LAUNCH(T, RR, X) \
+    hipLaunchKernelGGL((k_fast<T, RR, X>), dim3(blocks), dim3(256), 0, st, a)
+#define HBT_LAUNCH_R(T, X)                    \
+    do {                                      \
+        if (R == 1) HBT_LAUNCH(T, 1, X);      \
+        else if (R == 4) HBT_LAUNCH(T, 4, X); \
+        else HBT_LAUNCH(T, 2, X);             \
+    } while (0)
+    if (f32) {
+        if (ext) HBT_LAUNCH_R(float, true);
+        else HBT_LAUNCH_R(float, false);
+    } else {
+        if (ext) HBT_LAUNCH_R(double, true);
+        else HBT_LAUNCH_R(double, false);
+    }
+#undef HBT_LAUNCH_R
+#undef HBT_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(void *p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, st,
+                       (unsigned long long *)p, n);
+    return hipGetLastError();
+}
+
+}  // namespace hbt
 
 // =================================================================== C ABI
 
@@ -820,17 +723,10 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     if (!cfg || !out) return fail(nullptr, HBT_ERR_ARG, "null argument");
     *out = nullptr;
     const hbt_config &k = *cfg;
-    if (k.qnpts < 2) return fail(nullptr, HBT_ERR_ARG, "qnpts must be >= 2");
-    if (k.n_KT < 2) return fail(nullptr, HBT_ERR_ARG, "n_KT (number of K_T edges) must be >= 2");
-    if (k.azimuthal_flag && k.n_Kphi < 1) return fail(nullptr, HBT_ERR_ARG, "n_Kphi must be >= 1");
-    if (!(k.q_max > k.q_min)) return fail(nullptr, HBT_ERR_ARG, "q_max must exceed q_min");
-    if (!(k.KT_max > k.KT_min)) return fail(nullptr, HBT_ERR_ARG, "KT_max must exceed KT_min");
-    if (k.precision != HBT_PREC_FP64 && k.precision != HBT_PREC_FP32C)
-        return fail(nullptr, HBT_ERR_ARG, "unknown precision");
-    if (k.kernel != HBT_KERNEL_PLAIN && k.kernel != HBT_KERNEL_FAST)
-        return fail(nullptr, HBT_ERR_ARG, "unknown kernel");
-    if (k.precision == HBT_PREC_FP32C && k.kernel != HBT_KERNEL_FAST)
-        return fail(nullptr, HBT_ERR_ARG, "the precision form runs on the fast kernel only");
+    Derived D;
+    std::string err;
+    int rc0 = derive(k, D, err);
+    if (rc0) return fail(nullptr, rc0, err);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(nullptr, HBT_ERR_NODEVICE, "no HIP device visible");
@@ -847,58 +743,21 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     std::unique_ptr<hbt_ctx> c(new hbt_ctx());
     c->cfg = k;
     c->cu_count = prop.multiProcessorCount;
-    DevParams &P = c->P;
-    std::memset(&P, 0, sizeof(P));
-    // HBT_correlation.cpp:25-61 and the loop constants of :253-256, :289-290,
-    // :363-369, each evaluated in the reference's expression order.
-    P.qnpts = k.qnpts;
-    P.n_KT = k.n_KT;
-    P.n_Kphi = k.n_Kphi;
-    P.nphi = k.azimuthal_flag ? k.n_Kphi : 1;
-    P.azimuthal = k.azimuthal_flag == 1;
-    P.qinv = k.invariant_radius_flag == 1;
-    P.boost = k.long_comoving_boost == 1;
-    P.delta_q = (k.q_max - k.q_min) / (k.qnpts - 1);
-    P.KT_min = k.KT_min;
-    P.dKT = (k.KT_max - k.KT_min) / (k.n_KT - 1);
-    P.dKphi = 2 * M_PI / (k.n_Kphi > 0 ? k.n_Kphi : 1);
-    P.q_base = k.q_min - P.delta_q / 2.;
-    P.q_lo_cut = k.q_min - P.delta_q / 2. + 1e-8;
-    P.q_hi_cut = k.q_max + P.delta_q / 2. - 1e-8;
-    P.KT_min_sq = k.KT_min * k.KT_min;
-    P.KT_max_sq = k.KT_max * k.KT_max;
-    P.hbarC_inv = 1. / kHbarC;
-    P.two_pi = 2. * M_PI;
-    P.inv_dKT = 1.0 / P.dKT;
-    P.inv_dq = 1.0 / P.delta_q;
-    P.inv_two_pi = 1.0 / (2.0 * M_PI);
-    P.two_pi_hi = 6.28318530717958623199592693709e+00;  // 2*pi rounded to double
-    P.two_pi_lo = 2.44929359829470635445e-16;            // 2*pi - two_pi_hi
-    // Prefilter (hbt_fast.h).  T = 4 K_perp^2 must lie in
-    // [4 KT_min_sq, 4 KT_max_sq]; Q <= 2 H^2.  Margins of 1e-9 relative
-    // (+1e-12 absolute) dominate the ~1e-15 rounding of either side.
-    const double t_lo = 4.0 * P.KT_min_sq, t_hi = 4.0 * P.KT_max_sq;
-    c->mid = 0.5 * (t_lo + t_hi);
-    P.pf_mid = c->mid;
-    P.pf_half = 0.5 * (t_hi - t_lo) * (1.0 + 1e-9) + 1e-9 * t_hi + 1e-12;
-    c->H = std::max(std::fabs(P.q_lo_cut), std::fabs(P.q_hi_cut));
-    const double Hm = c->H * (1.0 + 1e-9) + 1e-12;
-    P.pf_qthr = P.qinv ? INFINITY : (2.0 * Hm * Hm * (1.0 + 1e-9) + 1e-12 - c->mid);
-    c->c_beta = 0.5 * Hm;
-    P.f_mid = (float)c->mid;
-    P.f_half = (float)P.pf_half;
-    P.f_qthr = P.qinv ? INFINITY : (float)P.pf_qthr;
-    c->needed = (uint64_t)(unsigned long long)k.needed_number_of_pairs;
-    c->needed_qinv = (uint64_t)50 * c->needed;
+    c->D = D;
+    c->P = D.P;
+    c->mid = D.mid;
+    c->H = D.H;
+    c->c_beta = D.c_beta;
+    c->needed = D.needed;
+    c->needed_qinv = D.needed_qinv;
 
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(nullptr, HBT_ERR_HIP, "hipStreamCreate failed");
     for (auto &ev : c->ev)
         if (hipEventCreate(&ev) != hipSuccess)
             return fail(nullptr, HBT_ERR_HIP, "hipEventCreate failed");
-    const long q = k.qnpts;
-    c->nbins = (long)k.n_KT * P.nphi * q * q * q;
-    c->nq = (long)k.n_KT * q;
+    c->nbins = D.nbins;
+    c->nq = D.nq;
     hbt_ctx *cp = c.get();
     int rc = dev_alloc(cp, &c->d_bins, (size_t)c->nbins * BIN_STRIDE);
     if (!rc) rc = dev_alloc(cp, &c->d_qinv_count, c->nq);
@@ -907,7 +766,7 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     if (!rc) rc = dev_alloc(cp, &c->d_qinv_num, c->nq);
     if (!rc) rc = dev_alloc(cp, &c->d_stage, 10);
     if (!rc) rc = dev_alloc(cp, &c->d_binned, 2);
-    if (!rc) rc = dev_alloc(cp, &c->d_sums, (size_t)k.n_KT * std::max(1, P.nphi));
+    if (!rc) rc = dev_alloc(cp, &c->d_sums, (size_t)k.n_KT * std::max(1, D.P.nphi));
     if (!rc) rc = dev_alloc(cp, &c->d_counter, 1);
     if (!rc) rc = dev_alloc(cp, &c->d_view, 1);
     if (!rc) rc = zero_hist(cp);

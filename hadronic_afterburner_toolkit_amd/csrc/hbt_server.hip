@@ -1,0 +1,789 @@
+// hbt_server.hip -- request server for independent HBT analyses on one GPU
+// (C ABI in include/hbt_hip.h, "request server").
+//
+// Threads (each owns one stream, all at the default priority):
+//   caller     hbt_server_submit: rapidity cut, prefilter records, mixing
+//              plan and work items on the host; queues the request.
+//   admission  takes queued requests in arrival order among those that fit
+//              (device arena block + resident slot + result slot), uploads
+//              them on the upload stream, marks them active.
+//   launcher   each launch takes up to step_pairs enumerated pairs of work
+//              items from the active requests (water-filling, smallest
+//              remaining first), copies the items and launches k_fast on
+//              the compute stream; at most two launches are in flight.  A
+//              request whose last items went into a launch gets its
+//              histograms copied to its pinned result slot on the copy
+//              stream after that launch.
+//   completion waits for those copies in order, stamps t_done, and frees the
+//              request's arena block and view slot.
+//
+// HBM layout of one resident request (one arena block, 256-byte aligned
+// parts): histogram bins [nbins][8] doubles, then the four q_inv arrays
+// (count, den, sum, num) of nq words each -- copied back in one piece --
+// then particle records (64 B), prefilter records (32 or 16 B), c/E bounds,
+// event offsets, plan row offsets, partner ids, (cos, sin) per row, and
+// psi_ref per group.  The same-event list also serves as the mixed list.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "hbt_internal.h"
+
+using namespace hbt;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// First-fit allocator over the device arena.
+class Arena {
+  public:
+    void init(size_t size) {
+        free_.clear();
+        if (size) free_[0] = size;
+        total_ = size;
+        used_ = peak_ = 0;
+    }
+    bool alloc(size_t n, size_t &off) {
+        n = align256(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            off = it->first;
+            const size_t rest = it->second - n, start = it->first + n;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            used_ += n;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = align256(n);
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t total() const { return total_; }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+
+  private:
+    std::map<size_t, size_t> free_;  // offset -> size
+    size_t total_ = 0, used_ = 0, peak_ = 0;
+};
+
+struct Request {
+    int64_t id = 0;
+    int state = HBT_REQ_QUEUED;
+    std::string err;
+    int nev = 0, ngroups = 0;
+    HostParticles hp;
+    std::vector<int> grp_ev;
+    std::vector<long long> rowoff;
+    std::vector<int> plan_ids;  // index into the request's events
+    std::vector<double> plan_cs;
+    std::vector<WorkItem> items;
+    std::vector<uint64_t> item_pairs;
+    size_t next = 0;           // next item to launch
+    uint64_t pairs_left = 0;   // enumerated pairs not yet launched
+    uint64_t enum_same = 0, enum_mixed = 0;
+    double pmax2 = 0.0;
+    // device residency
+    size_t foot = 0, arena_off = 0;
+    size_t o_bins = 0, o_rec = 0, o_pf = 0, o_cw = 0, o_off = 0, o_rowoff = 0, o_ids = 0,
+           o_cs = 0, o_psi = 0;
+    int view = -1, rslot = -1;
+    View v{};
+    hipEvent_t done_ev = nullptr;
+    double t_submit = 0, t_admit = 0, t_active = 0, t_done = 0;
+};
+
+}  // namespace
+
+struct hbt_server {
+    hbt_config cfg;
+    Derived D;
+    hbt_server_config sc;
+    ItemShape shape;
+    bool f32 = false, ext = false;
+    int device = 0, cu_count = 0;
+    hipStream_t s_compute = nullptr, s_copy = nullptr, s_upload = nullptr;
+    char *d_arena = nullptr;
+    Arena arena;
+    View *d_views = nullptr;
+    unsigned long long *d_misc = nullptr;  // [0..9] stage words, [16..17] binned counters
+    int *d_counter = nullptr;
+    std::vector<int> free_views;
+    std::vector<char *> h_results;
+    std::vector<int> free_results;
+    size_t result_bytes = 0;
+    WorkItem *d_items[2] = {};
+    WorkItem *h_items[2] = {};
+    hipEvent_t step_ev[2] = {};
+    bool step_pending[2] = {};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request *> queue;
+    std::vector<Request *> active;
+    std::deque<Request *> draining;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::thread t_admission, t_launcher, t_completion;
+    bool stop = false;
+    bool launcher_done = false;  // no more readbacks will be queued
+    int64_t next_id = 1;
+    hbt_server_stats st{};
+    Clock::time_point t0;
+    double busy_since = -1.0;
+    int inflight = 0;
+    std::string err;
+
+    double now() const {
+        return std::chrono::duration<double>(Clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+std::string g_server_create_err;
+
+int sfail(hbt_server *s, int code, const std::string &msg) {
+    if (s) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = msg;
+    } else {
+        g_server_create_err = msg;
+    }
+    return code;
+}
+
+// Request layout in its arena block; returns the footprint.
+size_t layout(const hbt_server *s, Request &r) {
+    const long long n = r.hp.n, rows = (long long)r.plan_ids.size();
+    size_t o = 0;
+    r.o_bins = o;
+    o = align256(o + s->result_bytes);
+    r.o_rec = o;
+    o = align256(o + (size_t)n * sizeof(PartRec));
+    r.o_pf = o;
+    o = align256(o + (size_t)n * (s->f32 ? sizeof(PfRec32) : sizeof(PfRec64)));
+    r.o_cw = o;
+    o = align256(o + (size_t)n * (s->f32 ? 4 : 8));
+    r.o_off = o;
+    o = align256(o + (size_t)(r.nev + 1) * 8);
+    r.o_rowoff = o;
+    o = align256(o + (size_t)(r.nev + 1) * 8);
+    r.o_ids = o;
+    o = align256(o + (size_t)std::max<long long>(rows, 1) * 4);
+    r.o_cs = o;
+    o = align256(o + (size_t)std::max<long long>(rows, 1) * 16);
+    r.o_psi = o;
+    o = align256(o + (size_t)std::max(r.ngroups, 1) * 8);
+    return o;
+}
+
+void make_request_view(hbt_server *s, Request &r) {
+    char *b = s->d_arena + r.arena_off;
+    View &v = r.v;
+    v.H.bins = (double *)(b + r.o_bins);
+    unsigned long long *q = (unsigned long long *)(b + r.o_bins + (size_t)s->D.nbins * 64);
+    v.H.qinv_count = q;
+    v.H.qinv_den = q + s->D.nq;
+    v.H.qinv_sum = (double *)(q + 2 * s->D.nq);
+    v.H.qinv_num = (double *)(q + 3 * s->D.nq);
+    v.H.stage = s->d_misc;
+    v.A = v.B = (const PartRec *)(b + r.o_rec);
+    v.pfA = v.pfB = (const void *)(b + r.o_pf);
+    v.cwA = v.cwB = (const void *)(b + r.o_cw);
+    v.mix_off = (const long long *)(b + r.o_off);
+    v.rowoff = (const long long *)(b + r.o_rowoff);
+    v.plan_ids = (const int *)(b + r.o_ids);
+    v.plan_cs = (const double *)(b + r.o_cs);
+    v.psi = (const double *)(b + r.o_psi);
+}
+
+// Uploads a request into its arena block on the upload stream and waits.
+hipError_t upload_request(hbt_server *s, Request &r) {
+    char *b = s->d_arena + r.arena_off;
+    const long long n = r.hp.n;
+    hipError_t e = launch_zero_words(b + r.o_bins, s->result_bytes / 8, s->s_upload);
+    auto cp = [&](size_t off, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes)
+            e = hipMemcpyAsync(b + off, src, bytes, hipMemcpyHostToDevice, s->s_upload);
+    };
+    cp(r.o_rec, r.hp.rec.data(), (size_t)n * sizeof(PartRec));
+    if (s->f32) {
+        cp(r.o_pf, r.hp.pf32.data(), (size_t)n * sizeof(PfRec32));
+        cp(r.o_cw, r.hp.cw32.data(), (size_t)n * 4);
+    } else {
+        cp(r.o_pf, r.hp.pf64.data(), (size_t)n * sizeof(PfRec64));
+        cp(r.o_cw, r.hp.cw64.data(), (size_t)n * 8);
+    }
+    cp(r.o_off, r.hp.off.data(), (size_t)(r.nev + 1) * 8);
+    cp(r.o_rowoff, r.rowoff.data(), (size_t)(r.nev + 1) * 8);
+    cp(r.o_ids, r.plan_ids.data(), r.plan_ids.size() * 4);
+    cp(r.o_cs, r.plan_cs.data(), r.plan_cs.size() * 8);
+    cp(r.o_psi, r.hp.psi2.data(), r.hp.psi2.size() * 8);
+    cp(0, nullptr, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_views + r.view, &r.v, sizeof(View), hipMemcpyHostToDevice,
+                           s->s_upload);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->s_upload);
+    return e;
+}
+
+// Frees what a finished or failed request holds on the device (not its
+// result slot).  Caller holds the lock.
+void retire_device(hbt_server *s, Request &r) {
+    if (r.foot) s->arena.release(r.arena_off, r.foot);
+    r.foot = 0;
+    if (r.view >= 0) s->free_views.push_back(r.view);
+    r.view = -1;
+}
+
+void admission_loop(hbt_server *s) {
+    hipSetDevice(s->device);
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stop) return;
+                // first request, in arrival order, that fits
+                if (!s->free_views.empty() && !s->free_results.empty()) {
+                    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+                        size_t off = 0;
+                        if (s->arena.alloc((*it)->foot, off)) {
+                            r = *it;
+                            r->arena_off = off;
+                            s->queue.erase(it);
+                            break;
+                        }
+                    }
+                }
+                if (r) break;
+                s->cv.wait(lk);
+            }
+            r->view = s->free_views.back();
+            s->free_views.pop_back();
+            r->rslot = s->free_results.back();
+            s->free_results.pop_back();
+            r->state = HBT_REQ_LOADING;
+            r->t_admit = s->now();
+            s->st.resident++;
+            s->st.resident_peak = std::max(s->st.resident_peak, s->st.resident);
+        }
+        make_request_view(s, *r);
+        const hipError_t e = upload_request(s, *r);
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (e != hipSuccess) {
+            r->state = HBT_REQ_FAILED;
+            r->err = std::string("upload: ") + hipGetErrorString(e);
+            s->st.failed++;
+            retire_device(s, *r);
+        } else {
+            r->state = HBT_REQ_ACTIVE;
+            r->t_active = s->now();
+            // host copies are on the device now
+            HostParticles keep_meta;
+            keep_meta.n = r->hp.n;
+            keep_meta.n_in = r->hp.n_in;
+            r->hp = std::move(keep_meta);
+            r->plan_cs.clear();
+            r->plan_cs.shrink_to_fit();
+            s->active.push_back(r);
+        }
+        s->cv.notify_all();
+    }
+}
+
+// One launch: items of the active requests, water-filling over requests
+// ordered by remaining pairs.  Caller holds the lock.  Requests whose last
+// items are taken move to `finishing`.
+void build_step(hbt_server *s, std::vector<WorkItem> &items, std::vector<Request *> &finishing,
+                double &pmax2, double &pairs) {
+    std::vector<Request *> act = s->active;
+    std::sort(act.begin(), act.end(), [](const Request *a, const Request *b) {
+        return a->pairs_left < b->pairs_left;
+    });
+    double budget = s->sc.step_pairs;
+    const size_t cap = (size_t)s->sc.max_step_items;
+    pmax2 = 0.0;
+    pairs = 0.0;
+    for (size_t k = 0; k < act.size() && items.size() < cap; k++) {
+        Request *r = act[k];
+        const double share = budget / (double)(act.size() - k);
+        double took = 0.0;
+        bool any = false;
+        while (r->next < r->items.size() && items.size() < cap && (!any || took < share)) {
+            WorkItem w = r->items[r->next];
+            w.view = r->view;
+            items.push_back(w);
+            took += (double)r->item_pairs[r->next];
+            r->pairs_left -= r->item_pairs[r->next];
+            r->next++;
+            any = true;
+        }
+        budget = std::max(0.0, budget - took);
+        pairs += took;
+        if (any) pmax2 = std::max(pmax2, r->pmax2);
+        if (r->next == r->items.size()) finishing.push_back(r);
+    }
+    for (Request *r : finishing) {
+        r->state = HBT_REQ_DRAINING;
+        s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+    }
+}
+
+void launcher_loop(hbt_server *s) {
+    hipSetDevice(s->device);
+    std::vector<WorkItem> items;
+    std::vector<Request *> finishing;
+    int step = 0;
+    for (;;) {
+        items.clear();
+        finishing.clear();
+        double pmax2 = 0.0, pairs = 0.0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->active.empty(); });
+            if (s->stop) break;
+            build_step(s, items, finishing, pmax2, pairs);
+        }
+        const int buf = step & 1;
+        hipError_t e = hipSuccess;
+        if (s->step_pending[buf]) {
+            e = hipEventSynchronize(s->step_ev[buf]);
+            s->step_pending[buf] = false;
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (--s->inflight == 0 && s->busy_since >= 0) {
+                s->st.busy_s += s->now() - s->busy_since;
+                s->busy_since = -1.0;
+            }
+        }
+        if (!items.empty() && e == hipSuccess) {
+            std::memcpy(s->h_items[buf], items.data(), items.size() * sizeof(WorkItem));
+            e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], items.size() * sizeof(WorkItem),
+                               hipMemcpyHostToDevice, s->s_compute);
+            FastArgs a;
+            a.P = s->D.P;
+            if (s->f32) fp32_margins(s->D, pmax2, a.P);
+            a.views = s->d_views;
+            a.items = s->d_items[buf];
+            a.nitems = (int)items.size();
+            a.counter = s->d_counter;
+            a.binned = s->d_misc + 16;
+            const long blocks = std::max(
+                1l, std::min<long>((long)s->cu_count * s->sc.tune.blocks_per_cu,
+                                   ((long)items.size() + 3) / 4));
+            if (e == hipSuccess)
+                e = launch_fast_kernel(a, s->f32, s->sc.tune.R, s->ext, blocks, s->s_compute);
+            if (e == hipSuccess) e = hipEventRecord(s->step_ev[buf], s->s_compute);
+            if (e == hipSuccess) {
+                s->step_pending[buf] = true;
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (s->inflight++ == 0) s->busy_since = s->now();
+                s->st.launches++;
+                s->st.items += (int64_t)items.size();
+                s->st.pairs += pairs;
+            }
+            step++;
+        }
+        // histograms of requests whose work is all launched: copied back once
+        // this launch has finished (a request that never had items needs
+        // only its upload, which has completed)
+        for (Request *r : finishing) {
+            hipError_t e2 = e;
+            if (e2 == hipSuccess && !items.empty())
+                e2 = hipStreamWaitEvent(s->s_copy, s->step_ev[buf], 0);
+            if (e2 == hipSuccess)
+                e2 = hipMemcpyAsync(s->h_results[r->rslot], s->d_arena + r->arena_off + r->o_bins,
+                                    s->result_bytes, hipMemcpyDeviceToHost, s->s_copy);
+            if (e2 == hipSuccess) e2 = hipEventRecord(r->done_ev, s->s_copy);
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (e2 != hipSuccess) {
+                r->state = HBT_REQ_FAILED;
+                r->err = std::string("launch/readback: ") + hipGetErrorString(e2);
+                s->st.failed++;
+                s->st.resident--;
+                retire_device(s, *r);
+            } else {
+                s->draining.push_back(r);
+            }
+            s->cv.notify_all();
+        }
+    }
+    // drain what is in flight before the streams go away
+    for (int b = 0; b < 2; b++)
+        if (s->step_pending[b]) hipEventSynchronize(s->step_ev[b]);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->launcher_done = true;
+    s->cv.notify_all();
+}
+
+void completion_loop(hbt_server *s) {
+    hipSetDevice(s->device);
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->launcher_done || !s->draining.empty(); });
+            if (s->draining.empty()) return;  // stopped and nothing left to read back
+            r = s->draining.front();
+        }
+        const hipError_t e = hipEventSynchronize(r->done_ev);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->draining.pop_front();
+        r->t_done = s->now();
+        if (e != hipSuccess) {
+            r->state = HBT_REQ_FAILED;
+            r->err = std::string("readback: ") + hipGetErrorString(e);
+            s->st.failed++;
+        } else {
+            r->state = HBT_REQ_DONE;
+            s->st.completed++;
+        }
+        s->st.resident--;
+        retire_device(s, *r);
+        s->cv.notify_all();
+    }
+}
+
+Request *find(hbt_server *s, int64_t id) {
+    auto it = s->reqs.find(id);
+    return it == s->reqs.end() ? nullptr : it->second.get();
+}
+
+void destroy_resources(hbt_server *s) {
+    if (s->s_compute) hipStreamSynchronize(s->s_compute);
+    if (s->s_copy) hipStreamSynchronize(s->s_copy);
+    if (s->s_upload) hipStreamSynchronize(s->s_upload);
+    for (auto &kv : s->reqs)
+        if (kv.second->done_ev) hipEventDestroy(kv.second->done_ev);
+    for (char *p : s->h_results)
+        if (p) hipHostFree(p);
+    for (int b = 0; b < 2; b++) {
+        if (s->h_items[b]) hipHostFree(s->h_items[b]);
+        if (s->d_items[b]) hipFree(s->d_items[b]);
+        if (s->step_ev[b]) hipEventDestroy(s->step_ev[b]);
+    }
+    if (s->d_views) hipFree(s->d_views);
+    if (s->d_misc) hipFree(s->d_misc);
+    if (s->d_counter) hipFree(s->d_counter);
+    if (s->d_arena) hipFree(s->d_arena);
+    if (s->s_compute) hipStreamDestroy(s->s_compute);
+    if (s->s_copy) hipStreamDestroy(s->s_copy);
+    if (s->s_upload) hipStreamDestroy(s->s_upload);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbt_server_create(const hbt_config *cfg, const hbt_server_config *sc_in, hbt_server **out) {
+    if (!cfg || !out) return sfail(nullptr, HBT_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->kernel != HBT_KERNEL_FAST)
+        return sfail(nullptr, HBT_ERR_ARG, "the server runs the fast kernel (kernel = 1)");
+    std::unique_ptr<hbt_server> s(new hbt_server());
+    std::string err;
+    int rc = derive(*cfg, s->D, err);
+    if (rc) return sfail(nullptr, rc, err);
+    s->cfg = *cfg;
+    hbt_server_config sc{};
+    if (sc_in) {
+        sc = *sc_in;
+    } else {
+        sc.hbm_bytes = 16ll << 30;
+        sc.max_resident = 16;
+        sc.max_step_items = 1 << 18;
+        sc.step_pairs = 2e9;
+        sc.tune = hbt_tuning{2, 1024, 4, 8, 0.0};
+    }
+    const hbt_tuning &t = sc.tune;
+    if ((t.R != 1 && t.R != 2 && t.R != 4) || t.jchunk < 64 || t.jchunk > (1 << 24) ||
+        t.kchunk < 1 || t.kchunk > 64 || t.blocks_per_cu < 1 || t.blocks_per_cu > 64)
+        return sfail(nullptr, HBT_ERR_ARG, "tuning out of range");
+    if (sc.hbm_bytes < (1 << 20) || sc.max_resident < 1 || sc.max_resident > 4096 ||
+        sc.max_step_items < 1 || !(sc.step_pairs > 0))
+        return sfail(nullptr, HBT_ERR_ARG, "bad server configuration");
+    s->sc = sc;
+    s->shape = item_shape(sc.tune);
+    s->f32 = cfg->precision == HBT_PREC_FP32C;
+    s->ext = s->D.P.qinv || s->D.P.azimuthal;
+    s->result_bytes = (size_t)s->D.nbins * 64 + (size_t)s->D.nq * 32;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return sfail(nullptr, HBT_ERR_NODEVICE, "no HIP device visible");
+    if (cfg->device < 0 || cfg->device >= ndev)
+        return sfail(nullptr, HBT_ERR_NODEVICE, "bad device");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess)
+        return sfail(nullptr, HBT_ERR_NODEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return sfail(nullptr, HBT_ERR_NODEVICE,
+                     std::string("built for gfx950, device is ") + prop.gcnArchName);
+    s->device = cfg->device;
+    s->cu_count = prop.multiProcessorCount;
+    if (hipSetDevice(s->device) != hipSuccess)
+        return sfail(nullptr, HBT_ERR_NODEVICE, "hipSetDevice failed");
+    auto bad = [&](hipError_t e, const char *what) {
+        if (e == hipSuccess) return false;
+        g_server_create_err = std::string(what) + ": " + hipGetErrorString(e);
+        return true;
+    };
+    bool ok = !bad(hipStreamCreateWithFlags(&s->s_compute, hipStreamNonBlocking), "stream") &&
+              !bad(hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking), "stream") &&
+              !bad(hipStreamCreateWithFlags(&s->s_upload, hipStreamNonBlocking), "stream");
+    if (ok && bad(hipMalloc((void **)&s->d_arena, (size_t)sc.hbm_bytes), "arena hipMalloc")) {
+        s->d_arena = nullptr;
+        ok = false;
+    }
+    ok = ok && !bad(hipMalloc((void **)&s->d_views, sizeof(View) * sc.max_resident), "hipMalloc");
+    ok = ok && !bad(hipMalloc((void **)&s->d_misc, 32 * 8), "hipMalloc");
+    ok = ok && !bad(hipMalloc((void **)&s->d_counter, 4), "hipMalloc");
+    ok = ok && !bad(hipMemset(s->d_misc, 0, 32 * 8), "hipMemset");
+    for (int b = 0; b < 2 && ok; b++) {
+        ok = !bad(hipMalloc((void **)&s->d_items[b], sizeof(WorkItem) * sc.max_step_items),
+                  "hipMalloc") &&
+             !bad(hipHostMalloc((void **)&s->h_items[b], sizeof(WorkItem) * sc.max_step_items),
+                  "hipHostMalloc") &&
+             !bad(hipEventCreateWithFlags(&s->step_ev[b], hipEventDisableTiming), "event");
+    }
+    s->h_results.assign(sc.max_resident, nullptr);
+    for (int k = 0; k < sc.max_resident && ok; k++)
+        ok = !bad(hipHostMalloc((void **)&s->h_results[k], s->result_bytes), "hipHostMalloc");
+    if (!ok) {
+        const int code = s->d_arena ? HBT_ERR_HIP : HBT_ERR_NOMEM;
+        destroy_resources(s.get());
+        return code;
+    }
+    s->arena.init((size_t)sc.hbm_bytes);
+    for (int k = sc.max_resident - 1; k >= 0; k--) {
+        s->free_views.push_back(k);
+        s->free_results.push_back(k);
+    }
+    s->st.arena_bytes = sc.hbm_bytes;
+    s->t0 = Clock::now();
+    hbt_server *sp = s.get();
+    sp->t_admission = std::thread(admission_loop, sp);
+    sp->t_launcher = std::thread(launcher_loop, sp);
+    sp->t_completion = std::thread(completion_loop, sp);
+    *out = s.release();
+    return HBT_OK;
+}
+
+int hbt_server_destroy(hbt_server *s) {
+    if (!s) return HBT_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    // the admission thread may be inside an upload and the launcher inside a
+    // launch: each ends its current piece of work, then sees stop
+    if (s->t_admission.joinable()) s->t_admission.join();
+    if (s->t_launcher.joinable()) s->t_launcher.join();
+    if (s->t_completion.joinable()) s->t_completion.join();
+    destroy_resources(s);
+    delete s;
+    return HBT_OK;
+}
+
+const char *hbt_server_last_error(const hbt_server *s) {
+    return s ? s->err.c_str() : g_server_create_err.c_str();
+}
+
+int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
+                      const double *const soa[8], int32_t events_per_group, int32_t seed,
+                      int64_t *req_id) {
+    if (!s || !req_id) return HBT_ERR_ARG;
+    if (nev < 1 || events_per_group < 1)
+        return sfail(s, HBT_ERR_ARG, "need nev >= 1 and events_per_group >= 1");
+    std::unique_ptr<Request> r(new Request());
+    r->t_submit = s->now();
+    r->nev = nev;
+    r->ngroups = (nev + events_per_group - 1) / events_per_group;
+    r->grp_ev.resize(r->ngroups + 1);
+    for (int g = 0; g <= r->ngroups; g++) r->grp_ev[g] = std::min(nev, g * events_per_group);
+    std::string err;
+    int rc = gather_particles(s->D, s->cfg, r->ngroups, r->grp_ev.data(), nev, ev_off, soa,
+                              r->hp, err);
+    if (rc) return sfail(s, rc, err);
+    r->pmax2 = r->hp.pmax2;
+    // mixing plan, buffer after buffer from one stream (HBT_correlation.cpp:200-216)
+    std::mt19937 gen((unsigned)seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    std::uniform_int_distribution<int> uint_{};
+    r->rowoff.assign(nev + 1, 0);
+    const std::vector<long long> &off = r->hp.off;
+    for (int g = 0; g < r->ngroups; g++) {
+        const int e0 = r->grp_ev[g], ne = r->grp_ev[g + 1] - e0;
+        const int M = s->cfg.number_of_mixed_events > 0 ? s->cfg.number_of_mixed_events
+                                                        : ne / 2 + 1;
+        for (int iev = 0; iev < ne; iev++) {
+            const size_t base = r->plan_ids.size();
+            for (int count = 0; count < M; count++) {
+                int id = uint_(gen) % ne;
+                while (iev == id && ne != 1) id = uint_(gen) % ne;
+                r->plan_ids.push_back(e0 + id);
+            }
+            for (int k = 0; k < M; k++) {
+                const double random_rotation = uni(gen) * 2 * M_PI;
+                double sn, cn;
+                sincos(random_rotation, &sn, &cn);  // as the reference's build evaluates it
+                r->plan_cs.push_back(cn);
+                r->plan_cs.push_back(sn);
+            }
+            const int e = e0 + iev;
+            r->rowoff[e + 1] = r->rowoff[e] + M;
+            const long long n1 = off[e + 1] - off[e];
+            for (size_t q = base; q < r->plan_ids.size(); q++) {
+                const int p = r->plan_ids[q];
+                r->enum_mixed += (uint64_t)(n1 * (off[p + 1] - off[p]));
+            }
+        }
+    }
+    // pair cap: counters of a fresh analysis start at 0, so it cannot bind
+    // while every enumerated total stays within it
+    const bool qinv = s->cfg.invariant_radius_flag == 1;
+    r->items.clear();
+    items_same(s->shape, off, r->grp_ev, 0, r->items, &r->item_pairs, r->enum_same);
+    items_mixed(s->shape, off, r->grp_ev, r->rowoff, r->plan_ids, off, 0, r->items,
+                &r->item_pairs);
+    for (uint64_t t : {r->enum_same, r->enum_mixed})
+        if (t > s->D.needed + 1 || (qinv && t > s->D.needed_qinv))
+            return sfail(s, HBT_ERR_CAPACITY,
+                         "needed_number_of_pairs may bind (order-dependent pair cap)");
+    for (uint64_t p : r->item_pairs) r->pairs_left += p;
+    r->foot = layout(s, *r);
+    if (hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming) != hipSuccess)
+        return sfail(s, HBT_ERR_HIP, "hipEventCreate failed");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->foot > s->arena.total()) {
+        hipEventDestroy(r->done_ev);
+        s->err = "request needs more HBM than the server's arena";
+        return HBT_ERR_CAPACITY;
+    }
+    if (s->stop) {
+        hipEventDestroy(r->done_ev);
+        s->err = "server is stopping";
+        return HBT_ERR_STATE;
+    }
+    r->id = s->next_id++;
+    *req_id = r->id;
+    s->queue.push_back(r.get());
+    s->reqs[r->id] = std::move(r);
+    s->st.submitted++;
+    s->cv.notify_all();
+    return HBT_OK;
+}
+
+int hbt_server_info(hbt_server *s, int64_t id, hbt_request_info *info) {
+    if (!s || !info) return HBT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request *r = find(s, id);
+    if (!r) return HBT_ERR_ARG;
+    info->state = r->state;
+    info->nev = r->nev;
+    info->n_kept = r->hp.n;
+    info->enum_same = r->enum_same;
+    info->enum_mixed = r->enum_mixed;
+    info->footprint = (int64_t)(r->foot ? r->foot : layout(s, *r));
+    info->t_submit = r->t_submit;
+    info->t_admit = r->t_admit;
+    info->t_active = r->t_active;
+    info->t_done = r->t_done;
+    return HBT_OK;
+}
+
+int hbt_server_wait(hbt_server *s, int64_t id, double timeout_s) {
+    if (!s) return HBT_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    Request *r = find(s, id);
+    if (!r) return HBT_ERR_ARG;
+    auto done = [&] { return r->state == HBT_REQ_DONE || r->state == HBT_REQ_FAILED; };
+    if (timeout_s < 0) {
+        s->cv.wait(lk, done);
+    } else if (!s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
+        return HBT_ERR_STATE;
+    }
+    if (r->state == HBT_REQ_FAILED) {
+        s->err = r->err;
+        return HBT_ERR_HIP;
+    }
+    return HBT_OK;
+}
+
+int hbt_server_result(hbt_server *s, int64_t id, hbt_result *o) {
+    if (!s || !o) return HBT_ERR_ARG;
+    const char *buf = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        Request *r = find(s, id);
+        if (!r) return HBT_ERR_ARG;
+        if (r->state != HBT_REQ_DONE) {
+            s->err = "request is not done";
+            return HBT_ERR_STATE;
+        }
+        buf = s->h_results[r->rslot];
+    }
+    // the result slot stays with the request until it is released
+    const double *aos = (const double *)buf;
+    const unsigned long long *q = (const unsigned long long *)(buf + (size_t)s->D.nbins * 64);
+    const long nq = s->D.nq;
+    result_from_host(s->D, s->cfg, aos, q, q + nq, (const double *)(q + 2 * nq),
+                     (const double *)(q + 3 * nq), o);
+    return HBT_OK;
+}
+
+int hbt_server_release(hbt_server *s, int64_t id) {
+    if (!s) return HBT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request *r = find(s, id);
+    if (!r) return HBT_ERR_ARG;
+    if (r->state != HBT_REQ_DONE && r->state != HBT_REQ_FAILED) {
+        s->err = "request is still running";
+        return HBT_ERR_STATE;
+    }
+    if (r->rslot >= 0) s->free_results.push_back(r->rslot);
+    if (r->done_ev) hipEventDestroy(r->done_ev);
+    s->reqs.erase(id);
+    s->cv.notify_all();
+    return HBT_OK;
+}
+
+int hbt_server_get_stats(hbt_server *s, hbt_server_stats *out) {
+    if (!s || !out) return HBT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    out->arena_used = (int64_t)s->arena.used();
+    out->arena_peak = (int64_t)s->arena.peak();
+    out->queued = (int32_t)s->queue.size();
+    if (s->busy_since >= 0) out->busy_s += s->now() - s->busy_since;
+    return HBT_OK;
+}
+
+}  // extern "C"

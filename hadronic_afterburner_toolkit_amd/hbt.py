@@ -16,7 +16,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import HbtConfig, HbtError, HbtResult, HbtTuning
+from ._lib import (HbtConfig, HbtError, HbtRequestInfo, HbtResult, HbtServerConfig,
+                   HbtServerStats, HbtTuning)
 
 DEFAULTS = dict(qnpts=31, q_min=-0.15, q_max=0.15, n_KT=5, KT_min=0.15, KT_max=0.55,
                 n_Kphi=48, azimuthal_flag=0, invariant_radius_flag=0,
@@ -163,29 +164,8 @@ class HBTCorrelation:
 
     def __init__(self, params: dict | None = None, precision: int = _lib.HBT_PREC_FP64,
                  kernel: int = _lib.HBT_KERNEL_FAST, device: int = 0):
-        p = dict(DEFAULTS)
-        if params:
-            p.update({k: v for k, v in params.items() if k in DEFAULTS})
-        self.params = p
+        self.params, cfg = _make_config(params, precision, kernel, device)
         self._lib = _lib.load()
-        cfg = HbtConfig()
-        cfg.qnpts = int(p["qnpts"])
-        cfg.q_min = float(p["q_min"])
-        cfg.q_max = float(p["q_max"])
-        cfg.n_KT = int(p["n_KT"])
-        cfg.KT_min = float(p["KT_min"])
-        cfg.KT_max = float(p["KT_max"])
-        cfg.n_Kphi = int(p["n_Kphi"])
-        cfg.azimuthal_flag = int(p["azimuthal_flag"])
-        cfg.invariant_radius_flag = int(p["invariant_radius_flag"])
-        cfg.long_comoving_boost = int(p["long_comoving_boost"])
-        cfg.HBTrap_min = float(p["HBTrap_min"])
-        cfg.HBTrap_max = float(p["HBTrap_max"])
-        cfg.needed_number_of_pairs = float(p["needed_number_of_pairs"])
-        cfg.number_of_mixed_events = int(p["number_of_mixed_events"])
-        cfg.precision = precision
-        cfg.kernel = kernel
-        cfg.device = device
         h = C.c_void_p()
         rc = self._lib.hbt_create(C.byref(cfg), C.byref(h))
         if rc:
@@ -323,23 +303,135 @@ class HBTCorrelation:
         return s, m
 
     def results(self) -> Results:
-        p = self.params
-        q = int(p["qnpts"])
-        nkt = int(p["n_KT"])
-        nb = nkt * self.nphi * q ** 3
-        nq = nkt * q
-        r = Results(
-            num_count=np.zeros(nb, np.uint64), den=np.zeros(nb, np.uint64),
-            q_out_sum=np.zeros(nb), q_side_sum=np.zeros(nb), q_long_sum=np.zeros(nb),
-            num=np.zeros(nb), qinv_count=np.zeros(nq, np.uint64),
-            qinv_den=np.zeros(nq, np.uint64), qinv_sum=np.zeros(nq), qinv_num=np.zeros(nq),
-            cnt_num_kt=np.zeros(nkt, np.uint64), cnt_den_kt=np.zeros(nkt, np.uint64),
-            cnt_num_qinv=np.zeros(nkt, np.uint64), cnt_den_qinv=np.zeros(nkt, np.uint64),
-            cnt_num_ktphi=np.zeros(nkt * self.nphi, np.uint64),
-            cnt_den_ktphi=np.zeros(nkt * self.nphi, np.uint64))
-        res = HbtResult()
-        for name, _ in HbtResult._fields_:
-            a = getattr(r, name)
-            setattr(res, name, _u64p(a) if a.dtype == np.uint64 else _f64p(a))
+        r, res = _alloc_results(self.params, self.nphi)
         self._check(self._lib.hbt_read(self.h, C.byref(res)), "hbt_read")
         return r
+
+
+def _make_config(params, precision, kernel, device):
+    p = dict(DEFAULTS)
+    if params:
+        p.update({k: v for k, v in params.items() if k in DEFAULTS})
+    cfg = HbtConfig()
+    cfg.qnpts = int(p["qnpts"])
+    cfg.q_min = float(p["q_min"])
+    cfg.q_max = float(p["q_max"])
+    cfg.n_KT = int(p["n_KT"])
+    cfg.KT_min = float(p["KT_min"])
+    cfg.KT_max = float(p["KT_max"])
+    cfg.n_Kphi = int(p["n_Kphi"])
+    cfg.azimuthal_flag = int(p["azimuthal_flag"])
+    cfg.invariant_radius_flag = int(p["invariant_radius_flag"])
+    cfg.long_comoving_boost = int(p["long_comoving_boost"])
+    cfg.HBTrap_min = float(p["HBTrap_min"])
+    cfg.HBTrap_max = float(p["HBTrap_max"])
+    cfg.needed_number_of_pairs = float(p["needed_number_of_pairs"])
+    cfg.number_of_mixed_events = int(p["number_of_mixed_events"])
+    cfg.precision = precision
+    cfg.kernel = kernel
+    cfg.device = device
+    return p, cfg
+
+
+def _alloc_results(p, nphi):
+    """Host arrays of one result and the hbt_result pointing at them."""
+    q = int(p["qnpts"])
+    nkt = int(p["n_KT"])
+    nb = nkt * nphi * q ** 3
+    nq = nkt * q
+    r = Results(
+        num_count=np.zeros(nb, np.uint64), den=np.zeros(nb, np.uint64),
+        q_out_sum=np.zeros(nb), q_side_sum=np.zeros(nb), q_long_sum=np.zeros(nb),
+        num=np.zeros(nb), qinv_count=np.zeros(nq, np.uint64),
+        qinv_den=np.zeros(nq, np.uint64), qinv_sum=np.zeros(nq), qinv_num=np.zeros(nq),
+        cnt_num_kt=np.zeros(nkt, np.uint64), cnt_den_kt=np.zeros(nkt, np.uint64),
+        cnt_num_qinv=np.zeros(nkt, np.uint64), cnt_den_qinv=np.zeros(nkt, np.uint64),
+        cnt_num_ktphi=np.zeros(nkt * nphi, np.uint64),
+        cnt_den_ktphi=np.zeros(nkt * nphi, np.uint64))
+    res = HbtResult()
+    for name, _ in HbtResult._fields_:
+        a = getattr(r, name)
+        setattr(res, name, _u64p(a) if a.dtype == np.uint64 else _f64p(a))
+    return r, res
+
+
+class HBTServer:
+    """Request server (hbt_server_* of include/hbt_hip.h): independent
+    analyses submitted as they arrive, admitted by HBM footprint, batched
+    into shared launches of the fast kernel.  One configuration (the
+    reference's HBT keys) for all requests."""
+
+    def __init__(self, params: dict | None = None, precision: int = _lib.HBT_PREC_FP64,
+                 hbm_bytes: int = 16 << 30, max_resident: int = 16,
+                 max_step_items: int = 1 << 18, step_pairs: float = 2e9,
+                 tuning=(2, 1024, 4, 8), device: int = 0):
+        self.params, cfg = _make_config(params, precision, _lib.HBT_KERNEL_FAST, device)
+        self.nphi = cfg.n_Kphi if cfg.azimuthal_flag else 1
+        self._lib = _lib.load()
+        sc = HbtServerConfig()
+        sc.hbm_bytes = int(hbm_bytes)
+        sc.max_resident = int(max_resident)
+        sc.max_step_items = int(max_step_items)
+        sc.step_pairs = float(step_pairs)
+        sc.tune = HbtTuning(*[int(v) for v in tuning], 0.0)
+        h = C.c_void_p()
+        rc = self._lib.hbt_server_create(C.byref(cfg), C.byref(sc), C.byref(h))
+        if rc:
+            raise HbtError(rc, self._lib.hbt_server_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            raise HbtError(rc, f"{what}: {self._lib.hbt_server_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.hbt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, buf, events_per_group: int, seed: int) -> int:
+        """Queues one analysis of the events of buf (a samples.Buffer)."""
+        a0 = int(buf.ev_off[0])
+        arrs = [np.ascontiguousarray(a[a0:a0 + buf.n], dtype=np.float64) for a in buf.soa()]
+        off = np.ascontiguousarray(np.asarray(buf.ev_off, np.int64) - a0)
+        ptrs = (C.POINTER(C.c_double) * 8)(*[_f64p(a) for a in arrs])
+        rid = C.c_int64()
+        self._check(self._lib.hbt_server_submit(self.h, int(buf.nev),
+                                                off.ctypes.data_as(C.POINTER(C.c_int64)), ptrs,
+                                                int(events_per_group), int(seed),
+                                                C.byref(rid)), "hbt_server_submit")
+        return rid.value
+
+    def info(self, rid: int) -> dict:
+        i = HbtRequestInfo()
+        self._check(self._lib.hbt_server_info(self.h, int(rid), C.byref(i)), "hbt_server_info")
+        d = {k: getattr(i, k) for k, _ in HbtRequestInfo._fields_}
+        d["state"] = _lib.HBT_REQ_STATES.get(d["state"], d["state"])
+        return d
+
+    def wait(self, rid: int, timeout_s: float = -1.0) -> bool:
+        rc = self._lib.hbt_server_wait(self.h, int(rid), float(timeout_s))
+        if rc == -4:  # HBT_ERR_STATE: timed out
+            return False
+        self._check(rc, "hbt_server_wait")
+        return True
+
+    def result(self, rid: int) -> Results:
+        r, res = _alloc_results(self.params, self.nphi)
+        self._check(self._lib.hbt_server_result(self.h, int(rid), C.byref(res)),
+                    "hbt_server_result")
+        return r
+
+    def release(self, rid: int):
+        self._check(self._lib.hbt_server_release(self.h, int(rid)), "hbt_server_release")
+
+    def stats(self) -> dict:
+        st = HbtServerStats()
+        self._check(self._lib.hbt_server_get_stats(self.h, C.byref(st)), "hbt_server_get_stats")
+        return {k: getattr(st, k) for k, _ in HbtServerStats._fields_ if k != "reserved"}

@@ -219,6 +219,91 @@ int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *t);
  * the fastest (written to *best when not NULL).  Clears the histograms. */
 int hbt_autotune(hbt_ctx *ctx, hbt_tuning *best);
 
+/* ---------------- request server (continuous batching) ----------------
+ *
+ * Serves independent HBT analyses as they arrive.  The reference runs one
+ * analysis per process (Analysis::HBTAnalysis, Analysis.cpp:817-835, one
+ * HBT_correlation per run); a request here is that whole run: its events,
+ * grouped into buffers of events_per_group consecutive events (what the
+ * reference's event_buffer_size produces), each buffer processed as by
+ * calculate_HBT_correlation_function (HBT_correlation.cpp:177-218) with the
+ * mixing plan drawn from RandomUtil::Random(seed) buffer after buffer.  The
+ * server shares one configuration (q grid, K_T bins, flags, precision) among
+ * its requests.
+ *
+ * Requests are admitted, in arrival order among those that fit, when their
+ * HBM footprint (particles, plan, histograms) fits the device arena and a
+ * resident slot is free; their particle records and histograms then stay in
+ * HBM until the result is read back.  Every launch of the fast kernel
+ * carries work items of all resident requests (each with its own view of
+ * particles and histograms), a bounded number of pairs per launch, shared
+ * among them smallest-remaining-first, so requests that arrive while others
+ * run join the next launch.  Results equal those of the same analysis run
+ * alone (counts bit-identical; sums to FP64 rounding of the addition order).
+ *
+ * Calls are thread-safe.  A finished request keeps its result slot until
+ * hbt_server_release. */
+typedef struct hbt_server hbt_server;
+
+typedef struct {
+    int64_t hbm_bytes;       /* device arena for resident requests */
+    int32_t max_resident;    /* requests resident at once (admitted, not released) */
+    int32_t max_step_items;  /* work items per launch */
+    double step_pairs;       /* enumerated pairs per launch (scheduling quantum) */
+    hbt_tuning tune;         /* launch shape of the fast kernel */
+    int32_t reserved[8];
+} hbt_server_config;
+
+enum { HBT_REQ_QUEUED = 0, HBT_REQ_LOADING = 1, HBT_REQ_ACTIVE = 2, HBT_REQ_DRAINING = 3,
+       HBT_REQ_DONE = 4, HBT_REQ_FAILED = 5 };
+
+typedef struct {
+    int32_t state;            /* HBT_REQ_* */
+    int32_t nev;              /* events */
+    int64_t n_kept;           /* particles kept by the rapidity cut */
+    uint64_t enum_same;       /* enumerated same-event pairs */
+    uint64_t enum_mixed;      /* enumerated mixed pairs */
+    int64_t footprint;        /* bytes of HBM while resident */
+    double t_submit;          /* seconds since hbt_server_create */
+    double t_admit;           /* admitted (HBM allocated) */
+    double t_active;          /* particles resident, first launch possible */
+    double t_done;            /* result in host memory */
+} hbt_request_info;
+
+typedef struct {
+    int64_t submitted, completed, failed;
+    int64_t launches;         /* kernel launches */
+    int64_t items;            /* work items launched */
+    double pairs;             /* enumerated pairs launched */
+    int64_t arena_bytes, arena_used, arena_peak;
+    int32_t resident, resident_peak, queued, reserved;
+    double busy_s;            /* time with at least one launch in flight */
+} hbt_server_stats;
+
+/* cfg must select the fast kernel (either precision); sc may be NULL for
+ * defaults (hbm_bytes 16 GiB, max_resident 16, max_step_items 262144,
+ * step_pairs 2e9, tune {2, 1024, 4, 8}). */
+int hbt_server_create(const hbt_config *cfg, const hbt_server_config *sc, hbt_server **out);
+/* Stops the server: queued requests are dropped, launched work finishes. */
+int hbt_server_destroy(hbt_server *srv);
+const char *hbt_server_last_error(const hbt_server *srv);
+
+/* Copies one analysis (SoA {px,py,pz,E,x,y,z,t} of nev events, offsets
+ * nev+1) and returns at once with its id.  Fails with HBT_ERR_CAPACITY when
+ * it can never fit the arena or when the pair cap could bind. */
+int hbt_server_submit(hbt_server *srv, int32_t nev, const int64_t *ev_off,
+                      const double *const soa[8], int32_t events_per_group, int32_t seed,
+                      int64_t *req_id);
+int hbt_server_info(hbt_server *srv, int64_t req_id, hbt_request_info *info);
+/* Waits until the request is done or failed, at most timeout_s seconds
+ * (negative: no limit).  HBT_OK when done, HBT_ERR_STATE on timeout. */
+int hbt_server_wait(hbt_server *srv, int64_t req_id, double timeout_s);
+/* Histograms and counters of a finished request (fields may be NULL). */
+int hbt_server_result(hbt_server *srv, int64_t req_id, hbt_result *out);
+/* Frees a finished or failed request and its result slot. */
+int hbt_server_release(hbt_server *srv, int64_t req_id);
+int hbt_server_get_stats(hbt_server *srv, hbt_server_stats *st);
+
 /* ---------------- host-side pieces (no GPU needed) ---------------- */
 
 /* Replaces output_HBTcorrelation and the three writers it calls

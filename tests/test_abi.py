@@ -54,3 +54,25 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(RuntimeError, match="missing"):
         _lib.load()
+
+
+def test_server_create_without_gpu_or_bad_args_fails_cleanly():
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.hbt_server_create(None, None, C.byref(h)) == -1
+    cfg = _lib.HbtConfig()
+    cfg.qnpts, cfg.q_min, cfg.q_max, cfg.n_KT, cfg.KT_min, cfg.KT_max = 11, -0.1, 0.1, 3, 0.2, 0.6
+    cfg.kernel = _lib.HBT_KERNEL_PLAIN
+    assert lib.hbt_server_create(C.byref(cfg), None, C.byref(h)) == -1
+    assert b"fast kernel" in lib.hbt_server_last_error(None)
+    cfg.kernel = _lib.HBT_KERNEL_FAST
+    sc = _lib.HbtServerConfig()
+    sc.hbm_bytes, sc.max_resident, sc.max_step_items, sc.step_pairs = 1 << 30, 4, 1024, 1e9
+    sc.tune = _lib.HbtTuning(3, 1024, 4, 8, 0.0)  # R must be 1, 2 or 4
+    assert lib.hbt_server_create(C.byref(cfg), C.byref(sc), C.byref(h)) == -1
+    sc.tune = _lib.HbtTuning(2, 1024, 4, 8, 0.0)
+    rc = lib.hbt_server_create(C.byref(cfg), C.byref(sc), C.byref(h))
+    if rc == 0:  # a GPU is visible: nothing more to check here
+        lib.hbt_server_destroy(h)
+    else:
+        assert rc == -6 and h.value is None
