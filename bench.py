@@ -93,6 +93,90 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None
                    tune=tune, upload_s=upload_s)
 
 
+def serve(args, log):
+    """Serving form: a Poisson stream of independent analyses (rate 20/s),
+    N events log-uniform on [50, 5000], multiplicity Poisson(mu) with mu
+    uniform on [100, 1500], source radius sigma uniform on [2, 7] fm, groups
+    of 10 events, 10 mixed rotations per event, same q/K_T grid.  Requests
+    are generated before the clock starts, submitted at their arrival times
+    by a pool of submitter threads, and served by the request server.
+    Latency = arrival -> result in host memory."""
+    import concurrent.futures as cf
+    from hadronic_afterburner_toolkit_amd import _lib, hbt, samples
+    rng = np.random.default_rng(args.seed + 7)
+    n = args.serve_requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, n))
+    nevs = np.exp(rng.uniform(np.log(50), np.log(5000), n)).astype(int)
+    mus = rng.uniform(100, 1500, n)
+    sigmas = rng.uniform(2.0, 7.0, n)
+    log(f"serving: generating {n} requests")
+    reqs = [samples.generate(args.seed + 1000 + k, int(nevs[k]), float(mus[k]),
+                             sigma=float(sigmas[k])) for k in range(n)]
+    srv = hbt.HBTServer(PARAMS, precision=_lib.HBT_PREC_FP32C, hbm_bytes=64 << 30,
+                        max_resident=32, step_pairs=args.serve_step_pairs)
+    # warm-up request (first launch, code load), not timed
+    w = srv.submit(reqs[0].events(0, 20), 10, 1)
+    srv.wait(w, 60.0)
+    srv.release(w)
+    picked = [None] * n
+    infos = [None] * n
+    t0 = time.perf_counter()
+
+    def client(k):
+        """One client: submit at arrival, wait, take the result, release."""
+        picked[k] = time.perf_counter()
+        rid = srv.submit(reqs[k], 10, int(args.seed + k))
+        srv.wait(rid, 600.0)
+        infos[k] = srv.info(rid)
+        srv.result(rid)
+        srv.release(rid)
+
+    with cf.ThreadPoolExecutor(max_workers=64) as pool:
+        futs = []
+        for k in range(n):
+            dt = t0 + arrivals[k] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(pool.submit(client, k))
+        for f in futs:
+            f.result()
+    pairs = 0
+    t_end = 0.0
+    lat = [None] * n
+    stages = {"front": [], "host_prep": [], "admission_wait": [], "upload": [],
+              "to_first_launch": [], "launches": [], "readback": []}
+    for k in range(n):
+        info = infos[k]
+        assert info["state"] == "done", info
+        for name, a, b in (("host_prep", "t_submit", "t_queued"),
+                           ("admission_wait", "t_queued", "t_admit"),
+                           ("upload", "t_admit", "t_active"),
+                           ("to_first_launch", "t_active", "t_first_launch"),
+                           ("launches", "t_first_launch", "t_last_launch"),
+                           ("readback", "t_last_launch", "t_done")):
+            stages[name].append(1e3 * (info[b] - info[a]))
+        front = picked[k] - (t0 + arrivals[k])  # arrival -> submit call
+        stages["front"].append(1e3 * front)
+        lat[k] = front + (info["t_done"] - info["t_submit"])
+        pairs += info["enum_same"] + info["enum_mixed"]
+        t_end = max(t_end, arrivals[k] + lat[k])
+    st = srv.stats()
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    out = {"requests": n, "rate_hz": args.serve_rate, "p50_ms": float(np.percentile(lat_ms, 50)),
+           "p99_ms": float(np.percentile(lat_ms, 99)), "mean_ms": float(lat_ms.mean()),
+           "max_ms": float(lat_ms.max()), "pairs": float(pairs), "makespan_s": float(t_end),
+           "pairs_per_s": float(pairs / t_end), "busy_frac": float(st["busy_s"] / t_end),
+           "launches": int(st["launches"]), "resident_peak": int(st["resident_peak"]),
+           "arena_peak_bytes": int(st["arena_peak"]), "precision": "fp32c",
+           "step_pairs": args.serve_step_pairs,
+           "stages_ms_p50_p99": {k: [float(np.percentile(v, 50)), float(np.percentile(v, 99))]
+                                 for k, v in stages.items()}}
+    log(f"  serving: p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms "
+        f"{out['pairs_per_s']:.3e} pairs/s busy {out['busy_frac']:.2f}")
+    return out
+
+
 def cpu_baseline(group, plan_g, log):
     """The CPU oracle on group 0 (its same-event pairs and its events' mixed
     pairs with the same plan), one thread."""
@@ -154,6 +238,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--only", choices=["fp32c", "fp64", "plain"], default=None)
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-step-pairs", type=float, default=2e9)
     ap.add_argument("--tuning", default=None, metavar="R,JCHUNK,KCHUNK,BLOCKS_PER_CU",
                     help="fixed launch shape of the fast kernel (implies --no-autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -247,6 +335,8 @@ def main():
                            "frac": ach / peak, "traffic": measured_traffic(kname),
                            "kernel": kname,
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
+    if not args.no_serve:
+        out["serving"] = serve(args, log)
     if not args.no_cpu:
         g0 = groups[0]
         M = int(plan[0][0])

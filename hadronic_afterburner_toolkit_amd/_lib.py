@@ -65,8 +65,10 @@ HBT_REQ_STATES = {0: "queued", 1: "loading", 2: "active", 3: "draining", 4: "don
 class HbtRequestInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("nev", C.c_int32), ("n_kept", C.c_int64),
                 ("enum_same", C.c_uint64), ("enum_mixed", C.c_uint64),
-                ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
-                ("t_active", C.c_double), ("t_done", C.c_double)]
+                ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_queued", C.c_double),
+                ("t_admit", C.c_double), ("t_active", C.c_double),
+                ("t_first_launch", C.c_double), ("t_last_launch", C.c_double),
+                ("t_done", C.c_double)]
 
 
 class HbtServerStats(C.Structure):

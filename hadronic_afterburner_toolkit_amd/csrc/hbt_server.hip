@@ -118,7 +118,8 @@ struct Request {
     int view = -1, rslot = -1;
     View v{};
     hipEvent_t done_ev = nullptr;
-    double t_submit = 0, t_admit = 0, t_active = 0, t_done = 0;
+    double t_submit = 0, t_queued = 0, t_admit = 0, t_active = 0, t_first = 0, t_last = 0,
+           t_done = 0;
 };
 
 }  // namespace
@@ -142,7 +143,9 @@ struct hbt_server {
     size_t result_bytes = 0;
     WorkItem *d_items[2] = {};
     WorkItem *h_items[2] = {};
-    hipEvent_t step_ev[2] = {};
+    hipEvent_t step_ev[2] = {};     // end of launch (ordering)
+    hipEvent_t step_t0[2] = {};     // start of launch (timing)
+    hipEvent_t step_t1[2] = {};     // end of launch (timing)
     bool step_pending[2] = {};
     std::mutex mu;
     std::condition_variable cv;
@@ -156,8 +159,6 @@ struct hbt_server {
     int64_t next_id = 1;
     hbt_server_stats st{};
     Clock::time_point t0;
-    double busy_since = -1.0;
-    int inflight = 0;
     std::string err;
 
     double now() const {
@@ -323,7 +324,7 @@ void admission_loop(hbt_server *s) {
 // ordered by remaining pairs.  Caller holds the lock.  Requests whose last
 // items are taken move to `finishing`.
 void build_step(hbt_server *s, std::vector<WorkItem> &items, std::vector<Request *> &finishing,
-                double &pmax2, double &pairs) {
+                std::vector<Request *> &launched, double &pmax2, double &pairs) {
     std::vector<Request *> act = s->active;
     std::sort(act.begin(), act.end(), [](const Request *a, const Request *b) {
         return a->pairs_left < b->pairs_left;
@@ -348,7 +349,10 @@ void build_step(hbt_server *s, std::vector<WorkItem> &items, std::vector<Request
         }
         budget = std::max(0.0, budget - took);
         pairs += took;
-        if (any) pmax2 = std::max(pmax2, r->pmax2);
+        if (any) {
+            pmax2 = std::max(pmax2, r->pmax2);
+            launched.push_back(r);
+        }
         if (r->next == r->items.size()) finishing.push_back(r);
     }
     for (Request *r : finishing) {
@@ -357,36 +361,48 @@ void build_step(hbt_server *s, std::vector<WorkItem> &items, std::vector<Request
     }
 }
 
+// Waits for launch buffer b (if in flight) and adds its device time.
+hipError_t retire_launch(hbt_server *s, int b) {
+    if (!s->step_pending[b]) return hipSuccess;
+    hipError_t e = hipEventSynchronize(s->step_ev[b]);
+    s->step_pending[b] = false;
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, s->step_t0[b], s->step_t1[b]);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.busy_s += 1e-3 * ms;
+    return e;
+}
+
 void launcher_loop(hbt_server *s) {
     hipSetDevice(s->device);
     std::vector<WorkItem> items;
-    std::vector<Request *> finishing;
+    std::vector<Request *> finishing, launched;
     int step = 0;
     for (;;) {
         items.clear();
         finishing.clear();
+        launched.clear();
         double pmax2 = 0.0, pairs = 0.0;
         {
             std::unique_lock<std::mutex> lk(s->mu);
+            if (s->active.empty() && !s->stop && (s->step_pending[0] || s->step_pending[1])) {
+                // idle: account the launches still in flight first
+                lk.unlock();
+                retire_launch(s, 0);
+                retire_launch(s, 1);
+                lk.lock();
+            }
             s->cv.wait(lk, [&] { return s->stop || !s->active.empty(); });
             if (s->stop) break;
-            build_step(s, items, finishing, pmax2, pairs);
+            build_step(s, items, finishing, launched, pmax2, pairs);
         }
         const int buf = step & 1;
-        hipError_t e = hipSuccess;
-        if (s->step_pending[buf]) {
-            e = hipEventSynchronize(s->step_ev[buf]);
-            s->step_pending[buf] = false;
-            std::lock_guard<std::mutex> lk(s->mu);
-            if (--s->inflight == 0 && s->busy_since >= 0) {
-                s->st.busy_s += s->now() - s->busy_since;
-                s->busy_since = -1.0;
-            }
-        }
+        hipError_t e = retire_launch(s, buf);
         if (!items.empty() && e == hipSuccess) {
             std::memcpy(s->h_items[buf], items.data(), items.size() * sizeof(WorkItem));
             e = hipMemcpyAsync(s->d_items[buf], s->h_items[buf], items.size() * sizeof(WorkItem),
                                hipMemcpyHostToDevice, s->s_compute);
+            if (e == hipSuccess) e = hipEventRecord(s->step_t0[buf], s->s_compute);
             FastArgs a;
             a.P = s->D.P;
             if (s->f32) fp32_margins(s->D, pmax2, a.P);
@@ -400,14 +416,19 @@ void launcher_loop(hbt_server *s) {
                                    ((long)items.size() + 3) / 4));
             if (e == hipSuccess)
                 e = launch_fast_kernel(a, s->f32, s->sc.tune.R, s->ext, blocks, s->s_compute);
+            if (e == hipSuccess) e = hipEventRecord(s->step_t1[buf], s->s_compute);
             if (e == hipSuccess) e = hipEventRecord(s->step_ev[buf], s->s_compute);
             if (e == hipSuccess) {
                 s->step_pending[buf] = true;
+                const double tl = s->now();
                 std::lock_guard<std::mutex> lk(s->mu);
-                if (s->inflight++ == 0) s->busy_since = s->now();
                 s->st.launches++;
                 s->st.items += (int64_t)items.size();
                 s->st.pairs += pairs;
+                for (Request *r : launched) {
+                    if (r->t_first == 0) r->t_first = tl;
+                    r->t_last = tl;
+                }
             }
             step++;
         }
@@ -436,8 +457,7 @@ void launcher_loop(hbt_server *s) {
         }
     }
     // drain what is in flight before the streams go away
-    for (int b = 0; b < 2; b++)
-        if (s->step_pending[b]) hipEventSynchronize(s->step_ev[b]);
+    for (int b = 0; b < 2; b++) retire_launch(s, b);
     std::lock_guard<std::mutex> lk(s->mu);
     s->launcher_done = true;
     s->cv.notify_all();
@@ -488,6 +508,8 @@ void destroy_resources(hbt_server *s) {
         if (s->h_items[b]) hipHostFree(s->h_items[b]);
         if (s->d_items[b]) hipFree(s->d_items[b]);
         if (s->step_ev[b]) hipEventDestroy(s->step_ev[b]);
+        if (s->step_t0[b]) hipEventDestroy(s->step_t0[b]);
+        if (s->step_t1[b]) hipEventDestroy(s->step_t1[b]);
     }
     if (s->d_views) hipFree(s->d_views);
     if (s->d_misc) hipFree(s->d_misc);
@@ -570,7 +592,9 @@ int hbt_server_create(const hbt_config *cfg, const hbt_server_config *sc_in, hbt
                   "hipMalloc") &&
              !bad(hipHostMalloc((void **)&s->h_items[b], sizeof(WorkItem) * sc.max_step_items),
                   "hipHostMalloc") &&
-             !bad(hipEventCreateWithFlags(&s->step_ev[b], hipEventDisableTiming), "event");
+             !bad(hipEventCreateWithFlags(&s->step_ev[b], hipEventDisableTiming), "event") &&
+             !bad(hipEventCreate(&s->step_t0[b]), "event") &&
+             !bad(hipEventCreate(&s->step_t1[b]), "event");
     }
     s->h_results.assign(sc.max_resident, nullptr);
     for (int k = 0; k < sc.max_resident && ok; k++)
@@ -693,6 +717,7 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
         return HBT_ERR_STATE;
     }
     r->id = s->next_id++;
+    r->t_queued = s->now();
     *req_id = r->id;
     s->queue.push_back(r.get());
     s->reqs[r->id] = std::move(r);
@@ -713,8 +738,11 @@ int hbt_server_info(hbt_server *s, int64_t id, hbt_request_info *info) {
     info->enum_mixed = r->enum_mixed;
     info->footprint = (int64_t)(r->foot ? r->foot : layout(s, *r));
     info->t_submit = r->t_submit;
+    info->t_queued = r->t_queued;
     info->t_admit = r->t_admit;
     info->t_active = r->t_active;
+    info->t_first_launch = r->t_first;
+    info->t_last_launch = r->t_last;
     info->t_done = r->t_done;
     return HBT_OK;
 }
@@ -782,7 +810,6 @@ int hbt_server_get_stats(hbt_server *s, hbt_server_stats *out) {
     out->arena_used = (int64_t)s->arena.used();
     out->arena_peak = (int64_t)s->arena.peak();
     out->queued = (int32_t)s->queue.size();
-    if (s->busy_since >= 0) out->busy_s += s->now() - s->busy_since;
     return HBT_OK;
 }
 

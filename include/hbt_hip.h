@@ -264,9 +264,12 @@ typedef struct {
     uint64_t enum_same;       /* enumerated same-event pairs */
     uint64_t enum_mixed;      /* enumerated mixed pairs */
     int64_t footprint;        /* bytes of HBM while resident */
-    double t_submit;          /* seconds since hbt_server_create */
+    double t_submit;          /* seconds since hbt_server_create: submit called */
+    double t_queued;          /* host preparation done, queued for admission */
     double t_admit;           /* admitted (HBM allocated) */
     double t_active;          /* particles resident, first launch possible */
+    double t_first_launch;    /* first launch carrying its work */
+    double t_last_launch;     /* launch carrying its last work items */
     double t_done;            /* result in host memory */
 } hbt_request_info;
 
@@ -277,7 +280,7 @@ typedef struct {
     double pairs;             /* enumerated pairs launched */
     int64_t arena_bytes, arena_used, arena_peak;
     int32_t resident, resident_peak, queued, reserved;
-    double busy_s;            /* time with at least one launch in flight */
+    double busy_s;            /* summed device time of the launches (HIP events) */
 } hbt_server_stats;
 
 /* cfg must select the fast kernel (either precision); sc may be NULL for
