@@ -34,6 +34,7 @@
 
 #include "../../include/hbt_hip.h"
 #include "hbt_device.h"
+#include "hbt_cap.h"
 #include "hbt_fast.h"
 #include "hbt_internal.h"
 
@@ -148,6 +149,94 @@ __global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, De
     wave_add_stage(&H.stage[9], c_bin);
 }
 
+// Pair-cap passes (hbt_cap.h) with the plain kernels' tiling: COUNT adds
+// arrivals per (key, row), otherwise admitted pairs are binned.  Rows are
+// particle indices of the same-event list; same-event positions are j.
+template <bool COUNT>
+__global__ __launch_bounds__(TILE) void k_same_cap(DevParams P, DevHist H, DevSoA A, CapView C,
+                                                   long long g0, int n, double psi) {
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj < ti) return;
+    __shared__ double s[8][TILE];
+    const int j0 = tj * TILE;
+    const int jn = min(TILE, n - j0);
+    const int tid = threadIdx.x;
+    if (tid < jn) {
+        const long long j = g0 + j0 + tid;
+        s[0][tid] = A.px[j];
+        s[1][tid] = A.py[j];
+        s[2][tid] = A.pz[j];
+        s[3][tid] = A.E[j];
+        s[4][tid] = A.x[j];
+        s[5][tid] = A.y[j];
+        s[6][tid] = A.z[j];
+        s[7][tid] = A.t[j];
+    }
+    __syncthreads();
+    const int i = ti * TILE + tid;
+    unsigned c_bin = 0;
+    if (i < n) {
+        const long long gi = g0 + i;
+        const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
+        const double x1 = A.x[gi], y1 = A.y[gi], z1 = A.z[gi], t1 = A.t[gi];
+        const int jstart = (ti == tj) ? tid + 1 : 0;
+        for (int jj = jstart; jj < jn; jj++) {
+            const int st = cap_same_pair<COUNT>(P, H, C, psi, gi, g0 + j0 + jj, p1x, p1y, p1z,
+                                                p1E, x1, y1, z1, t1, s[0][jj], s[1][jj],
+                                                s[2][jj], s[3][jj], s[4][jj], s[5][jj],
+                                                s[6][jj], s[7][jj]);
+            c_bin += (st >> 3) & 1;
+        }
+    }
+    if (!COUNT) wave_add_stage(&H.stage[4], c_bin);
+}
+
+// Mixed positions: (plan row k of the event) << 32 | j in the partner.
+template <bool COUNT>
+__global__ __launch_bounds__(TILE) void k_mixed_cap(DevParams P, DevHist H, DevSoA A, DevSoA B,
+                                                    CapView C, const long long *a_off,
+                                                    const long long *b_off, const int *ids,
+                                                    const double *cs, int M, int e0,
+                                                    long long row0, double psi) {
+    const long long row = row0 + blockIdx.z;
+    const long long k = blockIdx.z % M;
+    const int e = e0 + (int)(blockIdx.z / M);
+    const long long ib = a_off[e];
+    const int n1 = (int)(a_off[e + 1] - ib);
+    const int pid = ids[row];
+    const long long jb = b_off[pid];
+    const int n2 = (int)(b_off[pid + 1] - jb);
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (ti * TILE >= n1 || tj * TILE >= n2) return;
+    const double c = cs[2 * row], sn = cs[2 * row + 1];
+    __shared__ double s[4][TILE];
+    const int j0 = tj * TILE;
+    const int jn = min(TILE, n2 - j0);
+    const int tid = threadIdx.x;
+    if (tid < jn) {
+        const long long j = jb + j0 + tid;
+        const double px = B.px[j], py = B.py[j];
+        s[0][tid] = px * c - py * sn;
+        s[1][tid] = px * sn + py * c;
+        s[2][tid] = B.pz[j];
+        s[3][tid] = B.E[j];
+    }
+    __syncthreads();
+    const int i = ti * TILE + tid;
+    unsigned c_bin = 0;
+    if (i < n1) {
+        const long long gi = ib + i;
+        const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
+        for (int jj = 0; jj < jn; jj++) {
+            const int st = cap_mixed_pair<COUNT>(P, H, C, psi, gi, (k << 32) | (j0 + jj), p1x,
+                                                 p1y, p1z, p1E, s[0][jj], s[1][jj], s[2][jj],
+                                                 s[3][jj]);
+            c_bin += (st >> 3) & 1;
+        }
+    }
+    if (!COUNT) wave_add_stage(&H.stage[9], c_bin);
+}
+
 __global__ void k_zero_int(int *p) { *p = 0; }
 
 __global__ void k_zero_words(unsigned long long *p, size_t n) {
@@ -231,6 +320,14 @@ struct hbt_ctx {
     int nitems = 0, nitems_same = 0;
     int *d_counter = nullptr;
     View *d_view = nullptr;  // the context's one analysis view
+    // ordered pair-cap path (allocated when a cap binds)
+    unsigned *d_cnt = nullptr;  // [nkey3 + n_KT][nrows] arrivals
+    size_t cnt_cap = 0;
+    CapCut *d_cut = nullptr;    // [nkey3 + n_KT]
+    unsigned long long *d_lim = nullptr;  // [nkey3 + n_KT] remaining capacity
+    size_t cut_cap = 0, lim_cap = 0;
+    long long *d_gpoff = nullptr;  // group particle offsets of slot 0
+    size_t gpoff_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool batch_ready = false;
@@ -360,9 +457,12 @@ int group_sums(hbt_ctx *c, const unsigned long long *h, long per, int groups, in
 
 // The pair cap (HBT_correlation.cpp:402-406, :651-655, q_inv :340-341,
 // :600-603) admits pairs in enumeration order while a counter is at most
-// needed_number_of_pairs.  The kernels bin without ordering, which is exact
-// as long as no counter can pass the cap inside this call.
-int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
+// needed_number_of_pairs.  The fast and plain kernels bin without ordering,
+// which is exact as long as no counter can pass the cap inside this call;
+// binds is set when one might, and the call then runs the ordered path
+// (run_capped).
+int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call, bool &binds) {
+    binds = false;
     const uint64_t total = (mixed ? c->enum_mixed_total : c->enum_same_total) + enum_call;
     const bool qinv = c->cfg.invariant_radius_flag == 1;
     if (total <= c->needed + 1 && (!qinv || total <= c->needed_qinv)) return HBT_OK;
@@ -374,18 +474,14 @@ int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call) {
                         cnt);
     if (rc) return rc;
     for (auto v : cnt)
-        if (v + enum_call > c->needed + 1)
-            return fail(c, HBT_ERR_CAPACITY,
-                        "needed_number_of_pairs may bind inside this call (order-dependent "
-                        "pair cap); raise needed_number_of_pairs or split the buffer");
+        if (v + enum_call > c->needed + 1) binds = true;
     if (qinv) {
         std::vector<unsigned long long> qc;
         rc = group_sums(c, mixed ? c->d_qinv_den : c->d_qinv_count, c->P.qnpts, c->cfg.n_KT, 1,
                         qc);
         if (rc) return rc;
         for (auto v : qc)
-            if (v + enum_call > c->needed_qinv)
-                return fail(c, HBT_ERR_CAPACITY, "q_inv pair cap may bind inside this call");
+            if (v + enum_call > c->needed_qinv) binds = true;
     }
     return HBT_OK;
 }
@@ -579,6 +675,111 @@ int run_plain(hbt_ctx *c, bool same, bool mixed) {
                 HIPCHECK(c, hipGetLastError());
             }
         }
+    }
+    return HBT_OK;
+}
+
+// The ordered path for one side (same-event or mixed) of the current call:
+// arrivals per (key, row), the cut per key, then binning of the admitted
+// pairs (hbt_cap.h).  FP64 with the literal cascade in every precision.
+int run_capped(hbt_ctx *c, bool mixed) {
+    const Slot &s = c->slot[0];
+    const Slot &m = mixed_slot(c);
+    const long long nrows = std::max(1ll, s.n);
+    const int nkey3 = c->cfg.n_KT * c->P.nphi, nkq = c->cfg.n_KT, nkeys = nkey3 + nkq;
+    int rc = ensure(c, &c->d_cnt, c->cnt_cap, (size_t)nkeys * nrows);
+    if (!rc) rc = ensure(c, &c->d_cut, c->cut_cap, (size_t)nkeys);
+    if (!rc) rc = ensure(c, &c->d_lim, c->lim_cap, (size_t)nkeys);
+    if (!rc) rc = ensure(c, &c->d_gpoff, c->gpoff_cap, (size_t)s.ngroups + 1);
+    if (rc) return rc;
+    HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, (size_t)nkeys * nrows * 4, c->stream));
+    // remaining capacity per key from the counters so far (sums of the bins)
+    std::vector<unsigned long long> c3, cq(nkq, 0);
+    const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
+    rc = group_sums(c, (const unsigned long long *)(c->d_bins + (mixed ? BIN_DEN : BIN_COUNT)),
+                    per, nkey3, BIN_STRIDE, c3);
+    if (!rc && c->P.qinv)
+        rc = group_sums(c, mixed ? c->d_qinv_den : c->d_qinv_count, c->P.qnpts, nkq, 1, cq);
+    if (rc) return rc;
+    std::vector<unsigned long long> lim(nkeys);
+    for (int k = 0; k < nkey3; k++) lim[k] = c3[k] > c->needed ? 0 : c->needed + 1 - c3[k];
+    for (int k = 0; k < nkq; k++)
+        lim[nkey3 + k] = cq[k] >= c->needed_qinv ? 0 : c->needed_qinv - cq[k];
+    std::vector<long long> gpoff(s.ngroups + 1);
+    for (int g = 0; g <= s.ngroups; g++) gpoff[g] = s.off[s.grp_ev[g]];
+    HIPCHECK(c, hipMemcpy(c->d_lim, lim.data(), nkeys * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_gpoff, gpoff.data(), gpoff.size() * 8, hipMemcpyHostToDevice));
+    CapView C;
+    C.cnt3 = c->d_cnt;
+    C.cntq = c->d_cnt + (size_t)nkey3 * nrows;
+    C.cut3 = c->d_cut;
+    C.cutq = c->d_cut + nkey3;
+    C.nrows = nrows;
+    const DevHist H = hist_view(c);
+    const DevSoA A = soa_view(s), B = soa_view(m);
+    for (int pass = 0; pass < 2; pass++) {
+        const bool count = pass == 0;
+        if (!mixed) {
+            for (int g = 0; g < s.ngroups; g++) {
+                const long long g0 = gpoff[g], n = gpoff[g + 1] - g0;
+                if (n < 2) continue;
+                const int T = (int)((n + TILE - 1) / TILE);
+                if (T > 65535) return fail(c, HBT_ERR_CAPACITY, "group too large for the cap path");
+                if (count)
+                    hipLaunchKernelGGL(k_same_cap<true>, dim3(T, T), dim3(TILE), 0, c->stream,
+                                       c->P, H, A, C, g0, (int)n, c->psi[g]);
+                else
+                    hipLaunchKernelGGL(k_same_cap<false>, dim3(T, T), dim3(TILE), 0, c->stream,
+                                       c->P, H, A, C, g0, (int)n, c->psi[g]);
+                HIPCHECK(c, hipGetLastError());
+            }
+        } else {
+            long long max2 = 0;
+            for (int e = 0; e < m.nev; e++) max2 = std::max(max2, m.off[e + 1] - m.off[e]);
+            const int Tj = (int)((max2 + TILE - 1) / TILE);
+            for (int g = 0; g < s.ngroups && Tj > 0; g++) {
+                const int e0 = s.grp_ev[g], e1 = s.grp_ev[g + 1];
+                if (e1 == e0) continue;
+                const int M = (int)(c->rowoff[e0 + 1] - c->rowoff[e0]);
+                long long max1 = 0;
+                for (int e = e0; e < e1; e++) max1 = std::max(max1, s.off[e + 1] - s.off[e]);
+                const int Ti = (int)((max1 + TILE - 1) / TILE);
+                if (Ti == 0 || M == 0) continue;
+                const int perl = std::max(1, 65535 / M);
+                for (int eb = e0; eb < e1; eb += perl) {
+                    const int ne = std::min(perl, e1 - eb);
+                    if (count)
+                        hipLaunchKernelGGL(k_mixed_cap<true>, dim3(Tj, Ti, ne * M), dim3(TILE), 0,
+                                           c->stream, c->P, H, A, B, C, s.d_off, m.d_off,
+                                           c->d_ids, c->d_cs, M, eb, c->rowoff[eb], c->psi[g]);
+                    else
+                        hipLaunchKernelGGL(k_mixed_cap<false>, dim3(Tj, Ti, ne * M), dim3(TILE),
+                                           0, c->stream, c->P, H, A, B, C, s.d_off, m.d_off,
+                                           c->d_ids, c->d_cs, M, eb, c->rowoff[eb], c->psi[g]);
+                    HIPCHECK(c, hipGetLastError());
+                }
+            }
+        }
+        if (!count) break;
+        // cut row per key, then the position inside it
+        hipLaunchKernelGGL(k_cap_scan, dim3(nkeys), dim3(1024), 0, c->stream, c->d_cnt, nrows,
+                           c->d_lim, c->d_cut);
+        HIPCHECK(c, hipGetLastError());
+        const double *psi = c->d_psi;
+        if (!mixed) {
+            hipLaunchKernelGGL(k_cap_pos_same, dim3((nkey3 + 63) / 64), dim3(64), 0, c->stream,
+                               c->P, A, c->d_gpoff, s.ngroups, psi, c->d_cut, nkey3, false);
+            hipLaunchKernelGGL(k_cap_pos_same, dim3((nkq + 63) / 64), dim3(64), 0, c->stream,
+                               c->P, A, c->d_gpoff, s.ngroups, psi, c->d_cut + nkey3, nkq, true);
+        } else {
+            hipLaunchKernelGGL(k_cap_pos_mixed, dim3((nkey3 + 63) / 64), dim3(64), 0, c->stream,
+                               c->P, A, B, s.d_off, s.nev, m.d_off, c->d_rowoff, c->d_ids,
+                               c->d_cs, c->d_gpoff, s.ngroups, psi, c->d_cut, nkey3, false);
+            hipLaunchKernelGGL(k_cap_pos_mixed, dim3((nkq + 63) / 64), dim3(64), 0, c->stream,
+                               c->P, A, B, s.d_off, s.nev, m.d_off, c->d_rowoff, c->d_ids,
+                               c->d_cs, c->d_gpoff, s.ngroups, psi, c->d_cut + nkey3, nkq, true);
+        }
+        HIPCHECK(c, hipGetLastError());
     }
     return HBT_OK;
 }
@@ -786,7 +987,8 @@ int hbt_destroy(hbt_ctx *c) {
     void *ptrs[] = {c->d_bins,      c->d_qinv_count, c->d_qinv_den,
                     c->d_qinv_sum,  c->d_qinv_num,  c->d_stage,      c->d_binned,
                     c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
-                    c->d_items,     c->d_counter,   c->d_psi,        c->d_view};
+                    c->d_items,     c->d_counter,   c->d_psi,        c->d_view,
+                    c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -838,13 +1040,14 @@ int hbt_run_same(hbt_ctx *c) {
     std::vector<WorkItem> items;
     uint64_t es = 0;
     build_items(c, true, false, items, es);
-    rc = check_cap(c, false, es);
+    bool binds = false;
+    rc = check_cap(c, false, es, binds);
     if (rc) return rc;
     rc = upload_psi(c);
     if (rc) return rc;
-    if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
+    if (c->cfg.kernel == HBT_KERNEL_PLAIN || binds) {
         HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-        rc = run_plain(c, true, false);
+        rc = binds ? run_capped(c, false) : run_plain(c, true, false);
         if (rc) return rc;
         HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
         HIPCHECK(c, hipEventSynchronize(c->ev[1]));
@@ -872,13 +1075,14 @@ int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const doubl
     c->batch_ready = false;
     rc = upload_plan(c, Me, partner_ids, cos_sin, em);
     if (rc) return rc;
-    rc = check_cap(c, true, em);
+    bool binds = false;
+    rc = check_cap(c, true, em, binds);
     if (rc) return rc;
     rc = upload_psi(c);
     if (rc) return rc;
-    if (c->cfg.kernel == HBT_KERNEL_PLAIN) {
+    if (c->cfg.kernel == HBT_KERNEL_PLAIN || binds) {
         HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-        rc = run_plain(c, false, true);
+        rc = binds ? run_capped(c, true) : run_plain(c, false, true);
         if (rc) return rc;
         HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
         HIPCHECK(c, hipEventSynchronize(c->ev[3]));
@@ -920,11 +1124,16 @@ int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *par
 int hbt_launch_batch(hbt_ctx *c) {
     if (!c) return HBT_ERR_ARG;
     if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
-    int rc = check_cap(c, false, c->batch_enum_same);
-    if (!rc) rc = check_cap(c, true, c->batch_enum_mixed);
+    bool bs = false, bm = false;
+    int rc = check_cap(c, false, c->batch_enum_same, bs);
+    if (!rc) rc = check_cap(c, true, c->batch_enum_mixed, bm);
     if (rc) return rc;
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-    if (c->cfg.kernel == HBT_KERNEL_FAST) {
+    if (bs || bm) {  // a cap may bind: each side in enumeration order
+        rc = bs ? run_capped(c, false) : run_plain(c, true, false);
+        if (!rc) rc = bm ? run_capped(c, true) : (c->plan_valid ? run_plain(c, false, true) : 0);
+        if (rc) return rc;
+    } else if (c->cfg.kernel == HBT_KERNEL_FAST) {
         if (c->nitems > 0) HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
     } else {
         rc = run_plain(c, true, true);

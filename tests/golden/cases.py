@@ -20,7 +20,8 @@ FIXTURE_GZ = os.path.join(HERE, "ref_fixture", "particle_samples.gz")
 
 CASE_NAMES = ["fixture_3d", "fixture_q11", "fixture_kphi_qinv", "fixture_budget700",
               "fixture_qinv", "synth_cfg0",
-              "synth_noboost", "synth_cap", "synth_mixedfile", "synth_budget_exact"]
+              "synth_noboost", "synth_cap", "synth_mixedfile", "synth_budget_exact",
+              "fixture_cap_kphi_qinv", "synth_cap_qinv"]
 
 HBT_KEYS = ("qnpts", "q_min", "q_max", "n_KT", "KT_min", "KT_max", "n_Kphi",
             "azimuthal_flag", "invariant_radius_flag", "long_comoving_boost", "HBTrap_min",

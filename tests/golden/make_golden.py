@@ -57,6 +57,15 @@ CASES = {
     "synth_cap": dict(input=("synth", 1003, 16, 100.0), params=dict(
         randomSeed=13, qnpts=21, n_KT=4, KT_min=0.2, KT_max=0.8, HBTrap_min=-1.0,
         HBTrap_max=1.0, needed_number_of_pairs=40, event_buffer_size=500)),
+    # binding caps with K_phi counters (per K_T x K_phi bin) and the q_inv cap
+    # (50 x needed per K_T bin)
+    "fixture_cap_kphi_qinv": dict(input="fixture", params=dict(
+        randomSeed=23, qnpts=11, n_KT=3, KT_min=0.2, KT_max=0.6, azimuthal_flag=1, n_Kphi=4,
+        invariant_radius_flag=1, needed_number_of_pairs=6)),
+    "synth_cap_qinv": dict(input=("synth", 1007, 12, 100.0), params=dict(
+        randomSeed=29, qnpts=15, n_KT=3, KT_min=0.2, KT_max=0.6, HBTrap_min=-1.0,
+        HBTrap_max=1.0, invariant_radius_flag=1, needed_number_of_pairs=5,
+        event_buffer_size=400)),
     # separate mixed-event file (read_in_real_mixed_events=1)
     "synth_mixedfile": dict(input=("synth2", 1004, 1005, 10, 120.0), params=dict(
         randomSeed=17, qnpts=21, n_KT=4, KT_min=0.2, KT_max=0.8, HBTrap_min=-1.0,

@@ -35,7 +35,7 @@ def write_parameters(path, params):
 
 
 @pytest.mark.parametrize("name", ["fixture_3d", "fixture_kphi_qinv", "synth_cfg0",
-                                  "synth_mixedfile", "synth_budget_exact"])
+                                  "synth_mixedfile", "synth_budget_exact", "synth_cap_qinv"])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_cli_matches_reference_files(name, precision, tmp_path):
     spec, ref, outs, dig = cases.load(name)
