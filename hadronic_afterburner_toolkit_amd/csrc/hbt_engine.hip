@@ -267,6 +267,15 @@ __global__ void k_sum_groups(const unsigned long long *h, long per, int stride,
 
 // ------------------------------------------------------------------ host
 
+// Fast-kernel records of one ordering of a slot (RecordSet on the device).
+struct DevRecords {
+    PartRec *rec = nullptr;
+    PfRec64 *pf64 = nullptr;
+    PfRec32 *pf32 = nullptr;
+    double *cw64 = nullptr;
+    float *cw32 = nullptr;
+};
+
 struct Slot {
     int ngroups = 0, nev = 0;
     long long n = 0;        // kept particles
@@ -275,14 +284,12 @@ struct Slot {
     std::vector<long long> off;    // nev+1, kept-particle offsets
     std::vector<double> psi2;      // psi_ref (n=2) per group, over all particles passed in
     double *d_soa[8] = {};   // plain kernels (structure of arrays)
-    PartRec *d_rec = nullptr;  // fast kernel (64-byte records)
     long long *d_off = nullptr;
-    PfRec64 *d_pf64 = nullptr;
-    PfRec32 *d_pf32 = nullptr;
-    double *d_cw64 = nullptr;
-    float *d_cw32 = nullptr;
+    DevRecords ev, srt;        // fast kernel: gather order, px-sorted per group
+    int *d_orig = nullptr;     // srt position -> gather-order index
     size_t cap = 0, cap_ev = 0;
     double pmax2 = 0.0;     // max px^2+py^2 of kept particles
+    std::vector<double> srt_px;  // px of the px-sorted list (work-item windows)
     bool valid = false;
 };
 
@@ -385,11 +392,12 @@ void free_slot(Slot &s) {
         p = nullptr;
     }
     if (s.d_off) hipFree(s.d_off);
-    if (s.d_rec) hipFree(s.d_rec);
-    if (s.d_pf64) hipFree(s.d_pf64);
-    if (s.d_pf32) hipFree(s.d_pf32);
-    if (s.d_cw64) hipFree(s.d_cw64);
-    if (s.d_cw32) hipFree(s.d_cw32);
+    for (DevRecords *r : {&s.ev, &s.srt}) {
+        void *ptrs[] = {r->rec, r->pf64, r->pf32, r->cw64, r->cw32};
+        for (void *q : ptrs)
+            if (q) hipFree(q);
+    }
+    if (s.d_orig) hipFree(s.d_orig);
     s = Slot();
 }
 
@@ -514,13 +522,23 @@ View make_view(hbt_ctx *c) {
     v.H = hist_view(c);
     const Slot &s0 = c->slot[0];
     const Slot &s1 = mixed_slot(c);
-    v.A = s0.d_rec;
-    v.B = s1.d_rec;
+    v.A = s0.ev.rec;
+    v.B = s1.ev.rec;
+    v.S = s0.srt.rec;
     const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
-    v.pfA = f32 ? (const void *)s0.d_pf32 : (const void *)s0.d_pf64;
-    v.pfB = f32 ? (const void *)s1.d_pf32 : (const void *)s1.d_pf64;
-    v.cwA = f32 ? (const void *)s0.d_cw32 : (const void *)s0.d_cw64;
-    v.cwB = f32 ? (const void *)s1.d_cw32 : (const void *)s1.d_cw64;
+    auto pf = [&](const DevRecords &r) {
+        return f32 ? (const void *)r.pf32 : (const void *)r.pf64;
+    };
+    auto cw = [&](const DevRecords &r) {
+        return f32 ? (const void *)r.cw32 : (const void *)r.cw64;
+    };
+    v.pfA = pf(s0.ev);
+    v.pfB = pf(s1.ev);
+    v.pfS = pf(s0.srt);
+    v.cwA = cw(s0.ev);
+    v.cwB = cw(s1.ev);
+    v.cwS = cw(s0.srt);
+    v.origS = s0.d_orig;
     v.mix_off = s1.d_off;
     v.rowoff = c->d_rowoff;
     v.plan_ids = c->d_ids;
@@ -563,7 +581,7 @@ void build_items(hbt_ctx *c, bool same, bool mixed, std::vector<WorkItem> &items
     const ItemShape sh = item_shape(c->tune);
     items.clear();
     enum_same = 0;
-    if (same) items_same(sh, s.off, s.grp_ev, 0, items, nullptr, enum_same);
+    if (same) items_same(sh, s.off, s.grp_ev, s.srt_px, px_window(c->D), 0, items, nullptr, enum_same);
     c->nitems_same = (int)items.size();
     if (mixed && c->plan_valid)
         items_mixed(sh, s.off, s.grp_ev, c->rowoff, c->plan_gids, mixed_slot(c).off, 0, items,
@@ -829,11 +847,14 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
             rc = dev_alloc(c, &p, (size_t)n);
             if (rc) return rc;
         }
-        rc = dev_alloc(c, &s.d_pf64, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_rec, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_pf32, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_cw64, (size_t)n);
-        if (!rc) rc = dev_alloc(c, &s.d_cw32, (size_t)n);
+        for (DevRecords *r : {&s.ev, &s.srt}) {
+            if (!rc) rc = dev_alloc(c, &r->rec, (size_t)n);
+            if (!rc) rc = dev_alloc(c, &r->pf64, (size_t)n);
+            if (!rc) rc = dev_alloc(c, &r->pf32, (size_t)n);
+            if (!rc) rc = dev_alloc(c, &r->cw64, (size_t)n);
+            if (!rc) rc = dev_alloc(c, &r->cw32, (size_t)n);
+        }
+        if (!rc) rc = dev_alloc(c, &s.d_orig, (size_t)n);
         if (rc) return rc;
         s.cap = (size_t)n;
     }
@@ -848,12 +869,20 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
     }
     if (n > 0) {
-        HIPCHECK(c, hipMemcpy(s.d_rec, h.rec.data(), n * sizeof(PartRec), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_pf64, h.pf64.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_pf32, h.pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_cw64, h.cw64.data(), n * 8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(s.d_cw32, h.cw32.data(), n * 4, hipMemcpyHostToDevice));
+        const std::pair<DevRecords *, const RecordSet *> sets[] = {{&s.ev, &h.ev},
+                                                                   {&s.srt, &h.srt}};
+        for (const auto &x : sets) {
+            DevRecords &d = *x.first;
+            const RecordSet &r = *x.second;
+            HIPCHECK(c, hipMemcpy(d.rec, r.rec.data(), n * sizeof(PartRec), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.pf64, r.pf64.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.pf32, r.pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), n * 8, hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.cw32, r.cw32.data(), n * 4, hipMemcpyHostToDevice));
+        }
+        HIPCHECK(c, hipMemcpy(s.d_orig, h.orig.data(), n * 4, hipMemcpyHostToDevice));
     }
+    s.srt_px = std::move(h.srt_px);
     HIPCHECK(c, hipMemcpy(s.d_off, h.off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
     s.ngroups = ngroups;
     s.grp_ev.assign(grp_ev, grp_ev + ngroups + 1);

@@ -53,12 +53,16 @@ struct WorkItem {
 // them); every index in a WorkItem is relative to its own View.
 struct View {
     DevHist H;
-    const PartRec *A;    // same-event list (cut), 64 B per particle
+    const PartRec *A;    // same-event list (cut) in gather order, 64 B per particle
     const PartRec *B;    // mixed list (separate, or an alias of A)
+    const PartRec *S;    // same-event list sorted by px inside each group
     const void *pfA;     // PfRec64 (FP64 kernel) or PfRec32 (precision form) of A
     const void *pfB;     // of B
+    const void *pfS;     // of S
     const void *cwA;     // per-particle c/E bound of A, double or float like pfA
     const void *cwB;
+    const void *cwS;
+    const int *origS;    // S position -> gather-order index (pair orientation)
     const long long *mix_off;  // event offsets of the mixed list (cut)
     const long long *rowoff;   // plan rows of event e: [rowoff[e], rowoff[e+1])
     const int *plan_ids;       // partner event (index in the mixed list)
@@ -127,9 +131,16 @@ struct Exact;
 
 template <bool EXT>
 struct Exact<double, EXT> {
+    // i, j are positions in the px-sorted list; the pair is evaluated in the
+    // reference's orientation (particle 1 = the earlier in gather order).
     static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
                                                long long i, long long j, SameRec &rec) {
-        const PartRec p = v.A[i], r = v.A[j];
+        if (v.origS[i] > v.origS[j]) {
+            const long long t = i;
+            i = j;
+            j = t;
+        }
+        const PartRec p = v.S[i], r = v.S[j];
         return eval_same_f64<EXT>(P, v.H, psi, p.px, p.py, p.pz, p.E, p.x, p.y, p.z, p.t, r.px,
                                   r.py, r.pz, r.E, r.x, r.y, r.z, r.t, rec);
     }
@@ -148,7 +159,12 @@ template <bool EXT>
 struct Exact<float, EXT> {
     static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
                                                long long i, long long j, SameRec &rec) {
-        return eval_same_fast<EXT>(P, v.H, psi, v.A[i], v.A[j], rec);
+        if (v.origS[i] > v.origS[j]) {
+            const long long t = i;
+            i = j;
+            j = t;
+        }
+        return eval_same_fast<EXT>(P, v.H, psi, v.S[i], v.S[j], rec);
     }
     static __device__ __forceinline__ int mixed(const DevParams &P, const View &v, double psi,
                                                 long long i, long long j, double c, double s,
@@ -281,14 +297,14 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         if (it >= a.nitems) break;
         const WorkItem w = a.items[it];
         const View v = HBT_LOAD_VIEW(a.views, w.view);
-        const Rec *__restrict__ pfA = (const Rec *)v.pfA;
-        const Rec *__restrict__ pfB = (const Rec *)v.pfB;
-        const Prec *__restrict__ cwA = (const Prec *)v.cwA;
-        const Prec *__restrict__ cwB = (const Prec *)v.cwB;
         const double psi = v.psi[w.grp];
         const bool mixed = w.kind != 0;
-        const Rec *__restrict__ pf = mixed ? pfB : pfA;
-        const Prec *__restrict__ cwj = mixed ? cwB : cwA;
+        // same-event tiles run over the px-sorted list, mixed ones over the
+        // gather-order lists
+        const Rec *__restrict__ pfi = (const Rec *)(mixed ? v.pfA : v.pfS);
+        const Prec *__restrict__ cwi = (const Prec *)(mixed ? v.cwA : v.cwS);
+        const Rec *__restrict__ pf = (const Rec *)(mixed ? v.pfB : v.pfS);
+        const Prec *__restrict__ cwj = (const Prec *)(mixed ? v.cwB : v.cwS);
         // i particles, R per lane
         Prec ix[R], iy[R], in[R], ib[R], icw[R], rx[R], ry[R];
         unsigned long long vmask[R];
@@ -298,12 +314,12 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
             slot[r] = (unsigned)(r * 64 + lane);
             vmask[r] = low_lanes((long long)w.ni - r * 64);
             const long long i = w.i0 + min((int)slot[r], w.ni - 1);
-            const Rec p = pfA[i];
+            const Rec p = pfi[i];
             ix[r] = rx[r] = p.px;
             iy[r] = ry[r] = p.py;
             in[r] = p.nrm;
             ib[r] = p.beta;
-            icw[r] = cwA[i];
+            icw[r] = cwi[i];
         }
         // segment state
         const int nseg = mixed ? (w.k1 - w.k0) : 1;

@@ -102,18 +102,42 @@ inline void fp32_margins(const Derived &d, double pmax2, DevParams &P) {
     P.f_qthr = std::isfinite(d.P.pf_qthr) ? (float)(d.P.pf_qthr + m) : INFINITY;
 }
 
-// Particles of one list after the rapidity cut, in the reference's gather
-// order, with the fast kernel's records.
-struct HostParticles {
-    long long n = 0, n_in = 0;
-    std::vector<long long> keep;  // index into the input of each kept particle
-    std::vector<long long> off;   // nev+1 kept-particle offsets
-    std::vector<double> psi2;     // psi_ref (n=2) per group, over all input particles
+// Fast-kernel records of one ordering of a particle list.
+struct RecordSet {
     std::vector<PartRec> rec;
     std::vector<PfRec64> pf64;
     std::vector<PfRec32> pf32;
     std::vector<double> cw64;
     std::vector<float> cw32;
+    void resize(size_t n) {
+        rec.resize(n);
+        pf64.resize(n);
+        pf32.resize(n);
+        cw64.resize(n);
+        cw32.resize(n);
+    }
+    void copy_from(const RecordSet &o, size_t dst, size_t src) {
+        rec[dst] = o.rec[src];
+        pf64[dst] = o.pf64[src];
+        pf32[dst] = o.pf32[src];
+        cw64[dst] = o.cw64[src];
+        cw32[dst] = o.cw32[src];
+    }
+};
+
+// Particles of one list after the rapidity cut, in the reference's gather
+// order (ev), and the same particles sorted by px inside each group (srt)
+// for the same-event tiles, whose j range is then limited to the px window
+// a binned pair can have: |px1 - px2| <= |p1T - p2T| <= sqrt(2) H.
+struct HostParticles {
+    long long n = 0, n_in = 0;
+    std::vector<long long> keep;  // index into the input of each kept particle
+    std::vector<long long> off;   // nev+1 kept-particle offsets
+    std::vector<double> psi2;     // psi_ref (n=2) per group, over all input particles
+    RecordSet ev;                 // gather order
+    RecordSet srt;                // px-sorted inside each group
+    std::vector<int> orig;        // srt position -> gather-order index
+    std::vector<double> srt_px;   // px in srt order (work-item windows)
     double pmax2 = 0.0;  // max px^2+py^2 of kept particles
 };
 
@@ -166,35 +190,50 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
             return err = "an event keeps more than 262143 particles", HBT_ERR_CAPACITY;
     if (n >= (1ll << 31)) return err = "buffer too large", HBT_ERR_CAPACITY;
     h.n = n;
-    h.rec.resize((size_t)n);
-    h.pf64.resize((size_t)n);
-    h.pf32.resize((size_t)n);
-    h.cw64.resize((size_t)n);
-    h.cw32.resize((size_t)n);
+    RecordSet &R = h.ev;
+    R.resize((size_t)n);
     double bmax = 0.0;
     for (long long q = 0; q < n; q++) {
         const long long i = h.keep[q];
         const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
-        h.rec[q] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
+        R.rec[q] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
         const double n2 = px * px + py * py;
         h.pmax2 = std::max(h.pmax2, n2);
-        PfRec64 &r = h.pf64[q];
+        PfRec64 &r = R.pf64[q];
         r.px = px;
         r.py = py;
         r.nrm = n2 - 0.5 * d.mid;
         r.beta = P.boost ? pz / E : pz;
         bmax = std::max(bmax, std::fabs(r.beta));
-        h.cw64[q] = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
+        R.cw64[q] = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
     }
     const double eps32 = 5.9604644775390625e-8;
     for (long long q = 0; q < n; q++) {
-        h.pf32[q].px = (float)h.pf64[q].px;
-        h.pf32[q].py = (float)h.pf64[q].py;
-        h.pf32[q].nrm = (float)h.pf64[q].nrm;
-        h.pf32[q].beta = (float)h.pf64[q].beta;
+        R.pf32[q].px = (float)R.pf64[q].px;
+        R.pf32[q].py = (float)R.pf64[q].py;
+        R.pf32[q].nrm = (float)R.pf64[q].nrm;
+        R.pf32[q].beta = (float)R.pf64[q].beta;
         // covers the FP32 rounding of beta (|beta| <= bmax) and of cw itself
-        h.cw32[q] = P.qinv ? INFINITY
-                           : (float)(h.cw64[q] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
+        R.cw32[q] = P.qinv ? INFINITY
+                           : (float)(R.cw64[q] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
+    }
+    // px-sorted copy per group (stable, so equal px keep gather order)
+    h.srt.resize((size_t)n);
+    h.orig.resize((size_t)n);
+    h.srt_px.resize((size_t)n);
+    std::vector<int> idx;
+    for (int g = 0; g < ngroups; g++) {
+        const long long g0 = h.off[grp_ev[g]], g1 = h.off[grp_ev[g + 1]];
+        idx.resize((size_t)(g1 - g0));
+        for (long long q = g0; q < g1; q++) idx[q - g0] = (int)q;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return R.rec[a].px < R.rec[b].px; });
+        for (long long q = g0; q < g1; q++) {
+            const int src = idx[q - g0];
+            h.srt.copy_from(R, (size_t)q, (size_t)src);
+            h.orig[q] = src;
+            h.srt_px[q] = R.rec[src].px;
+        }
     }
     return HBT_OK;
 }
@@ -210,25 +249,45 @@ inline ItemShape item_shape(const hbt_tuning &t) {
     return ItemShape{64 * t.R, std::max(64, t.jchunk), std::max(1, std::min(t.kchunk, 64))};
 }
 
-// Same-event items of the groups: i tile of TI particles x j chunk of JC
-// (j > i inside the tile).  pairs (optional) receives each item's
-// enumerated pair count; enum_same the total.
+// Half-width of the px window of a binnable same-event pair: a pair inside
+// the q_out and q_side windows has |px1 - px2| <= |p1T - p2T| <= sqrt(2) H;
+// margins as the prefilter's.  Infinite (no window) when the q_inv branch
+// is on, since it bins pairs outside the 3D windows.
+inline double px_window(const Derived &d) {
+    if (d.P.qinv) return INFINITY;
+    const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
+    return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
+}
+
+// Same-event items of the groups, in px-sorted order: i tile of TI
+// consecutive particles x its j window (particles after the tile start whose
+// px lies within dpx of the tile's px range), cut into chunks of JC.
+// pairs (optional) receives each item's evaluated pair count.  enum_same
+// receives the reference's enumerated pair count sum n(n-1)/2.
 inline void items_same(const ItemShape &s, const std::vector<long long> &off,
-                       const std::vector<int> &grp_ev, int view, std::vector<WorkItem> &items,
+                       const std::vector<int> &grp_ev, const std::vector<double> &srt_px,
+                       double dpx, int view, std::vector<WorkItem> &items,
                        std::vector<uint64_t> *pairs, uint64_t &enum_same) {
     const int ngroups = (int)grp_ev.size() - 1;
     for (int g = 0; g < ngroups; g++) {
         const long long g0 = off[grp_ev[g]], n = off[grp_ev[g + 1]] - g0;
         if (n < 2) continue;
         enum_same += (uint64_t)(n * (n - 1) / 2);
+        const double *px = srt_px.data() + g0;
         for (long long a = 0; a + 1 < n; a += s.TI) {
             const int ni = (int)std::min<long long>(s.TI, n - a);
-            for (long long jb = a + 1; jb < n; jb += s.JC) {
+            long long jhi = n;
+            if (std::isfinite(dpx)) {
+                // px ascending inside the group: the tile spans [px[a], px[a+ni-1]]
+                const double top = px[a + ni - 1] + dpx;
+                jhi = std::upper_bound(px + a, px + n, top) - px;
+            }
+            for (long long jb = a + 1; jb < jhi; jb += s.JC) {
                 WorkItem w{};
                 w.i0 = g0 + a;
                 w.j0 = g0 + jb;
                 w.ni = ni;
-                w.nj = (int)std::min<long long>(s.JC, n - jb);
+                w.nj = (int)std::min<long long>(s.JC, jhi - jb);
                 w.kind = 0;
                 w.grp = g;
                 w.view = view;

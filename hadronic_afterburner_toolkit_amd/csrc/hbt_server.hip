@@ -113,7 +113,8 @@ struct Request {
     double pmax2 = 0.0;
     // device residency
     size_t foot = 0, arena_off = 0;
-    size_t o_bins = 0, o_rec = 0, o_pf = 0, o_cw = 0, o_off = 0, o_rowoff = 0, o_ids = 0,
+    size_t o_bins = 0, o_rec = 0, o_pf = 0, o_cw = 0, o_srec = 0, o_spf = 0, o_scw = 0,
+           o_orig = 0, o_off = 0, o_rowoff = 0, o_ids = 0,
            o_cs = 0, o_psi = 0;
     int view = -1, rslot = -1;
     View v{};
@@ -186,12 +187,21 @@ size_t layout(const hbt_server *s, Request &r) {
     size_t o = 0;
     r.o_bins = o;
     o = align256(o + s->result_bytes);
-    r.o_rec = o;
-    o = align256(o + (size_t)n * sizeof(PartRec));
-    r.o_pf = o;
-    o = align256(o + (size_t)n * (s->f32 ? sizeof(PfRec32) : sizeof(PfRec64)));
-    r.o_cw = o;
-    o = align256(o + (size_t)n * (s->f32 ? 4 : 8));
+    // gather-order and px-sorted records
+    for (size_t *po : {&r.o_rec, &r.o_srec}) {
+        *po = o;
+        o = align256(o + (size_t)n * sizeof(PartRec));
+    }
+    for (size_t *po : {&r.o_pf, &r.o_spf}) {
+        *po = o;
+        o = align256(o + (size_t)n * (s->f32 ? sizeof(PfRec32) : sizeof(PfRec64)));
+    }
+    for (size_t *po : {&r.o_cw, &r.o_scw}) {
+        *po = o;
+        o = align256(o + (size_t)n * (s->f32 ? 4 : 8));
+    }
+    r.o_orig = o;
+    o = align256(o + (size_t)n * 4);
     r.o_off = o;
     o = align256(o + (size_t)(r.nev + 1) * 8);
     r.o_rowoff = o;
@@ -218,6 +228,10 @@ void make_request_view(hbt_server *s, Request &r) {
     v.A = v.B = (const PartRec *)(b + r.o_rec);
     v.pfA = v.pfB = (const void *)(b + r.o_pf);
     v.cwA = v.cwB = (const void *)(b + r.o_cw);
+    v.S = (const PartRec *)(b + r.o_srec);
+    v.pfS = (const void *)(b + r.o_spf);
+    v.cwS = (const void *)(b + r.o_scw);
+    v.origS = (const int *)(b + r.o_orig);
     v.mix_off = (const long long *)(b + r.o_off);
     v.rowoff = (const long long *)(b + r.o_rowoff);
     v.plan_ids = (const int *)(b + r.o_ids);
@@ -234,14 +248,20 @@ hipError_t upload_request(hbt_server *s, Request &r) {
         if (e == hipSuccess && bytes)
             e = hipMemcpyAsync(b + off, src, bytes, hipMemcpyHostToDevice, s->s_upload);
     };
-    cp(r.o_rec, r.hp.rec.data(), (size_t)n * sizeof(PartRec));
-    if (s->f32) {
-        cp(r.o_pf, r.hp.pf32.data(), (size_t)n * sizeof(PfRec32));
-        cp(r.o_cw, r.hp.cw32.data(), (size_t)n * 4);
-    } else {
-        cp(r.o_pf, r.hp.pf64.data(), (size_t)n * sizeof(PfRec64));
-        cp(r.o_cw, r.hp.cw64.data(), (size_t)n * 8);
+    const std::pair<const RecordSet *, size_t> sets[] = {{&r.hp.ev, 0}, {&r.hp.srt, 1}};
+    for (const auto &x : sets) {
+        const RecordSet &R = *x.first;
+        const bool srt = x.second == 1;
+        cp(srt ? r.o_srec : r.o_rec, R.rec.data(), (size_t)n * sizeof(PartRec));
+        if (s->f32) {
+            cp(srt ? r.o_spf : r.o_pf, R.pf32.data(), (size_t)n * sizeof(PfRec32));
+            cp(srt ? r.o_scw : r.o_cw, R.cw32.data(), (size_t)n * 4);
+        } else {
+            cp(srt ? r.o_spf : r.o_pf, R.pf64.data(), (size_t)n * sizeof(PfRec64));
+            cp(srt ? r.o_scw : r.o_cw, R.cw64.data(), (size_t)n * 8);
+        }
     }
+    cp(r.o_orig, r.hp.orig.data(), (size_t)n * 4);
     cp(r.o_off, r.hp.off.data(), (size_t)(r.nev + 1) * 8);
     cp(r.o_rowoff, r.rowoff.data(), (size_t)(r.nev + 1) * 8);
     cp(r.o_ids, r.plan_ids.data(), r.plan_ids.size() * 4);
@@ -694,7 +714,8 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
     // while every enumerated total stays within it
     const bool qinv = s->cfg.invariant_radius_flag == 1;
     r->items.clear();
-    items_same(s->shape, off, r->grp_ev, 0, r->items, &r->item_pairs, r->enum_same);
+    items_same(s->shape, off, r->grp_ev, r->hp.srt_px, px_window(s->D), 0, r->items,
+               &r->item_pairs, r->enum_same);
     items_mixed(s->shape, off, r->grp_ev, r->rowoff, r->plan_ids, off, 0, r->items,
                 &r->item_pairs);
     for (uint64_t t : {r->enum_same, r->enum_mixed})
