@@ -89,13 +89,6 @@ struct FastArgs {
 #define HBT_LOAD_VIEW(tab, k) ((tab)[k])
 #endif
 
-// LDS ring entries per wave (power of two).  Between drains the ring holds
-// fewer than 64 entries, and one j step appends at most 64*R.
-template <int R>
-struct Ring {
-    static constexpr int CAP = (64 * (R + 1) <= 256) ? 256 : 512;
-};
-
 // ------------------------------------------------------- wave helpers
 
 // Mask of lanes where a <= b (ordered: NaN compares false), as one SGPR pair.
@@ -105,6 +98,9 @@ __device__ __forceinline__ unsigned long long le_mask(float a, float b) {
 __device__ __forceinline__ unsigned long long le_mask(double a, double b) {
     return __builtin_amdgcn_fcmp(a, b, 5 /* FCMP_OLE */);
 }
+__host__ __device__ __forceinline__ float absv(float x) { return __builtin_fabsf(x); }
+__host__ __device__ __forceinline__ double absv(double x) { return __builtin_fabs(x); }
+
 // Value of lane t (t uniform) in every lane.
 __device__ __forceinline__ float bcast(float v, int t) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), t));
@@ -250,31 +246,33 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
     return binned ? 1 : 0;
 }
 
-// 64 j records of one wave, one per lane (coalesced load), broadcast with
-// readlane while the next 64 are in flight.
-template <typename Prec, typename Rec>
-struct JChunk {
-    Prec px, py, nrm, beta, cw;
-    __device__ __forceinline__ void load(const Rec *__restrict__ pf, const Prec *__restrict__ cw_,
-                                         long long jb, long long je) {
-        const long long j = min(jb + (long long)__lane_id(), je - 1);
-        const Rec r = pf[j];
-        px = r.px;
-        py = r.py;
-        nrm = r.nrm;
-        beta = r.beta;
-        cw = cw_[j];
-    }
+// j records are read with scalar loads (uniform address, read-only data
+// seen through the constant address space), so the j values arrive in SGPRs
+// and cost no vector instructions to broadcast.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HBT_CONST(T, p) ((const __attribute__((address_space(4))) T *)(p))
+#else
+#define HBT_CONST(T, p) ((const T *)(p))
+#endif
+
+// Survivors queued between drains: fewer than 64 plus one unrolled step of
+// JU j's times 64R lanes.
+constexpr int JU = 4;
+template <int R>
+struct Ring {
+    static constexpr int NEED = 64 + JU * 64 * R;
+    static constexpr int CAP = NEED <= 256 ? 256 : (NEED <= 512 ? 512 : 1024);
 };
 
 // ---------------------------------------------------------------- kernel
 // Persistent: each wave pulls work items from a global counter until none
 // are left.  Per item the wave holds 64R i particles (R per lane) and walks
-// its j segments (same-event: one range with j > i; mixed: one segment per
-// partner with the i particles rotated by -theta).  Survivors of the
-// prefilter go to the wave's LDS ring; whenever 64 are queued they are
-// binned, one per lane.  The loop is a flat state machine so the exact
-// cascade has a single call site.
+// its j segments (same-event: one range with j > i in the px-sorted list;
+// mixed: one segment per partner with the i particles rotated by -theta),
+// JU j's per step.  Survivors of the prefilter go to the wave's LDS ring;
+// whenever 64 are queued they are binned, one per lane, and the last
+// survivors of the item are flushed at its final step (the exact cascade
+// has a single call site).
 template <typename Prec, int R, bool EXT>
 __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
     using Rec = typename std::conditional<std::is_same<Prec, double>::value, PfRec64,
@@ -303,8 +301,8 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         // gather-order lists
         const Rec *__restrict__ pfi = (const Rec *)(mixed ? v.pfA : v.pfS);
         const Prec *__restrict__ cwi = (const Prec *)(mixed ? v.cwA : v.cwS);
-        const Rec *__restrict__ pf = (const Rec *)(mixed ? v.pfB : v.pfS);
-        const Prec *__restrict__ cwj = (const Prec *)(mixed ? v.cwB : v.cwS);
+        const Rec *pfj = (const Rec *)(mixed ? v.pfB : v.pfS);
+        const Prec *cwj = (const Prec *)(mixed ? v.cwB : v.cwS);
         // i particles, R per lane
         Prec ix[R], iy[R], in[R], ib[R], icw[R], rx[R], ry[R];
         unsigned long long vmask[R];
@@ -321,93 +319,102 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
             ib[r] = p.beta;
             icw[r] = cwi[i];
         }
-        // segment state
+        const bool partial = w.ni < 64 * R;
         const int nseg = mixed ? (w.k1 - w.k0) : 1;
         const long long row0 = mixed ? v.rowoff[w.e] + w.k0 : 0;
-        int seg = -1;
-        long long jb = 0, je = 0, jc = 0;  // segment range, current chunk start
-        int t = 0, cnt = 0;
-        bool finished = false;
-        JChunk<Prec, Rec> cur, nxt;
         unsigned head = 0, tail = 0;
-        for (;;) {
-            if (tail - head >= 64 || (finished && tail != head)) {
-                const int n = (int)min(64u, tail - head);
-                __builtin_amdgcn_wave_barrier();
-                const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, row0, psi);
-                if (mixed)
-                    nb_mixed += nb;
-                else
-                    nb_same += nb;
-                head += n;
-                continue;
+        for (int seg = 0; seg < nseg; seg++) {
+            long long pbase, jb, je;
+            unsigned segbits;
+            if (mixed) {
+                const long long row = row0 + seg;
+                const int pid = v.plan_ids[row];
+                pbase = v.mix_off[pid];
+                jb = pbase;
+                je = v.mix_off[pid + 1];
+                segbits = (unsigned)seg << 26;
+                const double c = v.plan_cs[2 * row], s = v.plan_cs[2 * row + 1];
+#pragma unroll
+                for (int r = 0; r < R; r++) {  // rotate i by -theta
+                    rx[r] = (Prec)((double)ix[r] * c + (double)iy[r] * s);
+                    ry[r] = (Prec)((double)iy[r] * c - (double)ix[r] * s);
+                }
+            } else {
+                pbase = jb = w.j0;
+                je = w.j0 + w.nj;
+                segbits = 0;
             }
-            if (finished) break;
-            if (t == cnt) {  // next chunk, or next segment
-                if (seg >= 0 && jc + 64 < je) {
-                    jc += 64;
-                    cur = nxt;
-                } else {
-                    if (++seg == nseg) {
-                        finished = true;
+            const bool last_seg = seg == nseg - 1;
+            long long jc = jb;
+            do {  // chunks of 64 j's (at least one pass, for the final flush)
+                const int cnt = (int)max(0ll, min(64ll, je - jc));
+                const bool last = last_seg && jc + 64 >= je;
+                // same-event: lanes with i < j matter only where the chunk
+                // reaches into the tile itself
+                const bool diag = !mixed && jc < w.i0 + 64 * R;
+                int tri = (int)min(jc - w.i0, 1ll << 20);
+                unsigned ebase = segbits | ((unsigned)(jc - pbase) << 8);
+                const auto *cpf = HBT_CONST(Rec, pfj + jc);
+                const auto *ccw = HBT_CONST(Prec, cwj + jc);
+                int t = 0;
+                for (;;) {
+                    if (tail - head >= 64 || (last && t >= cnt && tail != head)) {
+                        const int n = (int)min(64u, tail - head);
+                        __builtin_amdgcn_wave_barrier();
+                        const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, row0, psi);
+                        if (mixed)
+                            nb_mixed += nb;
+                        else
+                            nb_same += nb;
+                        head += n;
                         continue;
                     }
-                    if (mixed) {
-                        const long long row = row0 + seg;
-                        const int pid = v.plan_ids[row];
-                        jb = v.mix_off[pid];
-                        je = v.mix_off[pid + 1];
-                        const double c = v.plan_cs[2 * row], s = v.plan_cs[2 * row + 1];
+                    if (t >= cnt) break;
+                    // one j against the wave's 64R i particles
+                    auto pair_step = [&](int u, const Rec &jr, Prec jcw) {
 #pragma unroll
-                        for (int r = 0; r < R; r++) {  // rotate i by -theta
-                            rx[r] = (Prec)((double)ix[r] * c + (double)iy[r] * s);
-                            ry[r] = (Prec)((double)iy[r] * c - (double)ix[r] * s);
+                        for (int r = 0; r < R; r++) {
+                            const Prec d = fma(rx[r], jr.px, ry[r] * jr.py);
+                            const Prec sn = in[r] + jr.nrm;
+                            const Prec uu = fma(two, d, sn);
+                            const Prec vv = fma(-two, d, sn);
+                            const Prec db = ib[r] - jr.beta;
+                            const Prec rw = icw[r] + jcw;
+                            unsigned long long m = le_mask(absv(uu), half_t) &
+                                                   le_mask(vv, q_thr) & le_mask(absv(db), rw);
+                            if (partial) m &= vmask[r];
+                            if (diag) m &= low_lanes32(tri + u - r * 64);
+                            if (m) {
+                                const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                                    (unsigned)(m >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, tail));
+                                if (__builtin_amdgcn_inverse_ballot_w64(m))
+                                    q[pos & (QCAP - 1)] = (ebase + ((unsigned)u << 8)) | slot[r];
+                                tail += (unsigned)__popcll(m);
+                            }
                         }
-                    } else {
-                        jb = w.j0;
-                        je = w.j0 + w.nj;
-                    }
-                    if (jb >= je) {
-                        cnt = t = 0;
-                        continue;
-                    }
-                    jc = jb;
-                    cur.load(pf, cwj, jc, je);
-                }
-                if (jc + 64 < je) nxt.load(pf, cwj, jc + 64, je);
-                cnt = (int)min(64ll, je - jc);
-                t = 0;
-            }
-            // one j against the wave's 64R i particles
-            const long long j = jc + t;
-            const Prec jx = bcast(cur.px, t), jy = bcast(cur.py, t);
-            const Prec jn = bcast(cur.nrm, t), jbeta = bcast(cur.beta, t);
-            const Prec jcw = bcast(cur.cw, t);
-            const unsigned ebase = mixed ? ((unsigned)seg << 26) | ((unsigned)(j - jb) << 8)
-                                         : ((unsigned)(j - w.j0) << 8);
-            // same-event: lanes with i < j (only the first chunks of a tile
-            // overlap it; clamp so the 32-bit shift below stays defined)
-            const int tri = (int)min(max(j - w.i0, -1ll), 1024ll);
-            t++;
+                    };
+                    if (t + JU <= cnt) {  // whole step: the JU records load together
+                        Rec jr[JU];
+                        Prec jw[JU];
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                const Prec d = fma(rx[r], jx, ry[r] * jy);
-                const Prec sn = in[r] + jn;
-                const Prec u = fma(two, d, sn);
-                const Prec v = fma(-two, d, sn);
-                const Prec db = ib[r] - jbeta;
-                const Prec rw = icw[r] + jcw;
-                unsigned long long m = le_mask(fabs(u), half_t) & le_mask(v, q_thr) &
-                                       le_mask(fabs(db), rw) & vmask[r];
-                if (!mixed) m &= low_lanes32(tri - r * 64);
-                if (m) {
-                    const unsigned pos =
-                        tail + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    if (__builtin_amdgcn_inverse_ballot_w64(m)) q[pos & (QCAP - 1)] = ebase | slot[r];
-                    tail += (unsigned)__popcll(m);
+                        for (int u = 0; u < JU; u++) {
+                            jr[u] = cpf[t + u];
+                            jw[u] = ccw[t + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < JU; u++) pair_step(u, jr[u], jw[u]);
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < JU; u++)
+                            if (t + u < cnt) pair_step(u, cpf[t + u], ccw[t + u]);
+                    }
+                    t += JU;
+                    tri += JU;
+                    ebase += (unsigned)JU << 8;
                 }
-            }
+                jc += 64;
+            } while (jc < je);
         }
         __builtin_amdgcn_wave_barrier();
     }
