@@ -177,6 +177,80 @@ def serve(args, log):
     return out
 
 
+PI_K = ((211, 0.13957, 0.25, 6.0), (321, 0.49368, 0.30, 1.0))  # monval, mass, T, weight
+
+
+def batch_line(groups, params, seed, prec, steps, tuning, autotune, log, label):
+    """One analysis in the batched form: groups resident, plans from one
+    Random(seed) stream, one persistent launch per step (hipGraph replay)."""
+    from hadronic_afterburner_toolkit_amd import _lib, hbt
+    rng = hbt.Random(seed)
+    Mg, ids, cs = [], [], []
+    for b in groups:
+        m, i, c = rng.draw_plan(b.nev, b.nev, int(params.get("number_of_mixed_events", 0)))
+        Mg.append(m)
+        ids.append(i)
+        cs.append(c)
+    g = hbt.HBTCorrelation(params, precision=prec, kernel=_lib.HBT_KERNEL_FAST)
+    g.upload_groups(groups)
+    g.prepare_batch(np.array(Mg, np.int32), np.concatenate(ids), np.concatenate(cs))
+    if tuning:
+        g.set_tuning(*tuning)
+    tune = g.autotune() if autotune else g.tuning()
+    es, em, _ = g.batch_info()
+    g.launch_batch()
+    g.sync()
+    ms = []
+    for _ in range(steps):
+        g.launch_batch()
+        ms.append(g.sync())
+    res = g.results()
+    g.close()
+    t = float(np.mean(ms)) * 1e-3
+    log(f"  {label}: {es + em:.3e} pairs in {t * 1e3:.1f} ms -> {(es + em) / t:.3e} pairs/s")
+    return dict(pairs=es + em, same_pairs=es, mixed_pairs=em, ms=t * 1e3,
+                pairs_per_s=(es + em) / t, tuning=tune), res
+
+
+def batched_uneven(args, log):
+    """Batched uneven form: 512 events, multiplicity log-uniform on [20, 4000],
+    pi+ and K+ (6:1, K+ with T = 0.30 GeV) analysed as separate identical-
+    particle histograms, groups of 20 events, 16 K_phi bins about the event
+    plane (folded to 8 bins of Phi_K mod pi afterwards), K_T edges 0.2..0.7
+    (step 0.1), the reference's partner rule (11 per event)."""
+    from hadronic_afterburner_toolkit_amd import _lib, samples
+    full = samples.generate(args.seed + 20, 512, samples.log_uniform_mult(20, 4000), species=PI_K)
+    params = dict(PARAMS, azimuthal_flag=1, n_Kphi=16, number_of_mixed_events=0)
+    out = {"events": 512, "per_group": 20}
+    tot_pairs = tot_t = 0.0
+    for monval, name in ((211, "pi+"), (321, "K+")):
+        buf = full.select(monval)
+        groups = [buf.events(g, min(20, buf.nev - g)) for g in range(0, buf.nev, 20)]
+        line, _ = batch_line(groups, params, args.seed + monval, _lib.HBT_PREC_FP32C,
+                             max(1, args.steps), None, not args.no_autotune, log,
+                             f"batched uneven {name}")
+        out[name] = line
+        tot_pairs += line["pairs"]
+        tot_t += line["ms"] * 1e-3
+    out["pairs_per_s"] = tot_pairs / tot_t
+    return out
+
+
+def large_histogram(args, log, tuning):
+    """Large-histogram form: 1000 pi+ events Poisson(4000), groups of 100,
+    20 rotations per event, q 61^3 on [-0.2, 0.2] GeV, 8 K_T bins on
+    [0.15, 1.0] GeV, 16 K_phi bins: 3.2e7 bins (2.1 GB of histogram)."""
+    from hadronic_afterburner_toolkit_amd import _lib, samples
+    full = samples.generate(args.seed + 40, 1000, 4000.0)
+    params = dict(PARAMS, qnpts=61, q_min=-0.2, q_max=0.2, n_KT=9, KT_min=0.15, KT_max=1.0,
+                  azimuthal_flag=1, n_Kphi=16, number_of_mixed_events=20)
+    groups = [full.events(g, 100) for g in range(0, 1000, 100)]
+    line, _ = batch_line(groups, params, args.seed + 41, _lib.HBT_PREC_FP32C, 1, tuning,
+                         False, log, "large histogram")
+    line.update(events=1000, per_group=100, bins=61 ** 3 * 9 * 16)
+    return line
+
+
 def cpu_baseline(group, plan_g, log):
     """The CPU oracle on group 0 (its same-event pairs and its events' mixed
     pairs with the same plan), one thread."""
@@ -239,6 +313,8 @@ def main():
     ap.add_argument("--only", choices=["fp32c", "fp64", "plain"], default=None)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched-uneven and large-histogram lines")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-step-pairs", type=float, default=2e9)
@@ -337,6 +413,11 @@ def main():
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
     if not args.no_serve:
         out["serving"] = serve(args, log)
+    if not args.no_extra:
+        t = res[head]["tune"]
+        out["batched_uneven"] = batched_uneven(args, log)
+        out["large_histogram"] = large_histogram(
+            args, log, (t["R"], t["jchunk"], t["kchunk"], t["blocks_per_cu"]))
     if not args.no_cpu:
         g0 = groups[0]
         M = int(plan[0][0])

@@ -308,6 +308,34 @@ class HBTCorrelation:
         return r
 
 
+def fold_kphi_mod_pi(r: Results, params: dict) -> Results:
+    """K_phi bins of Phi_K mod pi: the reference bins Delta phi on [0, 2 pi)
+    (HBT_correlation.cpp:408-416); bin k and bin k + n_Kphi/2 cover the same
+    pair axis, so their sums are the n_Kphi/2 bins on [0, pi).  Exact: the
+    folding follows the reference's own indexing."""
+    p = dict(DEFAULTS)
+    p.update(params)
+    nkt, nphi, q3 = int(p["n_KT"]), int(p["n_Kphi"]), int(p["qnpts"]) ** 3
+    if not int(p["azimuthal_flag"]) or nphi % 2:
+        raise ValueError("folding needs azimuthal_flag=1 and an even n_Kphi")
+    h = nphi // 2
+
+    def fold(a, per):
+        x = a.reshape(nkt, nphi, per)
+        return (x[:, :h] + x[:, h:]).reshape(-1)
+
+    out = {}
+    for name in Results.__dataclass_fields__:
+        a = getattr(r, name)
+        if name in ("num_count", "den", "q_out_sum", "q_side_sum", "q_long_sum", "num"):
+            out[name] = fold(a, q3)
+        elif name in ("cnt_num_ktphi", "cnt_den_ktphi"):
+            out[name] = fold(a, 1)
+        else:
+            out[name] = a.copy()
+    return Results(**out)
+
+
 def _make_config(params, precision, kernel, device):
     p = dict(DEFAULTS)
     if params:
