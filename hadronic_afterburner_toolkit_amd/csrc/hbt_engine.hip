@@ -285,9 +285,14 @@ struct Slot {
     std::vector<double> psi2;      // psi_ref (n=2) per group, over all particles passed in
     double *d_soa[8] = {};   // plain kernels (structure of arrays)
     long long *d_off = nullptr;
-    DevRecords ev, srt;        // fast kernel: gather order, px-sorted per group
-    int *d_orig = nullptr;     // srt position -> gather-order index
-    size_t cap = 0, cap_ev = 0;
+    // fast kernel: the fast list (particles whose |pT| can reach a K_T bin,
+    // |pT|-sorted per event) and its px-sorted copy per group
+    long long nf = 0;
+    std::vector<long long> foff;  // nev+1 fast-list offsets
+    std::vector<double> f_pt;     // |pT| of the fast list (mixed windows)
+    DevRecords ev, srt;
+    OrigRank *d_orig = nullptr;   // srt position -> fast-list index, gather rank
+    size_t cap = 0, cap_f = 0, cap_ev = 0;
     double pmax2 = 0.0;     // max px^2+py^2 of kept particles
     std::vector<double> srt_px;  // px of the px-sorted list (work-item windows)
     bool valid = false;
@@ -324,6 +329,8 @@ struct hbt_ctx {
     std::vector<double> psi;  // per group (set from the upload or by the caller)
     WorkItem *d_items = nullptr;
     size_t items_cap = 0;
+    JWin *d_wins = nullptr;  // partner windows of the mixed items
+    size_t wins_cap = 0;
     int nitems = 0, nitems_same = 0;
     int *d_counter = nullptr;
     View *d_view = nullptr;  // the context's one analysis view
@@ -524,7 +531,6 @@ View make_view(hbt_ctx *c) {
     const Slot &s1 = mixed_slot(c);
     v.A = s0.ev.rec;
     v.B = s1.ev.rec;
-    v.S = s0.srt.rec;
     const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
     auto pf = [&](const DevRecords &r) {
         return f32 ? (const void *)r.pf32 : (const void *)r.pf64;
@@ -539,9 +545,7 @@ View make_view(hbt_ctx *c) {
     v.cwB = cw(s1.ev);
     v.cwS = cw(s0.srt);
     v.origS = s0.d_orig;
-    v.mix_off = s1.d_off;
-    v.rowoff = c->d_rowoff;
-    v.plan_ids = c->d_ids;
+    v.mwin = c->d_wins;
     v.plan_cs = c->d_cs;
     v.psi = c->d_psi;
     return v;
@@ -570,22 +574,30 @@ int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
     return HBT_OK;
 }
 
+struct ItemList {
+    std::vector<WorkItem> items;
+    std::vector<JWin> wins;
+};
+
 // Work items: same-event tiles of every group, then mixed tiles of every
-// event.  Same: i tile of 64R particles x j chunk (j > i inside the tile).
-// Mixed: i tile x up to kchunk partners of the event's plan rows.
-// enum_same receives the same-event pairs the items enumerate; the mixed
+// event.  Same: i tile of 64R particles x j chunk of its px window.
+// Mixed: i tile x up to kchunk partner windows of the event's plan rows.
+// enum_same receives the reference's same-event pair count; the mixed
 // count is known from the plan (upload_plan).
-void build_items(hbt_ctx *c, bool same, bool mixed, std::vector<WorkItem> &items,
-                 uint64_t &enum_same) {
+void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_same) {
     const Slot &s = c->slot[0];
     const ItemShape sh = item_shape(c->tune);
-    items.clear();
+    const double D = px_window(c->D);
+    L.items.clear();
+    L.wins.clear();
     enum_same = 0;
-    if (same) items_same(sh, s.off, s.grp_ev, s.srt_px, px_window(c->D), 0, items, nullptr, enum_same);
-    c->nitems_same = (int)items.size();
-    if (mixed && c->plan_valid)
-        items_mixed(sh, s.off, s.grp_ev, c->rowoff, c->plan_gids, mixed_slot(c).off, 0, items,
-                    nullptr);
+    if (same) items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px, D, 0, L.items, nullptr, enum_same);
+    c->nitems_same = (int)L.items.size();
+    if (mixed && c->plan_valid) {
+        const Slot &m = mixed_slot(c);
+        items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt, m.f_pt, D, 0,
+                    L.items, L.wins, nullptr);
+    }
 }
 
 // Uploads the plan: ids are relative to the event's own group and are
@@ -630,11 +642,17 @@ int upload_plan(hbt_ctx *c, const std::vector<int> &M_per_event, const int32_t *
     return HBT_OK;
 }
 
-int upload_items(hbt_ctx *c, const std::vector<WorkItem> &items) {
+int upload_items(hbt_ctx *c, const ItemList &L) {
+    const std::vector<WorkItem> &items = L.items;
+    if (items.size() >= (size_t)INT32_MAX) return fail(c, HBT_ERR_CAPACITY, "too many work items");
     int rc = ensure(c, &c->d_items, c->items_cap, std::max<size_t>(1, items.size()));
+    if (!rc) rc = ensure(c, &c->d_wins, c->wins_cap, std::max<size_t>(1, L.wins.size()));
     if (rc) return rc;
     if (!items.empty())
         HIPCHECK(c, hipMemcpy(c->d_items, items.data(), items.size() * sizeof(WorkItem),
+                              hipMemcpyHostToDevice));
+    if (!L.wins.empty())
+        HIPCHECK(c, hipMemcpy(c->d_wins, L.wins.data(), L.wins.size() * sizeof(JWin),
                               hipMemcpyHostToDevice));
     c->nitems = (int)items.size();
     return HBT_OK;
@@ -804,7 +822,7 @@ int run_capped(hbt_ctx *c, bool mixed) {
 
 int prepare_and_capture(hbt_ctx *c) {
     drop_graph(c);
-    std::vector<WorkItem> items;
+    ItemList items;
     build_items(c, true, true, items, c->batch_enum_same);
     c->batch_enum_mixed = c->plan_valid ? c->plan_enum_mixed : 0;
     int rc = upload_items(c, items);
@@ -818,14 +836,21 @@ int prepare_and_capture(hbt_ctx *c) {
     FastArgs a;
     rc = fast_args(c, a);
     if (rc) return rc;
-    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_fast(c, a, c->stream);
+    // The batch is one graph (counter fill + persistent kernel).  A runtime
+    // that refuses the capture (a counter-collecting profiler does) gets the
+    // same launches issued directly by hbt_launch_batch.
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(c, HBT_ERR_HIP, "hipStreamEndCapture failed");
-    c->graph = g;
-    HIPCHECK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    bool captured = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (captured) {
+        rc = launch_fast(c, a, c->stream);
+        captured = hipStreamEndCapture(c->stream, &g) == hipSuccess && !rc && g;
+    }
+    if (captured) captured = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) c->graph = g;
+    if (!captured) {
+        drop_graph(c);
+        (void)hipGetLastError();  // the refused capture's error is not the batch's
+    }
     c->batch_ready = true;
     return HBT_OK;
 }
@@ -841,22 +866,25 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     drop_graph(c);
     c->plan_valid = false;
     Slot &s = c->slot[slot];
-    const long long n = h.n;
+    const long long n = h.n, nf = h.nf;
     if ((size_t)n > s.cap || !s.d_soa[0]) {
         for (auto &p : s.d_soa) {
             rc = dev_alloc(c, &p, (size_t)n);
             if (rc) return rc;
         }
-        for (DevRecords *r : {&s.ev, &s.srt}) {
-            if (!rc) rc = dev_alloc(c, &r->rec, (size_t)n);
-            if (!rc) rc = dev_alloc(c, &r->pf64, (size_t)n);
-            if (!rc) rc = dev_alloc(c, &r->pf32, (size_t)n);
-            if (!rc) rc = dev_alloc(c, &r->cw64, (size_t)n);
-            if (!rc) rc = dev_alloc(c, &r->cw32, (size_t)n);
-        }
-        if (!rc) rc = dev_alloc(c, &s.d_orig, (size_t)n);
-        if (rc) return rc;
         s.cap = (size_t)n;
+    }
+    if ((size_t)nf > s.cap_f || !s.d_orig) {
+        for (DevRecords *r : {&s.ev, &s.srt}) {
+            if (!rc && r == &s.ev) rc = dev_alloc(c, &r->rec, (size_t)nf);
+            if (!rc) rc = dev_alloc(c, &r->pf64, (size_t)nf);
+            if (!rc) rc = dev_alloc(c, &r->pf32, (size_t)nf);
+            if (!rc) rc = dev_alloc(c, &r->cw64, (size_t)nf);
+            if (!rc) rc = dev_alloc(c, &r->cw32, (size_t)nf);
+        }
+        if (!rc) rc = dev_alloc(c, &s.d_orig, (size_t)nf);
+        if (rc) return rc;
+        s.cap_f = (size_t)nf;
     }
     if ((size_t)nev + 1 > s.cap_ev || !s.d_off) {
         rc = dev_alloc(c, &s.d_off, (size_t)nev + 1);
@@ -868,20 +896,26 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         for (long long k = 0; k < n; k++) col[k] = soa[a][h.keep[k]];
         HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
     }
-    if (n > 0) {
-        const std::pair<DevRecords *, const RecordSet *> sets[] = {{&s.ev, &h.ev},
+    if (nf > 0) {
+        const std::pair<DevRecords *, const RecordSet *> sets[] = {{&s.ev, &h.fast},
                                                                    {&s.srt, &h.srt}};
         for (const auto &x : sets) {
             DevRecords &d = *x.first;
             const RecordSet &r = *x.second;
-            HIPCHECK(c, hipMemcpy(d.rec, r.rec.data(), n * sizeof(PartRec), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.pf64, r.pf64.data(), n * sizeof(PfRec64), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.pf32, r.pf32.data(), n * sizeof(PfRec32), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), n * 8, hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.cw32, r.cw32.data(), n * 4, hipMemcpyHostToDevice));
+            if (d.rec)
+                HIPCHECK(c, hipMemcpy(d.rec, r.rec.data(), nf * sizeof(PartRec), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.pf64, r.pf64.data(), nf * sizeof(PfRec64), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.pf32, r.pf32.data(), nf * sizeof(PfRec32), hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), nf * 8, hipMemcpyHostToDevice));
+            HIPCHECK(c, hipMemcpy(d.cw32, r.cw32.data(), nf * 4, hipMemcpyHostToDevice));
         }
-        HIPCHECK(c, hipMemcpy(s.d_orig, h.orig.data(), n * 4, hipMemcpyHostToDevice));
+        std::vector<OrigRank> orr((size_t)nf);
+        for (long long q = 0; q < nf; q++) orr[q] = OrigRank{h.orig[q], h.rank[q]};
+        HIPCHECK(c, hipMemcpy(s.d_orig, orr.data(), nf * sizeof(OrigRank), hipMemcpyHostToDevice));
     }
+    s.nf = nf;
+    s.foff = std::move(h.foff);
+    s.f_pt = std::move(h.f_pt);
     s.srt_px = std::move(h.srt_px);
     HIPCHECK(c, hipMemcpy(s.d_off, h.off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
     s.ngroups = ngroups;
@@ -1017,7 +1051,8 @@ int hbt_destroy(hbt_ctx *c) {
                     c->d_qinv_sum,  c->d_qinv_num,  c->d_stage,      c->d_binned,
                     c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
                     c->d_items,     c->d_counter,   c->d_psi,        c->d_view,
-                    c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff};
+                    c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff,
+                    c->d_wins};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -1066,7 +1101,7 @@ int hbt_run_same(hbt_ctx *c) {
     if (!c) return HBT_ERR_ARG;
     int rc = check_slots(c);
     if (rc) return rc;
-    std::vector<WorkItem> items;
+    ItemList items;
     uint64_t es = 0;
     build_items(c, true, false, items, es);
     bool binds = false;
@@ -1117,7 +1152,7 @@ int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const doubl
         HIPCHECK(c, hipEventSynchronize(c->ev[3]));
         HIPCHECK(c, hipEventElapsedTime(&c->ms_mixed, c->ev[2], c->ev[3]));
     } else {
-        std::vector<WorkItem> items;
+        ItemList items;
         build_items(c, false, true, items, es);
         rc = upload_items(c, items);
         if (!rc) rc = timed_fast(c, &c->ms_mixed);
@@ -1163,7 +1198,14 @@ int hbt_launch_batch(hbt_ctx *c) {
         if (!rc) rc = bm ? run_capped(c, true) : (c->plan_valid ? run_plain(c, false, true) : 0);
         if (rc) return rc;
     } else if (c->cfg.kernel == HBT_KERNEL_FAST) {
-        if (c->nitems > 0) HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        if (c->nitems > 0 && c->graph_exec) {
+            HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        } else if (c->nitems > 0) {
+            FastArgs a;
+            rc = fast_args(c, a);
+            if (!rc) rc = launch_fast(c, a, c->stream);
+            if (rc) return rc;
+        }
     } else {
         rc = run_plain(c, true, true);
         if (rc) return rc;
@@ -1228,7 +1270,7 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
                 for (int bpc : BPCs) {
                     hbt_tuning t{R, jc, kc, bpc, 0.0};
                     c->tune = t;
-                    std::vector<WorkItem> items;
+                    ItemList items;
                     uint64_t es;
                     build_items(c, true, true, items, es);
                     int rc = upload_items(c, items);

@@ -35,38 +35,52 @@ struct PfRec32 {
     float px, py, nrm, beta;
 };
 
+// Partner window of a mixed work item: the j range [b, e) of the mixed
+// list (the particles of the partner event whose |pT| can pair with the
+// item's i tile) and the plan row that gives the partner's rotation.
+struct JWin {
+    int b, e;
+    int row;
+    int pad;
+};
+
+// Same-event pair orientation: for a position in the px-sorted list, the
+// particle's index in the fast list and its rank in gather order (the
+// reference's particle 1 is the earlier of the two in gather order).
+struct OrigRank {
+    int idx;
+    int rank;
+};
+
 // One unit of work for one wave.
 struct WorkItem {
-    long long i0;   // first i particle (index into the slot-0 cut arrays)
-    long long j0;   // same-event: first j particle (slot-0 index)
+    long long i0;   // first i particle (same-event: px-sorted list, mixed: fast list)
+    long long j0;   // same-event: first j particle; mixed: first window (View::mwin)
     int ni;         // number of i particles (<= 64*R)
-    int nj;         // same-event: number of j particles
+    int nj;         // same-event: number of j particles; mixed: number of windows
     int kind;       // 0 same-event, 1 mixed
-    int e;          // mixed: event index (its plan rows start at rowoff[e])
-    int k0, k1;     // mixed: partner range [k0,k1) of the event's plan rows
     int grp;        // group (buffer) of the pair, selects psi_ref
     int view;       // analysis the item belongs to (index into FastArgs::views)
 };
 
 // One analysis resident in HBM: its particles, mixing plan and histograms.
 // A launch may carry items of many analyses (the request server batches
-// them); every index in a WorkItem is relative to its own View.
+// them); every index in a WorkItem is relative to its own View.  The
+// particle lists are the fast lists: per event, the particles whose |pT|
+// can reach a K_T bin, sorted by |pT|.
 struct View {
     DevHist H;
-    const PartRec *A;    // same-event list (cut) in gather order, 64 B per particle
-    const PartRec *B;    // mixed list (separate, or an alias of A)
-    const PartRec *S;    // same-event list sorted by px inside each group
+    const PartRec *A;    // same-event fast list, 64 B per particle
+    const PartRec *B;    // mixed fast list (separate, or an alias of A)
     const void *pfA;     // PfRec64 (FP64 kernel) or PfRec32 (precision form) of A
     const void *pfB;     // of B
-    const void *pfS;     // of S
+    const void *pfS;     // of the same-event fast list sorted by px inside each group
     const void *cwA;     // per-particle c/E bound of A, double or float like pfA
     const void *cwB;
     const void *cwS;
-    const int *origS;    // S position -> gather-order index (pair orientation)
-    const long long *mix_off;  // event offsets of the mixed list (cut)
-    const long long *rowoff;   // plan rows of event e: [rowoff[e], rowoff[e+1])
-    const int *plan_ids;       // partner event (index in the mixed list)
-    const double *plan_cs;     // cos, sin of the partner's rotation
+    const OrigRank *origS;     // px-sorted position -> fast-list index and gather rank
+    const JWin *mwin;          // partner windows of the mixed items
+    const double *plan_cs;     // cos, sin of each plan row's rotation
     const double *psi;         // psi_ref per group
 };
 
@@ -127,16 +141,14 @@ struct Exact;
 
 template <bool EXT>
 struct Exact<double, EXT> {
-    // i, j are positions in the px-sorted list; the pair is evaluated in the
-    // reference's orientation (particle 1 = the earlier in gather order).
+    // i, j are positions in the px-sorted list; origS maps them to the
+    // fast list, and the pair is evaluated in the reference's orientation
+    // (particle 1 = the earlier in gather order).
     static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
                                                long long i, long long j, SameRec &rec) {
-        if (v.origS[i] > v.origS[j]) {
-            const long long t = i;
-            i = j;
-            j = t;
-        }
-        const PartRec p = v.S[i], r = v.S[j];
+        const OrigRank oi = v.origS[i], oj = v.origS[j];
+        const bool fwd = oi.rank < oj.rank;
+        const PartRec p = v.A[fwd ? oi.idx : oj.idx], r = v.A[fwd ? oj.idx : oi.idx];
         return eval_same_f64<EXT>(P, v.H, psi, p.px, p.py, p.pz, p.E, p.x, p.y, p.z, p.t, r.px,
                                   r.py, r.pz, r.E, r.x, r.y, r.z, r.t, rec);
     }
@@ -155,12 +167,10 @@ template <bool EXT>
 struct Exact<float, EXT> {
     static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
                                                long long i, long long j, SameRec &rec) {
-        if (v.origS[i] > v.origS[j]) {
-            const long long t = i;
-            i = j;
-            j = t;
-        }
-        return eval_same_fast<EXT>(P, v.H, psi, v.S[i], v.S[j], rec);
+        const OrigRank oi = v.origS[i], oj = v.origS[j];
+        const bool fwd = oi.rank < oj.rank;
+        return eval_same_fast<EXT>(P, v.H, psi, v.A[fwd ? oi.idx : oj.idx],
+                                   v.A[fwd ? oj.idx : oi.idx], rec);
     }
     static __device__ __forceinline__ int mixed(const DevParams &P, const View &v, double psi,
                                                 long long i, long long j, double c, double s,
@@ -182,8 +192,8 @@ struct Stage {
 
 // Each lane with lane < cnt takes one queued survivor and runs the exact
 // cascade.  Same-event entries: (j - j0) << 8 | slot; mixed entries:
-// (k - k0) << 26 | (j - partner start) << 8 | slot, slot = r*64 + lane of
-// the i particle in the tile.
+// (window - j0) << 26 | (j - window start) << 8 | slot, slot = r*64 + lane
+// of the i particle in the tile.
 // Binned same-event pairs are added cooperatively: they are staged in LDS,
 // then groups of 5 lanes add one pair's count and four sums, so one atomic
 // instruction touches 12 bins (12 cache lines) instead of 64, and the five
@@ -192,8 +202,7 @@ struct Stage {
 template <typename Prec, bool EXT, int QCAP>
 __device__ __forceinline__ int drain(const DevParams &P, const View &v, const unsigned *q,
                                      Stage &stg,
-                                     unsigned head, int cnt, const WorkItem &w, long long row0,
-                                     double psi) {
+                                     unsigned head, int cnt, const WorkItem &w, double psi) {
     const int lane = __lane_id();
     int st = 0;
     SameRec rec;
@@ -206,10 +215,10 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
             if (w.kind == 0) {
                 st = Exact<Prec, EXT>::same(P, v, psi, i, w.j0 + (ent >> 8), rec);
             } else {
-                const long long row = row0 + (long long)(ent >> 26);
-                const long long j = v.mix_off[v.plan_ids[row]] + ((ent >> 8) & 0x3ffffu);
-                st = Exact<Prec, EXT>::mixed(P, v, psi, i, j, v.plan_cs[2 * row],
-                                             v.plan_cs[2 * row + 1], kmix);
+                const JWin jw = v.mwin[w.j0 + (ent >> 26)];
+                const long long j = jw.b + ((ent >> 8) & 0x3ffffu);
+                st = Exact<Prec, EXT>::mixed(P, v, psi, i, j, v.plan_cs[2 * jw.row],
+                                             v.plan_cs[2 * jw.row + 1], kmix);
             }
         }
     }
@@ -268,7 +277,8 @@ struct Ring {
 // Persistent: each wave pulls work items from a global counter until none
 // are left.  Per item the wave holds 64R i particles (R per lane) and walks
 // its j segments (same-event: one range with j > i in the px-sorted list;
-// mixed: one segment per partner with the i particles rotated by -theta),
+// mixed: one partner window per segment with the i particles rotated by
+// -theta),
 // JU j's per step.  Survivors of the prefilter go to the wave's LDS ring;
 // whenever 64 are queued they are binned, one per lane, and the last
 // survivors of the item are flushed at its final step (the exact cascade
@@ -298,7 +308,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
         const double psi = v.psi[w.grp];
         const bool mixed = w.kind != 0;
         // same-event tiles run over the px-sorted list, mixed ones over the
-        // gather-order lists
+        // |pT|-sorted fast lists
         const Rec *__restrict__ pfi = (const Rec *)(mixed ? v.pfA : v.pfS);
         const Prec *__restrict__ cwi = (const Prec *)(mixed ? v.cwA : v.cwS);
         const Rec *pfj = (const Rec *)(mixed ? v.pfB : v.pfS);
@@ -320,20 +330,17 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
             icw[r] = cwi[i];
         }
         const bool partial = w.ni < 64 * R;
-        const int nseg = mixed ? (w.k1 - w.k0) : 1;
-        const long long row0 = mixed ? v.rowoff[w.e] + w.k0 : 0;
+        const int nseg = mixed ? w.nj : 1;
         unsigned head = 0, tail = 0;
         for (int seg = 0; seg < nseg; seg++) {
             long long pbase, jb, je;
             unsigned segbits;
             if (mixed) {
-                const long long row = row0 + seg;
-                const int pid = v.plan_ids[row];
-                pbase = v.mix_off[pid];
-                jb = pbase;
-                je = v.mix_off[pid + 1];
+                const JWin jw = v.mwin[w.j0 + seg];
+                pbase = jb = jw.b;
+                je = jw.e;
                 segbits = (unsigned)seg << 26;
-                const double c = v.plan_cs[2 * row], s = v.plan_cs[2 * row + 1];
+                const double c = v.plan_cs[2 * jw.row], s = v.plan_cs[2 * jw.row + 1];
 #pragma unroll
                 for (int r = 0; r < R; r++) {  // rotate i by -theta
                     rx[r] = (Prec)((double)ix[r] * c + (double)iy[r] * s);
@@ -361,7 +368,7 @@ __global__ __launch_bounds__(256) void k_fast(FastArgs a) {
                     if (tail - head >= 64 || (last && t >= cnt && tail != head)) {
                         const int n = (int)min(64u, tail - head);
                         __builtin_amdgcn_wave_barrier();
-                        const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, row0, psi);
+                        const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, psi);
                         if (mixed)
                             nb_mixed += nb;
                         else

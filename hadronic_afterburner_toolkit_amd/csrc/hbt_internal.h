@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hbt_hip.h"
@@ -116,8 +117,7 @@ struct RecordSet {
         cw64.resize(n);
         cw32.resize(n);
     }
-    void copy_from(const RecordSet &o, size_t dst, size_t src) {
-        rec[dst] = o.rec[src];
+    void copy_from(const RecordSet &o, size_t dst, size_t src) {  // prefilter records
         pf64[dst] = o.pf64[src];
         pf32[dst] = o.pf32[src];
         cw64[dst] = o.cw64[src];
@@ -126,25 +126,61 @@ struct RecordSet {
 };
 
 // Particles of one list after the rapidity cut, in the reference's gather
-// order (ev), and the same particles sorted by px inside each group (srt)
-// for the same-event tiles, whose j range is then limited to the px window
-// a binned pair can have: |px1 - px2| <= |p1T - p2T| <= sqrt(2) H.
+// order (keep, off: the plain kernels and the order of the pair cap), and
+// the fast kernel's lists:
+//  - fast: per event, the particles whose |pT| a binnable pair allows
+//    (|p1T| <= K_T + |q_T|/2 <= KT_max + D/2, >= KT_min - D/2 with
+//    D = sqrt(2) H, as |q_out|, |q_side| <= H), sorted by |pT|, so a mixed
+//    tile's partners are limited to a |pT| window (|pT1 - pT2| <= D);
+//  - srt: the fast list of each group sorted by px, so a same-event tile's
+//    j range is limited to a px window (|px1 - px2| <= D).
+// With the q_inv branch on (it bins pairs outside the 3D windows) neither
+// cut applies and D is infinite.
 struct HostParticles {
     long long n = 0, n_in = 0;
     std::vector<long long> keep;  // index into the input of each kept particle
     std::vector<long long> off;   // nev+1 kept-particle offsets
     std::vector<double> psi2;     // psi_ref (n=2) per group, over all input particles
-    RecordSet ev;                 // gather order
-    RecordSet srt;                // px-sorted inside each group
-    std::vector<int> orig;        // srt position -> gather-order index
-    std::vector<double> srt_px;   // px in srt order (work-item windows)
+    long long nf = 0;             // fast-list particles
+    std::vector<long long> foff;  // nev+1 fast-list offsets
+    RecordSet fast;               // fast list: records
+    std::vector<double> f_pt;     // |pT| of the fast list (mixed windows)
+    RecordSet srt;                // px-sorted per group (prefilter records only)
+    std::vector<int> orig;        // srt position -> fast-list index
+    std::vector<int> rank;        // srt position -> gather-order rank (pair orientation)
+    std::vector<double> srt_px;   // px in srt order (same-event windows)
     double pmax2 = 0.0;  // max px^2+py^2 of kept particles
 };
+
+// Half-width D of the px and |pT| windows of a binnable pair: a pair inside
+// the q_out and q_side windows has |q_T| <= sqrt(2) H (margins as the
+// prefilter's).  Infinite when the q_inv branch is on.
+inline double px_window(const Derived &d) {
+    if (d.P.qinv) return INFINITY;
+    const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
+    return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
+}
 
 // Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476),
 // psi_ref per group (:181-183, :233-249; over all particles passed in), and
 // the prefilter records.  Events keep < 2^18 particles (the fast kernel's
 // survivor encoding).
+// Runs fn(begin, end) over [0, n) split into contiguous ranges on up to 16
+// host threads (one when n is small).
+template <typename F>
+inline void parallel_ranges(long long n, long long min_per_thread, F fn) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const long long nt = std::max(1ll, std::min<long long>(hw, n / std::max(1ll, min_per_thread)));
+    if (nt <= 1) {
+        fn(0ll, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long long t = 0; t < nt; t++)
+        th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
+
 inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                             const int32_t *grp_ev, int nev, const int64_t *ev_off,
                             const double *const soa[8], HostParticles &h, std::string &err) {
@@ -163,78 +199,158 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     h.n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
     const double cut_max = tanh(k.HBTrap_max);
     const double cut_min = tanh(k.HBTrap_min);
-    h.keep.reserve((size_t)h.n_in);
+    auto kept = [&](int64_t i) {
+        const double ratio = soa[2][i] / soa[3][i];
+        return ratio > cut_min && ratio < cut_max;
+    };
+    // kept particles per event, then their offsets (gather order)
     h.off.assign(nev + 1, 0);
-    h.psi2.assign(ngroups, 0.0);
-    for (int g = 0; g < ngroups; g++) {
-        double vr = 0.0, vi = 0.0;
-        for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
+    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
+        for (long long e = e0; e < e1; e++) {
+            long long c = 0;
+            for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) c += kept(i);
+            h.off[e + 1] = c;
+        }
+    });
+    for (int e = 0; e < nev; e++) {
+        if (h.off[e + 1] >= (1 << 18))
+            return err = "an event keeps more than 262143 particles", HBT_ERR_CAPACITY;
+        h.off[e + 1] += h.off[e];
+    }
+    const long long n = h.off[nev];
+    if (n >= (1ll << 31)) return err = "buffer too large", HBT_ERR_CAPACITY;
+    h.n = n;
+    h.keep.resize((size_t)n);
+    RecordSet R;  // kept particles in gather order
+    R.resize((size_t)n);
+    const double eps32 = 5.9604644775390625e-8;
+    std::vector<double> pmax((size_t)nev + 1, 0.0), bmax((size_t)nev + 1, 0.0);
+    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
+        for (long long e = e0; e < e1; e++) {
+            long long q = h.off[e];
             for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
-                const double ratio = soa[2][i] / soa[3][i];
-                if (ratio > cut_min && ratio < cut_max) h.keep.push_back(i);
-                if (P.azimuthal) {
+                if (!kept(i)) continue;
+                h.keep[q] = i;
+                const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
+                R.rec[q] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
+                const double n2 = px * px + py * py;
+                pmax[e] = std::max(pmax[e], n2);
+                PfRec64 &r = R.pf64[q];
+                r.px = px;
+                r.py = py;
+                r.nrm = n2 - 0.5 * d.mid;
+                r.beta = P.boost ? pz / E : pz;
+                bmax[e] = std::max(bmax[e], std::fabs(r.beta));
+                R.cw64[q] = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
+                q++;
+            }
+        }
+    });
+    double bm = 0.0;
+    for (int e = 0; e < nev; e++) {
+        h.pmax2 = std::max(h.pmax2, pmax[e]);
+        bm = std::max(bm, bmax[e]);
+    }
+    parallel_ranges(n, 1 << 16, [&](long long q0, long long q1) {
+        for (long long q = q0; q < q1; q++) {
+            R.pf32[q].px = (float)R.pf64[q].px;
+            R.pf32[q].py = (float)R.pf64[q].py;
+            R.pf32[q].nrm = (float)R.pf64[q].nrm;
+            R.pf32[q].beta = (float)R.pf64[q].beta;
+            // covers the FP32 rounding of beta (|beta| <= bm) and of cw itself
+            R.cw32[q] = P.qinv ? INFINITY
+                               : (float)(R.cw64[q] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bm));
+        }
+    });
+    // psi_ref per group over all particles passed in, in order (a sum per group)
+    h.psi2.assign(ngroups, 0.0);
+    if (P.azimuthal) {
+        parallel_ranges(ngroups, 1, [&](long long g0, long long g1) {
+            for (long long g = g0; g < g1; g++) {
+                double vr = 0.0, vi = 0.0;
+                for (int64_t i = ev_off[grp_ev[g]]; i < ev_off[grp_ev[g + 1]]; i++) {
                     const double phi = atan2(soa[1][i], soa[0][i]);
                     double sn, cn;
                     sincos(2 * phi, &sn, &cn);  // as the reference's build evaluates it
                     vr += cn;
                     vi += sn;
                 }
+                h.psi2[g] = atan2(vi, vr) / 2;
             }
-            h.off[e + 1] = (long long)h.keep.size();
+        });
+    }
+    // fast list: |pT| range of binnable pairs, sorted by |pT| inside each event
+    const double D = px_window(d);
+    const double r_lo = std::isfinite(D) ? (P.KT_min - 0.5 * D) * (1.0 - 1e-9) : -INFINITY;
+    const double r_hi =
+        std::isfinite(D) ? (std::sqrt(P.KT_max_sq) + 0.5 * D) * (1.0 + 1e-9) : INFINITY;
+    auto pt_of = [&](long long q) {
+        return std::sqrt(R.pf64[q].px * R.pf64[q].px + R.pf64[q].py * R.pf64[q].py);
+    };
+    h.foff.assign(nev + 1, 0);
+    for (int e = 0; e < nev; e++) {
+        long long c = 0;
+        for (long long q = h.off[e]; q < h.off[e + 1]; q++) {
+            const double pt = pt_of(q);
+            c += (pt >= r_lo && pt <= r_hi);
         }
-        h.psi2[g] = P.azimuthal ? atan2(vi, vr) / 2 : 0.0;
+        h.foff[e + 1] = h.foff[e] + c;
     }
-    const long long n = (long long)h.keep.size();
-    for (int e = 0; e < nev; e++)
-        if (h.off[e + 1] - h.off[e] >= (1 << 18))
-            return err = "an event keeps more than 262143 particles", HBT_ERR_CAPACITY;
-    if (n >= (1ll << 31)) return err = "buffer too large", HBT_ERR_CAPACITY;
-    h.n = n;
-    RecordSet &R = h.ev;
-    R.resize((size_t)n);
-    double bmax = 0.0;
-    for (long long q = 0; q < n; q++) {
-        const long long i = h.keep[q];
-        const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
-        R.rec[q] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
-        const double n2 = px * px + py * py;
-        h.pmax2 = std::max(h.pmax2, n2);
-        PfRec64 &r = R.pf64[q];
-        r.px = px;
-        r.py = py;
-        r.nrm = n2 - 0.5 * d.mid;
-        r.beta = P.boost ? pz / E : pz;
-        bmax = std::max(bmax, std::fabs(r.beta));
-        R.cw64[q] = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
-    }
-    const double eps32 = 5.9604644775390625e-8;
-    for (long long q = 0; q < n; q++) {
-        R.pf32[q].px = (float)R.pf64[q].px;
-        R.pf32[q].py = (float)R.pf64[q].py;
-        R.pf32[q].nrm = (float)R.pf64[q].nrm;
-        R.pf32[q].beta = (float)R.pf64[q].beta;
-        // covers the FP32 rounding of beta (|beta| <= bmax) and of cw itself
-        R.cw32[q] = P.qinv ? INFINITY
-                           : (float)(R.cw64[q] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bmax));
-    }
-    // px-sorted copy per group (stable, so equal px keep gather order)
-    h.srt.resize((size_t)n);
-    h.orig.resize((size_t)n);
-    h.srt_px.resize((size_t)n);
-    std::vector<int> idx;
-    for (int g = 0; g < ngroups; g++) {
-        const long long g0 = h.off[grp_ev[g]], g1 = h.off[grp_ev[g + 1]];
-        idx.resize((size_t)(g1 - g0));
-        for (long long q = g0; q < g1; q++) idx[q - g0] = (int)q;
-        std::stable_sort(idx.begin(), idx.end(),
-                         [&](int a, int b) { return R.rec[a].px < R.rec[b].px; });
-        for (long long q = g0; q < g1; q++) {
-            const int src = idx[q - g0];
-            h.srt.copy_from(R, (size_t)q, (size_t)src);
-            h.orig[q] = src;
-            h.srt_px[q] = R.rec[src].px;
+    const long long nf = h.foff[nev];
+    h.nf = nf;
+    RecordSet &F = h.fast;
+    F.resize((size_t)nf);
+    h.f_pt.resize((size_t)nf);
+    std::vector<int> frank((size_t)nf);
+    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
+        std::vector<std::pair<double, long long>> ev;
+        for (long long e = e0; e < e1; e++) {
+            ev.clear();
+            for (long long q = h.off[e]; q < h.off[e + 1]; q++) {
+                const double pt = pt_of(q);
+                if (pt >= r_lo && pt <= r_hi) ev.emplace_back(pt, q);
+            }
+            std::stable_sort(ev.begin(), ev.end(),
+                             [](const std::pair<double, long long> &x,
+                                const std::pair<double, long long> &y) { return x.first < y.first; });
+            long long f = h.foff[e];
+            for (const auto &x : ev) {
+                F.rec[f] = R.rec[x.second];
+                F.pf64[f] = R.pf64[x.second];
+                F.pf32[f] = R.pf32[x.second];
+                F.cw64[f] = R.cw64[x.second];
+                F.cw32[f] = R.cw32[x.second];
+                h.f_pt[f] = x.first;
+                frank[f] = (int)x.second;
+                f++;
+            }
         }
-    }
+    });
+    // px-sorted prefilter records per group (stable)
+    h.srt.pf64.resize((size_t)nf);
+    h.srt.pf32.resize((size_t)nf);
+    h.srt.cw64.resize((size_t)nf);
+    h.srt.cw32.resize((size_t)nf);
+    h.orig.resize((size_t)nf);
+    h.rank.resize((size_t)nf);
+    h.srt_px.resize((size_t)nf);
+    parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
+        std::vector<int> idx;
+        for (long long g = ga; g < gb; g++) {
+            const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
+            idx.resize((size_t)(g1 - g0));
+            for (long long q = g0; q < g1; q++) idx[q - g0] = (int)q;
+            std::stable_sort(idx.begin(), idx.end(),
+                             [&](int a, int b) { return F.rec[a].px < F.rec[b].px; });
+            for (long long q = g0; q < g1; q++) {
+                const int src = idx[q - g0];
+                h.srt.copy_from(F, (size_t)q, (size_t)src);
+                h.orig[q] = src;
+                h.rank[q] = frank[src];
+                h.srt_px[q] = F.rec[src].px;
+            }
+        }
+    });
     return HBT_OK;
 }
 
@@ -242,37 +358,30 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
 struct ItemShape {
     int TI;  // i particles per item (64 R)
     int JC;  // same-event j chunk
-    int KC;  // mixed partners per item
+    int KC;  // mixed partner windows per item
 };
 
 inline ItemShape item_shape(const hbt_tuning &t) {
     return ItemShape{64 * t.R, std::max(64, t.jchunk), std::max(1, std::min(t.kchunk, 64))};
 }
 
-// Half-width of the px window of a binnable same-event pair: a pair inside
-// the q_out and q_side windows has |px1 - px2| <= |p1T - p2T| <= sqrt(2) H;
-// margins as the prefilter's.  Infinite (no window) when the q_inv branch
-// is on, since it bins pairs outside the 3D windows.
-inline double px_window(const Derived &d) {
-    if (d.P.qinv) return INFINITY;
-    const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
-    return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
-}
-
-// Same-event items of the groups, in px-sorted order: i tile of TI
-// consecutive particles x its j window (particles after the tile start whose
-// px lies within dpx of the tile's px range), cut into chunks of JC.
-// pairs (optional) receives each item's evaluated pair count.  enum_same
-// receives the reference's enumerated pair count sum n(n-1)/2.
+// Same-event items of the groups over the px-sorted fast lists (group g is
+// [foff[grp_ev[g]], foff[grp_ev[g+1]])): i tile of TI consecutive particles x
+// its j window (particles after the tile start whose px lies within dpx of
+// the tile's px range), cut into chunks of JC.  pairs (optional) receives
+// each item's evaluated pair count.  enum_same receives the reference's
+// enumerated pair count sum n(n-1)/2 over the kept lists (off).
 inline void items_same(const ItemShape &s, const std::vector<long long> &off,
-                       const std::vector<int> &grp_ev, const std::vector<double> &srt_px,
-                       double dpx, int view, std::vector<WorkItem> &items,
-                       std::vector<uint64_t> *pairs, uint64_t &enum_same) {
+                       const std::vector<long long> &foff, const std::vector<int> &grp_ev,
+                       const std::vector<double> &srt_px, double dpx, int view,
+                       std::vector<WorkItem> &items, std::vector<uint64_t> *pairs,
+                       uint64_t &enum_same) {
     const int ngroups = (int)grp_ev.size() - 1;
     for (int g = 0; g < ngroups; g++) {
-        const long long g0 = off[grp_ev[g]], n = off[grp_ev[g + 1]] - g0;
+        const long long nk = off[grp_ev[g + 1]] - off[grp_ev[g]];
+        if (nk >= 2) enum_same += (uint64_t)(nk * (nk - 1) / 2);
+        const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
         if (n < 2) continue;
-        enum_same += (uint64_t)(n * (n - 1) / 2);
         const double *px = srt_px.data() + g0;
         for (long long a = 0; a + 1 < n; a += s.TI) {
             const int ni = (int)std::min<long long>(s.TI, n - a);
@@ -307,39 +416,59 @@ inline void items_same(const ItemShape &s, const std::vector<long long> &off,
     }
 }
 
-// Mixed items: i tile of TI particles of event e x up to KC of its plan
-// rows [rowoff[e], rowoff[e+1]).  mix_off are the mixed list's event
-// offsets, plan_ids the partner of each row (index into the mixed list).
-inline void items_mixed(const ItemShape &s, const std::vector<long long> &off,
+// Mixed items over the fast lists (|pT|-sorted per event): i tile of TI
+// particles of event e x up to KC partner windows, one per plan row whose
+// partner has particles within dpt of the tile's |pT| range (rows with an
+// empty window are skipped).  Windows are appended to wins (absolute
+// indices into the mixed fast list, whose event offsets are mfoff); the
+// item's j0 / nj give its first window and their number.  plan_ids: the
+// partner of each plan row.  pairs (optional): evaluated pairs per item.
+inline void items_mixed(const ItemShape &s, const std::vector<long long> &foff,
                         const std::vector<int> &grp_ev, const std::vector<long long> &rowoff,
-                        const std::vector<int> &plan_ids, const std::vector<long long> &mix_off,
-                        int view, std::vector<WorkItem> &items, std::vector<uint64_t> *pairs) {
+                        const std::vector<int> &plan_ids, const std::vector<long long> &mfoff,
+                        const std::vector<double> &f_pt, const std::vector<double> &mf_pt,
+                        double dpt, int view, std::vector<WorkItem> &items,
+                        std::vector<JWin> &wins, std::vector<uint64_t> *pairs) {
     const int ngroups = (int)grp_ev.size() - 1;
+    const double *pt = mf_pt.data();
     for (int g = 0; g < ngroups; g++) {
         for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
-            const long long n1 = off[e + 1] - off[e];
-            const int M = (int)(rowoff[e + 1] - rowoff[e]);
+            const long long n1 = foff[e + 1] - foff[e];
             for (long long a = 0; a < n1; a += s.TI) {
-                for (int k0 = 0; k0 < M; k0 += s.KC) {
-                    WorkItem w{};
-                    w.i0 = off[e] + a;
-                    w.ni = (int)std::min<long long>(s.TI, n1 - a);
-                    w.kind = 1;
-                    w.e = e;
-                    w.k0 = k0;
-                    w.k1 = std::min(M, k0 + s.KC);
-                    w.grp = g;
-                    w.view = view;
+                const int ni = (int)std::min<long long>(s.TI, n1 - a);
+                const double lo = f_pt[foff[e] + a] - dpt;
+                const double hi = f_pt[foff[e] + a + ni - 1] + dpt;
+                WorkItem w{};
+                uint64_t cnt = 0;
+                auto flush = [&]() {
+                    if (w.nj == 0) return;
                     items.push_back(w);
-                    if (pairs) {
-                        uint64_t nj = 0;
-                        for (int k = w.k0; k < w.k1; k++) {
-                            const int p = plan_ids[rowoff[e] + k];
-                            nj += (uint64_t)(mix_off[p + 1] - mix_off[p]);
-                        }
-                        pairs->push_back((uint64_t)w.ni * nj);
+                    if (pairs) pairs->push_back(cnt);
+                    w.nj = 0;
+                    cnt = 0;
+                };
+                for (long long row = rowoff[e]; row < rowoff[e + 1]; row++) {
+                    const int p = plan_ids[row];
+                    long long jb = mfoff[p], je = mfoff[p + 1];
+                    if (std::isfinite(dpt)) {
+                        jb = std::lower_bound(pt + jb, pt + je, lo) - pt;
+                        je = std::upper_bound(pt + jb, pt + je, hi) - pt;
                     }
+                    if (je <= jb) continue;
+                    if (w.nj == 0) {
+                        w.i0 = foff[e] + a;
+                        w.ni = ni;
+                        w.j0 = (long long)wins.size();
+                        w.kind = 1;
+                        w.grp = g;
+                        w.view = view;
+                    }
+                    wins.push_back(JWin{(int)jb, (int)je, (int)row, 0});
+                    w.nj++;
+                    cnt += (uint64_t)ni * (uint64_t)(je - jb);
+                    if (w.nj == s.KC) flush();
                 }
+                flush();
             }
         }
     }

@@ -106,6 +106,8 @@ struct Request {
     std::vector<int> plan_ids;  // index into the request's events
     std::vector<double> plan_cs;
     std::vector<WorkItem> items;
+    std::vector<JWin> wins;  // partner windows of the mixed items
+    std::vector<OrigRank> orr;  // px-sorted position -> fast-list index, gather rank
     std::vector<uint64_t> item_pairs;
     size_t next = 0;           // next item to launch
     uint64_t pairs_left = 0;   // enumerated pairs not yet launched
@@ -113,9 +115,8 @@ struct Request {
     double pmax2 = 0.0;
     // device residency
     size_t foot = 0, arena_off = 0;
-    size_t o_bins = 0, o_rec = 0, o_pf = 0, o_cw = 0, o_srec = 0, o_spf = 0, o_scw = 0,
-           o_orig = 0, o_off = 0, o_rowoff = 0, o_ids = 0,
-           o_cs = 0, o_psi = 0;
+    size_t o_bins = 0, o_rec = 0, o_pf = 0, o_cw = 0, o_spf = 0, o_scw = 0,
+           o_orig = 0, o_win = 0, o_cs = 0, o_psi = 0;
     int view = -1, rslot = -1;
     View v{};
     hipEvent_t done_ev = nullptr;
@@ -183,31 +184,25 @@ int sfail(hbt_server *s, int code, const std::string &msg) {
 
 // Request layout in its arena block; returns the footprint.
 size_t layout(const hbt_server *s, Request &r) {
-    const long long n = r.hp.n, rows = (long long)r.plan_ids.size();
+    const long long nf = r.hp.nf, rows = (long long)r.plan_ids.size();
     size_t o = 0;
     r.o_bins = o;
     o = align256(o + s->result_bytes);
-    // gather-order and px-sorted records
-    for (size_t *po : {&r.o_rec, &r.o_srec}) {
-        *po = o;
-        o = align256(o + (size_t)n * sizeof(PartRec));
-    }
+    // fast-list records, their prefilter records, and the px-sorted copy
+    r.o_rec = o;
+    o = align256(o + (size_t)nf * sizeof(PartRec));
     for (size_t *po : {&r.o_pf, &r.o_spf}) {
         *po = o;
-        o = align256(o + (size_t)n * (s->f32 ? sizeof(PfRec32) : sizeof(PfRec64)));
+        o = align256(o + (size_t)nf * (s->f32 ? sizeof(PfRec32) : sizeof(PfRec64)));
     }
     for (size_t *po : {&r.o_cw, &r.o_scw}) {
         *po = o;
-        o = align256(o + (size_t)n * (s->f32 ? 4 : 8));
+        o = align256(o + (size_t)nf * (s->f32 ? 4 : 8));
     }
     r.o_orig = o;
-    o = align256(o + (size_t)n * 4);
-    r.o_off = o;
-    o = align256(o + (size_t)(r.nev + 1) * 8);
-    r.o_rowoff = o;
-    o = align256(o + (size_t)(r.nev + 1) * 8);
-    r.o_ids = o;
-    o = align256(o + (size_t)std::max<long long>(rows, 1) * 4);
+    o = align256(o + (size_t)nf * sizeof(OrigRank));
+    r.o_win = o;
+    o = align256(o + std::max<size_t>(r.wins.size(), 1) * sizeof(JWin));
     r.o_cs = o;
     o = align256(o + (size_t)std::max<long long>(rows, 1) * 16);
     r.o_psi = o;
@@ -228,13 +223,10 @@ void make_request_view(hbt_server *s, Request &r) {
     v.A = v.B = (const PartRec *)(b + r.o_rec);
     v.pfA = v.pfB = (const void *)(b + r.o_pf);
     v.cwA = v.cwB = (const void *)(b + r.o_cw);
-    v.S = (const PartRec *)(b + r.o_srec);
     v.pfS = (const void *)(b + r.o_spf);
     v.cwS = (const void *)(b + r.o_scw);
-    v.origS = (const int *)(b + r.o_orig);
-    v.mix_off = (const long long *)(b + r.o_off);
-    v.rowoff = (const long long *)(b + r.o_rowoff);
-    v.plan_ids = (const int *)(b + r.o_ids);
+    v.origS = (const OrigRank *)(b + r.o_orig);
+    v.mwin = (const JWin *)(b + r.o_win);
     v.plan_cs = (const double *)(b + r.o_cs);
     v.psi = (const double *)(b + r.o_psi);
 }
@@ -242,29 +234,27 @@ void make_request_view(hbt_server *s, Request &r) {
 // Uploads a request into its arena block on the upload stream and waits.
 hipError_t upload_request(hbt_server *s, Request &r) {
     char *b = s->d_arena + r.arena_off;
-    const long long n = r.hp.n;
+    const long long nf = r.hp.nf;
     hipError_t e = launch_zero_words(b + r.o_bins, s->result_bytes / 8, s->s_upload);
     auto cp = [&](size_t off, const void *src, size_t bytes) {
         if (e == hipSuccess && bytes)
             e = hipMemcpyAsync(b + off, src, bytes, hipMemcpyHostToDevice, s->s_upload);
     };
-    const std::pair<const RecordSet *, size_t> sets[] = {{&r.hp.ev, 0}, {&r.hp.srt, 1}};
+    const std::pair<const RecordSet *, size_t> sets[] = {{&r.hp.fast, 0}, {&r.hp.srt, 1}};
     for (const auto &x : sets) {
         const RecordSet &R = *x.first;
         const bool srt = x.second == 1;
-        cp(srt ? r.o_srec : r.o_rec, R.rec.data(), (size_t)n * sizeof(PartRec));
+        if (!srt) cp(r.o_rec, R.rec.data(), (size_t)nf * sizeof(PartRec));
         if (s->f32) {
-            cp(srt ? r.o_spf : r.o_pf, R.pf32.data(), (size_t)n * sizeof(PfRec32));
-            cp(srt ? r.o_scw : r.o_cw, R.cw32.data(), (size_t)n * 4);
+            cp(srt ? r.o_spf : r.o_pf, R.pf32.data(), (size_t)nf * sizeof(PfRec32));
+            cp(srt ? r.o_scw : r.o_cw, R.cw32.data(), (size_t)nf * 4);
         } else {
-            cp(srt ? r.o_spf : r.o_pf, R.pf64.data(), (size_t)n * sizeof(PfRec64));
-            cp(srt ? r.o_scw : r.o_cw, R.cw64.data(), (size_t)n * 8);
+            cp(srt ? r.o_spf : r.o_pf, R.pf64.data(), (size_t)nf * sizeof(PfRec64));
+            cp(srt ? r.o_scw : r.o_cw, R.cw64.data(), (size_t)nf * 8);
         }
     }
-    cp(r.o_orig, r.hp.orig.data(), (size_t)n * 4);
-    cp(r.o_off, r.hp.off.data(), (size_t)(r.nev + 1) * 8);
-    cp(r.o_rowoff, r.rowoff.data(), (size_t)(r.nev + 1) * 8);
-    cp(r.o_ids, r.plan_ids.data(), r.plan_ids.size() * 4);
+    cp(r.o_orig, r.orr.data(), r.orr.size() * sizeof(OrigRank));
+    cp(r.o_win, r.wins.data(), r.wins.size() * sizeof(JWin));
     cp(r.o_cs, r.plan_cs.data(), r.plan_cs.size() * 8);
     cp(r.o_psi, r.hp.psi2.data(), r.hp.psi2.size() * 8);
     cp(0, nullptr, 0);
@@ -334,6 +324,8 @@ void admission_loop(hbt_server *s) {
             r->hp = std::move(keep_meta);
             r->plan_cs.clear();
             r->plan_cs.shrink_to_fit();
+            std::vector<JWin>().swap(r->wins);
+            std::vector<OrigRank>().swap(r->orr);
             s->active.push_back(r);
         }
         s->cv.notify_all();
@@ -714,10 +706,14 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
     // while every enumerated total stays within it
     const bool qinv = s->cfg.invariant_radius_flag == 1;
     r->items.clear();
-    items_same(s->shape, off, r->grp_ev, r->hp.srt_px, px_window(s->D), 0, r->items,
-               &r->item_pairs, r->enum_same);
-    items_mixed(s->shape, off, r->grp_ev, r->rowoff, r->plan_ids, off, 0, r->items,
-                &r->item_pairs);
+    const double D = px_window(s->D);
+    const HostParticles &hp = r->hp;
+    items_same(s->shape, off, hp.foff, r->grp_ev, hp.srt_px, D, 0, r->items, &r->item_pairs,
+               r->enum_same);
+    items_mixed(s->shape, hp.foff, r->grp_ev, r->rowoff, r->plan_ids, hp.foff, hp.f_pt, hp.f_pt,
+                D, 0, r->items, r->wins, &r->item_pairs);
+    r->orr.resize((size_t)hp.nf);
+    for (long long q = 0; q < hp.nf; q++) r->orr[q] = OrigRank{hp.orig[q], hp.rank[q]};
     for (uint64_t t : {r->enum_same, r->enum_mixed})
         if (t > s->D.needed + 1 || (qinv && t > s->D.needed_qinv))
             return sfail(s, HBT_ERR_CAPACITY,
