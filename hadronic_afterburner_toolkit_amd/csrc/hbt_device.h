@@ -35,7 +35,9 @@ enum { BIN_COUNT = 0, BIN_QO = 1, BIN_QS = 2, BIN_QL = 3, BIN_NUM = 4, BIN_DEN =
        BIN_STRIDE = 8 };
 
 struct DevHist {
-    double *bins;  // [nbins][BIN_STRIDE]; COUNT and DEN hold u64 bit patterns
+    // [nbins][BIN_STRIDE]; COUNT holds the pair count as an FP64 integer
+    // (exact below 2^53 pairs per bin), DEN a u64 bit pattern
+    double *bins;
     unsigned long long *qinv_count, *qinv_den;
     double *qinv_sum, *qinv_num;
     unsigned long long *stage;  // [0..4] same, [5..9] mixed
@@ -241,7 +243,7 @@ __device__ __forceinline__ int eval_mixed_f64(const DevParams &P, const DevHist 
 // Direct accumulation of one pair (HBT_correlation.cpp:437-446, :681-683).
 __device__ __forceinline__ void add_same_direct(const DevHist &H, const SameRec &r) {
     double *b = H.bins + r.k * BIN_STRIDE;
-    atomic_add_u64((unsigned long long *)(b + BIN_COUNT), 1ull);
+    atomic_add_f64(b + BIN_COUNT, 1.0);
     atomic_add_f64(b + BIN_QO, r.qo);
     atomic_add_f64(b + BIN_QS, r.qs);
     atomic_add_f64(b + BIN_QL, r.ql);

@@ -248,12 +248,13 @@ __global__ void k_zero_words(unsigned long long *p, size_t n) {
 // Sum of a u64 field over the q bins of each (kt, kphi): the reference's
 // number_of_pairs_*_KTdiff / _KTKphidiff counters.  Element k of group g
 // is h[(g*per + k)*stride].
-__global__ void k_sum_groups(const unsigned long long *h, long per, int stride,
-                             unsigned long long *out) {
+// T = double for the count field (integers held exactly below 2^53).
+template <typename T>
+__global__ void k_sum_groups(const T *h, long per, int stride, unsigned long long *out) {
     const int g = blockIdx.x;
     unsigned long long acc = 0;
     for (long k = threadIdx.x; k < per; k += blockDim.x)
-        acc += h[((long)g * per + k) * stride];
+        acc += (unsigned long long)h[((long)g * per + k) * stride];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
     __shared__ unsigned long long part[16];
     if (__lane_id() == 0) part[threadIdx.x >> 6] = acc;
@@ -457,11 +458,12 @@ int zero_hist(hbt_ctx *c) {
 }
 
 // Per-(kt,kphi) sums of a u64 histogram field into host memory.
-int group_sums(hbt_ctx *c, const unsigned long long *h, long per, int groups, int stride,
+template <typename T>
+int group_sums(hbt_ctx *c, const T *h, long per, int groups, int stride,
                std::vector<unsigned long long> &out) {
     out.assign(groups, 0);
     if (groups <= 0) return HBT_OK;
-    hipLaunchKernelGGL(k_sum_groups, dim3(groups), dim3(1024), 0, c->stream, h, per, stride,
+    hipLaunchKernelGGL(k_sum_groups<T>, dim3(groups), dim3(1024), 0, c->stream, h, per, stride,
                        c->d_sums);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipMemcpyAsync(out.data(), c->d_sums, groups * 8, hipMemcpyDeviceToHost,
@@ -483,10 +485,11 @@ int check_cap(hbt_ctx *c, bool mixed, uint64_t enum_call, bool &binds) {
     if (total <= c->needed + 1 && (!qinv || total <= c->needed_qinv)) return HBT_OK;
     const int groups = c->cfg.n_KT * c->P.nphi;
     std::vector<unsigned long long> cnt;
-    const unsigned long long *field =
-        (const unsigned long long *)(c->d_bins + (mixed ? BIN_DEN : BIN_COUNT));
-    int rc = group_sums(c, field, (long)c->P.qnpts * c->P.qnpts * c->P.qnpts, groups, BIN_STRIDE,
-                        cnt);
+    const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
+    int rc = mixed ? group_sums(c, (const unsigned long long *)(c->d_bins + BIN_DEN), per, groups,
+                                BIN_STRIDE, cnt)
+                   : group_sums(c, (const double *)(c->d_bins + BIN_COUNT), per, groups,
+                                BIN_STRIDE, cnt);
     if (rc) return rc;
     for (auto v : cnt)
         if (v + enum_call > c->needed + 1) binds = true;
@@ -732,8 +735,10 @@ int run_capped(hbt_ctx *c, bool mixed) {
     // remaining capacity per key from the counters so far (sums of the bins)
     std::vector<unsigned long long> c3, cq(nkq, 0);
     const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
-    rc = group_sums(c, (const unsigned long long *)(c->d_bins + (mixed ? BIN_DEN : BIN_COUNT)),
-                    per, nkey3, BIN_STRIDE, c3);
+    rc = mixed ? group_sums(c, (const unsigned long long *)(c->d_bins + BIN_DEN), per, nkey3,
+                            BIN_STRIDE, c3)
+               : group_sums(c, (const double *)(c->d_bins + BIN_COUNT), per, nkey3, BIN_STRIDE,
+                            c3);
     if (!rc && c->P.qinv)
         rc = group_sums(c, mixed ? c->d_qinv_den : c->d_qinv_count, c->P.qnpts, nkq, 1, cq);
     if (rc) return rc;
@@ -1308,7 +1313,7 @@ int hbt_read(hbt_ctx *c, hbt_result *o) {
         HIPCHECK(c, hipMemcpy(aos.data(), c->d_bins, aos.size() * 8, hipMemcpyDeviceToHost));
         for (long k = 0; k < c->nbins; k++) {
             const double *b = aos.data() + k * BIN_STRIDE;
-            if (o->num_count) std::memcpy(&o->num_count[k], b + BIN_COUNT, 8);
+            if (o->num_count) o->num_count[k] = (uint64_t)b[BIN_COUNT];
             if (o->den) std::memcpy(&o->den[k], b + BIN_DEN, 8);
             if (o->q_out_sum) o->q_out_sum[k] = b[BIN_QO];
             if (o->q_side_sum) o->q_side_sum[k] = b[BIN_QS];
@@ -1326,7 +1331,7 @@ int hbt_read(hbt_ctx *c, hbt_result *o) {
     const int nKT = c->cfg.n_KT, nphi = c->P.nphi;
     const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
     std::vector<unsigned long long> sn, sd, qn, qd;
-    int rc = group_sums(c, (const unsigned long long *)(c->d_bins + BIN_COUNT), per, nKT * nphi,
+    int rc = group_sums(c, (const double *)(c->d_bins + BIN_COUNT), per, nKT * nphi,
                         BIN_STRIDE, sn);
     if (!rc)
         rc = group_sums(c, (const unsigned long long *)(c->d_bins + BIN_DEN), per, nKT * nphi,

@@ -196,9 +196,11 @@ struct Stage {
 // of the i particle in the tile.
 // Binned same-event pairs are added cooperatively: they are staged in LDS,
 // then groups of 5 lanes add one pair's count and four sums, so one atomic
-// instruction touches 12 bins (12 cache lines) instead of 64, and the five
-// values of a bin travel together (atomics on this part execute at the
-// memory side, where the cost is per line request).
+// instruction touches 12 bins (12 cache lines) instead of 64.  Atomic
+// throughput on this part is set by (instruction, cache line) accesses,
+// about 2.4e10 per second whatever the histogram's size or placement, so
+// the count is an FP64 add in the same instruction as the sums: one access
+// per binned pair.
 template <typename Prec, bool EXT, int QCAP>
 __device__ __forceinline__ int drain(const DevParams &P, const View &v, const unsigned *q,
                                      Stage &stg,
@@ -240,16 +242,15 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
         stg.v[idx][3] = rec.c;
     }
     __builtin_amdgcn_wave_barrier();
+    // lane sub of a group adds field sub of its pair's bin: the count (an
+    // exact FP64 integer) and the four sums go out as one FP64 atomic
+    // instruction, one cache line per pair
     const int grp = lane / 5, sub = lane - 5 * grp;
     for (int base = 0; base < nb; base += 12) {
         const int p = base + grp;
-        if (lane < 60 && p < nb) {
-            double *b = v.H.bins + stg.k[p] * BIN_STRIDE;
-            if (sub == 0)
-                atomic_add_u64((unsigned long long *)(b + BIN_COUNT), 1ull);
-            else
-                atomic_add_f64(b + sub, stg.v[p][sub - 1]);
-        }
+        if (lane < 60 && p < nb)
+            atomic_add_f64(v.H.bins + stg.k[p] * BIN_STRIDE + sub,
+                           sub == 0 ? 1.0 : stg.v[p][max(sub - 1, 0)]);
     }
     __builtin_amdgcn_wave_barrier();
     return binned ? 1 : 0;

@@ -486,8 +486,8 @@ inline void result_from_host(const Derived &d, const hbt_config &k, const double
     std::vector<unsigned long long> sn((size_t)nKT * nphi, 0), sd((size_t)nKT * nphi, 0);
     for (long b = 0; b < d.nbins; b++) {
         const double *x = aos + b * BIN_STRIDE;
-        unsigned long long cnt, den;
-        std::memcpy(&cnt, x + BIN_COUNT, 8);
+        unsigned long long den;
+        const unsigned long long cnt = (unsigned long long)x[BIN_COUNT];
         std::memcpy(&den, x + BIN_DEN, 8);
         sn[b / per] += cnt;
         sd[b / per] += den;

@@ -12,6 +12,14 @@
 //   u64_1_per_lane     one u64 per pair (the mixed-event denominator)
 //   lds_4_per_lane     four FP64 LDS atomics per pair into a 16K-entry table
 //
+// Locality of the same-event scatter (aos64_coop5 throughout):
+//   coop5_bins_N       histogram of N 64-byte bins (from 4 MB-class L2 size to
+//                      8x the workload's histogram)
+//   coop5_xcd_private  one histogram copy per XCD, the copy chosen by the
+//                      wave's XCC_ID hardware register
+//   coop5_xcd_part     one histogram, each XCD touching only the bins
+//                      b = xcc (mod 8)
+//
 // Prints one line per pattern: ms per launch and binned pairs/s.
 #include <hip/hip_runtime.h>
 
@@ -71,6 +79,41 @@ __global__ void k_coop5(double *h, unsigned nbins, unsigned per) {
     }
 }
 
+// XCD of the executing wave (hardware register XCC_ID, bits 3:0), 0..7.
+__device__ __forceinline__ unsigned xcc_id() {
+    return (unsigned)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;
+}
+
+__global__ void k_coop5_private(double *h, unsigned nbins, unsigned per) {
+    const unsigned lane = threadIdx.x & 63u;
+    if (lane >= 60) return;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned pair = wave * 12u + lane / 5u, sub = lane % 5u;
+    double *copy = h + (size_t)xcc_id() * nbins * 8;
+    for (unsigned it = 0; it < per; it++) {
+        double *p = copy + (size_t)(mix32(pair * 977u + it * 131071u) % nbins) * 8;
+        if (sub == 0)
+            add_u64(p);
+        else
+            unsafeAtomicAdd(p + sub, 0.1 * sub);
+    }
+}
+
+__global__ void k_coop5_part(double *h, unsigned nbins, unsigned per) {
+    const unsigned lane = threadIdx.x & 63u;
+    if (lane >= 60) return;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned pair = wave * 12u + lane / 5u, sub = lane % 5u;
+    const unsigned x = xcc_id(), n8 = nbins / 8u;
+    for (unsigned it = 0; it < per; it++) {
+        double *p = h + (size_t)((mix32(pair * 977u + it * 131071u) % n8) * 8u + x) * 8;
+        if (sub == 0)
+            add_u64(p);
+        else
+            unsafeAtomicAdd(p + sub, 0.1 * sub);
+    }
+}
+
 __global__ void k_u64(double *h, unsigned nbins, unsigned per) {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (unsigned it = 0; it < per; it++) add_u64(&h[mix32(tid * 977u + it * 131071u) % nbins]);
@@ -91,7 +134,7 @@ __global__ void k_lds(double *h, unsigned nbins, unsigned per) {
 
 int main() {
     const unsigned nbins = 41u * 41u * 41u * 5u;
-    const size_t bytes = (size_t)nbins * 8 * sizeof(double);
+    const size_t bytes = (size_t)nbins * 8 * sizeof(double) * 8;  // room for 8 copies
     double *h = nullptr;
     CK(hipMalloc(&h, bytes));
     CK(hipMemset(h, 0, bytes));
@@ -124,6 +167,32 @@ int main() {
         const double per_launch = ms / reps;
         std::printf("%-18s %8.3f ms/launch  %.3e binned pairs/s\n", p.name, per_launch,
                     lanes * p.pairs_per_lane_iter / (per_launch * 1e-3));
+        std::fflush(stdout);
+    }
+    const struct {
+        const char *name;
+        void (*k)(double *, unsigned, unsigned);
+        unsigned nbins;
+    } loc[] = {{"coop5_bins_16384", k_coop5, 16384u},
+               {"coop5_bins_65536", k_coop5, 65536u},
+               {"coop5_bins_344605", k_coop5, nbins},
+               {"coop5_bins_2756840", k_coop5, nbins * 8u},
+               {"coop5_xcd_private", k_coop5_private, nbins},
+               {"coop5_xcd_part", k_coop5_part, nbins}};
+    for (const auto &p : loc) {
+        hipLaunchKernelGGL(p.k, dim3(blocks), dim3(threads), 0, 0, h, p.nbins, per);
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+        CK(hipEventRecord(e0));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(p.k, dim3(blocks), dim3(threads), 0, 0, h, p.nbins, per);
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        const double per_launch = ms / reps;
+        std::printf("%-18s %8.3f ms/launch  %.3e binned pairs/s\n", p.name, per_launch,
+                    lanes * (12.0 / 64.0) / (per_launch * 1e-3));
         std::fflush(stdout);
     }
     CK(hipFree(h));
