@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -282,6 +283,29 @@ def copy_calibration(nbytes, reps):
 
 
 TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "k_fast_traffic.md")
+ATOMICS_BENCH = os.path.join(REPO, "hadronic_afterburner_toolkit_amd", "hist_atomics_bench.e")
+
+
+def atomic_ceiling(log):
+    """Chip-wide rate of histogram atomics, in (instruction, cache line)
+    accesses per second: the f64coop5 pattern of the histogram-atomics
+    micro-benchmark (5 lanes per bin, 12 bins per FP64 atomic instruction,
+    as the fast kernel's scatter), run as a child process before this
+    process touches the GPU.  None when the binary is missing or fails."""
+    if not os.path.exists(ATOMICS_BENCH):
+        return None
+    try:
+        r = subprocess.run([ATOMICS_BENCH], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        log(f"atomics micro-benchmark failed: {e}")
+        return None
+    for line in r.stdout.splitlines():
+        parts = line.split()
+        if r.returncode == 0 and len(parts) >= 4 and parts[0] == "f64coop5":
+            return float(parts[3])
+    log("atomics micro-benchmark printed no f64coop5 line")
+    return None
 
 
 def measured_traffic(kernel_prefix):
@@ -337,6 +361,8 @@ def main():
         return
 
     tuning = [int(v) for v in args.tuning.split(",")] if args.tuning else None
+    # the roofline needs the plain kernel's stage counts
+    atom_ceil = atomic_ceiling(log) if args.only in (None, "plain") else None
     from hadronic_afterburner_toolkit_amd import _lib
     log(f"generating {args.events} events, Poisson({args.mult}), groups of {args.per_group}")
     groups = make_workload(args.events, args.mult, args.per_group, args.seed)
@@ -411,6 +437,16 @@ def main():
                            "frac": ach / peak, "traffic": measured_traffic(kname),
                            "kernel": kname,
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
+        # the bound that holds: every binned pair is one atomic cache-line
+        # access (same-event: count + four sums in one FP64 instruction;
+        # mixed: the u64 denominator), against the measured chip-wide rate
+        acc = s[4] + m[4]
+        out["atomic_roofline"] = {
+            "bound": "l2_atomics", "achieved": acc / dev_s, "ceiling": atom_ceil,
+            "unit": "atomic line accesses/s",
+            "frac": (acc / dev_s / atom_ceil) if atom_ceil else None,
+            "accesses_per_launch": acc, "kernel": kname,
+            "ceiling_source": "hist_atomics_bench.e f64coop5, measured in this run"}
     if not args.no_serve:
         out["serving"] = serve(args, log)
     if not args.no_extra:

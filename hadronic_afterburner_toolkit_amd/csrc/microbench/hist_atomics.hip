@@ -11,6 +11,11 @@
 //                      (the fast kernel's same-event scatter)
 //   u64_1_per_lane     one u64 per pair (the mixed-event denominator)
 //   lds_4_per_lane     four FP64 LDS atomics per pair into a 16K-entry table
+//   f64coop5           64-byte bins, 5 lanes per pair, count as an FP64 add:
+//                      one instruction per 12 pairs (the fast kernel's scatter)
+//   u32_1_per_lane     one u32 per pair
+//   f64coop5_quarter   f64coop5 on a quarter of the workgroups (scaling with
+//                      the number of issuing CUs)
 //
 // Locality of the same-event scatter (aos64_coop5 throughout):
 //   coop5_bins_N       histogram of N 64-byte bins (from 4 MB-class L2 size to
@@ -77,6 +82,25 @@ __global__ void k_coop5(double *h, unsigned nbins, unsigned per) {
         else
             unsafeAtomicAdd(p + sub, 0.1 * sub);
     }
+}
+
+__global__ void k_f64coop5(double *h, unsigned nbins, unsigned per) {
+    const unsigned lane = threadIdx.x & 63u;
+    if (lane >= 60) return;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const unsigned pair = wave * 12u + lane / 5u, sub = lane % 5u;
+    for (unsigned it = 0; it < per; it++) {
+        double *p = h + (size_t)(mix32(pair * 977u + it * 131071u) % nbins) * 8;
+        unsafeAtomicAdd(p + sub, sub == 0 ? 1.0 : 0.1 * sub);
+    }
+}
+
+__global__ void k_u32(double *h, unsigned nbins, unsigned per) {
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned *u = (unsigned *)h;
+    for (unsigned it = 0; it < per; it++)
+        __hip_atomic_fetch_add(u + 2 * (mix32(tid * 977u + it * 131071u) % nbins), 1u,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // XCD of the executing wave (hardware register XCC_ID, bits 3:0), 0..7.
@@ -193,6 +217,31 @@ int main() {
         const double per_launch = ms / reps;
         std::printf("%-18s %8.3f ms/launch  %.3e binned pairs/s\n", p.name, per_launch,
                     lanes * (12.0 / 64.0) / (per_launch * 1e-3));
+        std::fflush(stdout);
+    }
+    const struct {
+        const char *name;
+        void (*k)(double *, unsigned, unsigned);
+        unsigned blocks;
+        double pairs_per_lane_iter;
+    } more[] = {{"f64coop5", k_f64coop5, blocks, 12.0 / 64.0},
+                {"u32_1_per_lane", k_u32, blocks, 1.0},
+                {"f64coop5_quarter", k_f64coop5, blocks / 4, 12.0 / 64.0}};
+    for (const auto &p : more) {
+        hipLaunchKernelGGL(p.k, dim3(p.blocks), dim3(threads), 0, 0, h, nbins, per);
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+        CK(hipEventRecord(e0));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(p.k, dim3(p.blocks), dim3(threads), 0, 0, h, nbins, per);
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        const double per_launch = ms / reps;
+        const double n = (double)p.blocks * threads * per * p.pairs_per_lane_iter;
+        std::printf("%-18s %8.3f ms/launch  %.3e binned pairs/s\n", p.name, per_launch,
+                    n / (per_launch * 1e-3));
         std::fflush(stdout);
     }
     CK(hipFree(h));
