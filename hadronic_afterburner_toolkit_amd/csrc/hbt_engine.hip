@@ -879,13 +879,14 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         }
         s.cap = (size_t)n;
     }
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;  // prefilter records of one precision
     if ((size_t)nf > s.cap_f || !s.d_orig) {
         for (DevRecords *r : {&s.ev, &s.srt}) {
             if (!rc && r == &s.ev) rc = dev_alloc(c, &r->rec, (size_t)nf);
-            if (!rc) rc = dev_alloc(c, &r->pf64, (size_t)nf);
-            if (!rc) rc = dev_alloc(c, &r->pf32, (size_t)nf);
-            if (!rc) rc = dev_alloc(c, &r->cw64, (size_t)nf);
-            if (!rc) rc = dev_alloc(c, &r->cw32, (size_t)nf);
+            if (!rc && !f32) rc = dev_alloc(c, &r->pf64, (size_t)nf);
+            if (!rc && f32) rc = dev_alloc(c, &r->pf32, (size_t)nf);
+            if (!rc && !f32) rc = dev_alloc(c, &r->cw64, (size_t)nf);
+            if (!rc && f32) rc = dev_alloc(c, &r->cw32, (size_t)nf);
         }
         if (!rc) rc = dev_alloc(c, &s.d_orig, (size_t)nf);
         if (rc) return rc;
@@ -909,10 +910,13 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
             const RecordSet &r = *x.second;
             if (d.rec)
                 HIPCHECK(c, hipMemcpy(d.rec, r.rec.data(), nf * sizeof(PartRec), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.pf64, r.pf64.data(), nf * sizeof(PfRec64), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.pf32, r.pf32.data(), nf * sizeof(PfRec32), hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), nf * 8, hipMemcpyHostToDevice));
-            HIPCHECK(c, hipMemcpy(d.cw32, r.cw32.data(), nf * 4, hipMemcpyHostToDevice));
+            if (f32) {
+                HIPCHECK(c, hipMemcpy(d.pf32, r.pf32.data(), nf * sizeof(PfRec32), hipMemcpyHostToDevice));
+                HIPCHECK(c, hipMemcpy(d.cw32, r.cw32.data(), nf * 4, hipMemcpyHostToDevice));
+            } else {
+                HIPCHECK(c, hipMemcpy(d.pf64, r.pf64.data(), nf * sizeof(PfRec64), hipMemcpyHostToDevice));
+                HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), nf * 8, hipMemcpyHostToDevice));
+            }
         }
         std::vector<OrigRank> orr((size_t)nf);
         for (long long q = 0; q < nf; q++) orr[q] = OrigRank{h.orig[q], h.rank[q]};

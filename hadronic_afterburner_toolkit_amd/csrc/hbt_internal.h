@@ -103,25 +103,33 @@ inline void fp32_margins(const Derived &d, double pmax2, DevParams &P) {
     P.f_qthr = std::isfinite(d.P.pf_qthr) ? (float)(d.P.pf_qthr + m) : INFINITY;
 }
 
-// Fast-kernel records of one ordering of a particle list.
+// Fast-kernel records of one ordering of a particle list: the 64-byte
+// particle records and the prefilter records of one precision (FP32 for
+// the precision form, FP64 otherwise; the other pair stays empty).
 struct RecordSet {
     std::vector<PartRec> rec;
     std::vector<PfRec64> pf64;
     std::vector<PfRec32> pf32;
     std::vector<double> cw64;
     std::vector<float> cw32;
-    void resize(size_t n) {
-        rec.resize(n);
-        pf64.resize(n);
-        pf32.resize(n);
-        cw64.resize(n);
-        cw32.resize(n);
+    void resize_prefilter(size_t n, bool f32) {
+        pf64.resize(f32 ? 0 : n);
+        cw64.resize(f32 ? 0 : n);
+        pf32.resize(f32 ? n : 0);
+        cw32.resize(f32 ? n : 0);
     }
-    void copy_from(const RecordSet &o, size_t dst, size_t src) {  // prefilter records
-        pf64[dst] = o.pf64[src];
-        pf32[dst] = o.pf32[src];
-        cw64[dst] = o.cw64[src];
-        cw32[dst] = o.cw32[src];
+    void resize(size_t n, bool f32) {
+        rec.resize(n);
+        resize_prefilter(n, f32);
+    }
+    void copy_from(const RecordSet &o, size_t dst, size_t src, bool f32) {  // prefilter records
+        if (f32) {
+            pf32[dst] = o.pf32[src];
+            cw32[dst] = o.cw32[src];
+        } else {
+            pf64[dst] = o.pf64[src];
+            cw64[dst] = o.cw64[src];
+        }
     }
 };
 
@@ -149,7 +157,7 @@ struct HostParticles {
     std::vector<int> orig;        // srt position -> fast-list index
     std::vector<int> rank;        // srt position -> gather-order rank (pair orientation)
     std::vector<double> srt_px;   // px in srt order (same-event windows)
-    double pmax2 = 0.0;  // max px^2+py^2 of kept particles
+    double pmax2 = 0.0;  // max px^2+py^2 over the fast list (FP32 prefilter margins)
 };
 
 // Half-width D of the px and |pT| windows of a binnable pair: a pair inside
@@ -195,6 +203,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     for (int g = 0; g < ngroups; g++)
         if (grp_ev[g + 1] < grp_ev[g]) return err = "group offsets decrease", HBT_ERR_ARG;
     const DevParams &P = d.P;
+    const bool f32 = k.precision == HBT_PREC_FP32C;
     h = HostParticles();
     h.n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
     const double cut_max = tanh(k.HBTrap_max);
@@ -203,63 +212,93 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
         const double ratio = soa[2][i] / soa[3][i];
         return ratio > cut_min && ratio < cut_max;
     };
-    // kept particles per event, then their offsets (gather order)
+    // the fast lists' |pT| range: particles a binnable pair can contain
+    const double D = px_window(d);
+    const double r_lo = std::isfinite(D) ? (P.KT_min - 0.5 * D) * (1.0 - 1e-9) : -INFINITY;
+    const double r_hi =
+        std::isfinite(D) ? (std::sqrt(P.KT_max_sq) + 0.5 * D) * (1.0 + 1e-9) : INFINITY;
+    auto pt_of = [&](int64_t i) { return std::sqrt(soa[0][i] * soa[0][i] + soa[1][i] * soa[1][i]); };
+    auto fast = [&](double pt) { return pt >= r_lo && pt <= r_hi; };
+    auto beta_of = [&](int64_t i) { return P.boost ? soa[2][i] / soa[3][i] : soa[2][i]; };
+    // pass 1, per event: kept and fast-list counts, and the fast list's
+    // momentum and beta scales (FP32 margins)
     h.off.assign(nev + 1, 0);
-    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
-        for (long long e = e0; e < e1; e++) {
-            long long c = 0;
-            for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) c += kept(i);
-            h.off[e + 1] = c;
-        }
-    });
-    for (int e = 0; e < nev; e++) {
-        if (h.off[e + 1] >= (1 << 18))
-            return err = "an event keeps more than 262143 particles", HBT_ERR_CAPACITY;
-        h.off[e + 1] += h.off[e];
-    }
-    const long long n = h.off[nev];
-    if (n >= (1ll << 31)) return err = "buffer too large", HBT_ERR_CAPACITY;
-    h.n = n;
-    h.keep.resize((size_t)n);
-    RecordSet R;  // kept particles in gather order
-    R.resize((size_t)n);
-    const double eps32 = 5.9604644775390625e-8;
+    h.foff.assign(nev + 1, 0);
     std::vector<double> pmax((size_t)nev + 1, 0.0), bmax((size_t)nev + 1, 0.0);
     parallel_ranges(nev, 64, [&](long long e0, long long e1) {
         for (long long e = e0; e < e1; e++) {
-            long long q = h.off[e];
+            long long c = 0, cf = 0;
             for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
                 if (!kept(i)) continue;
-                h.keep[q] = i;
-                const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
-                R.rec[q] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
-                const double n2 = px * px + py * py;
-                pmax[e] = std::max(pmax[e], n2);
-                PfRec64 &r = R.pf64[q];
-                r.px = px;
-                r.py = py;
-                r.nrm = n2 - 0.5 * d.mid;
-                r.beta = P.boost ? pz / E : pz;
-                bmax[e] = std::max(bmax[e], std::fabs(r.beta));
-                R.cw64[q] = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
-                q++;
+                c++;
+                const double pt = pt_of(i);
+                if (!fast(pt)) continue;
+                cf++;
+                pmax[e] = std::max(pmax[e], soa[0][i] * soa[0][i] + soa[1][i] * soa[1][i]);
+                bmax[e] = std::max(bmax[e], std::fabs(beta_of(i)));
             }
+            h.off[e + 1] = c;
+            h.foff[e + 1] = cf;
         }
     });
     double bm = 0.0;
     for (int e = 0; e < nev; e++) {
+        if (h.off[e + 1] >= (1 << 18))
+            return err = "an event keeps more than 262143 particles", HBT_ERR_CAPACITY;
+        h.off[e + 1] += h.off[e];
+        h.foff[e + 1] += h.foff[e];
         h.pmax2 = std::max(h.pmax2, pmax[e]);
         bm = std::max(bm, bmax[e]);
     }
-    parallel_ranges(n, 1 << 16, [&](long long q0, long long q1) {
-        for (long long q = q0; q < q1; q++) {
-            R.pf32[q].px = (float)R.pf64[q].px;
-            R.pf32[q].py = (float)R.pf64[q].py;
-            R.pf32[q].nrm = (float)R.pf64[q].nrm;
-            R.pf32[q].beta = (float)R.pf64[q].beta;
-            // covers the FP32 rounding of beta (|beta| <= bm) and of cw itself
-            R.cw32[q] = P.qinv ? INFINITY
-                               : (float)(R.cw64[q] * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bm));
+    const long long n = h.off[nev], nf = h.foff[nev];
+    if (n >= (1ll << 31)) return err = "buffer too large", HBT_ERR_CAPACITY;
+    h.n = n;
+    h.nf = nf;
+    h.keep.resize((size_t)n);
+    RecordSet &F = h.fast;
+    F.resize((size_t)nf, f32);
+    h.f_pt.resize((size_t)nf);
+    std::vector<int> frank((size_t)nf);
+    const double eps32 = 5.9604644775390625e-8;
+    // pass 2, per event: gather order, then the fast list sorted by |pT|
+    // (stable, so equal |pT| keep gather order)
+    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
+        std::vector<std::pair<double, long long>> ev;  // |pT|, kept index
+        std::vector<int64_t> src;
+        for (long long e = e0; e < e1; e++) {
+            ev.clear();
+            long long q = h.off[e];
+            for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
+                if (!kept(i)) continue;
+                h.keep[q] = i;
+                const double pt = pt_of(i);
+                if (fast(pt)) ev.emplace_back(pt, q);
+                q++;
+            }
+            std::stable_sort(ev.begin(), ev.end(),
+                             [](const std::pair<double, long long> &x,
+                                const std::pair<double, long long> &y) { return x.first < y.first; });
+            long long f = h.foff[e];
+            for (const auto &x : ev) {
+                const int64_t i = h.keep[x.second];
+                const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
+                F.rec[f] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
+                const double nrm = px * px + py * py - 0.5 * d.mid;
+                const double beta = P.boost ? pz / E : pz;
+                const double cw = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
+                if (f32) {
+                    F.pf32[f] = PfRec32{(float)px, (float)py, (float)nrm, (float)beta};
+                    // covers the FP32 rounding of beta (|beta| <= bm) and of cw itself
+                    F.cw32[f] = P.qinv ? INFINITY
+                                       : (float)(cw * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bm));
+                } else {
+                    F.pf64[f] = PfRec64{px, py, nrm, beta};
+                    F.cw64[f] = cw;
+                }
+                h.f_pt[f] = x.first;
+                frank[f] = (int)x.second;
+                f++;
+            }
         }
     });
     // psi_ref per group over all particles passed in, in order (a sum per group)
@@ -279,58 +318,8 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
             }
         });
     }
-    // fast list: |pT| range of binnable pairs, sorted by |pT| inside each event
-    const double D = px_window(d);
-    const double r_lo = std::isfinite(D) ? (P.KT_min - 0.5 * D) * (1.0 - 1e-9) : -INFINITY;
-    const double r_hi =
-        std::isfinite(D) ? (std::sqrt(P.KT_max_sq) + 0.5 * D) * (1.0 + 1e-9) : INFINITY;
-    auto pt_of = [&](long long q) {
-        return std::sqrt(R.pf64[q].px * R.pf64[q].px + R.pf64[q].py * R.pf64[q].py);
-    };
-    h.foff.assign(nev + 1, 0);
-    for (int e = 0; e < nev; e++) {
-        long long c = 0;
-        for (long long q = h.off[e]; q < h.off[e + 1]; q++) {
-            const double pt = pt_of(q);
-            c += (pt >= r_lo && pt <= r_hi);
-        }
-        h.foff[e + 1] = h.foff[e] + c;
-    }
-    const long long nf = h.foff[nev];
-    h.nf = nf;
-    RecordSet &F = h.fast;
-    F.resize((size_t)nf);
-    h.f_pt.resize((size_t)nf);
-    std::vector<int> frank((size_t)nf);
-    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
-        std::vector<std::pair<double, long long>> ev;
-        for (long long e = e0; e < e1; e++) {
-            ev.clear();
-            for (long long q = h.off[e]; q < h.off[e + 1]; q++) {
-                const double pt = pt_of(q);
-                if (pt >= r_lo && pt <= r_hi) ev.emplace_back(pt, q);
-            }
-            std::stable_sort(ev.begin(), ev.end(),
-                             [](const std::pair<double, long long> &x,
-                                const std::pair<double, long long> &y) { return x.first < y.first; });
-            long long f = h.foff[e];
-            for (const auto &x : ev) {
-                F.rec[f] = R.rec[x.second];
-                F.pf64[f] = R.pf64[x.second];
-                F.pf32[f] = R.pf32[x.second];
-                F.cw64[f] = R.cw64[x.second];
-                F.cw32[f] = R.cw32[x.second];
-                h.f_pt[f] = x.first;
-                frank[f] = (int)x.second;
-                f++;
-            }
-        }
-    });
     // px-sorted prefilter records per group (stable)
-    h.srt.pf64.resize((size_t)nf);
-    h.srt.pf32.resize((size_t)nf);
-    h.srt.cw64.resize((size_t)nf);
-    h.srt.cw32.resize((size_t)nf);
+    h.srt.resize_prefilter((size_t)nf, f32);
     h.orig.resize((size_t)nf);
     h.rank.resize((size_t)nf);
     h.srt_px.resize((size_t)nf);
@@ -344,7 +333,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                              [&](int a, int b) { return F.rec[a].px < F.rec[b].px; });
             for (long long q = g0; q < g1; q++) {
                 const int src = idx[q - g0];
-                h.srt.copy_from(F, (size_t)q, (size_t)src);
+                h.srt.copy_from(F, (size_t)q, (size_t)src, f32);
                 h.orig[q] = src;
                 h.rank[q] = frank[src];
                 h.srt_px[q] = F.rec[src].px;
