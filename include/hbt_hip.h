@@ -202,8 +202,10 @@ int hbt_sync(hbt_ctx *ctx, double *batch_ms);
 int hbt_batch_info(hbt_ctx *ctx, uint64_t *enum_same, uint64_t *enum_mixed, int32_t *nitems);
 
 /* Launch shape of the fast kernel: R i-particles per lane (1, 2, 4), j
- * chunk of same-event tiles, partners per mixed tile, persistent blocks per
- * CU; ms is the measured batch time when set by hbt_autotune. */
+ * chunk of same-event tiles, partner windows per mixed work item (one per
+ * plan row whose partner has particles in the tile's |pT| window),
+ * persistent blocks per CU; ms is the measured batch time when set by
+ * hbt_autotune. */
 typedef struct {
     int32_t R;
     int32_t jchunk;
