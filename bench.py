@@ -240,16 +240,22 @@ def batched_uneven(args, log):
 def large_histogram(args, log, tuning):
     """Large-histogram form: 1000 pi+ events Poisson(4000), groups of 100,
     20 rotations per event, q 61^3 on [-0.2, 0.2] GeV, 8 K_T bins on
-    [0.15, 1.0] GeV, 16 K_phi bins: 3.2e7 bins (2.1 GB of histogram)."""
+    [0.15, 1.0] GeV, 16 K_phi bins: 3.2e7 bins (2.1 GB of histogram).
+    One launch takes seconds, so instead of the autotuner's sweep the
+    headline's launch shape runs at R = 1 and R = 2 and the faster counts."""
     from hadronic_afterburner_toolkit_amd import _lib, samples
     full = samples.generate(args.seed + 40, 1000, 4000.0)
     params = dict(PARAMS, qnpts=61, q_min=-0.2, q_max=0.2, n_KT=9, KT_min=0.15, KT_max=1.0,
                   azimuthal_flag=1, n_Kphi=16, number_of_mixed_events=20)
     groups = [full.events(g, 100) for g in range(0, 1000, 100)]
-    line, _ = batch_line(groups, params, args.seed + 41, _lib.HBT_PREC_FP32C, 1, tuning,
-                         False, log, "large histogram")
-    line.update(events=1000, per_group=100, bins=61 ** 3 * 9 * 16)
-    return line
+    best = None
+    for R in (1, 2):
+        line, _ = batch_line(groups, params, args.seed + 41, _lib.HBT_PREC_FP32C, 1,
+                             (R,) + tuple(tuning[1:]), False, log, f"large histogram R={R}")
+        if best is None or line["ms"] < best["ms"]:
+            best = line
+    best.update(events=1000, per_group=100, bins=61 ** 3 * 9 * 16)
+    return best
 
 
 def cpu_baseline(group, plan_g, log):

@@ -285,13 +285,14 @@ struct Ring {
 // survivors of the item are flushed at its final step (the exact cascade
 // has a single call site).
 // Occupancy: the exact stage is latency-bound (dependent loads of the two
-// particle records per lane), so the common-case kernels trade a few spilled
-// registers for more waves per SIMD: 5 at R=1 (96 VGPRs), 4 at R=2
-// (128 VGPRs); measured 5 % faster on the bench workload.  The K_phi / q_inv
-// variants and R=4 keep the compiler's choice.
+// particle records per lane), so the kernels trade a few spilled registers
+// for more waves per SIMD: common case 5 at R=1 (96 VGPRs) and 4 at R=2
+// (128 VGPRs), 5 % faster on the bench workload; K_phi / q_inv variants 3
+// at R=1, 2 (168 VGPRs), 11-16 % faster on the K_phi workloads.  R=4 keeps
+// the compiler's choice.
 template <int R, bool EXT>
 constexpr int fast_waves_per_eu() {
-    return EXT ? 1 : (R == 1 ? 5 : (R == 2 ? 4 : 1));
+    return R == 4 ? 1 : (EXT ? 3 : (R == 1 ? 5 : 4));
 }
 
 template <typename Prec, int R, bool EXT>
