@@ -290,12 +290,12 @@ struct Slot {
     // |pT|-sorted per event) and its px-sorted copy per group
     long long nf = 0;
     std::vector<long long> foff;  // nev+1 fast-list offsets
-    std::vector<double> f_pt;     // |pT| of the fast list (mixed windows)
+    hvec<double> f_pt;            // |pT| of the fast list (mixed windows)
     DevRecords ev, srt;
     OrigRank *d_orig = nullptr;   // srt position -> fast-list index, gather rank
     size_t cap = 0, cap_f = 0, cap_ev = 0;
     double pmax2 = 0.0;     // max px^2+py^2 of kept particles
-    std::vector<double> srt_px;  // px of the px-sorted list (work-item windows)
+    hvec<double> srt_px;          // px of the px-sorted list (work-item windows)
     bool valid = false;
 };
 
@@ -594,11 +594,14 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
     L.items.clear();
     L.wins.clear();
     enum_same = 0;
-    if (same) items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px, D, 0, L.items, nullptr, enum_same);
+    if (same)
+        items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px.data(), D, 0, L.items, nullptr,
+                   enum_same);
     c->nitems_same = (int)L.items.size();
     if (mixed && c->plan_valid) {
         const Slot &m = mixed_slot(c);
-        items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt, m.f_pt, D, 0,
+        items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt.data(),
+                    m.f_pt.data(), D, 0,
                     L.items, L.wins, nullptr);
     }
 }
@@ -897,7 +900,7 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         if (rc) return rc;
         s.cap_ev = (size_t)nev + 1;
     }
-    std::vector<double> col((size_t)n);
+    hvec<double> col((size_t)n);
     for (int a = 0; a < 8; a++) {
         for (long long k = 0; k < n; k++) col[k] = soa[a][h.keep[k]];
         HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
@@ -918,9 +921,7 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
                 HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), nf * 8, hipMemcpyHostToDevice));
             }
         }
-        std::vector<OrigRank> orr((size_t)nf);
-        for (long long q = 0; q < nf; q++) orr[q] = OrigRank{h.orig[q], h.rank[q]};
-        HIPCHECK(c, hipMemcpy(s.d_orig, orr.data(), nf * sizeof(OrigRank), hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(s.d_orig, h.orr.data(), nf * sizeof(OrigRank), hipMemcpyHostToDevice));
     }
     s.nf = nf;
     s.foff = std::move(h.foff);

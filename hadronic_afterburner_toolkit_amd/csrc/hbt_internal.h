@@ -10,8 +10,11 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/hbt_hip.h"
@@ -103,15 +106,42 @@ inline void fp32_margins(const Derived &d, double pmax2, DevParams &P) {
     P.f_qthr = std::isfinite(d.P.pf_qthr) ? (float)(d.P.pf_qthr + m) : INFINITY;
 }
 
+// Allocator whose construct() without arguments default-initializes: a
+// resize() of plain data leaves the memory untouched, so the pages are
+// first written (and faulted in) by the parallel loops that fill them
+// rather than by one thread zeroing hundreds of MB.
+template <typename T, typename A = std::allocator<T>>
+struct default_init_allocator : A {
+    using traits = std::allocator_traits<A>;
+    template <typename U>
+    struct rebind {
+        using other = default_init_allocator<U, typename traits::template rebind_alloc<U>>;
+    };
+    using A::A;
+    default_init_allocator() = default;
+    template <typename U, typename B>
+    default_init_allocator(const default_init_allocator<U, B> &o) : A(o) {}
+    template <typename U>
+    void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U *p, Args &&...args) {
+        traits::construct(static_cast<A &>(*this), p, std::forward<Args>(args)...);
+    }
+};
+template <typename T>
+using hvec = std::vector<T, default_init_allocator<T>>;
+
 // Fast-kernel records of one ordering of a particle list: the 64-byte
 // particle records and the prefilter records of one precision (FP32 for
 // the precision form, FP64 otherwise; the other pair stays empty).
 struct RecordSet {
-    std::vector<PartRec> rec;
-    std::vector<PfRec64> pf64;
-    std::vector<PfRec32> pf32;
-    std::vector<double> cw64;
-    std::vector<float> cw32;
+    hvec<PartRec> rec;
+    hvec<PfRec64> pf64;
+    hvec<PfRec32> pf32;
+    hvec<double> cw64;
+    hvec<float> cw32;
     void resize_prefilter(size_t n, bool f32) {
         pf64.resize(f32 ? 0 : n);
         cw64.resize(f32 ? 0 : n);
@@ -146,17 +176,16 @@ struct RecordSet {
 // cut applies and D is infinite.
 struct HostParticles {
     long long n = 0, n_in = 0;
-    std::vector<long long> keep;  // index into the input of each kept particle
+    hvec<long long> keep;         // index into the input of each kept particle
     std::vector<long long> off;   // nev+1 kept-particle offsets
     std::vector<double> psi2;     // psi_ref (n=2) per group, over all input particles
     long long nf = 0;             // fast-list particles
     std::vector<long long> foff;  // nev+1 fast-list offsets
     RecordSet fast;               // fast list: records
-    std::vector<double> f_pt;     // |pT| of the fast list (mixed windows)
+    hvec<double> f_pt;            // |pT| of the fast list (mixed windows)
     RecordSet srt;                // px-sorted per group (prefilter records only)
-    std::vector<int> orig;        // srt position -> fast-list index
-    std::vector<int> rank;        // srt position -> gather-order rank (pair orientation)
-    std::vector<double> srt_px;   // px in srt order (same-event windows)
+    hvec<OrigRank> orr;           // srt position -> fast-list index, gather rank
+    hvec<double> srt_px;          // px in srt order (same-event windows)
     double pmax2 = 0.0;  // max px^2+py^2 over the fast list (FP32 prefilter margins)
 };
 
@@ -258,7 +287,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     RecordSet &F = h.fast;
     F.resize((size_t)nf, f32);
     h.f_pt.resize((size_t)nf);
-    std::vector<int> frank((size_t)nf);
+    hvec<int> frank((size_t)nf);
     const double eps32 = 5.9604644775390625e-8;
     // pass 2, per event: gather order, then the fast list sorted by |pT|
     // (stable, so equal |pT| keep gather order)
@@ -275,9 +304,8 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                 if (fast(pt)) ev.emplace_back(pt, q);
                 q++;
             }
-            std::stable_sort(ev.begin(), ev.end(),
-                             [](const std::pair<double, long long> &x,
-                                const std::pair<double, long long> &y) { return x.first < y.first; });
+            // (|pT|, gather index) pairs are distinct: the order equals a stable sort by |pT|
+            std::sort(ev.begin(), ev.end());
             long long f = h.foff[e];
             for (const auto &x : ev) {
                 const int64_t i = h.keep[x.second];
@@ -320,22 +348,20 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     }
     // px-sorted prefilter records per group (stable)
     h.srt.resize_prefilter((size_t)nf, f32);
-    h.orig.resize((size_t)nf);
-    h.rank.resize((size_t)nf);
+    h.orr.resize((size_t)nf);
     h.srt_px.resize((size_t)nf);
     parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
-        std::vector<int> idx;
+        std::vector<std::pair<double, int>> idx;  // (px, fast-list index)
         for (long long g = ga; g < gb; g++) {
             const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
             idx.resize((size_t)(g1 - g0));
-            for (long long q = g0; q < g1; q++) idx[q - g0] = (int)q;
-            std::stable_sort(idx.begin(), idx.end(),
-                             [&](int a, int b) { return F.rec[a].px < F.rec[b].px; });
+            for (long long q = g0; q < g1; q++) idx[q - g0] = {F.rec[q].px, (int)q};
+            // distinct pairs: the order equals a stable sort by px
+            std::sort(idx.begin(), idx.end());
             for (long long q = g0; q < g1; q++) {
-                const int src = idx[q - g0];
+                const int src = idx[q - g0].second;
                 h.srt.copy_from(F, (size_t)q, (size_t)src, f32);
-                h.orig[q] = src;
-                h.rank[q] = frank[src];
+                h.orr[q] = OrigRank{src, frank[src]};
                 h.srt_px[q] = F.rec[src].px;
             }
         }
@@ -354,55 +380,94 @@ inline ItemShape item_shape(const hbt_tuning &t) {
     return ItemShape{64 * t.R, std::max(64, t.jchunk), std::max(1, std::min(t.kchunk, 64))};
 }
 
+// Work items of consecutive chunks of groups, built on several host threads
+// and concatenated in group order (the result is the serial one).
+struct ItemChunk {
+    std::vector<WorkItem> items;
+    std::vector<JWin> wins;
+    std::vector<uint64_t> pairs;
+    uint64_t enum_same = 0;
+};
+
+template <typename F>
+inline void items_by_group_chunks(int ngroups, std::vector<WorkItem> &items,
+                                  std::vector<JWin> *wins, std::vector<uint64_t> *pairs,
+                                  uint64_t *enum_same, F build) {
+    const int nchunks = std::max(1, std::min(ngroups, 64));
+    std::vector<ItemChunk> ch((size_t)nchunks);
+    parallel_ranges(nchunks, 1, [&](long long c0, long long c1) {
+        for (long long c = c0; c < c1; c++)
+            build((int)((long long)ngroups * c / nchunks),
+                  (int)((long long)ngroups * (c + 1) / nchunks), ch[(size_t)c]);
+    });
+    size_t nw = wins ? wins->size() : 0;
+    for (const ItemChunk &c : ch) {
+        for (WorkItem w : c.items) {
+            if (w.kind != 0) w.j0 += (long long)nw;  // window index in the joint table
+            items.push_back(w);
+        }
+        if (wins) {
+            wins->insert(wins->end(), c.wins.begin(), c.wins.end());
+            nw = wins->size();
+        }
+        if (pairs) pairs->insert(pairs->end(), c.pairs.begin(), c.pairs.end());
+        if (enum_same) *enum_same += c.enum_same;
+    }
+}
+
 // Same-event items of the groups over the px-sorted fast lists (group g is
 // [foff[grp_ev[g]], foff[grp_ev[g+1]])): i tile of TI consecutive particles x
 // its j window (particles after the tile start whose px lies within dpx of
-// the tile's px range), cut into chunks of JC.  pairs (optional) receives
-// each item's evaluated pair count.  enum_same receives the reference's
-// enumerated pair count sum n(n-1)/2 over the kept lists (off).
+// the tile's px range), cut into chunks of JC.  srt_px: px in that order.
+// pairs (optional) receives each item's evaluated pair count.  enum_same
+// receives the reference's enumerated pair count sum n(n-1)/2 over the kept
+// lists (off).
 inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                        const std::vector<long long> &foff, const std::vector<int> &grp_ev,
-                       const std::vector<double> &srt_px, double dpx, int view,
+                       const double *srt_px, double dpx, int view,
                        std::vector<WorkItem> &items, std::vector<uint64_t> *pairs,
                        uint64_t &enum_same) {
     const int ngroups = (int)grp_ev.size() - 1;
-    for (int g = 0; g < ngroups; g++) {
-        const long long nk = off[grp_ev[g + 1]] - off[grp_ev[g]];
-        if (nk >= 2) enum_same += (uint64_t)(nk * (nk - 1) / 2);
-        const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
-        if (n < 2) continue;
-        const double *px = srt_px.data() + g0;
-        for (long long a = 0; a + 1 < n; a += s.TI) {
-            const int ni = (int)std::min<long long>(s.TI, n - a);
-            long long jhi = n;
-            if (std::isfinite(dpx)) {
-                // px ascending inside the group: the tile spans [px[a], px[a+ni-1]]
-                const double top = px[a + ni - 1] + dpx;
-                jhi = std::upper_bound(px + a, px + n, top) - px;
-            }
-            for (long long jb = a + 1; jb < jhi; jb += s.JC) {
-                WorkItem w{};
-                w.i0 = g0 + a;
-                w.j0 = g0 + jb;
-                w.ni = ni;
-                w.nj = (int)std::min<long long>(s.JC, jhi - jb);
-                w.kind = 0;
-                w.grp = g;
-                w.view = view;
-                items.push_back(w);
-                if (pairs) {
-                    // pairs (i, j) with i in [a, a+ni), j in [jb, jb+nj), i < j
-                    uint64_t cnt = 0;
-                    const long long j1 = jb + w.nj;
-                    for (long long i = a; i < a + ni; i++) {
-                        const long long lo = std::max(jb, i + 1);
-                        if (j1 > lo) cnt += (uint64_t)(j1 - lo);
+    items_by_group_chunks(ngroups, items, nullptr, pairs, &enum_same,
+                          [&](int ga, int gb, ItemChunk &out) {
+        for (int g = ga; g < gb; g++) {
+            const long long nk = off[grp_ev[g + 1]] - off[grp_ev[g]];
+            if (nk >= 2) out.enum_same += (uint64_t)(nk * (nk - 1) / 2);
+            const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
+            if (n < 2) continue;
+            const double *px = srt_px + g0;
+            for (long long a = 0; a + 1 < n; a += s.TI) {
+                const int ni = (int)std::min<long long>(s.TI, n - a);
+                long long jhi = n;
+                if (std::isfinite(dpx)) {
+                    // px ascending inside the group: the tile spans [px[a], px[a+ni-1]]
+                    const double top = px[a + ni - 1] + dpx;
+                    jhi = std::upper_bound(px + a, px + n, top) - px;
+                }
+                for (long long jb = a + 1; jb < jhi; jb += s.JC) {
+                    WorkItem w{};
+                    w.i0 = g0 + a;
+                    w.j0 = g0 + jb;
+                    w.ni = ni;
+                    w.nj = (int)std::min<long long>(s.JC, jhi - jb);
+                    w.kind = 0;
+                    w.grp = g;
+                    w.view = view;
+                    out.items.push_back(w);
+                    if (pairs) {
+                        // pairs (i, j) with i in [a, a+ni), j in [jb, jb+nj), i < j
+                        uint64_t cnt = 0;
+                        const long long j1 = jb + w.nj;
+                        for (long long i = a; i < a + ni; i++) {
+                            const long long lo = std::max(jb, i + 1);
+                            if (j1 > lo) cnt += (uint64_t)(j1 - lo);
+                        }
+                        out.pairs.push_back(cnt);
                     }
-                    pairs->push_back(cnt);
                 }
             }
         }
-    }
+    });
 }
 
 // Mixed items over the fast lists (|pT|-sorted per event): i tile of TI
@@ -411,56 +476,60 @@ inline void items_same(const ItemShape &s, const std::vector<long long> &off,
 // empty window are skipped).  Windows are appended to wins (absolute
 // indices into the mixed fast list, whose event offsets are mfoff); the
 // item's j0 / nj give its first window and their number.  plan_ids: the
-// partner of each plan row.  pairs (optional): evaluated pairs per item.
+// partner of each plan row; f_pt / mf_pt: |pT| of the two fast lists.
+// pairs (optional): evaluated pairs per item.
 inline void items_mixed(const ItemShape &s, const std::vector<long long> &foff,
                         const std::vector<int> &grp_ev, const std::vector<long long> &rowoff,
                         const std::vector<int> &plan_ids, const std::vector<long long> &mfoff,
-                        const std::vector<double> &f_pt, const std::vector<double> &mf_pt,
-                        double dpt, int view, std::vector<WorkItem> &items,
-                        std::vector<JWin> &wins, std::vector<uint64_t> *pairs) {
+                        const double *f_pt, const double *mf_pt, double dpt, int view,
+                        std::vector<WorkItem> &items, std::vector<JWin> &wins,
+                        std::vector<uint64_t> *pairs) {
     const int ngroups = (int)grp_ev.size() - 1;
-    const double *pt = mf_pt.data();
-    for (int g = 0; g < ngroups; g++) {
-        for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
-            const long long n1 = foff[e + 1] - foff[e];
-            for (long long a = 0; a < n1; a += s.TI) {
-                const int ni = (int)std::min<long long>(s.TI, n1 - a);
-                const double lo = f_pt[foff[e] + a] - dpt;
-                const double hi = f_pt[foff[e] + a + ni - 1] + dpt;
-                WorkItem w{};
-                uint64_t cnt = 0;
-                auto flush = [&]() {
-                    if (w.nj == 0) return;
-                    items.push_back(w);
-                    if (pairs) pairs->push_back(cnt);
-                    w.nj = 0;
-                    cnt = 0;
-                };
-                for (long long row = rowoff[e]; row < rowoff[e + 1]; row++) {
-                    const int p = plan_ids[row];
-                    long long jb = mfoff[p], je = mfoff[p + 1];
-                    if (std::isfinite(dpt)) {
-                        jb = std::lower_bound(pt + jb, pt + je, lo) - pt;
-                        je = std::upper_bound(pt + jb, pt + je, hi) - pt;
+    const double *pt = mf_pt;
+    items_by_group_chunks(ngroups, items, &wins, pairs, nullptr,
+                          [&](int ga, int gb, ItemChunk &out) {
+        for (int g = ga; g < gb; g++) {
+            for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
+                const long long n1 = foff[e + 1] - foff[e];
+                for (long long a = 0; a < n1; a += s.TI) {
+                    const int ni = (int)std::min<long long>(s.TI, n1 - a);
+                    const double lo = f_pt[foff[e] + a] - dpt;
+                    const double hi = f_pt[foff[e] + a + ni - 1] + dpt;
+                    WorkItem w{};
+                    uint64_t cnt = 0;
+                    auto flush = [&]() {
+                        if (w.nj == 0) return;
+                        out.items.push_back(w);
+                        if (pairs) out.pairs.push_back(cnt);
+                        w.nj = 0;
+                        cnt = 0;
+                    };
+                    for (long long row = rowoff[e]; row < rowoff[e + 1]; row++) {
+                        const int p = plan_ids[row];
+                        long long jb = mfoff[p], je = mfoff[p + 1];
+                        if (std::isfinite(dpt)) {
+                            jb = std::lower_bound(pt + jb, pt + je, lo) - pt;
+                            je = std::upper_bound(pt + jb, pt + je, hi) - pt;
+                        }
+                        if (je <= jb) continue;
+                        if (w.nj == 0) {
+                            w.i0 = foff[e] + a;
+                            w.ni = ni;
+                            w.j0 = (long long)out.wins.size();  // chunk-local until joined
+                            w.kind = 1;
+                            w.grp = g;
+                            w.view = view;
+                        }
+                        out.wins.push_back(JWin{(int)jb, (int)je, (int)row, 0});
+                        w.nj++;
+                        cnt += (uint64_t)ni * (uint64_t)(je - jb);
+                        if (w.nj == s.KC) flush();
                     }
-                    if (je <= jb) continue;
-                    if (w.nj == 0) {
-                        w.i0 = foff[e] + a;
-                        w.ni = ni;
-                        w.j0 = (long long)wins.size();
-                        w.kind = 1;
-                        w.grp = g;
-                        w.view = view;
-                    }
-                    wins.push_back(JWin{(int)jb, (int)je, (int)row, 0});
-                    w.nj++;
-                    cnt += (uint64_t)ni * (uint64_t)(je - jb);
-                    if (w.nj == s.KC) flush();
+                    flush();
                 }
-                flush();
             }
         }
-    }
+    });
 }
 
 // hbt_result from the AoS bins and q_inv arrays in host memory; the pair

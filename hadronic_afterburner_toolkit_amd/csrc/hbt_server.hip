@@ -107,7 +107,6 @@ struct Request {
     std::vector<double> plan_cs;
     std::vector<WorkItem> items;
     std::vector<JWin> wins;  // partner windows of the mixed items
-    std::vector<OrigRank> orr;  // px-sorted position -> fast-list index, gather rank
     std::vector<uint64_t> item_pairs;
     size_t next = 0;           // next item to launch
     uint64_t pairs_left = 0;   // enumerated pairs not yet launched
@@ -253,7 +252,7 @@ hipError_t upload_request(hbt_server *s, Request &r) {
             cp(srt ? r.o_scw : r.o_cw, R.cw64.data(), (size_t)nf * 8);
         }
     }
-    cp(r.o_orig, r.orr.data(), r.orr.size() * sizeof(OrigRank));
+    cp(r.o_orig, r.hp.orr.data(), r.hp.orr.size() * sizeof(OrigRank));
     cp(r.o_win, r.wins.data(), r.wins.size() * sizeof(JWin));
     cp(r.o_cs, r.plan_cs.data(), r.plan_cs.size() * 8);
     cp(r.o_psi, r.hp.psi2.data(), r.hp.psi2.size() * 8);
@@ -325,7 +324,6 @@ void admission_loop(hbt_server *s) {
             r->plan_cs.clear();
             r->plan_cs.shrink_to_fit();
             std::vector<JWin>().swap(r->wins);
-            std::vector<OrigRank>().swap(r->orr);
             s->active.push_back(r);
         }
         s->cv.notify_all();
@@ -708,12 +706,10 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
     r->items.clear();
     const double D = px_window(s->D);
     const HostParticles &hp = r->hp;
-    items_same(s->shape, off, hp.foff, r->grp_ev, hp.srt_px, D, 0, r->items, &r->item_pairs,
-               r->enum_same);
-    items_mixed(s->shape, hp.foff, r->grp_ev, r->rowoff, r->plan_ids, hp.foff, hp.f_pt, hp.f_pt,
-                D, 0, r->items, r->wins, &r->item_pairs);
-    r->orr.resize((size_t)hp.nf);
-    for (long long q = 0; q < hp.nf; q++) r->orr[q] = OrigRank{hp.orig[q], hp.rank[q]};
+    items_same(s->shape, off, hp.foff, r->grp_ev, hp.srt_px.data(), D, 0, r->items,
+               &r->item_pairs, r->enum_same);
+    items_mixed(s->shape, hp.foff, r->grp_ev, r->rowoff, r->plan_ids, hp.foff, hp.f_pt.data(),
+                hp.f_pt.data(), D, 0, r->items, r->wins, &r->item_pairs);
     for (uint64_t t : {r->enum_same, r->enum_mixed})
         if (t > s->D.needed + 1 || (qinv && t > s->D.needed_qinv))
             return sfail(s, HBT_ERR_CAPACITY,
