@@ -293,11 +293,13 @@ ATOMICS_BENCH = os.path.join(REPO, "hadronic_afterburner_toolkit_amd", "hist_ato
 
 
 def atomic_ceiling(log):
-    """Chip-wide rate of histogram atomics, in (instruction, cache line)
-    accesses per second: the f64coop5 pattern of the histogram-atomics
-    micro-benchmark (5 lanes per bin, 12 bins per FP64 atomic instruction,
-    as the fast kernel's scatter), run as a child process before this
-    process touches the GPU.  None when the binary is missing or fails."""
+    """Chip-wide rates of histogram atomics, in (instruction, cache line)
+    accesses per second, from the histogram-atomics micro-benchmark run as a
+    child process before this process touches the GPU: "same" = the f64coop5
+    pattern (5 lanes per bin, 12 bins per FP64 atomic instruction, the fast
+    kernel's same-event scatter), "mixed" = u32carry_1_per_lane (the
+    denominator's u32 add with carry).  None when the binary is missing or
+    fails."""
     if not os.path.exists(ATOMICS_BENCH):
         return None
     try:
@@ -306,12 +308,18 @@ def atomic_ceiling(log):
     except (OSError, subprocess.TimeoutExpired) as e:
         log(f"atomics micro-benchmark failed: {e}")
         return None
+    rates = {}
     for line in r.stdout.splitlines():
         parts = line.split()
-        if r.returncode == 0 and len(parts) >= 4 and parts[0] == "f64coop5":
-            return float(parts[3])
-    log("atomics micro-benchmark printed no f64coop5 line")
-    return None
+        if r.returncode == 0 and len(parts) >= 4:
+            if parts[0] == "f64coop5":
+                rates["same"] = float(parts[3])
+            elif parts[0] == "u32carry_1_per_lane":
+                rates["mixed"] = float(parts[3])
+    if len(rates) != 2:
+        log("atomics micro-benchmark printed no f64coop5 / u32carry_1_per_lane line")
+        return None
+    return rates
 
 
 def measured_traffic(kernel_prefix):
@@ -445,14 +453,18 @@ def main():
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
         # the bound that holds: every binned pair is one atomic cache-line
         # access (same-event: count + four sums in one FP64 instruction;
-        # mixed: the u64 denominator), against the measured chip-wide rate
+        # mixed: the denominator's u32 add), against the measured chip-wide
+        # rates of those two patterns; the ceiling is the blended rate
         acc = s[4] + m[4]
+        ceil = acc / (s[4] / atom_ceil["same"] + m[4] / atom_ceil["mixed"]) if atom_ceil else None
         out["atomic_roofline"] = {
-            "bound": "l2_atomics", "achieved": acc / dev_s, "ceiling": atom_ceil,
+            "bound": "memory_side_atomics", "achieved": acc / dev_s, "ceiling": ceil,
             "unit": "atomic line accesses/s",
-            "frac": (acc / dev_s / atom_ceil) if atom_ceil else None,
-            "accesses_per_launch": acc, "kernel": kname,
-            "ceiling_source": "hist_atomics_bench.e f64coop5, measured in this run"}
+            "frac": (acc / dev_s / ceil) if ceil else None,
+            "accesses_per_launch": acc, "accesses_same": s[4], "accesses_mixed": m[4],
+            "floor_ms": 1e3 * acc / ceil if ceil else None, "kernel": kname,
+            "ceiling_source": "hist_atomics_bench.e f64coop5 (same-event) and "
+                              "u32carry_1_per_lane (mixed), measured in this run"}
     if not args.no_serve:
         out["serving"] = serve(args, log)
     if not args.no_extra:

@@ -14,6 +14,8 @@
 //   f64coop5           64-byte bins, 5 lanes per pair, count as an FP64 add:
 //                      one instruction per 12 pairs (the fast kernel's scatter)
 //   u32_1_per_lane     one u32 per pair
+//   u32carry_1_per_lane  a u64 counter as a returning u32 add on its low word,
+//                      the carry added to the high word when the add wraps
 //   f64coop5_quarter   f64coop5 on a quarter of the workgroups (scaling with
 //                      the number of issuing CUs)
 //
@@ -101,6 +103,23 @@ __global__ void k_u32(double *h, unsigned nbins, unsigned per) {
     for (unsigned it = 0; it < per; it++)
         __hip_atomic_fetch_add(u + 2 * (mix32(tid * 977u + it * 131071u) % nbins), 1u,
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// u64 counter as a u32 add on its low word, with the carry added to the
+// high word by the lane whose add wrapped (returning atomic).
+__global__ void k_u32carry(double *h, unsigned nbins, unsigned per) {
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned *u = (unsigned *)h;
+    unsigned wraps = 0;
+    for (unsigned it = 0; it < per; it++) {
+        unsigned *lo = u + 2 * (mix32(tid * 977u + it * 131071u) % nbins);
+        const unsigned old = __hip_atomic_fetch_add(lo, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0xffffffffu) {
+            __hip_atomic_fetch_add(lo + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            wraps++;
+        }
+    }
+    (void)wraps;
 }
 
 // XCD of the executing wave (hardware register XCC_ID, bits 3:0), 0..7.
@@ -226,6 +245,7 @@ int main() {
         double pairs_per_lane_iter;
     } more[] = {{"f64coop5", k_f64coop5, blocks, 12.0 / 64.0},
                 {"u32_1_per_lane", k_u32, blocks, 1.0},
+                {"u32carry_1_per_lane", k_u32carry, blocks, 1.0},
                 {"f64coop5_quarter", k_f64coop5, blocks / 4, 12.0 / 64.0}};
     for (const auto &p : more) {
         hipLaunchKernelGGL(p.k, dim3(p.blocks), dim3(threads), 0, 0, h, nbins, per);
