@@ -206,11 +206,22 @@ def batch_line(groups, params, seed, prec, steps, tuning, autotune, log, label):
         g.launch_batch()
         ms.append(g.sync())
     res = g.results()
+    # binned pairs per launch (= atomic accesses), from the kernel's counters
+    sc, mc = g.stage_counts()
+    launches = max(1, int(sc[0]) // es if es else int(mc[0]) // max(1, em))
     g.close()
     t = float(np.mean(ms)) * 1e-3
     log(f"  {label}: {es + em:.3e} pairs in {t * 1e3:.1f} ms -> {(es + em) / t:.3e} pairs/s")
     return dict(pairs=es + em, same_pairs=es, mixed_pairs=em, ms=t * 1e3,
-                pairs_per_s=(es + em) / t, tuning=tune), res
+                pairs_per_s=(es + em) / t, tuning=tune,
+                binned_same=int(sc[4]) // launches, binned_mixed=int(mc[4]) // launches), res
+
+
+def add_atomic_floor(line, rates):
+    """The launch's atomic floor: binned pairs over the measured rates."""
+    if rates:
+        line["atomic_floor_ms"] = 1e3 * (line["binned_same"] / rates["same"]
+                                         + line["binned_mixed"] / rates["mixed"])
 
 
 def batched_uneven(args, log):
@@ -472,6 +483,9 @@ def main():
         out["batched_uneven"] = batched_uneven(args, log)
         out["large_histogram"] = large_histogram(
             args, log, (t["R"], t["jchunk"], t["kchunk"], t["blocks_per_cu"]))
+        for line in (out["batched_uneven"]["pi+"], out["batched_uneven"]["K+"],
+                     out["large_histogram"]):
+            add_atomic_floor(line, atom_ceil)
     if not args.no_cpu:
         g0 = groups[0]
         M = int(plan[0][0])

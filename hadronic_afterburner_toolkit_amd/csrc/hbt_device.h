@@ -26,7 +26,7 @@ struct DevParams {
     double pf_mid;
     float f_half, f_qthr, f_mid, f_pad;
     // precision-form exact-stage constants
-    double inv_dKT, inv_dq, inv_two_pi, two_pi_hi, two_pi_lo;
+    double inv_dKT, inv_dq, inv_two_pi, two_pi_hi, two_pi_lo, inv_dKphi;
 };
 
 // Histogram bins, array of structures: 8 doubles (64 bytes, one aligned

@@ -332,7 +332,7 @@ struct hbt_ctx {
     size_t items_cap = 0;
     JWin *d_wins = nullptr;  // partner windows of the mixed items
     size_t wins_cap = 0;
-    int nitems = 0, nitems_same = 0;
+    int nitems = 0;
     int *d_counter = nullptr;
     View *d_view = nullptr;  // the context's one analysis view
     // ordered pair-cap path (allocated when a cap binds)
@@ -594,16 +594,36 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
     L.items.clear();
     L.wins.clear();
     enum_same = 0;
+    std::vector<uint64_t> pairs;
     if (same)
-        items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px.data(), D, 0, L.items, nullptr,
+        items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px.data(), D, 0, L.items, &pairs,
                    enum_same);
-    c->nitems_same = (int)L.items.size();
     if (mixed && c->plan_valid) {
         const Slot &m = mixed_slot(c);
         items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt.data(),
-                    m.f_pt.data(), D, 0,
-                    L.items, L.wins, nullptr);
+                    m.f_pt.data(), D, 0, L.items, L.wins, &pairs);
     }
+    // larger items first, by factor-4 size classes (stable inside a class,
+    // which keeps a group's items together): the persistent waves pull items
+    // in queue order, so the launch ends on small items instead of one wave
+    // finishing a large one (uneven multiplicities make items differ by 10x
+    // and more)
+    auto size_class = [&](uint32_t k) {
+        int c = 0;
+        for (uint64_t v = pairs[k]; v >= 4; v >>= 2) c++;
+        return c;
+    };
+    std::vector<uint32_t> order(L.items.size());
+    std::vector<int> cls(L.items.size());
+    for (size_t k = 0; k < order.size(); k++) {
+        order[k] = (uint32_t)k;
+        cls[k] = size_class((uint32_t)k);
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return cls[a] > cls[b]; });
+    std::vector<WorkItem> sorted(L.items.size());
+    for (size_t k = 0; k < order.size(); k++) sorted[k] = L.items[order[k]];
+    L.items.swap(sorted);
 }
 
 // Uploads the plan: ids are relative to the event's own group and are

@@ -68,13 +68,31 @@ __device__ __forceinline__ int guarded_window(double v, double lo, double hi, do
 
 constexpr double GUARD_IDX = 1e-9;   // in bin-index units
 constexpr double GUARD_REL = 1e-12;  // relative to the q scale of the pair
+constexpr double GUARD_PHI = 4e-6;   // radians, around K_phi bin edges
+
+// K_phi bin of the precision form (the reference's wrap loops,
+// HBT_correlation.cpp:408-423): atan2 on the FP32 unit (error below 1e-6
+// rad); an angle within GUARD_PHI of a bin edge, 0 or 2 pi returns -2 and
+// the pair is re-run through the literal cascade.
+__device__ __forceinline__ int kphi_index_fast(const DevParams &P, double K_x, double K_y,
+                                               double psi_ref) {
+    double delta = (double)atan2f((float)K_y, (float)K_x) - psi_ref;
+    while (delta < 0.) delta += P.two_pi;
+    while (delta > P.two_pi) delta -= P.two_pi;
+    const double w = delta * P.inv_dKphi;
+    const double n = rint(w);
+    if (fabs(w - n) * P.dKphi < GUARD_PHI) return -2;
+    const int idx = (int)w;
+    return (idx < 0 || idx >= P.n_Kphi) ? -2 : idx;
+}
 
 // The K/q cascade of the precision form.  Returns the stage mask as
 // cascade_f64 does, or -1 when a decision is ambiguous.
-template <bool MIXED>
+template <bool MIXED, bool EXT = false>
 __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, double p1y,
                                             double p1z, double p1E, double p2x, double p2y,
-                                            double p2z, double p2E, PairBins &b) {
+                                            double p2z, double p2E, double psi_ref,
+                                            PairBins &b) {
     const double K_z = 0.5 * (p1z + p2z);
     const double K_E = 0.5 * (p1E + p2E);
     const double K_x = 0.5 * (p1x + p2x);
@@ -125,6 +143,10 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
     if (b.il < -1) return -1;
     if (b.il >= P.qnpts) return st;
     b.kphi = 0;
+    if (EXT && P.azimuthal) {
+        b.kphi = kphi_index_fast(P, K_x, K_y, psi_ref);
+        if (b.kphi < 0) return -1;
+    }
     b.q_out = q_out;
     b.q_side = q_side;
     b.q_long = q_long;
@@ -132,17 +154,17 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
 }
 
 // Same-event pair in the precision form (see eval_same_f64 for the
-// contract).  The q_inv and K_phi variants, and ambiguous pairs, use the
-// literal FP64 cascade.
+// contract).  The q_inv variant and ambiguous pairs use the literal FP64
+// cascade.
 template <bool EXT = true>
 __device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist &H, double psi,
                                               const PartRec &p1, const PartRec &p2,
                                               SameRec &rec) {
     PairBins b;
-    const int st = (EXT && (P.qinv | P.azimuthal))
+    const int st = (EXT && P.qinv)
                        ? -1
-                       : cascade_fast<false>(P, p1.px, p1.py, p1.pz, p1.E, p2.px, p2.py, p2.pz,
-                                             p2.E, b);
+                       : cascade_fast<false, EXT>(P, p1.px, p1.py, p1.pz, p1.E, p2.px, p2.py,
+                                                  p2.pz, p2.E, psi, b);
     if (st < 0)
         return eval_same_f64<EXT>(P, H, psi, p1.px, p1.py, p1.pz, p1.E, p1.x, p1.y, p1.z, p1.t,
                                   p2.px, p2.py, p2.pz, p2.E, p2.x, p2.y, p2.z, p2.t, rec);
@@ -153,7 +175,7 @@ __device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist 
     const double z_diff = p1.z - p2.z;
     const double phase =
         P.hbarC_inv * (b.q_E * t_diff - b.q_x * x_diff - b.q_y * y_diff - b.q_z * z_diff);
-    rec.k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
+    rec.k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
     rec.qo = b.q_out;
     rec.qs = b.q_side;
     rec.ql = b.q_long;
@@ -171,12 +193,13 @@ __device__ __forceinline__ int eval_mixed_fast(const DevParams &P, const DevHist
     PairBins b;
     // mixed windows are half-open: v < hi (HBT_correlation.cpp:613); the
     // guard band around the cut sends v == hi to the literal cascade
-    const int st = (EXT && (P.qinv | P.azimuthal))
+    const int st = (EXT && P.qinv)
                        ? -1
-                       : cascade_fast<true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, b);
+                       : cascade_fast<true, EXT>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi,
+                                                 b);
     if (st < 0)
         return eval_mixed_f64<EXT>(P, H, psi, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, k);
-    if (st & ST_BIN) k = bin_index(P, b.kt, 0, b.io, b.is, b.il);
+    if (st & ST_BIN) k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
     return st;
 }
 

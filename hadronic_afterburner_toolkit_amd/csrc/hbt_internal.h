@@ -71,6 +71,7 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     P.inv_dKT = 1.0 / P.dKT;
     P.inv_dq = 1.0 / P.delta_q;
     P.inv_two_pi = 1.0 / (2.0 * M_PI);
+    P.inv_dKphi = 1.0 / P.dKphi;
     P.two_pi_hi = 6.28318530717958623199592693709e+00;  // 2*pi rounded to double
     P.two_pi_lo = 2.44929359829470635445e-16;            // 2*pi - two_pi_hi
     // Prefilter (hbt_fast.h).  T = 4 K_perp^2 must lie in
