@@ -1288,9 +1288,11 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
         if (best_out) *best_out = c->tune;
         return HBT_OK;
     }
+    // small j chunks and one window per mixed item give small batches
+    // enough items to fill the chip; large ones amortize the per-item setup
     const int Rs[] = {1, 2, 4};
-    const int JCs[] = {1024, 4096};
-    const int KCs[] = {4, 16};
+    const int JCs[] = {256, 1024, 4096};
+    const int KCs[] = {1, 4, 16};
     const int BPCs[] = {4, 8, 16};
     hbt_tuning best = c->tune;
     best.ms = 1e30;
