@@ -287,6 +287,46 @@ def cpu_baseline(group, plan_g, log):
     return o, a, pairs / dt, dt, pairs
 
 
+def cpu_baseline_multicore(groups, plan, log):
+    """The CPU oracle on several groups at once, one thread per group (groups
+    are independent once the plans are drawn; the oracle's C calls release
+    the GIL): threads = min(16, cores, groups), the box's CPU share."""
+    import threading
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    nthreads = max(1, min(16, os.cpu_count() or 1, len(groups)))
+    Mg, ids, cs = plan
+    offs = np.concatenate([[0], np.cumsum([int(Mg[g]) * groups[g].nev for g in range(len(groups))])])
+    pairs = [0.0] * nthreads
+
+    def work(g):
+        M = int(Mg[g])
+        o = oracle.Oracle(PARAMS)
+        o.combine_same(groups[g])
+        o.run_mixed_plan(groups[g], M, ids[offs[g]:offs[g + 1]], cs[2 * offs[g]:2 * offs[g + 1]])
+        a = o.arrays()
+        pairs[g] = float(a["same_st"][0] + a["mixed_st"][0])
+
+    th = [threading.Thread(target=work, args=(g,)) for g in range(nthreads)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    log(f"  cpu oracle, {nthreads} threads: {sum(pairs):.3e} pairs in {dt:.2f}s")
+    return {"value": sum(pairs) / dt, "unit": "pairs/s", "cores": nthreads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": model,
+            "sample": f"groups 0..{nthreads - 1} of {len(groups)}, one thread each: "
+                      f"{sum(pairs):.3e} same+mixed pairs in {dt:.1f}s, oracle/hbt_oracle.c -O2"}
+
+
 def copy_calibration(nbytes, reps):
     """A plain device copy of known size (torch's copy kernel), for scaling
     the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -495,6 +535,7 @@ def main():
                                "sample": f"group 0 of {len(groups)} ({g0.nev} events, "
                                          f"{g0.n} particles): {pairs:.3e} same+mixed pairs in "
                                          f"{dt:.1f}s, oracle/hbt_oracle.c -O2"}
+        out["cpu_baseline_multicore"] = cpu_baseline_multicore(groups, plan, log)
         # precision form vs the FP64 oracle on that sample, per bin
         if "fp32c" in res:
             from hadronic_afterburner_toolkit_amd import hbt
