@@ -327,6 +327,64 @@ def cpu_baseline_multicore(groups, plan, log):
                       f"{sum(pairs):.3e} same+mixed pairs in {dt:.1f}s, oracle/hbt_oracle.c -O2"}
 
 
+def ingest(args, log):
+    """Sample ingest (host only): a 200-event Poisson(600) sample written as
+    mode-10 text (gzip and plain, %.7e like the reference's sample files) and
+    as mode-9 binary records, read back with the engine's readers into SoA
+    buffers, beside the oracle's restatement of the reference's mode-10
+    reader (byte-at-a-time gz_readline, strtod per field)."""
+    import tempfile
+    from hadronic_afterburner_toolkit_amd import hbt, samples
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    buf = samples.generate(args.seed + 60, 200, 600.0)
+    cols = ("mass", "t", "x", "y", "z", "E", "px", "py", "pz")
+    vals = np.stack([getattr(buf, k) for k in cols], axis=1)
+    out = {"particles": int(buf.n), "events": int(buf.nev)}
+    with tempfile.TemporaryDirectory() as d:
+        lines = []
+        for e in range(buf.nev):
+            a, b = int(buf.ev_off[e]), int(buf.ev_off[e + 1])
+            lines.append(f"{b - a}\n")
+            for i in range(a, b):
+                lines.append("%d %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.7e %.7e\n"
+                             % ((buf.monval[i],) + tuple(vals[i])))
+        text = "".join(lines).encode()
+        import gzip
+        paths = {"text_plain": os.path.join(d, "s.txt"), "text_gz": os.path.join(d, "s.gz"),
+                 "binary_mode9": os.path.join(d, "s.bin")}
+        with open(paths["text_plain"], "wb") as f:
+            f.write(text)
+        with open(paths["text_gz"], "wb") as f:
+            f.write(gzip.compress(text, 6))
+        with open(paths["binary_mode9"], "wb") as f:
+            for e in range(buf.nev):
+                a, b = int(buf.ev_off[e]), int(buf.ev_off[e + 1])
+                np.array([b - a], np.int32).tofile(f)
+                rec = np.zeros(b - a, dtype=[("m", "<i4"), ("v", "<f4", (9,))])
+                rec["m"] = buf.monval[a:b]
+                rec["v"] = vals[a:b]
+                rec.tofile(f)
+        out["text_bytes"] = len(text)
+        for name, path in paths.items():
+            t0 = time.perf_counter()
+            got = hbt.read_buffers(path, 10 ** 9, 211, read_in_mode=9 if "mode9" in name else 10)
+            dt = time.perf_counter() - t0
+            assert sum(b.n for b, _ in got) == buf.n
+            out[name] = {"s": dt, "particles_per_s": buf.n / dt}
+        t0 = time.perf_counter()
+        ref = oracle.read_mode10(paths["text_gz"], 10 ** 9, 211)
+        dt = time.perf_counter() - t0
+        assert sum(b.n for b, _ in ref) == buf.n
+        out["oracle_text_gz"] = {"s": dt, "particles_per_s": buf.n / dt,
+                                 "what": "oracle/hbt_oracle.c restatement of the reference's "
+                                         "mode-10 reader (gz_readline, strtod)"}
+    out["speedup_text_gz_vs_oracle"] = out["oracle_text_gz"]["s"] / out["text_gz"]["s"]
+    log(f"  ingest: text.gz {out['text_gz']['particles_per_s']:.3e} particles/s, "
+        f"oracle reader {out['oracle_text_gz']['particles_per_s']:.3e}")
+    return out
+
+
 def copy_calibration(nbytes, reps):
     """A plain device copy of known size (torch's copy kernel), for scaling
     the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -526,6 +584,8 @@ def main():
         for line in (out["batched_uneven"]["pi+"], out["batched_uneven"]["K+"],
                      out["large_histogram"]):
             add_atomic_floor(line, atom_ceil)
+    if not args.no_extra:
+        out["ingest"] = ingest(args, log)
     if not args.no_cpu:
         g0 = groups[0]
         M = int(plan[0][0])

@@ -117,6 +117,8 @@ SIGNATURES = [
                                     C.c_int32, C.c_int32]),
     ("hbt_flow_event_plane_angle", C.c_double, [C.c_int32, _F64P, _F64P, C.c_int32]),
     ("hbt_reader_open", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("hbt_reader_open_mode", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_void_p)]),
     ("hbt_reader_next", C.c_int32, [C.c_void_p]),
     ("hbt_reader_eof", C.c_int32, [C.c_void_p]),
     ("hbt_reader_buffer", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_int64)),

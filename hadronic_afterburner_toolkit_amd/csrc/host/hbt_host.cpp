@@ -1,9 +1,12 @@
 // hbt_host.cpp -- host-side pieces of the C ABI that need no GPU: the
-// reference's text writers, its mode-10 sample reader and psi_ref.
+// reference's text writers, its mode-10 and mode-9 sample readers and
+// psi_ref.
 // Compiled into libhbt_hip.so next to the engine.
 #include <math.h>
 #include <zlib.h>
 
+#include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -11,6 +14,7 @@
 #include <iomanip>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/hbt_hip.h"
@@ -163,13 +167,22 @@ double hbt_flow_event_plane_angle(int32_t n, const double *px, const double *py,
     return atan2(vn_imag, vn_real) / n_order;
 }
 
-// ------------------------------------------------------------ mode-10 reader
+}  // extern "C"
+
+// ---------------------------------------------- mode-10 and mode-9 readers
 
 struct hbt_reader {
+    int mode = 10;  // 10: text (gzip or plain), 9: binary records
     gzFile gz = nullptr;
+    FILE *bin = nullptr;
     int event_buffer_size = 0;
     int monval = 0;
     bool eof = false;
+    // decompressed text not yet consumed: win[pos, len), NUL after len;
+    // the window grows by doubling (realloc, no zero fill)
+    char *win = nullptr;
+    size_t cap = 0, pos = 0, len = 0;
+    bool no_more_data = false;
     // last buffer: filtered particles (SoA) and their event offsets
     std::vector<int64_t> off;
     std::vector<double> cols[8];  // px py pz E x y z t
@@ -179,89 +192,330 @@ struct hbt_reader {
 
 namespace {
 
-// One line without its '\n', byte at a time like gz_readline
-// (particleSamples.cpp:2209-2218); end of file is flagged only when a read
-// finds no more data, as gzeof does.
-bool read_line(hbt_reader *rd, std::string &line) {
-    line.clear();
-    for (;;) {
-        const int ch = gzgetc(rd->gz);
-        if (ch < 0) {
-            rd->eof = true;
-            return false;
+// ---- number parsing, bit-identical to strtod / strtol
+
+// Exact powers of ten representable in double.
+const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                           1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+
+inline bool is_blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+inline bool is_delim(char c) { return is_blank(c) || c == '\n' || c == '\0'; }
+
+// One decimal number at *p (leading blanks skipped), as strtod reads it.
+// Clinger's fast path (W. D. Clinger, "How to read floating point numbers
+// accurately", PLDI 1990): when the significant digits form an integer
+// m < 2^53 and the decimal exponent e lies in [-22, 22], m and 10^|e| are
+// exact doubles and one IEEE multiply or divide gives the correctly rounded
+// value, which is what glibc's strtod returns.  Anything else (more digits,
+// larger exponents, hex, inf/nan, a malformed field) is handed to strtod.
+inline double parse_double(const char *&p) {
+    const char *s = p;
+    while (is_blank(*s)) s++;
+    const char *q = s;
+    bool neg = false;
+    if (*q == '-' || *q == '+') neg = *q++ == '-';
+    uint64_t m = 0;
+    int sig = 0, e10 = 0;
+    bool any = false, fast = true;
+    for (; *q >= '0' && *q <= '9'; q++) {
+        any = true;
+        if (m == 0 && *q == '0') continue;
+        if (sig < 19) {
+            m = m * 10 + (uint64_t)(*q - '0');
+            sig++;
+        } else {
+            fast = false;
         }
-        if (ch == '\n') return true;
-        line.push_back((char)ch);
     }
+    if (*q == '.') {
+        for (q++; *q >= '0' && *q <= '9'; q++) {
+            any = true;
+            if (m == 0 && *q == '0') {
+                e10--;
+                continue;
+            }
+            if (sig < 19) {
+                m = m * 10 + (uint64_t)(*q - '0');
+                sig++;
+                e10--;
+            } else {
+                fast = false;
+            }
+        }
+    }
+    if (any && (*q == 'e' || *q == 'E')) {
+        const char *r = q + 1;
+        bool eneg = false;
+        if (*r == '-' || *r == '+') eneg = *r++ == '-';
+        if (*r >= '0' && *r <= '9') {
+            int x = 0;
+            for (; *r >= '0' && *r <= '9'; r++)
+                if (x < 100000) x = x * 10 + (*r - '0');
+            e10 += eneg ? -x : x;
+            q = r;
+        }
+    }
+    if (!any || !fast || !is_delim(*q) || m >= (1ull << 53) || e10 < -22 || e10 > 22) {
+        char *end = nullptr;
+        const double v = strtod(s, &end);
+        p = end;
+        return v;
+    }
+    double v = (double)m;
+    v = e10 >= 0 ? v * kPow10[e10] : v / kPow10[-e10];
+    p = q;
+    return neg ? -v : v;
+}
+
+// One integer at *p as strtol(p, &end, 10) reads it (leading blanks, sign);
+// long digit strings and malformed fields go to strtol.
+inline long parse_long(const char *&p) {
+    const char *s = p;
+    while (is_blank(*s)) s++;
+    const char *q = s;
+    bool neg = false;
+    if (*q == '-' || *q == '+') neg = *q++ == '-';
+    long v = 0;
+    int nd = 0;
+    for (; *q >= '0' && *q <= '9' && nd < 18; q++, nd++) v = v * 10 + (*q - '0');
+    if (nd == 0 || (*q >= '0' && *q <= '9')) {
+        char *end = nullptr;
+        const long r = strtol(s, &end, 10);
+        p = end;
+        return r;
+    }
+    p = q;
+    return neg ? -v : v;
+}
+
+// ---- text window
+
+constexpr size_t kChunk = size_t(4) << 20;
+
+// Room for n more bytes and the NUL after them.
+bool reserve_window(hbt_reader *rd, size_t n) {
+    if (rd->len + n + 1 <= rd->cap) return true;
+    size_t cap = std::max<size_t>(rd->cap * 2, rd->len + n + 1);
+    char *w = (char *)realloc(rd->win, cap);
+    if (!w) return false;
+    rd->win = w;
+    rd->cap = cap;
+    return true;
+}
+
+// Appends up to kChunk decompressed bytes; false when the file has no more.
+bool refill(hbt_reader *rd) {
+    if (rd->no_more_data || !reserve_window(rd, kChunk)) {
+        rd->no_more_data = true;
+        return false;
+    }
+    const int got = gzread(rd->gz, rd->win + rd->len, (unsigned)kChunk);
+    if (got <= 0) {
+        rd->no_more_data = true;
+        rd->win[rd->len] = '\0';
+        return false;
+    }
+    rd->len += (size_t)got;
+    rd->win[rd->len] = '\0';
+    return true;
+}
+
+// The next line [start, end) of the window, without its '\n'.  As
+// gz_readline + gzeof (particleSamples.cpp:2209-2218): a line that reaches
+// the end of the data without '\n' sets eof (and is returned as read).
+void next_line(hbt_reader *rd, size_t &start, size_t &end) {
+    for (;;) {
+        const char *b = rd->win + rd->pos;
+        const void *nl = rd->len > rd->pos ? memchr(b, '\n', rd->len - rd->pos) : nullptr;
+        if (nl) {
+            start = rd->pos;
+            end = (size_t)((const char *)nl - rd->win);
+            rd->pos = end + 1;
+            return;
+        }
+        if (!refill(rd)) {
+            start = rd->pos;
+            end = rd->len;
+            rd->pos = rd->len;
+            rd->eof = true;
+            return;
+        }
+    }
+}
+
+// Runs fn(begin, end) over [0, n) on up to 16 host threads.
+template <typename F>
+void parallel_lines(size_t n, F fn) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nt = std::max<size_t>(1, std::min<size_t>(hw, n / 4096));
+    if (nt <= 1) {
+        fn((size_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
+
+// Mode 10 (particleSamples.cpp:1247-1286): whole events while the running
+// particle total (all species) is below event_buffer_size.  The event
+// structure is scanned first (line boundaries only), then the particle
+// lines "monval mass t x y z E px py pz" are parsed on several threads and
+// filtered by species (:1329-1346) in file order.
+int32_t read_text_buffer(hbt_reader *rd) {
+    // drop what earlier buffers consumed
+    if (rd->pos > 0) {
+        memmove(rd->win, rd->win + rd->pos, rd->len - rd->pos);
+        rd->len -= rd->pos;
+        rd->pos = 0;
+        rd->win[rd->len] = '\0';
+    }
+    if (!rd->win && !reserve_window(rd, 0)) return HBT_ERR_NOMEM;
+    rd->win[rd->len] = '\0';
+    std::vector<size_t> ls, le;  // particle line bounds
+    std::vector<int> nper;       // particles per event
+    long long num_particles = 0;
+    while (num_particles < rd->event_buffer_size) {
+        size_t a, b;
+        next_line(rd, a, b);
+        if (rd->eof) break;
+        rd->win[b] = '\0';  // the '\n' (or the end) terminates the line
+        const char *h = rd->win + a;
+        const int n_particle = (int)parse_long(h);
+        for (int ip = 0; ip < n_particle; ip++) {
+            next_line(rd, a, b);
+            ls.push_back(a);
+            le.push_back(b);
+        }
+        nper.push_back(n_particle);
+        num_particles += n_particle;
+    }
+    const size_t nl = ls.size();
+    for (size_t k = 0; k < nl; k++) rd->win[le[k]] = '\0';
+    std::vector<int> mon(nl);
+    std::vector<double> val(nl * 9);
+    const char *base = rd->win;
+    parallel_lines(nl, [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; k++) {
+            const char *p = base + ls[k];
+            mon[k] = (int)parse_long(p);
+            double *v = &val[k * 9];  // mass t x y z E px py pz
+            for (int f = 0; f < 9; f++) v[f] = parse_double(p);
+        }
+    });
+    rd->off.assign(1, 0);
+    for (auto &c : rd->cols) c.clear();
+    rd->mass.clear();
+    size_t k = 0;
+    for (int n : nper) {
+        for (int ip = 0; ip < n; ip++, k++) {
+            if (rd->monval != 99999 && mon[k] != rd->monval) continue;
+            const double *v = &val[k * 9];
+            rd->cols[0].push_back(v[6]);
+            rd->cols[1].push_back(v[7]);
+            rd->cols[2].push_back(v[8]);
+            rd->cols[3].push_back(v[5]);
+            rd->cols[4].push_back(v[2]);
+            rd->cols[5].push_back(v[3]);
+            rd->cols[6].push_back(v[4]);
+            rd->cols[7].push_back(v[1]);
+            rd->mass.push_back(v[0]);
+        }
+        rd->off.push_back((int64_t)rd->cols[0].size());
+    }
+    rd->total = num_particles;
+    return (int32_t)nper.size();
+}
+
+// Mode 9 (particleSamples.cpp:1203-1245): per event an int n, then n
+// records of an int Monte-Carlo id and 9 floats (mass t x y z E px py pz),
+// native byte order; the values widen to double.  The reference files every
+// particle under event 0 (its event index never advances); here each record
+// belongs to its own event, as the file lays them out.
+int32_t read_binary_buffer(hbt_reader *rd) {
+    rd->off.assign(1, 0);
+    for (auto &c : rd->cols) c.clear();
+    rd->mass.clear();
+    long long num_particles = 0;
+    int nev = 0;
+    std::vector<char> rec;
+    while (num_particles < rd->event_buffer_size) {
+        int32_t n_particle = 0;
+        if (fread(&n_particle, sizeof(n_particle), 1, rd->bin) != 1) {
+            rd->eof = true;
+            break;
+        }
+        const size_t rb = 4 + 9 * sizeof(float);
+        rec.assign((size_t)std::max(0, n_particle) * rb, 0);
+        const size_t got = rec.empty() ? 0 : fread(rec.data(), 1, rec.size(), rd->bin);
+        if (got < rec.size()) rd->eof = true;  // truncated: missing values read as 0
+        for (int ip = 0; ip < n_particle; ip++) {
+            const char *r = rec.data() + (size_t)ip * rb;
+            int32_t mon;
+            float a[9];
+            memcpy(&mon, r, 4);
+            memcpy(a, r + 4, sizeof(a));
+            if (rd->monval != 99999 && mon != rd->monval) continue;
+            rd->cols[0].push_back(a[6]);
+            rd->cols[1].push_back(a[7]);
+            rd->cols[2].push_back(a[8]);
+            rd->cols[3].push_back(a[5]);
+            rd->cols[4].push_back(a[2]);
+            rd->cols[5].push_back(a[3]);
+            rd->cols[6].push_back(a[4]);
+            rd->cols[7].push_back(a[1]);
+            rd->mass.push_back(a[0]);
+        }
+        num_particles += std::max(0, n_particle);
+        nev++;
+        rd->off.push_back((int64_t)rd->cols[0].size());
+        if (rd->eof) break;
+    }
+    rd->total = num_particles;
+    return nev;
 }
 
 }  // namespace
 
-int hbt_reader_open(const char *filename, int32_t event_buffer_size, int32_t particle_monval,
-                    hbt_reader **out) {
+extern "C" {
+
+int hbt_reader_open_mode(const char *filename, int32_t read_in_mode, int32_t event_buffer_size,
+                         int32_t particle_monval, hbt_reader **out) {
     if (!filename || !out) return HBT_ERR_ARG;
     *out = nullptr;
+    if (read_in_mode != 9 && read_in_mode != 10) return HBT_ERR_ARG;
     // species selections other than one Monte-Carlo id need the PDG tables
     // (particleSamples.cpp:625-678), which this reader does not load
     if (particle_monval == 9999 || particle_monval == 9998 || particle_monval == -9998 ||
         particle_monval == 9997 || particle_monval == -9997 || particle_monval == 9996 ||
         particle_monval == -9996)
         return HBT_ERR_ARG;
-    gzFile gz = gzopen(filename, "rb");
-    if (!gz) return HBT_ERR_ARG;
     hbt_reader *rd = new hbt_reader();
-    rd->gz = gz;
+    rd->mode = read_in_mode;
     rd->event_buffer_size = event_buffer_size;
     rd->monval = particle_monval;
+    if (read_in_mode == 10) {
+        rd->gz = gzopen(filename, "rb");
+        if (rd->gz) gzbuffer(rd->gz, 1u << 20);
+    } else {
+        rd->bin = fopen(filename, "rb");
+    }
+    if (!rd->gz && !rd->bin) {
+        delete rd;
+        return HBT_ERR_ARG;
+    }
     *out = rd;
     return HBT_OK;
 }
 
-// particleSamples.cpp:1247-1286 (whole events while the running particle
-// total of all species is below event_buffer_size) and :1329-1346 (species
-// filter by exact Monte-Carlo id; 99999 keeps everything).
+int hbt_reader_open(const char *filename, int32_t event_buffer_size, int32_t particle_monval,
+                    hbt_reader **out) {
+    return hbt_reader_open_mode(filename, 10, event_buffer_size, particle_monval, out);
+}
+
 int32_t hbt_reader_next(hbt_reader *rd) {
     if (!rd) return HBT_ERR_ARG;
-    rd->off.assign(1, 0);
-    for (auto &c : rd->cols) c.clear();
-    rd->mass.clear();
-    rd->total = 0;
-    long long num_particles = 0;
-    int nev = 0;
-    std::string line;
-    while (num_particles < rd->event_buffer_size) {
-        read_line(rd, line);
-        if (rd->eof) break;
-        const int n_particle = (int)strtol(line.c_str(), nullptr, 10);
-        for (int ip = 0; ip < n_particle; ip++) {
-            read_line(rd, line);
-            char *p = const_cast<char *>(line.c_str());
-            char *e = nullptr;
-            const int mon = (int)strtol(p, &e, 10);
-            double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // mass t x y z E px py pz
-            p = e;
-            for (int k = 0; k < 9; k++) {
-                v[k] = strtod(p, &e);
-                p = e;
-            }
-            if (rd->monval == 99999 || mon == rd->monval) {
-                rd->cols[0].push_back(v[6]);
-                rd->cols[1].push_back(v[7]);
-                rd->cols[2].push_back(v[8]);
-                rd->cols[3].push_back(v[5]);
-                rd->cols[4].push_back(v[2]);
-                rd->cols[5].push_back(v[3]);
-                rd->cols[6].push_back(v[4]);
-                rd->cols[7].push_back(v[1]);
-                rd->mass.push_back(v[0]);
-            }
-        }
-        num_particles += n_particle;
-        nev++;
-        rd->off.push_back((int64_t)rd->cols[0].size());
-    }
-    rd->total = num_particles;
-    return nev;
+    return rd->mode == 9 ? read_binary_buffer(rd) : read_text_buffer(rd);
 }
 
 int32_t hbt_reader_eof(const hbt_reader *rd) { return rd && rd->eof ? 1 : 0; }
@@ -280,6 +534,8 @@ int hbt_reader_buffer(const hbt_reader *rd, const int64_t **ev_off, const double
 int hbt_reader_close(hbt_reader *rd) {
     if (!rd) return HBT_OK;
     if (rd->gz) gzclose(rd->gz);
+    if (rd->bin) fclose(rd->bin);
+    free(rd->win);
     delete rd;
     return HBT_OK;
 }

@@ -2,8 +2,8 @@
 // analyze_HBT=1 branch of the reference's hadronic_afterburner_tools.e
 // (src/main.cpp:21-46, Analysis::HBTAnalysis Analysis.cpp:817-835):
 // reads ./parameters.dat and key=value overrides, samples from
-// results/particle_samples.gz (read_in_mode 10), writes the HBT files to
-// results/.
+// results/particle_samples.gz (read_in_mode 10) or results/particle_list.bin
+// (read_in_mode 9), writes the HBT files to results/.
 #include <chrono>
 #include <iostream>
 #include <memory>

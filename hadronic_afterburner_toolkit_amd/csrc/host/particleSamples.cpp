@@ -1,6 +1,7 @@
-// particleSamples.cpp -- see particleSamples.h.  Reader mode 10 of the
-// reference (particleSamples.cpp:20-221 for the keys it reads, :480-552 for
-// the buffer calls, :1247-1346 for the format and the species filter).
+// particleSamples.cpp -- see particleSamples.h.  Reader modes 10 and 9 of
+// the reference (particleSamples.cpp:20-221 for the keys it reads and the
+// file names :131-152, :480-552 for the buffer calls, :1203-1346 for the
+// formats and the species filter).
 #include "particleSamples.h"
 
 #include <cmath>
@@ -15,21 +16,25 @@ particleSamples::particleSamples(ParameterReader &paraRdr, std::string path,
     const int read_in_mode = (int)paraRdr_.getVal("read_in_mode");
     particle_monval = (int)paraRdr_.getVal("particle_monval");
     read_mixed_events = paraRdr_.getVal("read_in_real_mixed_events") == 1;
-    if (read_in_mode != 10) {
+    if (read_in_mode != 10 && read_in_mode != 9) {
         std::cout << "particleSamples:: Error: read_in_mode " << read_in_mode
-                  << " is not supported by this build (mode 10, gzip text, only)" << std::endl;
+                  << " is not supported by this build (modes 10, gzip text, and 9, binary)"
+                  << std::endl;
         exit(1);
     }
-    const std::string fn = path_ + "/particle_samples.gz";
-    const std::string fn_mixed = path_ + "/particle_samples_mixed_event.gz";
-    if (hbt_reader_open(fn.c_str(), event_buffer_size, particle_monval, &reader) != HBT_OK) {
+    const bool bin = read_in_mode == 9;
+    const std::string fn = path_ + (bin ? "/particle_list.bin" : "/particle_samples.gz");
+    const std::string fn_mixed =
+        path_ + (bin ? "/particle_list_mixed_event.bin" : "/particle_samples_mixed_event.gz");
+    if (hbt_reader_open_mode(fn.c_str(), read_in_mode, event_buffer_size, particle_monval,
+                             &reader) != HBT_OK) {
         std::cout << "particleSamples:: Error: input file: " << fn << " can not open!"
                   << std::endl;
         exit(1);
     }
     if (read_mixed_events &&
-        hbt_reader_open(fn_mixed.c_str(), event_buffer_size, particle_monval, &reader_mixed) !=
-            HBT_OK) {
+        hbt_reader_open_mode(fn_mixed.c_str(), read_in_mode, event_buffer_size, particle_monval,
+                             &reader_mixed) != HBT_OK) {
         std::cout << "particleSamples:: Error: input file: " << fn_mixed << " can not open!"
                   << std::endl;
         exit(1);

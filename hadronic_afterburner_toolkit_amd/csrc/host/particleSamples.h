@@ -1,6 +1,7 @@
 // particleSamples -- the part of the reference's event container that the
 // HBT path uses (src/particleSamples.h): read_in_mode 10 (gzip text,
-// particleSamples.cpp:1247-1286) in buffers of whole events, species filter
+// particleSamples.cpp:1247-1286) and 9 (binary records, :1203-1245) in
+// buffers of whole events, species filter
 // by one Monte-Carlo id, optional separate mixed-event file
 // (read_in_real_mixed_events), the same getters.  Particles are kept as
 // structure-of-arrays so that HBT_correlation can hand them to the GPU

@@ -75,14 +75,15 @@ def flow_event_plane_angle(px, py, n_order: int) -> float:
     return float(lib.hbt_flow_event_plane_angle(len(px), _f64p(px), _f64p(py), int(n_order)))
 
 
-def read_buffers(filename: str, event_buffer_size: int, particle_monval: int):
-    """The mode-10 reader of the engine library, buffer by buffer (the
-    reference's read loop, Analysis.cpp:821-833)."""
+def read_buffers(filename: str, event_buffer_size: int, particle_monval: int,
+                 read_in_mode: int = 10):
+    """The mode-10 (or mode-9 binary) reader of the engine library, buffer by
+    buffer (the reference's read loop, Analysis.cpp:821-833)."""
     from .samples import Buffer
     lib = _lib.load()
     rd = C.c_void_p()
-    rc = lib.hbt_reader_open(filename.encode(), int(event_buffer_size), int(particle_monval),
-                             C.byref(rd))
+    rc = lib.hbt_reader_open_mode(filename.encode(), int(read_in_mode), int(event_buffer_size),
+                                  int(particle_monval), C.byref(rd))
     if rc:
         raise HbtError(rc, f"hbt_reader_open {filename}")
     out = []

@@ -335,10 +335,20 @@ double hbt_flow_event_plane_angle(int32_t n, const double *px, const double *py,
  * species) is below event_buffer_size and returns the number of events
  * (possibly 0 at end of file); hbt_reader_buffer exposes the kept
  * particles (SoA {px,py,pz,E,x,y,z,t}, event offsets) until the next call.
- * particle_monval selects one Monte-Carlo id (99999 keeps all). */
+ * particle_monval selects one Monte-Carlo id (99999 keeps all).  Numbers
+ * are read as strtod reads them (bit-identical values); lines are parsed on
+ * several host threads. */
 typedef struct hbt_reader hbt_reader;
 int hbt_reader_open(const char *filename, int32_t event_buffer_size, int32_t particle_monval,
                     hbt_reader **out);
+/* The same for read_in_mode 10 (gzip or plain text, as above) or 9
+ * (particleSamples::read_in_particle_samples_binary, particleSamples.cpp:
+ * 1203-1245: per event an int count, then per particle an int Monte-Carlo id
+ * and 9 floats mass t x y z E px py pz, widened to double).  The reference's
+ * mode-9 reader files every particle under event 0; this one keeps the
+ * file's events. */
+int hbt_reader_open_mode(const char *filename, int32_t read_in_mode, int32_t event_buffer_size,
+                         int32_t particle_monval, hbt_reader **out);
 int32_t hbt_reader_next(hbt_reader *rd);
 int32_t hbt_reader_eof(const hbt_reader *rd);
 int hbt_reader_buffer(const hbt_reader *rd, const int64_t **ev_off, const double *soa[8],

@@ -85,3 +85,51 @@ def test_cli_matches_reference_files(name, precision, tmp_path):
             assert ok.all(), (fn, la, lb)
     if precision == 0:
         assert same >= 0.999 * total, (same, total)
+
+
+def test_cli_mode9_binary(tmp_path):
+    """read_in_mode 9 (results/particle_list.bin, float records) end to end:
+    the CLI's files against the oracle's on the same particles (the floats
+    widened to double) and the same buffering; bar as for FP64 above."""
+    full = samples.generate(1011, 14, 120.0)
+    f32 = {k: getattr(full, k).astype(np.float32) for k in
+           ("mass", "t", "x", "y", "z", "E", "px", "py", "pz")}
+    work = tmp_path / "work"
+    res = work / "results"
+    res.mkdir(parents=True)
+    with open(res / "particle_list.bin", "wb") as f:
+        for e in range(full.nev):
+            a, b = int(full.ev_off[e]), int(full.ev_off[e + 1])
+            np.array([b - a], np.int32).tofile(f)
+            rec = np.zeros(b - a, dtype=[("m", "<i4"), ("v", "<f4", (9,))])
+            rec["m"] = full.monval[a:b]
+            rec["v"] = np.stack([f32[k][a:b] for k in ("mass", "t", "x", "y", "z", "E", "px",
+                                                        "py", "pz")], axis=1)
+            rec.tofile(f)
+    hp = dict(qnpts=21, q_min=-0.15, q_max=0.15, n_KT=4, KT_min=0.2, KT_max=0.8,
+              HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18)
+    keys = dict(hp, analyze_HBT=1, read_in_mode=9, particle_monval=211, event_buffer_size=500,
+                read_in_real_mixed_events=0, randomSeed=31, hbt_precision=0)
+    write_parameters(str(work / "parameters.dat"), keys)
+    r = subprocess.run([CLI], cwd=str(work), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    wide = samples.Buffer(ev_off=full.ev_off, **{k: v.astype(np.float64) for k, v in f32.items()},
+                          monval=full.monval)
+    o = oracle.Oracle(hp)
+    rng = oracle.OracleRng(31)
+    for first, count in samples.buffers_by_budget(wide, 500):
+        o.calculate_buffer(wide.events(first, count), rng)
+    od = tmp_path / "oracle"
+    od.mkdir()
+    o.write_outputs(str(od), 0)
+    got = sorted(f for f in os.listdir(res) if f.endswith(".dat"))
+    assert got == sorted(os.listdir(od)) and got
+    for fn in got:
+        a = (res / fn).read_text().splitlines()
+        b = (od / fn).read_text().splitlines()
+        assert len(a) == len(b)
+        for la, lb in zip(a, b):
+            if la != lb:
+                va = np.array([float(x) for x in la.split()])
+                vb = np.array([float(x) for x in lb.split()])
+                assert (np.isclose(va, vb, rtol=2e-8, atol=0) | (np.isnan(va) & np.isnan(vb))).all(), (fn, la, lb)

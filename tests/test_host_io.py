@@ -97,3 +97,113 @@ def test_psi_ref_golden():
     for n, g in zip((1, 2, 3), golden):
         v = hbt.flow_event_plane_angle(z["px"], z["py"], n)
         assert abs(v - g) < 1e-8 and v == z["psi_ref"][n - 1]
+
+
+def _number_strings(rng, n):
+    """Decimal spellings a sample file may hold: %g / %e / %f at every
+    precision, signs, leading and trailing zeros, missing integer or fraction
+    digits, long digit strings, and exponents beyond the fast path."""
+    out = []
+    mags = 10.0 ** rng.uniform(-30, 30, n)
+    vals = mags * rng.choice([-1.0, 1.0], n)
+    for k, v in enumerate(vals):
+        f = k % 9
+        if f == 0:
+            out.append("%.*g" % (1 + k % 17, v))
+        elif f == 1:
+            out.append("%.*e" % (k % 16, v))
+        elif f == 2:
+            out.append("%.*f" % (k % 12, v / 1e25))
+        elif f == 3:
+            out.append("%.*E" % (k % 10, v))
+        elif f == 4:
+            out.append("+%.7e" % abs(v))
+        elif f == 5:
+            out.append("%.17g" % v)
+        elif f == 6:
+            out.append("000" + ("%.9f" % (abs(v) / 1e25)))
+        elif f == 7:
+            out.append(("%.6f" % (abs(v) / 1e26)).lstrip("0") or "0")
+        else:
+            out.append("%d." % (k % 1000))
+    out += ["0", "-0", "0.000", "42", "5.", ".5", "-.25", "1e-400", "1e400", "-1e400",
+            "0.12345678901234567890123", "123456789012345678901234567890",
+            "9007199254740993", "9007199254740992", "1e22", "1e23", "1e-22", "1e-23",
+            "4.9406564584124654e-324", "2.2250738585072011e-308", "inf", "-inf"]
+    return out
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_reader_numbers_bit_identical_to_strtod(tmp_path, gz):
+    """Every field is read as strtod reads it: the reader's fast decimal path
+    and its fallback against Python's float() (correctly rounded, as glibc
+    strtod), on plain and gzip text, with tabs and repeated blanks."""
+    import gzip
+    rng = np.random.default_rng(7)
+    nums = _number_strings(rng, 4000)
+    nums += ["1.0"] * ((-len(nums)) % 9)
+    rows = [nums[k:k + 9] for k in range(0, len(nums), 9)]
+    text = [f"{len(rows)}\n"]
+    for k, r in enumerate(rows):
+        sep = "\t" if k % 3 == 0 else ("  " if k % 3 == 1 else " ")
+        text.append(f"{211 + (k % 2)}{sep}" + sep.join(r) + "\n")
+    path = tmp_path / ("s.gz" if gz else "s.txt")
+    if gz:
+        with gzip.open(path, "wt") as f:
+            f.write("".join(text))
+    else:
+        path.write_text("".join(text))
+    (b, total), = hbt.read_buffers(str(path), 10 ** 9, 99999)
+    assert total == len(rows) and list(b.ev_off) == [0, len(rows)]
+    want = np.array([[float(x) for x in r] for r in rows])  # mass t x y z E px py pz
+    cols = {"px": 6, "py": 7, "pz": 8, "E": 5, "x": 2, "y": 3, "z": 4, "t": 1}
+    for name, c in cols.items():
+        got = getattr(b, name)
+        assert np.array_equal(got.view(np.uint64), want[:, c].view(np.uint64)), name
+
+
+def _write_mode9(path, events):
+    with open(path, "wb") as f:
+        for mon, vals in events:
+            np.array([len(mon)], np.int32).tofile(f)
+            rec = np.zeros(len(mon), dtype=[("m", "<i4"), ("v", "<f4", (9,))])
+            rec["m"] = mon
+            rec["v"] = vals
+            rec.tofile(f)
+
+
+@pytest.mark.parametrize("budget", [1, 250, 700, 10 ** 6])
+def test_mode9_binary_reader(tmp_path, budget):
+    """Mode 9 (particleSamples.cpp:1203-1245): int count, then per particle an
+    int id and 9 floats mass t x y z E px py pz, widened to double; whole
+    events while the running total (all species) is below the budget; the
+    species filter keeps one id.  Events keep their own particles."""
+    rng = np.random.default_rng(3)
+    events = []
+    for e in range(9):
+        n = int(rng.integers(0, 200))
+        mon = rng.choice([211, -211, 321], n).astype(np.int32)
+        vals = rng.normal(0, 2, (n, 9)).astype(np.float32)
+        events.append((mon, vals))
+    path = tmp_path / "particle_list.bin"
+    _write_mode9(path, events)
+    got = hbt.read_buffers(str(path), budget, 211, read_in_mode=9)
+    e = 0
+    for b, total in got:
+        first, tot = e, 0
+        while tot < budget and e < len(events):
+            tot += len(events[e][0])
+            e += 1
+        assert total == tot and b.nev == e - first
+        keep = [ev[0] == 211 for ev in events[first:e]]
+        assert list(np.diff(b.ev_off)) == [int(k.sum()) for k in keep]
+        v = np.concatenate([ev[1][k] for ev, k in zip(events[first:e], keep)] or
+                           [np.zeros((0, 9), np.float32)]).astype(np.float64)
+        for name, c in {"px": 6, "py": 7, "pz": 8, "E": 5, "x": 2, "y": 3, "z": 4, "t": 1}.items():
+            assert np.array_equal(getattr(b, name), v[:, c]), name
+    # every event read once; a buffer that fills up exactly with the last
+    # event is followed by one empty buffer (the end is noticed by a read
+    # that finds nothing), which the loop above checks as a 0-event buffer
+    assert e == len(events)
+    if budget == 10 ** 6:
+        assert [b.nev for b, _ in got] == [9]
