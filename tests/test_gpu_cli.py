@@ -106,8 +106,9 @@ def test_cli_mode9_binary(tmp_path):
             rec["v"] = np.stack([f32[k][a:b] for k in ("mass", "t", "x", "y", "z", "E", "px",
                                                         "py", "pz")], axis=1)
             rec.tofile(f)
-    hp = dict(qnpts=21, q_min=-0.15, q_max=0.15, n_KT=4, KT_min=0.2, KT_max=0.8,
-              HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18)
+    hp = cases.hbt_params({"params": dict(qnpts=21, q_min=-0.15, q_max=0.15, n_KT=4, KT_min=0.2,
+                                          KT_max=0.8, HBTrap_min=-1.0, HBTrap_max=1.0,
+                                          needed_number_of_pairs=1e18)})
     keys = dict(hp, analyze_HBT=1, read_in_mode=9, particle_monval=211, event_buffer_size=500,
                 read_in_real_mixed_events=0, randomSeed=31, hbt_precision=0)
     write_parameters(str(work / "parameters.dat"), keys)
