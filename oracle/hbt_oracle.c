@@ -700,3 +700,234 @@ void orc_reader_close(orc_reader *rd) {
     free(rd->off);
     free(rd);
 }
+
+/* ======================================================================
+ * Balance function (BalanceFunction.cpp).  Restated from
+ *   constructor  :15-61   (bins: drap = 2|Brap_max|/(Bnpts-1),
+ *                          Brap_min = -|Brap_max| - drap/2, 20 phi bins of
+ *                          2 pi/20 from -pi/2)
+ *   per buffer   :63-101  (four same-event histograms, then four mixed ones,
+ *                          each mixed call drawing per event a partner
+ *                          rand_int_uniform() % nev_mixed and a rotation
+ *                          rand_uniform() * 2 * pi)
+ *   pair loops   :120-197 (pT cut on both, phi bin floor(...) % Bnphi made
+ *                          non-negative, |dy| < 1e-10 skipped, dy < Brap_min
+ *                          skipped, y bin truncation, 0 <= bin < Bnpts)
+ *   N_b, N_bbar  :68-69, :199-209
+ *   writer       :211-318
+ * with the per-particle quantities of particleSamples::boostParticles
+ * (particleSamples.cpp:442-468, rap_shift 0): pT, phi = atan2(py, px),
+ * y = asinh(pz / mT), eta = asinh(pz / pT), and the species lists of
+ * filter_particles (:1329-1460): exact Monte-Carlo id, 99999 = all.
+ * ====================================================================== */
+
+int orc_bf_init(orc_bf_state *s, const orc_bf_params *p) {
+    memset(s, 0, sizeof(*s));
+    s->p = *p;
+    if (p->Bnpts < 2) return -1;
+    s->drap = 2. * fabs(p->Brap_max) / (p->Bnpts - 1);
+    s->Brap_min = -fabs(p->Brap_max) - 0.5 * s->drap;
+    s->Bnphi = 20;
+    s->dphi = 2. * M_PI / s->Bnphi;
+    s->Bphi_min = -M_PI / 2.;
+    s->hist = (double *)calloc((size_t)8 * p->Bnpts * s->Bnphi, sizeof(double));
+    return s->hist ? 0 : -1;
+}
+
+void orc_bf_free(orc_bf_state *s) {
+    free(s->hist);
+    s->hist = NULL;
+}
+
+typedef struct {
+    int n;
+    double *pT, *phi, *y;
+    int64_t *off; /* nev+1 */
+} bf_list;
+
+static int bf_pick(int poi, int monval) { return poi == 99999 || monval == poi; }
+
+static void bf_list_build(bf_list *L, const orc_bf_buffer *b, int poi, int rap_type) {
+    const int64_t n = b->nev > 0 ? b->ev_off[b->nev] - b->ev_off[0] : 0;
+    L->pT = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    L->phi = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    L->y = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    L->off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(b->nev + 1));
+    int k = 0;
+    L->off[0] = 0;
+    for (int e = 0; e < b->nev; e++) {
+        for (int64_t i = b->ev_off[e]; i < b->ev_off[e + 1]; i++) {
+            if (!bf_pick(poi, b->monval[i])) continue;
+            const double px = b->px[i], py = b->py[i], pz = b->pz[i];
+            const double pT = sqrt(px * px + py * py);
+            const double mT = sqrt(pT * pT + b->mass[i] * b->mass[i]);
+            L->pT[k] = pT;
+            L->phi[k] = atan2(py, px);
+            L->y[k] = rap_type == 0 ? asinh(pz / pT) : asinh(pz / mT);
+            k++;
+        }
+        L->off[e + 1] = k;
+    }
+    L->n = k;
+}
+
+static void bf_list_free(bf_list *L) {
+    free(L->pT);
+    free(L->phi);
+    free(L->y);
+    free(L->off);
+}
+
+static void bf_pair(const orc_bf_state *s, double *hist, double phi_a, double y_a, double phi_b,
+                    double y_b, double rot, int mixed) {
+    const double delta_phi_local = mixed ? (phi_a - phi_b + rot) : (phi_a - phi_b);
+    int phi_idx = ((int)floor((delta_phi_local - s->Bphi_min) / s->dphi)) % s->Bnphi;
+    if (phi_idx < 0) phi_idx += s->Bnphi;
+    const double delta_y_local = y_a - y_b;
+    if (fabs(delta_y_local) < 1e-10) return;
+    if (delta_y_local < s->Brap_min) return;
+    const int y_bin_idx = (int)((delta_y_local - s->Brap_min) / s->drap);
+    if (y_bin_idx >= 0 && y_bin_idx < s->p.Bnpts) hist[(size_t)y_bin_idx * s->Bnphi + phi_idx] += 1.;
+}
+
+static int bf_cut(const orc_bf_state *s, double pT) {
+    return pT < s->p.BpT_min || pT > s->p.BpT_max;
+}
+
+static void bf_same(const orc_bf_state *s, double *hist, const bf_list *A, const bf_list *B,
+                    int nev) {
+    for (int e = 0; e < nev; e++)
+        for (int64_t i = A->off[e]; i < A->off[e + 1]; i++) {
+            if (bf_cut(s, A->pT[i])) continue;
+            for (int64_t j = B->off[e]; j < B->off[e + 1]; j++) {
+                if (bf_cut(s, B->pT[j])) continue;
+                bf_pair(s, hist, A->phi[i], A->y[i], B->phi[j], B->y[j], 0.0, 0);
+            }
+        }
+}
+
+static void bf_mixed(const orc_bf_state *s, orc_rng *r, double *hist, const bf_list *A,
+                     const bf_list *B, int nev, int nev_mixed) {
+    for (int e = 0; e < nev; e++) {
+        const int em = orc_rand_int_uniform(r) % nev_mixed;
+        const double rot = orc_rand_uniform(r) * 2. * M_PI;
+        for (int64_t i = A->off[e]; i < A->off[e + 1]; i++) {
+            if (bf_cut(s, A->pT[i])) continue;
+            for (int64_t j = B->off[em]; j < B->off[em + 1]; j++) {
+                if (bf_cut(s, B->pT[j])) continue;
+                bf_pair(s, hist, A->phi[i], A->y[i], B->phi[j], B->y[j], rot, 1);
+            }
+        }
+    }
+}
+
+static long bf_count(const orc_bf_state *s, const bf_list *L) {
+    long c = 0;
+    for (int k = 0; k < L->n; k++) c += !bf_cut(s, L->pT[k]);
+    return c;
+}
+
+void orc_bf_calculate(orc_bf_state *s, orc_rng *r, const orc_bf_buffer *b,
+                      const orc_bf_buffer *mix) {
+    const int rt = s->p.rap_type;
+    bf_list a, abar, bb, bbar, bm, bbarm;
+    bf_list_build(&a, b, s->p.alpha, rt);
+    bf_list_build(&abar, b, -s->p.alpha, rt);
+    bf_list_build(&bb, b, s->p.beta, rt);
+    bf_list_build(&bbar, b, -s->p.beta, rt);
+    bf_list_build(&bm, mix, s->p.beta, rt);
+    bf_list_build(&bbarm, mix, -s->p.beta, rt);
+    s->N_b += bf_count(s, &bb);
+    s->N_bbar += bf_count(s, &bbar);
+    const size_t H = (size_t)s->p.Bnpts * s->Bnphi;
+    bf_same(s, s->hist + 0 * H, &a, &bb, b->nev);
+    bf_same(s, s->hist + 1 * H, &abar, &bbar, b->nev);
+    bf_same(s, s->hist + 2 * H, &a, &bbar, b->nev);
+    bf_same(s, s->hist + 3 * H, &abar, &bb, b->nev);
+    bf_mixed(s, r, s->hist + 4 * H, &a, &bm, b->nev, mix->nev);
+    bf_mixed(s, r, s->hist + 5 * H, &abar, &bbarm, b->nev, mix->nev);
+    bf_mixed(s, r, s->hist + 6 * H, &a, &bbarm, b->nev, mix->nev);
+    bf_mixed(s, r, s->hist + 7 * H, &abar, &bm, b->nev, mix->nev);
+    bf_list_free(&a);
+    bf_list_free(&abar);
+    bf_list_free(&bb);
+    bf_list_free(&bbar);
+    bf_list_free(&bm);
+    bf_list_free(&bbarm);
+}
+
+int orc_bf_write_outputs(const orc_bf_state *s, const char *path) {
+    const int np = s->p.Bnpts, nf = s->Bnphi;
+    const double *C[8];
+    for (int a = 0; a < 8; a++) C[a] = s->hist + (size_t)a * np * nf;
+#define H(a, i, j) C[a][(size_t)(i) * nf + (j)]
+    double N_OS = 0., N_OS_mixed = 0., N_SS = 0., N_SS_mixed = 0.;
+    double *os_y = calloc(np, 8), *ss_y = calloc(np, 8), *osm_y = calloc(np, 8),
+           *ssm_y = calloc(np, 8);
+    double *os_p = calloc(nf, 8), *ss_p = calloc(nf, 8), *osm_p = calloc(nf, 8),
+           *ssm_p = calloc(nf, 8);
+    for (int i = 0; i < np; i++) {
+        for (int j = 0; j < nf; j++) {
+            os_y[i] += H(0, i, j) + H(1, i, j);
+            ss_y[i] += H(2, i, j) + H(3, i, j);
+            osm_y[i] += H(4, i, j) + H(5, i, j);
+            ssm_y[i] += H(6, i, j) + H(7, i, j);
+        }
+        N_OS += os_y[i];
+        N_SS += ss_y[i];
+        N_OS_mixed += osm_y[i];
+        N_SS_mixed += ssm_y[i];
+    }
+    for (int j = 0; j < nf; j++)
+        for (int i = 0; i < np; i++) {
+            os_p[j] += H(0, i, j) + H(1, i, j);
+            ss_p[j] += H(2, i, j) + H(3, i, j);
+            osm_p[j] += H(4, i, j) + H(5, i, j);
+            ssm_p[j] += H(6, i, j) + H(7, i, j);
+        }
+    char fn[4096];
+    snprintf(fn, sizeof fn, "%s/Balance_function_%d_%d_Delta_y.dat", path, s->p.alpha, s->p.beta);
+    FILE *f = fopen(fn, "w");
+    if (!f) return -1;
+    fprintf(f, "# DeltaY  Delta_C2  C2(OS)  rho2(OS)  rho1^2(OS)  C2(SS) rho2(SS)  rho1^2(SS)\n");
+    for (int i = 0; i < np; i++) {
+        const double dy = s->Brap_min + (i + 0.5) * s->drap;
+        const double c_os = os_y[i] / osm_y[i] * N_OS_mixed / N_OS;
+        const double c_ss = ss_y[i] / ssm_y[i] * N_SS_mixed / N_SS;
+        fprintf(f, "%18.8e   %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e\n", dy, c_os - c_ss, c_os,
+                os_y[i], osm_y[i], c_ss, ss_y[i], ssm_y[i]);
+    }
+    fclose(f);
+    snprintf(fn, sizeof fn, "%s/Balance_function_%d_%d_Delta_phi.dat", path, s->p.alpha,
+             s->p.beta);
+    f = fopen(fn, "w");
+    if (!f) return -1;
+    fprintf(f, "# Delta_phi  Delta_C2  C2(OS)  rho2(OS)  rho1^2(OS)  C2(SS) rho2(SS)  rho1^2(SS)\n");
+    for (int j = 0; j < nf; j++) {
+        const double dp = s->Bphi_min + (j + 0.5) * s->dphi;
+        const double c_os = (os_p[j] / osm_p[j] * N_OS_mixed / N_OS);
+        const double c_ss = (ss_p[j] / ssm_p[j] * N_SS_mixed / N_SS);
+        fprintf(f, "%18.8e   %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e\n", dp, c_os - c_ss, c_os,
+                os_p[j], osm_p[j], c_ss, ss_p[j], ssm_p[j]);
+    }
+    fclose(f);
+    snprintf(fn, sizeof fn, "%s/Correlation_function_%d_%d_2D.dat", path, s->p.alpha, s->p.beta);
+    f = fopen(fn, "w");
+    if (!f) return -1;
+    fprintf(f, "# DY  Dphi  C2(OS)  rho2(OS)  rho1^2(OS)  C2(SS)  rho2(SS)  rho1^2(SS)\n");
+    for (int i = 0; i < np; i++)
+        for (int j = 0; j < nf; j++) {
+            const double dy = s->Brap_min + (i + 0.5) * s->drap;
+            const double dp = s->Bphi_min + (j + 0.5) * s->dphi;
+            const double c_os = ((H(0, i, j) + H(1, i, j)) / (H(4, i, j) + H(5, i, j) + 1e-15));
+            const double c_ss = ((H(2, i, j) + H(3, i, j)) / (H(6, i, j) + H(7, i, j) + 1e-15));
+            fprintf(f, "%18.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e  %.8e\n", dy, dp,
+                    c_os * N_OS_mixed / N_OS, H(0, i, j) + H(1, i, j), H(4, i, j) + H(5, i, j),
+                    c_ss * N_SS_mixed / N_SS, H(2, i, j) + H(3, i, j), H(6, i, j) + H(7, i, j));
+        }
+    fclose(f);
+#undef H
+    free(os_y); free(ss_y); free(osm_y); free(ssm_y);
+    free(os_p); free(ss_p); free(osm_p); free(ssm_p);
+    return 0;
+}

@@ -137,6 +137,39 @@ const orc_buffer *orc_reader_buffer(const orc_reader *rd);
 long orc_reader_total_particles(const orc_reader *rd);
 void orc_reader_close(orc_reader *rd);
 
+/* ---------------- balance function (BalanceFunction.cpp) ---------------- */
+typedef struct {
+    int alpha, beta;             /* particle_alpha, particle_beta */
+    double BpT_min, BpT_max, Brap_max;
+    int Bnpts, rap_type;         /* rap_type 0: pseudo-rapidity, 1: rapidity */
+} orc_bf_params;
+
+typedef struct {
+    orc_bf_params p;
+    int Bnphi;
+    double dphi, Bphi_min, drap, Brap_min;
+    double *hist;                /* [8][Bnpts][Bnphi]: C_ab, C_abarbbar, C_abbar,
+                                    C_abarb, then the four mixed-event ones */
+    long N_b, N_bbar;
+} orc_bf_state;
+
+/* One buffer of events with all species (monval, mass and momenta). */
+typedef struct {
+    int nev;
+    const int64_t *ev_off;
+    const int *monval;
+    const double *mass, *px, *py, *pz, *E;
+} orc_bf_buffer;
+
+int orc_bf_init(orc_bf_state *s, const orc_bf_params *p);
+void orc_bf_free(orc_bf_state *s);
+/* calculate_balance_function on one buffer (BalanceFunction.cpp:63-101);
+ * mix may equal b (the aliased mixed list). */
+void orc_bf_calculate(orc_bf_state *s, orc_rng *r, const orc_bf_buffer *b,
+                      const orc_bf_buffer *mix);
+/* output_balance_function (BalanceFunction.cpp:211-318). */
+int orc_bf_write_outputs(const orc_bf_state *s, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

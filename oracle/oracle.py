@@ -267,3 +267,81 @@ def read_mode10(filename: str, event_buffer_size: int, monval: int):
     finally:
         L.orc_reader_close(rd)
     return bufs
+
+
+# ---------------------------------------------------------------- balance function
+
+class BfParams(C.Structure):
+    _fields_ = [("alpha", C.c_int), ("beta", C.c_int), ("BpT_min", C.c_double),
+                ("BpT_max", C.c_double), ("Brap_max", C.c_double), ("Bnpts", C.c_int),
+                ("rap_type", C.c_int)]
+
+
+class BfState(C.Structure):
+    _fields_ = [("p", BfParams), ("Bnphi", C.c_int), ("dphi", C.c_double),
+                ("Bphi_min", C.c_double), ("drap", C.c_double), ("Brap_min", C.c_double),
+                ("hist", _D), ("N_b", C.c_long), ("N_bbar", C.c_long)]
+
+
+class BfBuffer(C.Structure):
+    _fields_ = [("nev", C.c_int), ("ev_off", C.POINTER(C.c_int64)),
+                ("monval", C.POINTER(C.c_int))] + [
+        (n, _D) for n in ("mass", "px", "py", "pz", "E")]
+
+
+class CBfBuffer:
+    """Keeps the numpy arrays alive behind a BfBuffer."""
+
+    def __init__(self, buf):
+        self.off = np.ascontiguousarray(buf.ev_off, dtype=np.int64)
+        self.mon = np.ascontiguousarray(buf.monval, dtype=np.int32)
+        self.cols = [np.ascontiguousarray(getattr(buf, k), dtype=np.float64)
+                     for k in ("mass", "px", "py", "pz", "E")]
+        self.c = BfBuffer(len(self.off) - 1, self.off.ctypes.data_as(C.POINTER(C.c_int64)),
+                          self.mon.ctypes.data_as(C.POINTER(C.c_int)),
+                          *[a.ctypes.data_as(_D) for a in self.cols])
+
+
+BF_KEYS = ("particle_alpha", "particle_beta", "BpT_min", "BpT_max", "Brap_max", "Bnpts",
+           "rap_type")
+
+
+class BfOracle:
+    """CPU restatement of BalanceFunction (BalanceFunction.cpp)."""
+
+    def __init__(self, params: dict):
+        L = lib()
+        if not hasattr(L, "_bf_ready"):
+            L.orc_bf_init.argtypes = [C.POINTER(BfState), C.POINTER(BfParams)]
+            L.orc_bf_free.argtypes = [C.POINTER(BfState)]
+            L.orc_bf_calculate.argtypes = [C.POINTER(BfState), C.POINTER(Rng),
+                                           C.POINTER(BfBuffer), C.POINTER(BfBuffer)]
+            L.orc_bf_write_outputs.argtypes = [C.POINTER(BfState), C.c_char_p]
+            L._bf_ready = True
+        self.L = L
+        p = BfParams(int(params["particle_alpha"]), int(params["particle_beta"]),
+                     float(params["BpT_min"]), float(params["BpT_max"]),
+                     float(params["Brap_max"]), int(params["Bnpts"]), int(params["rap_type"]))
+        self.s = BfState()
+        if L.orc_bf_init(C.byref(self.s), C.byref(p)):
+            raise ValueError("orc_bf_init")
+
+    def __del__(self):
+        try:
+            self.L.orc_bf_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    def calculate_buffer(self, buf, rng: "OracleRng", mix=None):
+        b = CBfBuffer(buf)
+        m = CBfBuffer(mix) if mix is not None else b
+        self.L.orc_bf_calculate(C.byref(self.s), C.byref(rng.r), C.byref(b.c), C.byref(m.c))
+
+    def write_outputs(self, path: str) -> int:
+        return self.L.orc_bf_write_outputs(C.byref(self.s), path.encode())
+
+    def arrays(self) -> dict:
+        n = 8 * self.s.p.Bnpts * self.s.Bnphi
+        h = np.ctypeslib.as_array(self.s.hist, shape=(n,)).copy()
+        return dict(hist=h.reshape(8, self.s.p.Bnpts, self.s.Bnphi), N_b=int(self.s.N_b),
+                    N_bbar=int(self.s.N_bbar))

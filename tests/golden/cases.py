@@ -92,3 +92,26 @@ def ref_arrays_as_u64(ref):
             assert np.all(v == np.floor(v))
             out[k] = v.astype(np.uint64)
     return out
+
+
+BF_CASE_NAMES = ["bf_pipi", "bf_kpi_eta", "bf_pi_antipi"]
+
+
+def bf_input(spec, digests):
+    """The balance-function case's sample (all species), regenerated from its
+    seed and checked against the digest of the mode-10 text the reference
+    read."""
+    seed, nev, mult = spec["input"]
+    species = tuple(tuple(s) for s in spec["species"])
+    full = samples.generate(seed, nev, mult, species=species)
+    text = io.StringIO()
+    for e in range(full.nev):
+        a, b = int(full.ev_off[e]), int(full.ev_off[e + 1])
+        text.write(f"{b - a}\n")
+        for i in range(a, b):
+            text.write("%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n" % (
+                full.monval[i], full.mass[i], full.t[i], full.x[i], full.y[i], full.z[i],
+                full.E[i], full.px[i], full.py[i], full.pz[i]))
+    digest = hashlib.sha256(text.getvalue().encode()).hexdigest()
+    assert digest == digests["particle_samples.gz"], "regenerated input differs"
+    return full
