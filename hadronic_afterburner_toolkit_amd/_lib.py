@@ -52,6 +52,13 @@ class HbtResult(C.Structure):
     ]
 
 
+class HbtBfConfig(C.Structure):
+    _fields_ = [("particle_alpha", C.c_int32), ("particle_beta", C.c_int32),
+                ("BpT_min", C.c_double), ("BpT_max", C.c_double), ("Brap_max", C.c_double),
+                ("Bnpts", C.c_int32), ("rap_type", C.c_int32), ("device", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class HbtServerConfig(C.Structure):
     _fields_ = [("hbm_bytes", C.c_int64), ("max_resident", C.c_int32),
                 ("max_step_items", C.c_int32), ("step_pairs", C.c_double),
@@ -137,6 +144,17 @@ SIGNATURES = [
     ("hbt_server_result", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(HbtResult)]),
     ("hbt_server_release", C.c_int, [C.c_void_p, C.c_int64]),
     ("hbt_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(HbtServerStats)]),
+    ("hbt_bf_create", C.c_int, [C.POINTER(HbtBfConfig), C.POINTER(C.c_void_p)]),
+    ("hbt_bf_destroy", C.c_int, [C.c_void_p]),
+    ("hbt_bf_last_error", C.c_char_p, [C.c_void_p]),
+    ("hbt_bf_reset", C.c_int, [C.c_void_p]),
+    ("hbt_bf_calculate", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int32), C.POINTER(_F64P), C.c_int32,
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                   C.POINTER(_F64P), C.c_void_p]),
+    ("hbt_bf_read", C.c_int, [C.c_void_p, _F64P, C.POINTER(C.c_int64),
+                              C.POINTER(C.c_int64)]),
+    ("hbt_bf_write_outputs", C.c_int, [C.POINTER(HbtBfConfig), _F64P, C.c_char_p]),
 ]
 
 _lib = None

@@ -531,6 +531,88 @@ int hbt_reader_buffer(const hbt_reader *rd, const int64_t **ev_off, const double
     return HBT_OK;
 }
 
+// output_balance_function (BalanceFunction.cpp:211-318): the sums over
+// Delta phi and over Delta y in the reference's loop order, the mixed-event
+// normalization N_mixed / N per charge combination, and its three files.
+int hbt_bf_write_outputs(const hbt_bf_config *cfg, const double *hist, const char *path) {
+    if (!cfg || !hist || !path || cfg->Bnpts < 2) return HBT_ERR_ARG;
+    const int np = cfg->Bnpts, nf = 20;
+    const double drap = 2. * fabs(cfg->Brap_max) / (np - 1);
+    const double Brap_min = -fabs(cfg->Brap_max) - 0.5 * drap;
+    const double dphi = 2. * M_PI / nf, Bphi_min = -M_PI / 2.;
+    auto H = [&](int a, int i, int j) { return hist[((size_t)a * np + i) * nf + j]; };
+    double N_OS = 0., N_OS_mixed = 0., N_SS = 0., N_SS_mixed = 0.;
+    std::vector<double> os_y(np, 0.), ss_y(np, 0.), osm_y(np, 0.), ssm_y(np, 0.);
+    std::vector<double> os_p(nf, 0.), ss_p(nf, 0.), osm_p(nf, 0.), ssm_p(nf, 0.);
+    for (int i = 0; i < np; i++) {
+        for (int j = 0; j < nf; j++) {
+            os_y[i] += H(0, i, j) + H(1, i, j);
+            ss_y[i] += H(2, i, j) + H(3, i, j);
+            osm_y[i] += H(4, i, j) + H(5, i, j);
+            ssm_y[i] += H(6, i, j) + H(7, i, j);
+        }
+        N_OS += os_y[i];
+        N_SS += ss_y[i];
+        N_OS_mixed += osm_y[i];
+        N_SS_mixed += ssm_y[i];
+    }
+    for (int j = 0; j < nf; j++)
+        for (int i = 0; i < np; i++) {
+            os_p[j] += H(0, i, j) + H(1, i, j);
+            ss_p[j] += H(2, i, j) + H(3, i, j);
+            osm_p[j] += H(4, i, j) + H(5, i, j);
+            ssm_p[j] += H(6, i, j) + H(7, i, j);
+        }
+    const std::string base = std::string(path) + "/";
+    const std::string ab = std::to_string(cfg->particle_alpha) + "_" +
+                           std::to_string(cfg->particle_beta);
+    // one line: the first field right-aligned in 18 characters, then the
+    // rest, %.8e like std::scientific with precision 8
+    auto line = [](FILE *f, const char *sep0, const double *v, int n) {
+        fprintf(f, "%18.8e", v[0]);
+        for (int k = 1; k < n; k++) fprintf(f, "%s%.8e", k == 1 ? sep0 : "  ", v[k]);
+        fputc('\n', f);
+    };
+    FILE *f = fopen((base + "Balance_function_" + ab + "_Delta_y.dat").c_str(), "w");
+    if (!f) return HBT_ERR_ARG;
+    fputs("# DeltaY  Delta_C2  C2(OS)  rho2(OS)  rho1^2(OS)  C2(SS) rho2(SS)  rho1^2(SS)\n", f);
+    for (int i = 0; i < np; i++) {
+        const double c_os = os_y[i] / osm_y[i] * N_OS_mixed / N_OS;
+        const double c_ss = ss_y[i] / ssm_y[i] * N_SS_mixed / N_SS;
+        const double v[8] = {Brap_min + (i + 0.5) * drap, c_os - c_ss, c_os, os_y[i], osm_y[i],
+                             c_ss, ss_y[i], ssm_y[i]};
+        line(f, "   ", v, 8);
+    }
+    fclose(f);
+    f = fopen((base + "Balance_function_" + ab + "_Delta_phi.dat").c_str(), "w");
+    if (!f) return HBT_ERR_ARG;
+    fputs("# Delta_phi  Delta_C2  C2(OS)  rho2(OS)  rho1^2(OS)  C2(SS) rho2(SS)  rho1^2(SS)\n",
+          f);
+    for (int j = 0; j < nf; j++) {
+        const double c_os = (os_p[j] / osm_p[j] * N_OS_mixed / N_OS);
+        const double c_ss = (ss_p[j] / ssm_p[j] * N_SS_mixed / N_SS);
+        const double v[8] = {Bphi_min + (j + 0.5) * dphi, c_os - c_ss, c_os, os_p[j], osm_p[j],
+                             c_ss, ss_p[j], ssm_p[j]};
+        line(f, "   ", v, 8);
+    }
+    fclose(f);
+    f = fopen((base + "Correlation_function_" + ab + "_2D.dat").c_str(), "w");
+    if (!f) return HBT_ERR_ARG;
+    fputs("# DY  Dphi  C2(OS)  rho2(OS)  rho1^2(OS)  C2(SS)  rho2(SS)  rho1^2(SS)\n", f);
+    for (int i = 0; i < np; i++)
+        for (int j = 0; j < nf; j++) {
+            const double c_os = ((H(0, i, j) + H(1, i, j)) / (H(4, i, j) + H(5, i, j) + 1e-15));
+            const double c_ss = ((H(2, i, j) + H(3, i, j)) / (H(6, i, j) + H(7, i, j) + 1e-15));
+            const double v[8] = {Brap_min + (i + 0.5) * drap, Bphi_min + (j + 0.5) * dphi,
+                                 c_os * N_OS_mixed / N_OS, H(0, i, j) + H(1, i, j),
+                                 H(4, i, j) + H(5, i, j), c_ss * N_SS_mixed / N_SS,
+                                 H(2, i, j) + H(3, i, j), H(6, i, j) + H(7, i, j)};
+            line(f, "  ", v, 8);
+        }
+    fclose(f);
+    return HBT_OK;
+}
+
 int hbt_reader_close(hbt_reader *rd) {
     if (!rd) return HBT_OK;
     if (rd->gz) gzclose(rd->gz);

@@ -464,3 +464,95 @@ class HBTServer:
         st = HbtServerStats()
         self._check(self._lib.hbt_server_get_stats(self.h, C.byref(st)), "hbt_server_get_stats")
         return {k: getattr(st, k) for k, _ in HbtServerStats._fields_ if k != "reserved"}
+
+
+BF_KEYS = ("particle_alpha", "particle_beta", "BpT_min", "BpT_max", "Brap_max", "Bnpts",
+           "rap_type")
+
+
+def bf_config_from_params(params: dict, device: int = 0) -> "_lib.HbtBfConfig":
+    return _lib.HbtBfConfig(int(params["particle_alpha"]), int(params["particle_beta"]),
+                            float(params["BpT_min"]), float(params["BpT_max"]),
+                            float(params["Brap_max"]), int(params["Bnpts"]),
+                            int(params["rap_type"]), int(device), 0)
+
+
+class BalanceFunction:
+    """BalanceFunction (BalanceFunction.h) on the GPU: calculate per buffer,
+    histograms and counts, the reference's output files."""
+
+    def __init__(self, params: dict, device: int = 0):
+        self._lib = _lib.load()
+        self.params = {k: params[k] for k in BF_KEYS}
+        self.cfg = bf_config_from_params(params, device)
+        h = C.c_void_p()
+        rc = self._lib.hbt_bf_create(C.byref(self.cfg), C.byref(h))
+        if rc:
+            raise HbtError(rc, "hbt_bf_create: " + self._lib.hbt_bf_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc:
+            raise HbtError(rc, f"{what}: {self._lib.hbt_bf_last_error(self.h).decode()}")
+
+    @staticmethod
+    def _buf_args(buf):
+        off = np.ascontiguousarray(buf.ev_off, dtype=np.int64)
+        mon = np.ascontiguousarray(buf.monval, dtype=np.int32)
+        cols = [np.ascontiguousarray(getattr(buf, k), dtype=np.float64)
+                for k in ("mass", "px", "py", "pz", "E")]
+        ptrs = (_lib._F64P * 5)(*[c.ctypes.data_as(_lib._F64P) for c in cols])
+        return off, mon, cols, ptrs
+
+    def calculate_balance_function(self, buf, rng: "Random", mix=None):
+        """One buffer (all species; mass and monval needed), as
+        BalanceFunction::calculate_balance_function."""
+        off, mon, cols, ptrs = self._buf_args(buf)
+        if mix is not None:
+            moff, mmon, mcols, mptrs = self._buf_args(mix)
+            margs = (len(moff) - 1, moff.ctypes.data_as(C.POINTER(C.c_int64)),
+                     mmon.ctypes.data_as(C.POINTER(C.c_int32)), mptrs)
+        else:
+            margs = (0, None, None, None)
+        self._check(self._lib.hbt_bf_calculate(
+            self.h, len(off) - 1, off.ctypes.data_as(C.POINTER(C.c_int64)),
+            mon.ctypes.data_as(C.POINTER(C.c_int32)), ptrs, *margs, rng.h), "hbt_bf_calculate")
+
+    def results(self) -> dict:
+        n = 8 * int(self.params["Bnpts"]) * 20
+        hist = np.zeros(n)
+        nb, nbb = C.c_int64(), C.c_int64()
+        self._check(self._lib.hbt_bf_read(self.h, hist.ctypes.data_as(_lib._F64P), C.byref(nb),
+                                          C.byref(nbb)), "hbt_bf_read")
+        return dict(hist=hist.reshape(8, int(self.params["Bnpts"]), 20), N_b=int(nb.value),
+                    N_bbar=int(nbb.value))
+
+    def output_balance_function(self, path: str):
+        hist = np.ascontiguousarray(self.results()["hist"].ravel())
+        self._check(self._lib.hbt_bf_write_outputs(C.byref(self.cfg),
+                                                   hist.ctypes.data_as(_lib._F64P),
+                                                   path.encode()), "hbt_bf_write_outputs")
+
+    def reset(self):
+        self._check(self._lib.hbt_bf_reset(self.h), "hbt_bf_reset")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.hbt_bf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_bf_outputs(params: dict, hist, path: str):
+    """The reference's balance-function files from histograms (host only)."""
+    lib = _lib.load()
+    cfg = bf_config_from_params(params)
+    h = np.ascontiguousarray(np.asarray(hist, dtype=np.float64).ravel())
+    rc = lib.hbt_bf_write_outputs(C.byref(cfg), h.ctypes.data_as(_lib._F64P), path.encode())
+    if rc:
+        raise HbtError(rc, "hbt_bf_write_outputs")

@@ -362,6 +362,49 @@ int hbt_last_kernel_ms(hbt_ctx *ctx, double *same_ms, double *mixed_ms);
 /* Number of particles kept by the rapidity cut in a slot, and of events. */
 int hbt_slot_info(hbt_ctx *ctx, int32_t slot, int32_t *nev, int64_t *n_kept);
 
+/* ---------------- balance function (BalanceFunction.cpp) ----------------
+ * The pair histograms of BalanceFunction (BalanceFunction.cpp:15-197) on the
+ * GPU: for a = particle_alpha, b = particle_beta and their antiparticles
+ * (ids -a, -b), same-event (a,b), (abar,bbar), (a,bbar), (abar,b) and the
+ * four mixed-event histograms, each [Bnpts][20] over (Delta y or Delta eta,
+ * Delta phi), counts in doubles as the reference keeps them.  Particles
+ * outside [BpT_min, BpT_max] are skipped.  Ids name one Monte-Carlo id;
+ * the species classes 9999, +-9998, +-9997, +-9996 need the PDG tables and
+ * are refused. */
+typedef struct {
+    int32_t particle_alpha, particle_beta;
+    double BpT_min, BpT_max, Brap_max;
+    int32_t Bnpts;     /* Delta y bins, 2..819 */
+    int32_t rap_type;  /* 0: pseudo-rapidity, 1: rapidity */
+    int32_t device;
+    int32_t reserved;
+} hbt_bf_config;
+
+typedef struct hbt_bf_ctx hbt_bf_ctx;
+/* Replaces the BalanceFunction constructor (BalanceFunction.cpp:15-61). */
+int hbt_bf_create(const hbt_bf_config *cfg, hbt_bf_ctx **out);
+int hbt_bf_destroy(hbt_bf_ctx *ctx);
+const char *hbt_bf_last_error(const hbt_bf_ctx *ctx);
+/* Zeroes the histograms and the N_b / N_bbar counts. */
+int hbt_bf_reset(hbt_bf_ctx *ctx);
+/* Replaces calculate_balance_function on one buffer (BalanceFunction.cpp:
+ * 63-101): nev events (ev_off nev+1) with all species, cols = {mass, px,
+ * py, pz, E}; the mixed list is a separate buffer or, with mix_off NULL,
+ * the same one (particleSamples.cpp:528-531).  The mixing draws come from
+ * rng in the reference's order: per mixed histogram, per event a partner
+ * rand_int_uniform() % mix_nev and a rotation 2 pi rand_uniform(). */
+int hbt_bf_calculate(hbt_bf_ctx *ctx, int32_t nev, const int64_t *ev_off,
+                     const int32_t *monval, const double *const cols[5], int32_t mix_nev,
+                     const int64_t *mix_off, const int32_t *mix_monval,
+                     const double *const mix_cols[5], hbt_rng *rng);
+/* hist: 8 * Bnpts * 20 doubles (C_ab, C_abarbbar, C_abbar, C_abarb, then the
+ * mixed four), N_b / N_bbar: b / bbar particles inside the pT cut. */
+int hbt_bf_read(hbt_bf_ctx *ctx, double *hist, int64_t *N_b, int64_t *N_bbar);
+/* Replaces output_balance_function (BalanceFunction.cpp:211-318): writes
+ * Balance_function_<a>_<b>_Delta_y.dat, _Delta_phi.dat and
+ * Correlation_function_<a>_<b>_2D.dat into path. */
+int hbt_bf_write_outputs(const hbt_bf_config *cfg, const double *hist, const char *path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,3 +40,16 @@ def test_bf_oracle_equals_reference(name, tmp_path):
     for fn, meta in outs.items():
         with open(tmp_path / fn, "rb") as f:
             assert hashlib.sha256(f.read()).hexdigest() == meta["sha256"], fn
+
+
+@pytest.mark.parametrize("name", cases.BF_CASE_NAMES)
+def test_bf_engine_writer_byte_identical(name, tmp_path):
+    """The engine library's writer (host code) on the reference's histograms
+    writes the reference's three files byte for byte."""
+    from hadronic_afterburner_toolkit_amd import hbt
+    spec, ref, outs, _ = cases.load(name)
+    hbt.write_bf_outputs(bf_params(spec), ref["hist"], str(tmp_path))
+    assert sorted(os.listdir(tmp_path)) == sorted(outs)
+    for fn, meta in outs.items():
+        with open(tmp_path / fn, "rb") as f:
+            assert hashlib.sha256(f.read()).hexdigest() == meta["sha256"], fn
