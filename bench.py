@@ -269,6 +269,78 @@ def large_histogram(args, log, tuning):
     return best
 
 
+BF_PARAMS = dict(particle_alpha=211, particle_beta=211, BpT_min=0.2, BpT_max=3.0,
+                 Brap_max=2.0, Bnpts=21, rap_type=1)  # parameters.dat:122-128 with pi+
+BF_SPECIES = ((211, 0.13957, 0.25, 3.0), (-211, 0.13957, 0.25, 3.0),
+              (321, 0.49368, 0.30, 1.0), (-321, 0.49368, 0.30, 1.0))
+BF_FAST_VALU_PER_PAIR = 15  # vector instructions per pair, k_bf fast form (gfx950 assembly)
+MI355X_CUS, MI355X_GHZ = 256, 2.4
+
+
+def balance_function(args, log):
+    """Balance-function pair histograms (BalanceFunction.cpp:120-197): 2000
+    events, Poisson(900) particles of pi+-, K+- (3:3:1:1), alpha = beta = 211
+    with the reference's parameters.dat cuts (pT in [0.2, 3] GeV, 21 Delta y
+    bins up to 2, rapidity), one buffer; the fast and the exact forms, and
+    the oracle on the first 40 events (one thread)."""
+    from hadronic_afterburner_toolkit_amd import _lib, hbt, samples
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    buf = samples.generate(args.seed + 70, 2000, 900.0, species=BF_SPECIES)
+    out = {"workload": "2000 events Poisson(900) pi+-,K+- 3:3:1:1, alpha=beta=211, "
+                       "Bnpts 21, Brap_max 2, pT 0.2-3 GeV, rapidity",
+           "events": buf.nev, "particles": int(buf.n)}
+    hists = {}
+    for name, kern in (("fast", _lib.HBT_BF_KERNEL_FAST), ("exact", _lib.HBT_BF_KERNEL_EXACT)):
+        bf = hbt.BalanceFunction(BF_PARAMS, kernel=kern)
+        runs = []
+        for rep in range(4):
+            bf.reset()
+            t0 = time.perf_counter()
+            bf.calculate_balance_function(buf, hbt.Random(args.seed + 71))
+            wall = time.perf_counter() - t0
+            runs.append((bf.info(), wall))
+        hists[name] = bf.results()["hist"]
+        bf.close()
+        runs = runs[1:]  # the first call allocates
+        info = runs[0][0]
+        kms = float(np.median([r[0]["kernel_ms"] for r in runs]))
+        line = {"kernel_ms": kms, "prep_ms": float(np.median([r[0]["prep_ms"] for r in runs])),
+                "host_ms": float(np.median([r[0]["host_ms"] for r in runs])),
+                "call_ms": 1e3 * float(np.median([r[1] for r in runs])),
+                "pairs": int(info["pairs"]), "pairs_per_s": info["pairs"] / (kms * 1e-3),
+                "fallback_fraction": info["fallback"] / max(1, info["pairs"]),
+                "launches": int(info["launches"])}
+        line["pairs_per_s_call"] = info["pairs"] / (line["call_ms"] * 1e-3)
+        out[name] = line
+        log(f"  balance function {name}: {kms:.3f} ms, {line['pairs_per_s']:.3e} pairs/s "
+            f"(call {line['call_ms']:.1f} ms)")
+    out["identical_fast_vs_exact"] = bool(np.array_equal(hists["fast"], hists["exact"]))
+    ceil = MI355X_CUS * MI355X_GHZ * 1e9 * 64 / BF_FAST_VALU_PER_PAIR
+    out["valu_roofline"] = {"bound": "valu", "ops_per_pair": BF_FAST_VALU_PER_PAIR,
+                            "ceiling_pairs_per_s": ceil,
+                            "frac": out["fast"]["pairs_per_s"] / ceil, "kernel": "k_bf fast"}
+    if not args.no_cpu:
+        oracle.build()
+        sub = buf.events(0, 40)
+        o = oracle.BfOracle(BF_PARAMS)
+        t0 = time.perf_counter()
+        o.calculate_buffer(sub, oracle.OracleRng(args.seed + 71))
+        dt = time.perf_counter() - t0
+        bf = hbt.BalanceFunction(BF_PARAMS)
+        bf.calculate_balance_function(sub, hbt.Random(args.seed + 71))
+        pairs = bf.info()["pairs"]
+        same = bool(np.array_equal(bf.results()["hist"], o.arrays()["hist"]))
+        bf.close()
+        out["cpu_baseline"] = {"value": pairs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"first 40 events: {pairs:.3e} pairs in {dt:.2f}s, "
+                                         "oracle/hbt_oracle.c -O2 (lists + pair loops)",
+                               "gpu_equals_oracle": same}
+        out["speedup_kernel_vs_cpu_1core"] = out["fast"]["pairs_per_s"] / (pairs / dt)
+        log(f"  balance function cpu oracle: {pairs / dt:.3e} pairs/s")
+    return out
+
+
 def cpu_baseline(group, plan_g, log):
     """The CPU oracle on group 0 (its same-event pairs and its events' mixed
     pairs with the same plan), one thread."""
@@ -467,6 +539,8 @@ def main():
     ap.add_argument("--serve-step-pairs", type=float, default=2e9)
     ap.add_argument("--tuning", default=None, metavar="R,JCHUNK,KCHUNK,BLOCKS_PER_CU",
                     help="fixed launch shape of the fast kernel (implies --no-autotune)")
+    ap.add_argument("--only-bf", action="store_true",
+                    help="only the balance-function line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -481,6 +555,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, max(1, args.steps))
+        return
+    if args.only_bf:
+        print(json.dumps({"balance_function": balance_function(args, log)}), flush=True)
         return
 
     tuning = [int(v) for v in args.tuning.split(",")] if args.tuning else None
@@ -586,6 +663,7 @@ def main():
             add_atomic_floor(line, atom_ceil)
     if not args.no_extra:
         out["ingest"] = ingest(args, log)
+        out["balance_function"] = balance_function(args, log)
     if not args.no_cpu:
         g0 = groups[0]
         M = int(plan[0][0])

@@ -56,7 +56,17 @@ class HbtBfConfig(C.Structure):
     _fields_ = [("particle_alpha", C.c_int32), ("particle_beta", C.c_int32),
                 ("BpT_min", C.c_double), ("BpT_max", C.c_double), ("Brap_max", C.c_double),
                 ("Bnpts", C.c_int32), ("rap_type", C.c_int32), ("device", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("kernel", C.c_int32)]
+
+
+HBT_BF_KERNEL_FAST = 0
+HBT_BF_KERNEL_EXACT = 1
+
+
+class HbtBfInfo(C.Structure):
+    _fields_ = [("host_ms", C.c_double), ("prep_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("pairs", C.c_uint64), ("fallback", C.c_uint64), ("kernel", C.c_int32),
+                ("launches", C.c_int32)]
 
 
 class HbtServerConfig(C.Structure):
@@ -155,6 +165,7 @@ SIGNATURES = [
     ("hbt_bf_read", C.c_int, [C.c_void_p, _F64P, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)]),
     ("hbt_bf_write_outputs", C.c_int, [C.POINTER(HbtBfConfig), _F64P, C.c_char_p]),
+    ("hbt_bf_last_info", C.c_int, [C.c_void_p, C.POINTER(HbtBfInfo)]),
 ]
 
 _lib = None

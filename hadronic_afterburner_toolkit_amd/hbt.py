@@ -470,21 +470,22 @@ BF_KEYS = ("particle_alpha", "particle_beta", "BpT_min", "BpT_max", "Brap_max", 
            "rap_type")
 
 
-def bf_config_from_params(params: dict, device: int = 0) -> "_lib.HbtBfConfig":
+def bf_config_from_params(params: dict, device: int = 0,
+                          kernel: int = _lib.HBT_BF_KERNEL_FAST) -> "_lib.HbtBfConfig":
     return _lib.HbtBfConfig(int(params["particle_alpha"]), int(params["particle_beta"]),
                             float(params["BpT_min"]), float(params["BpT_max"]),
                             float(params["Brap_max"]), int(params["Bnpts"]),
-                            int(params["rap_type"]), int(device), 0)
+                            int(params["rap_type"]), int(device), int(kernel))
 
 
 class BalanceFunction:
     """BalanceFunction (BalanceFunction.h) on the GPU: calculate per buffer,
     histograms and counts, the reference's output files."""
 
-    def __init__(self, params: dict, device: int = 0):
+    def __init__(self, params: dict, device: int = 0, kernel: int = _lib.HBT_BF_KERNEL_FAST):
         self._lib = _lib.load()
         self.params = {k: params[k] for k in BF_KEYS}
-        self.cfg = bf_config_from_params(params, device)
+        self.cfg = bf_config_from_params(params, device, kernel)
         h = C.c_void_p()
         rc = self._lib.hbt_bf_create(C.byref(self.cfg), C.byref(h))
         if rc:
@@ -526,6 +527,12 @@ class BalanceFunction:
                                           C.byref(nbb)), "hbt_bf_read")
         return dict(hist=hist.reshape(8, int(self.params["Bnpts"]), 20), N_b=int(nb.value),
                     N_bbar=int(nbb.value))
+
+    def info(self) -> dict:
+        """Timing and counts of the last calculate_balance_function call."""
+        i = _lib.HbtBfInfo()
+        self._check(self._lib.hbt_bf_last_info(self.h, C.byref(i)), "hbt_bf_last_info")
+        return {f: getattr(i, f) for f, _ in _lib.HbtBfInfo._fields_}
 
     def output_balance_function(self, path: str):
         hist = np.ascontiguousarray(self.results()["hist"].ravel())

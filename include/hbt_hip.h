@@ -374,11 +374,31 @@ int hbt_slot_info(hbt_ctx *ctx, int32_t slot, int32_t *nev, int64_t *n_kept);
 typedef struct {
     int32_t particle_alpha, particle_beta;
     double BpT_min, BpT_max, Brap_max;
-    int32_t Bnpts;     /* Delta y bins, 2..819 */
+    int32_t Bnpts;     /* Delta y bins, 2..817 */
     int32_t rap_type;  /* 0: pseudo-rapidity, 1: rapidity */
     int32_t device;
-    int32_t reserved;
+    int32_t kernel;    /* HBT_BF_KERNEL_FAST (default) or HBT_BF_KERNEL_EXACT */
 } hbt_bf_config;
+
+/* Per-pair forms of the balance-function kernel; both give the reference's
+ * counts.  EXACT evaluates the reference's FP64 bin arithmetic for every
+ * pair; FAST takes the bins from FP32 differences of per-particle values in
+ * bin units and sends each pair within a guard band of a bin edge (or of
+ * Delta y = 0) to the FP64 arithmetic. */
+#define HBT_BF_KERNEL_FAST 0
+#define HBT_BF_KERNEL_EXACT 1
+
+/* What the last hbt_bf_calculate did: host_ms = species lists, draws and
+ * uploads on the host; prep_ms / kernel_ms = the per-particle preparation
+ * and the pair kernels (HIP events on the context's stream); pairs =
+ * enumerated pairs over the eight histograms; fallback = pairs of the FAST
+ * form that took the FP64 arithmetic; kernel = the form that ran; launches
+ * = pair-kernel launches (each below 2^32 pairs per histogram). */
+typedef struct {
+    double host_ms, prep_ms, kernel_ms;
+    uint64_t pairs, fallback;
+    int32_t kernel, launches;
+} hbt_bf_info;
 
 typedef struct hbt_bf_ctx hbt_bf_ctx;
 /* Replaces the BalanceFunction constructor (BalanceFunction.cpp:15-61). */
@@ -400,6 +420,9 @@ int hbt_bf_calculate(hbt_bf_ctx *ctx, int32_t nev, const int64_t *ev_off,
 /* hist: 8 * Bnpts * 20 doubles (C_ab, C_abarbbar, C_abbar, C_abarb, then the
  * mixed four), N_b / N_bbar: b / bbar particles inside the pT cut. */
 int hbt_bf_read(hbt_bf_ctx *ctx, double *hist, int64_t *N_b, int64_t *N_bbar);
+/* Timing and counts of the last hbt_bf_calculate (no counterpart in the
+ * reference, which has no timing). */
+int hbt_bf_last_info(hbt_bf_ctx *ctx, hbt_bf_info *info);
 /* Replaces output_balance_function (BalanceFunction.cpp:211-318): writes
  * Balance_function_<a>_<b>_Delta_y.dat, _Delta_phi.dat and
  * Correlation_function_<a>_<b>_2D.dat into path. */
