@@ -117,7 +117,8 @@ __device__ __forceinline__ int next_unit(int *queue) {
 }
 
 // LDS layout: rows 0 .. Bnpts+1 (row y_idx + 1 holds Delta y bin y_idx; rows
-// 0 and Bnpts + 1 collect the fast form's pairs outside the Delta y range),
+// 0 and Bnpts + 1 collect the fast form's pairs outside the Delta y range and
+// the lanes that bin nothing; they are never read),
 // S columns: the Delta phi bin (S = 20) or, FOLD, the unfolded index
 // floor(t_phi) in [5, 65], folded mod 20 when the block adds its histogram.
 template <bool EXACT, bool FOLD, bool ODD>
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_bf(BfArgs a) {
     const int nunits = a.nunits[k];
     const int lane = threadIdx.x & 63;
     const float top = (float)(a.g.Bnpts + 1);
+    const unsigned junk = (unsigned)(lane % S) * 4u;  // this lane's word of row 0
     unsigned nfb = 0;
 
     for (;;) {
@@ -167,47 +169,67 @@ __global__ __launch_bounds__(kThreads) void k_bf(BfArgs a) {
         // scalar loads, eight rows per load in the unrolled loop
         BF_CONST_PTR(BfPrep) ap_row = BF_CONST(BfPrep, Aprep + a0);
         BF_CONST_PTR(BfRec) ar_row = BF_CONST(BfRec, Arec + a0);
-        auto pair = [&](BfPrep ap, int i) {
-            bool slow = active;
-            if (!EXACT) {
-                const float tp = ap.p - bp.p;
-                const float ty = ap.y - bp.y;
-                const float ep = fabsf(__builtin_amdgcn_fractf(tp) - 0.5f);
-                const float ey = fabsf(__builtin_amdgcn_fractf(ty) - 0.5f);
-                bool ok = ep < a.Hp && ey < a.Hy;
-                if (ODD) ok = ok && ey > a.Gy;
-                if (active && ok) {
-                    const unsigned row = (unsigned)__builtin_amdgcn_fmed3f(ty, 0.f, top);
-                    const unsigned x = (unsigned)tp;
-                    const unsigned col = FOLD ? x : x % (unsigned)kBnphi;
-                    const unsigned off = __umul24(row, 4u * S) + (col << 2);
-                    atomicAdd((unsigned *)((char *)h + off), 1u);
-                }
-                slow = active && !ok;
-            }
-            if (slow) {
-                const BfRec ra = ar_row[i];
-                const double d = ra.phi - rb.phi;
-                int p;
-                const int y = bf_bin_exact(a.g, mixed ? d + rot : d, ra.y - rb.y, p);
-                if (y >= 0) atomicAdd(&h[(y + 1) * S + p], 1u);
-                if (!EXACT) nfb++;
-            }
+        // Fast form of one pair: bins it and returns whether it needs the
+        // exact form.  Branch-free: lanes that bin nothing (inactive, or
+        // inside a guard band) add to their own word of LDS row 0, which is
+        // never read, so the eight rows of a batch form one basic block the
+        // compiler can interleave.
+        auto fastpair = [&](BfPrep ap) -> bool {
+            const float tp = ap.p - bp.p;
+            const float ty = ap.y - bp.y;
+            const float ep = fabsf(__builtin_amdgcn_fractf(tp) - 0.5f);
+            const float ey = fabsf(__builtin_amdgcn_fractf(ty) - 0.5f);
+            bool ok = ep < a.Hp && ey < a.Hy;
+            if (ODD) ok = ok && ey > a.Gy;
+            const unsigned row = (unsigned)__builtin_amdgcn_fmed3f(ty, 0.f, top);
+            const unsigned x = (unsigned)tp;
+            const unsigned col = FOLD ? x : x % (unsigned)kBnphi;
+            unsigned off = __umul24(row, 4u * S) + (col << 2);
+            asm volatile("" : "+v"(off));  // keep the select below a select
+            atomicAdd((unsigned *)((char *)h + (active && ok ? off : junk)), 1u);
+            return active && !ok;
+        };
+        // Exact form (the reference's FP64 arithmetic) of the pair with row i.
+        auto slowpair = [&](int i) {
+            const BfRec ra = ar_row[i];
+            const double d = ra.phi - rb.phi;
+            int p;
+            const int y = bf_bin_exact(a.g, mixed ? d + rot : d, ra.y - rb.y, p);
+            if (y >= 0) atomicAdd(&h[(y + 1) * S + p], 1u);
+            if (!EXACT) nfb++;
         };
         int i = un.i0;
-        if (!EXACT) {
+        if (!EXACT && i + 8 <= un.i1) {
+            // the next batch of rows is loaded while this one is binned
+            // (its start clamped to i1 - 8, so the load stays in the unit);
+            // the rare exact-form pairs of a batch follow it
+            BfPrep q[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) q[r] = ap_row[i + r];
             for (; i + 8 <= un.i1; i += 8) {
-                BfPrep q[8];
+                const int nx = min(i + 8, un.i1 - 8);
+                BfPrep qn[8];
 #pragma unroll
-                for (int r = 0; r < 8; r++) q[r] = ap_row[i + r];
+                for (int r = 0; r < 8; r++) qn[r] = ap_row[nx + r];
+                bool sl[8];
 #pragma unroll
-                for (int r = 0; r < 8; r++) pair(q[r], i + r);
+                for (int r = 0; r < 8; r++) sl[r] = fastpair(q[r]);
+                if (__builtin_amdgcn_ballot_w64(sl[0] || sl[1] || sl[2] || sl[3] || sl[4] ||
+                                                sl[5] || sl[6] || sl[7])) {
+#pragma unroll
+                    for (int r = 0; r < 8; r++)
+                        if (sl[r]) slowpair(i + r);
+                }
+#pragma unroll
+                for (int r = 0; r < 8; r++) q[r] = qn[r];
             }
         }
         for (; i < un.i1; i++) {
-            BfPrep q{0.f, 0.f};
-            if (!EXACT) q = ap_row[i];
-            pair(q, i);
+            if (EXACT) {
+                if (active) slowpair(i);
+            } else if (fastpair(ap_row[i])) {
+                slowpair(i);
+            }
         }
     }
     if (!EXACT && nfb) atomicAdd(a.fallback, (unsigned long long)nfb);
