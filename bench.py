@@ -273,7 +273,7 @@ BF_PARAMS = dict(particle_alpha=211, particle_beta=211, BpT_min=0.2, BpT_max=3.0
                  Brap_max=2.0, Bnpts=21, rap_type=1)  # parameters.dat:122-128 with pi+
 BF_SPECIES = ((211, 0.13957, 0.25, 3.0), (-211, 0.13957, 0.25, 3.0),
               (321, 0.49368, 0.30, 1.0), (-321, 0.49368, 0.30, 1.0))
-BF_FAST_VALU_PER_PAIR = 15  # vector instructions per pair, k_bf fast form (gfx950 assembly)
+BF_FAST_VALU_PER_PAIR = 15.25  # vector instructions per pair, k_bf fast form (gfx950 assembly: 122 per 8 rows)
 MI355X_CUS, MI355X_GHZ = 256, 2.4
 
 

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_bf(BfArgs a) {
     const int nunits = a.nunits[k];
     const int lane = threadIdx.x & 63;
     const float top = (float)(a.g.Bnpts + 1);
-    const unsigned junk = (unsigned)(lane % S) * 4u;  // this lane's word of row 0
+    const unsigned junk = (unsigned)(lane % S);  // this lane's word of row 0
     unsigned nfb = 0;
 
     for (;;) {
@@ -184,9 +184,9 @@ __global__ __launch_bounds__(kThreads) void k_bf(BfArgs a) {
             const unsigned row = (unsigned)__builtin_amdgcn_fmed3f(ty, 0.f, top);
             const unsigned x = (unsigned)tp;
             const unsigned col = FOLD ? x : x % (unsigned)kBnphi;
-            unsigned off = __umul24(row, 4u * S) + (col << 2);
-            asm volatile("" : "+v"(off));  // keep the select below a select
-            atomicAdd((unsigned *)((char *)h + (active && ok ? off : junk)), 1u);
+            unsigned word = __umul24(row, (unsigned)S) + col;
+            asm volatile("" : "+v"(word));  // keep the select below a select
+            atomicAdd(&h[active && ok ? word : junk], 1u);
             return active && !ok;
         };
         // Exact form (the reference's FP64 arithmetic) of the pair with row i.
@@ -299,6 +299,18 @@ __global__ __launch_bounds__(kThreads) void k_bf_prep(BfPrepArgs a) {
     }
 }
 
+// One species list of a buffer: (phi, y or eta) of the particles with the
+// id (99999: all) that pass the pT cut, in file order, and event offsets.
+// The records live in the context's pinned staging area.
+struct HostList {
+    BfRec *rec = nullptr;
+    size_t n = 0;
+    std::vector<long long> off;
+    double max_phi = 0, max_y = 0;  // largest finite |phi|, |y|
+};
+
+bool bf_match(int id, int mon) { return id == 99999 || mon == id; }
+
 template <typename F>
 void par_ranges(long long n, long long min_per_thread, F fn) {
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -313,44 +325,13 @@ void par_ranges(long long n, long long min_per_thread, F fn) {
     for (auto &x : th) x.join();
 }
 
-// Allocator whose resize leaves new elements uninitialized (they are filled
-// in parallel afterwards).
-template <typename T>
-struct NoInit : std::allocator<T> {
-    template <typename U>
-    struct rebind {
-        using other = NoInit<U>;
-    };
-    NoInit() = default;
-    template <typename U>
-    NoInit(const NoInit<U> &) {}
-    template <typename U>
-    void construct(U *p) {
-        ::new ((void *)p) U;
-    }
-    template <typename U, typename... Args>
-    void construct(U *p, Args &&...args) {
-        ::new ((void *)p) U(std::forward<Args>(args)...);
-    }
-};
-
-// One species list of a buffer: (phi, y or eta) of the particles with the
-// id (99999: all) that pass the pT cut, in file order, and event offsets.
-struct HostList {
-    std::vector<BfRec, NoInit<BfRec>> rec;
-    std::vector<long long> off;
-    double max_phi = 0, max_y = 0;  // largest finite |phi|, |y|
-};
-
-bool bf_match(int id, int mon) { return id == 99999 || mon == id; }
-
-// The lists of ids[0..nl) over one buffer: counts, then values, each pass
-// over event ranges on up to 16 threads (particleSamples.cpp:447-465 with
-// rap_shift 0; the pT cut of BalanceFunction.cpp:127-129).
-void build_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
+// First pass over one buffer: event offsets and sizes of the lists of
+// ids[0..nl), over event ranges on up to 16 threads (the pT cut of
+// BalanceFunction.cpp:127-129).
+void count_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
                  const int32_t *monval, const double *const cols[5], const int *ids, int nl,
                  HostList *L) {
-    const double *mass = cols[0], *px = cols[1], *py = cols[2], *pz = cols[3];
+    const double *px = cols[1], *py = cols[2];
     std::vector<long long> cnt((size_t)nl * std::max(nev, 1), 0);
     par_ranges(nev, 32, [&](long long e0, long long e1) {
         for (long long e = e0; e < e1; e++)
@@ -367,9 +348,19 @@ void build_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
     for (int l = 0; l < nl; l++) {
         L[l].off.assign((size_t)nev + 1, 0);
         for (int e = 0; e < nev; e++) L[l].off[e + 1] = L[l].off[e] + cnt[(size_t)l * nev + e];
-        L[l].rec.resize((size_t)L[l].off[nev]);
+        L[l].n = (size_t)L[l].off[nev];
         L[l].max_phi = L[l].max_y = 0;
     }
+}
+
+// Second pass: the records, into L[l].rec (particleSamples.cpp:447-465 with
+// rap_shift 0), and the largest finite |phi| and |y| of each list.  (One
+// pass into per-thread vectors copied into place afterwards measured slower
+// on 16 host threads.)
+void fill_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
+                const int32_t *monval, const double *const cols[5], const int *ids, int nl,
+                HostList *L) {
+    const double *mass = cols[0], *px = cols[1], *py = cols[2], *pz = cols[3];
     std::mutex mu;
     par_ranges(nev, 32, [&](long long e0, long long e1) {
         double mphi[4] = {0, 0, 0, 0}, my[4] = {0, 0, 0, 0};
@@ -437,6 +428,8 @@ struct hbt_bf_ctx {
     double *d_mix_rot = nullptr;
     size_t mix_ev_cap = 0, mix_rot_cap = 0;
     unsigned long long *d_fallback = nullptr;
+    BfRec *h_pin = nullptr;  // pinned staging of the species lists
+    size_t pin_cap = 0;      // records
     int cu_count = 0;
     hbt_bf_info info{};
     std::string err;
@@ -550,6 +543,7 @@ int hbt_bf_destroy(hbt_bf_ctx *c) {
                     (void *)c->d_mix_ev, (void *)c->d_mix_rot, (void *)c->d_fallback,
                     (void *)c->d_hist})
         if (p) hipFree(p);
+    if (c->h_pin) hipHostFree(c->h_pin);
     for (hipEvent_t e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -606,12 +600,33 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     }
     const int nlists = nu + nmu;
     HostList L[6];
-    build_lists(k, nev, ev_off, monval, cols, uid, nu, L);
-    if (nmu) build_lists(k, mix_nev, mix_off, mix_monval, mix_cols, muid, nmu, L + nu);
+    count_lists(k, nev, ev_off, monval, cols, uid, nu, L);
+    if (nmu) count_lists(k, mix_nev, mix_off, mix_monval, mix_cols, muid, nmu, L + nu);
+    // the records go to pinned memory (reused across calls: the previous
+    // call's copies have finished), so their upload is a DMA at link speed
+    size_t nrec = 0;
+    for (int l = 0; l < nlists; l++) nrec += L[l].n;
+    if (nrec > c->pin_cap) {
+        if (c->h_pin) hipHostFree(c->h_pin);
+        c->h_pin = nullptr;
+        c->pin_cap = 0;
+        const size_t want = nrec + nrec / 2;
+        if (hipHostMalloc((void **)&c->h_pin, want * sizeof(BfRec), hipHostMallocDefault) !=
+            hipSuccess)
+            return bf_fail(c, HBT_ERR_NOMEM, "hipHostMalloc failed");
+        c->pin_cap = want;
+    }
+    size_t at = 0;
+    for (int l = 0; l < nlists; l++) {
+        L[l].rec = c->h_pin + at;
+        at += L[l].n;
+    }
+    fill_lists(k, nev, ev_off, monval, cols, uid, nu, L);
+    if (nmu) fill_lists(k, mix_nev, mix_off, mix_monval, mix_cols, muid, nmu, L + nu);
     // N_b, N_bbar: b and bbar particles passing the pT cut
     // (BalanceFunction.cpp:68-69, :199-209)
-    c->N_b += (int64_t)L[sslot[2]].rec.size();
-    c->N_bbar += (int64_t)L[sslot[3]].rec.size();
+    c->N_b += (int64_t)L[sslot[2]].n;
+    c->N_bbar += (int64_t)L[sslot[3]].n;
     // mixing draws of the four mixed histograms, in call order, per event a
     // partner then a rotation (BalanceFunction.cpp:165-169)
     std::vector<int> mix_ev((size_t)4 * std::max(nev, 1));
@@ -706,11 +721,11 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
 
     // device buffers: lists, offsets, draws, units, queues, prepared values
     for (int l = 0; l < nlists; l++) {
-        int rc = bf_ensure(c, (BfRec **)&c->d_rec[l], c->rec_cap[l], L[l].rec.size());
+        int rc = bf_ensure(c, (BfRec **)&c->d_rec[l], c->rec_cap[l], L[l].n);
         if (!rc) rc = bf_ensure(c, &c->d_off[l], c->off_cap[l], L[l].off.size());
         if (rc) return rc;
-        if (!L[l].rec.empty())
-            BFCHECK(c, hipMemcpyAsync(c->d_rec[l], L[l].rec.data(), L[l].rec.size() * sizeof(BfRec),
+        if (L[l].n)
+            BFCHECK(c, hipMemcpyAsync(c->d_rec[l], L[l].rec, L[l].n * sizeof(BfRec),
                                       hipMemcpyHostToDevice, c->stream));
         BFCHECK(c, hipMemcpyAsync(c->d_off[l], L[l].off.data(), L[l].off.size() * sizeof(long long),
                                   hipMemcpyHostToDevice, c->stream));
@@ -728,11 +743,11 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     if (!exact) {
         for (int hh = 0; hh < 8; hh++) {
             poff_A[hh] = nprep;
-            nprep += L[Aslot[hh]].rec.size();
+            nprep += L[Aslot[hh]].n;
         }
         for (int l = 0; l < nlists; l++) {
             poff_L[l] = nprep;
-            nprep += L[l].rec.size();
+            nprep += L[l].n;
         }
         if (!rc) rc = bf_ensure(c, &c->d_prep, c->prep_cap, nprep);
     }
@@ -764,7 +779,7 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
         }
         for (int l = 0; l < nlists; l++) {
             pa.Lrec[l] = (const BfRec *)c->d_rec[l];
-            pa.Ln[l] = (long long)L[l].rec.size();
+            pa.Ln[l] = (long long)L[l].n;
             pa.Lprep[l] = c->d_prep + poff_L[l];
         }
         pa.nlists = nlists;
