@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void k_bf(BfArgs a) {
             const unsigned x = (unsigned)tp;
             const unsigned col = FOLD ? x : x % (unsigned)kBnphi;
             unsigned word = __umul24(row, (unsigned)S) + col;
-            asm volatile("" : "+v"(word));  // keep the select below a select
+            asm("" : "+v"(word));  // keep the select below a select
             atomicAdd(&h[active && ok ? word : junk], 1u);
             return active && !ok;
         };
