@@ -183,3 +183,25 @@ def test_bf_fast_equals_exact_large():
     f = run_bf(params, buf, 37, _lib.HBT_BF_KERNEL_FAST, mix=mix)
     e = run_bf(params, buf, 37, _lib.HBT_BF_KERNEL_EXACT, mix=mix)
     assert np.array_equal(f["hist"], e["hist"])
+
+
+def test_bf_errors_and_empty_buffer():
+    """Refused configurations (hbt_bf_create) and calls, and an empty buffer."""
+    base = dict(particle_alpha=211, particle_beta=-211, BpT_min=0.2, BpT_max=3.0,
+                Brap_max=2.0, Bnpts=21, rap_type=1)
+    for bad in (dict(Bnpts=818), dict(Bnpts=1), dict(rap_type=2), dict(particle_alpha=9998),
+                dict(particle_beta=-9997)):
+        with pytest.raises(hbt.HbtError) as ei:
+            hbt.BalanceFunction(dict(base, **bad))
+        assert ei.value.code == -1, bad  # HBT_ERR_ARG
+    with pytest.raises(hbt.HbtError):
+        hbt.BalanceFunction(base, kernel=7)
+    bf = hbt.BalanceFunction(base)
+    empty = samples.generate(2601, 0, 100.0, species=PI_K)
+    bf.calculate_balance_function(empty, hbt.Random(5))
+    r = bf.results()
+    assert r["hist"].sum() == 0 and (r["N_b"], r["N_bbar"]) == (0, 0)
+    buf = samples.generate(2602, 3, 50.0, species=PI_K)
+    with pytest.raises(hbt.HbtError):  # a separate mixed buffer without events
+        bf.calculate_balance_function(buf, hbt.Random(5), mix=empty)
+    bf.close()
