@@ -142,6 +142,8 @@ SIGNATURES = [
                                     C.POINTER(_F64P), C.POINTER(_F64P),
                                     C.POINTER(C.c_int64)]),
     ("hbt_reader_close", C.c_int, [C.c_void_p]),
+    ("hbt_reader_set_species", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hbt_reader_bad_monval", C.c_int64, [C.c_void_p]),
     ("hbt_server_create", C.c_int, [C.POINTER(HbtConfig), C.POINTER(HbtServerConfig),
                                     C.POINTER(C.c_void_p)]),
     ("hbt_server_destroy", C.c_int, [C.c_void_p]),
@@ -166,6 +168,15 @@ SIGNATURES = [
                               C.POINTER(C.c_int64)]),
     ("hbt_bf_write_outputs", C.c_int, [C.POINTER(HbtBfConfig), _F64P, C.c_char_p]),
     ("hbt_bf_last_info", C.c_int, [C.c_void_p, C.POINTER(HbtBfInfo)]),
+    ("hbt_bf_set_species", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]),
+    ("hbt_species_read", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("hbt_species_count", C.c_int32, [C.c_void_p]),
+    ("hbt_species_get", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]),
+    ("hbt_species_free", C.c_int, [C.c_void_p]),
 ]
 
 _lib = None

@@ -31,12 +31,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -309,7 +311,58 @@ struct HostList {
     double max_phi = 0, max_y = 0;  // largest finite |phi|, |y|
 };
 
-bool bf_match(int id, int mon) { return id == 99999 || mon == id; }
+// Charge, baryon number and strangeness of a species.
+struct SpQ {
+    int32_t charge, baryon, strange;
+};
+
+bool is_class(int id) {
+    return id == 9999 || id == 9998 || id == -9998 || id == 9997 || id == -9997 || id == 9996 ||
+           id == -9996;
+}
+
+// particleSamples::decide_to_pick_OSCAR (particleSamples.cpp:625-678).
+bool bf_pick(int poi, int mon, const SpQ *q) {
+    switch (poi) {
+        case 99999: return true;
+        case 9999: return q->charge != 0;
+        case 9998: return q->charge > 0;
+        case -9998: return q->charge < 0;
+        case 9997: return q->baryon > 0;
+        case -9997: return q->baryon < 0;
+        case 9996: return q->strange > 0;
+        case -9996: return q->strange < 0;
+        default: return mon == poi;
+    }
+}
+
+// Which of the lists ids[0..nl) a particle belongs to (bit l).  With a
+// class among the ids every particle's quantum numbers are looked up, as the
+// reference looks them up; a monval missing from the table is recorded in
+// bad and belongs to no list.
+struct Picker {
+    int nl = 0;
+    int ids[4] = {0, 0, 0, 0};
+    const std::unordered_map<int32_t, SpQ> *sp = nullptr;  // set when a class is used
+    std::atomic<long long> *bad = nullptr;
+    unsigned mask(int mon) const {
+        const SpQ *q = nullptr;
+        if (sp) {
+            const auto it = sp->find(mon);
+            if (it == sp->end()) {
+                long long none = kNoBad;
+                bad->compare_exchange_strong(none, mon);
+                return 0u;
+            }
+            q = &it->second;
+        }
+        unsigned m = 0u;
+        for (int l = 0; l < nl; l++)
+            if (bf_pick(ids[l], mon, q)) m |= 1u << l;
+        return m;
+    }
+    static constexpr long long kNoBad = 1ll << 40;
+};
 
 template <typename F>
 void par_ranges(long long n, long long min_per_thread, F fn) {
@@ -329,20 +382,20 @@ void par_ranges(long long n, long long min_per_thread, F fn) {
 // ids[0..nl), over event ranges on up to 16 threads (the pT cut of
 // BalanceFunction.cpp:127-129).
 void count_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
-                 const int32_t *monval, const double *const cols[5], const int *ids, int nl,
+                 const int32_t *monval, const double *const cols[5], const Picker &pk,
                  HostList *L) {
+    const int nl = pk.nl;
     const double *px = cols[1], *py = cols[2];
     std::vector<long long> cnt((size_t)nl * std::max(nev, 1), 0);
     par_ranges(nev, 32, [&](long long e0, long long e1) {
         for (long long e = e0; e < e1; e++)
             for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
-                bool any = false;
-                for (int l = 0; l < nl; l++) any = any || bf_match(ids[l], monval[i]);
-                if (!any) continue;
+                const unsigned m = pk.mask(monval[i]);
+                if (!m) continue;
                 const double pT = sqrt(px[i] * px[i] + py[i] * py[i]);
                 if (pT < cfg.BpT_min || pT > cfg.BpT_max) continue;
                 for (int l = 0; l < nl; l++)
-                    if (bf_match(ids[l], monval[i])) cnt[(size_t)l * nev + e]++;
+                    if (m >> l & 1u) cnt[(size_t)l * nev + e]++;
             }
     });
     for (int l = 0; l < nl; l++) {
@@ -358,8 +411,9 @@ void count_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
 // pass into per-thread vectors copied into place afterwards measured slower
 // on 16 host threads.)
 void fill_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
-                const int32_t *monval, const double *const cols[5], const int *ids, int nl,
+                const int32_t *monval, const double *const cols[5], const Picker &pk,
                 HostList *L) {
+    const int nl = pk.nl;
     const double *mass = cols[0], *px = cols[1], *py = cols[2], *pz = cols[3];
     std::mutex mu;
     par_ranges(nev, 32, [&](long long e0, long long e1) {
@@ -368,16 +422,15 @@ void fill_lists(const hbt_bf_config &cfg, int32_t nev, const int64_t *ev_off,
             long long pos[4] = {0, 0, 0, 0};
             for (int l = 0; l < nl; l++) pos[l] = L[l].off[e];
             for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
-                bool any = false;
-                for (int l = 0; l < nl; l++) any = any || bf_match(ids[l], monval[i]);
-                if (!any) continue;
+                const unsigned m = pk.mask(monval[i]);
+                if (!m) continue;
                 const double pT = sqrt(px[i] * px[i] + py[i] * py[i]);
                 if (pT < cfg.BpT_min || pT > cfg.BpT_max) continue;
                 const double mT = sqrt(pT * pT + mass[i] * mass[i]);
                 const double y = cfg.rap_type == 0 ? asinh(pz[i] / pT) : asinh(pz[i] / mT);
                 const BfRec r{atan2(py[i], px[i]), y};
                 for (int l = 0; l < nl; l++)
-                    if (bf_match(ids[l], monval[i])) {
+                    if (m >> l & 1u) {
                         L[l].rec[pos[l]++] = r;
                         if (std::isfinite(r.phi)) mphi[l] = std::max(mphi[l], std::fabs(r.phi));
                         if (std::isfinite(r.y)) my[l] = std::max(my[l], std::fabs(r.y));
@@ -428,6 +481,8 @@ struct hbt_bf_ctx {
     double *d_mix_rot = nullptr;
     size_t mix_ev_cap = 0, mix_rot_cap = 0;
     unsigned long long *d_fallback = nullptr;
+    std::unordered_map<int32_t, SpQ> species;  // for the species classes
+    bool has_species = false;
     BfRec *h_pin = nullptr;  // pinned staging of the species lists
     size_t pin_cap = 0;      // records
     int cu_count = 0;
@@ -493,11 +548,6 @@ int hbt_bf_create(const hbt_bf_config *cfg, hbt_bf_ctx **out) {
         return bf_fail(nullptr, HBT_ERR_ARG, "rap_type must be 0 or 1");
     if (cfg->kernel != HBT_BF_KERNEL_FAST && cfg->kernel != HBT_BF_KERNEL_EXACT)
         return bf_fail(nullptr, HBT_ERR_ARG, "kernel must be HBT_BF_KERNEL_FAST or _EXACT");
-    for (int code : {cfg->particle_alpha, cfg->particle_beta})
-        if (code == 9999 || code == 9998 || code == -9998 || code == 9997 || code == -9997 ||
-            code == 9996 || code == -9996)
-            return bf_fail(nullptr, HBT_ERR_ARG,
-                           "species classes need the PDG tables; give one Monte-Carlo id");
     hbt_bf_ctx *c = new hbt_bf_ctx();
     c->cfg = *cfg;
     // BalanceFunction.cpp:29-35
@@ -599,9 +649,26 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
         mslot[1] = nu + mu_slot[1];
     }
     const int nlists = nu + nmu;
+    bool classes = false;
+    for (int id : sid) classes = classes || is_class(id);
+    if (classes && !c->has_species)
+        return bf_fail(c, HBT_ERR_STATE, "species classes need hbt_bf_set_species");
+    std::atomic<long long> bad{Picker::kNoBad};
+    Picker pk_main, pk_mix;
+    pk_main.nl = nu;
+    std::copy(uid, uid + nu, pk_main.ids);
+    pk_mix.nl = nmu;
+    std::copy(muid, muid + nmu, pk_mix.ids);
+    for (Picker *pk : {&pk_main, &pk_mix}) {
+        pk->sp = classes ? &c->species : nullptr;
+        pk->bad = &bad;
+    }
     HostList L[6];
-    count_lists(k, nev, ev_off, monval, cols, uid, nu, L);
-    if (nmu) count_lists(k, mix_nev, mix_off, mix_monval, mix_cols, muid, nmu, L + nu);
+    count_lists(k, nev, ev_off, monval, cols, pk_main, L);
+    if (nmu) count_lists(k, mix_nev, mix_off, mix_monval, mix_cols, pk_mix, L + nu);
+    if (bad.load() != Picker::kNoBad)
+        return bf_fail(c, HBT_ERR_ARG, "monval " + std::to_string(bad.load()) +
+                                           " is not in the species table");
     // the records go to pinned memory (reused across calls: the previous
     // call's copies have finished), so their upload is a DMA at link speed
     size_t nrec = 0;
@@ -621,8 +688,8 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
         L[l].rec = c->h_pin + at;
         at += L[l].n;
     }
-    fill_lists(k, nev, ev_off, monval, cols, uid, nu, L);
-    if (nmu) fill_lists(k, mix_nev, mix_off, mix_monval, mix_cols, muid, nmu, L + nu);
+    fill_lists(k, nev, ev_off, monval, cols, pk_main, L);
+    if (nmu) fill_lists(k, mix_nev, mix_off, mix_monval, mix_cols, pk_mix, L + nu);
     // N_b, N_bbar: b and bbar particles passing the pT cut
     // (BalanceFunction.cpp:68-69, :199-209)
     c->N_b += (int64_t)L[sslot[2]].n;
@@ -832,6 +899,15 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     c->info.kernel_ms = ms_pairs;
     c->info.fallback = exact ? 0 : fb;
     c->info.launches = (int32_t)nlaunch;
+    return HBT_OK;
+}
+
+int hbt_bf_set_species(hbt_bf_ctx *c, int32_t n, const int32_t *monval, const int32_t *charge,
+                       const int32_t *baryon, const int32_t *strange) {
+    if (!c || n < 0 || (n > 0 && (!monval || !charge || !baryon || !strange))) return HBT_ERR_ARG;
+    c->species.clear();
+    for (int32_t i = 0; i < n; i++) c->species[monval[i]] = SpQ{charge[i], baryon[i], strange[i]};
+    c->has_species = true;
     return HBT_OK;
 }
 

@@ -11,6 +11,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <memory>
 #include <iomanip>
 #include <sstream>
 #include <string>
@@ -171,6 +173,15 @@ double hbt_flow_event_plane_angle(int32_t n, const double *px, const double *py,
 
 // ---------------------------------------------- mode-10 and mode-9 readers
 
+// Charge, baryon number and strangeness per monval (the species table).
+struct hbt_species_table {
+    struct Q {
+        int32_t charge, baryon, strange;
+    };
+    std::vector<int32_t> order;  // monval in order of first insertion
+    std::map<int32_t, Q> q;
+};
+
 struct hbt_reader {
     int mode = 10;  // 10: text (gzip or plain), 9: binary records
     gzFile gz = nullptr;
@@ -188,9 +199,44 @@ struct hbt_reader {
     std::vector<double> cols[8];  // px py pz E x y z t
     std::vector<double> mass;
     long long total = 0;
+    // species classes (particle_monval 9999, +-9998, +-9997, +-9996)
+    bool has_species = false;
+    std::map<int32_t, hbt_species_table::Q> species;
+    long long bad_monval = 0;
+    bool bad = false;
 };
 
 namespace {
+
+bool is_species_class(int id) {
+    return id == 9999 || id == 9998 || id == -9998 || id == 9997 || id == -9997 || id == 9996 ||
+           id == -9996;
+}
+
+// particleSamples::decide_to_pick_OSCAR (particleSamples.cpp:625-678); a
+// class evaluated for a monval missing from the table is recorded (the
+// reference's lookup throws) and the particle is not kept.
+bool reader_pick(hbt_reader *rd, int mon) {
+    const int poi = rd->monval;
+    if (poi == 99999) return true;
+    if (!is_species_class(poi)) return mon == poi;
+    const auto it = rd->species.find(mon);
+    if (it == rd->species.end()) {
+        if (!rd->bad) rd->bad_monval = mon;
+        rd->bad = true;
+        return false;
+    }
+    const hbt_species_table::Q &q = it->second;
+    switch (poi) {
+        case 9999: return q.charge != 0;
+        case 9998: return q.charge > 0;
+        case -9998: return q.charge < 0;
+        case 9997: return q.baryon > 0;
+        case -9997: return q.baryon < 0;
+        case 9996: return q.strange > 0;
+        default: return q.strange < 0;  // -9996
+    }
+}
 
 // ---- number parsing, bit-identical to strtod / strtol
 
@@ -409,7 +455,7 @@ int32_t read_text_buffer(hbt_reader *rd) {
     size_t k = 0;
     for (int n : nper) {
         for (int ip = 0; ip < n; ip++, k++) {
-            if (rd->monval != 99999 && mon[k] != rd->monval) continue;
+            if (!reader_pick(rd, mon[k])) continue;
             const double *v = &val[k * 9];
             rd->cols[0].push_back(v[6]);
             rd->cols[1].push_back(v[7]);
@@ -455,7 +501,7 @@ int32_t read_binary_buffer(hbt_reader *rd) {
             float a[9];
             memcpy(&mon, r, 4);
             memcpy(a, r + 4, sizeof(a));
-            if (rd->monval != 99999 && mon != rd->monval) continue;
+            if (!reader_pick(rd, mon)) continue;
             rd->cols[0].push_back(a[6]);
             rd->cols[1].push_back(a[7]);
             rd->cols[2].push_back(a[8]);
@@ -484,12 +530,6 @@ int hbt_reader_open_mode(const char *filename, int32_t read_in_mode, int32_t eve
     if (!filename || !out) return HBT_ERR_ARG;
     *out = nullptr;
     if (read_in_mode != 9 && read_in_mode != 10) return HBT_ERR_ARG;
-    // species selections other than one Monte-Carlo id need the PDG tables
-    // (particleSamples.cpp:625-678), which this reader does not load
-    if (particle_monval == 9999 || particle_monval == 9998 || particle_monval == -9998 ||
-        particle_monval == 9997 || particle_monval == -9997 || particle_monval == 9996 ||
-        particle_monval == -9996)
-        return HBT_ERR_ARG;
     hbt_reader *rd = new hbt_reader();
     rd->mode = read_in_mode;
     rd->event_buffer_size = event_buffer_size;
@@ -515,8 +555,19 @@ int hbt_reader_open(const char *filename, int32_t event_buffer_size, int32_t par
 
 int32_t hbt_reader_next(hbt_reader *rd) {
     if (!rd) return HBT_ERR_ARG;
-    return rd->mode == 9 ? read_binary_buffer(rd) : read_text_buffer(rd);
+    if (is_species_class(rd->monval) && !rd->has_species) return HBT_ERR_STATE;
+    const int32_t n = rd->mode == 9 ? read_binary_buffer(rd) : read_text_buffer(rd);
+    return rd->bad ? HBT_ERR_ARG : n;
 }
+
+int hbt_reader_set_species(hbt_reader *rd, const hbt_species_table *t) {
+    if (!rd || !t) return HBT_ERR_ARG;
+    rd->species = t->q;
+    rd->has_species = true;
+    return HBT_OK;
+}
+
+int64_t hbt_reader_bad_monval(const hbt_reader *rd) { return rd && rd->bad ? rd->bad_monval : 0; }
 
 int32_t hbt_reader_eof(const hbt_reader *rd) { return rd && rd->eof ? 1 : 0; }
 
@@ -623,3 +674,78 @@ int hbt_reader_close(hbt_reader *rd) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Species table: the quantum numbers of a PDG table file in the reference's
+// format (EOS/pdg.dat, read by particle_decay::read_resonances_list,
+// particle_decay.cpp:23-129).  Per species: monval name mass width gspin
+// baryon strange charm bottom gisospin charge decays, then `decays` channel
+// records of 8 numbers.  As the reference does, every species with baryon
+// number 1 gets an anti-species entry -monval with baryon, strangeness and
+// charge negated, and a decay product of such a baryon that has no entry yet
+// gets one with zero quantum numbers (the reference's map lookup creates it;
+// a later entry of that species replaces it).
+
+
+extern "C" {
+
+int hbt_species_read(const char *path, hbt_species_table **out) {
+    if (!path || !out) return HBT_ERR_ARG;
+    *out = nullptr;
+    std::ifstream f(path);
+    if (!f) return HBT_ERR_ARG;
+    std::unique_ptr<hbt_species_table> t(new hbt_species_table());
+    auto put = [&](int32_t m, hbt_species_table::Q v) {
+        if (t->q.find(m) == t->q.end()) t->order.push_back(m);
+        t->q[m] = v;
+    };
+    long long monval;
+    while (f >> monval) {
+        std::string name;
+        double mass, width;
+        long long gspin, baryon, strange, charm, bottom, gisospin, charge, decays;
+        if (!(f >> name >> mass >> width >> gspin >> baryon >> strange >> charm >> bottom >>
+              gisospin >> charge >> decays) ||
+            decays < 0)
+            return HBT_ERR_ARG;
+        std::vector<long long> parts;
+        for (long long d = 0; d < decays; d++) {
+            long long dummy, npart, p[5];
+            double br;
+            if (!(f >> dummy >> npart >> br >> p[0] >> p[1] >> p[2] >> p[3] >> p[4]))
+                return HBT_ERR_ARG;
+            parts.insert(parts.end(), p, p + 5);
+        }
+        put((int32_t)monval, {(int32_t)charge, (int32_t)baryon, (int32_t)strange});
+        if (baryon == 1) {
+            for (long long d : parts)
+                if (d != 0 && t->q.find((int32_t)d) == t->q.end()) put((int32_t)d, {0, 0, 0});
+            put((int32_t)-monval, {(int32_t)-charge, (int32_t)-baryon, (int32_t)-strange});
+        }
+    }
+    if (!f.eof()) return HBT_ERR_ARG;  // a token that is not a number
+    *out = t.release();
+    return HBT_OK;
+}
+
+int32_t hbt_species_count(const hbt_species_table *t) { return t ? (int32_t)t->order.size() : 0; }
+
+int hbt_species_get(const hbt_species_table *t, int32_t i, int32_t *monval, int32_t *charge,
+                    int32_t *baryon, int32_t *strange) {
+    if (!t || i < 0 || i >= (int32_t)t->order.size()) return HBT_ERR_ARG;
+    const int32_t m = t->order[i];
+    const hbt_species_table::Q &v = t->q.at(m);
+    if (monval) *monval = m;
+    if (charge) *charge = v.charge;
+    if (baryon) *baryon = v.baryon;
+    if (strange) *strange = v.strange;
+    return HBT_OK;
+}
+
+int hbt_species_free(hbt_species_table *t) {
+    delete t;
+    return HBT_OK;
+}
+
+}  // extern "C"
+

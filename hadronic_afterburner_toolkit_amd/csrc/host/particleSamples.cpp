@@ -39,6 +39,24 @@ particleSamples::particleSamples(ParameterReader &paraRdr, std::string path,
                   << std::endl;
         exit(1);
     }
+    // species classes pick by charge, baryon number or strangeness from the
+    // PDG table the reference's particle_decay reads (particle_decay.cpp:
+    // 26-36): EOS/pdg.dat, or EOS/pdg_weak.dat with weak feed-down
+    const int m = particle_monval;
+    if (m == 9999 || m == 9998 || m == -9998 || m == 9997 || m == -9997 || m == 9996 ||
+        m == -9996) {
+        const std::string pdg = paraRdr_.getVal("resonance_weak_feed_down_flag", 0) == 1
+                                    ? "EOS/pdg_weak.dat"
+                                    : "EOS/pdg.dat";
+        hbt_species_table *t = nullptr;
+        if (hbt_species_read(pdg.c_str(), &t) != HBT_OK) {
+            std::cout << "can not find the file " << pdg << "!" << std::endl;
+            exit(1);
+        }
+        hbt_reader_set_species(reader, t);
+        if (reader_mixed) hbt_reader_set_species(reader_mixed, t);
+        hbt_species_free(t);
+    }
 }
 
 particleSamples::~particleSamples() {
@@ -48,6 +66,13 @@ particleSamples::~particleSamples() {
 
 void particleSamples::read(hbt_reader *rd, Events &ev) {
     const int nev = hbt_reader_next(rd);
+    if (nev < 0) {
+        // the reference's particle_decay lookup throws for a monval its
+        // table does not hold
+        std::cout << "particleSamples:: Error: particle " << hbt_reader_bad_monval(rd)
+                  << " is not in the PDG table" << std::endl;
+        exit(1);
+    }
     const int64_t *off = nullptr;
     const double *soa[8] = {};
     const double *mass = nullptr;

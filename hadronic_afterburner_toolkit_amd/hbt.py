@@ -76,9 +76,11 @@ def flow_event_plane_angle(px, py, n_order: int) -> float:
 
 
 def read_buffers(filename: str, event_buffer_size: int, particle_monval: int,
-                 read_in_mode: int = 10):
+                 read_in_mode: int = 10, species_table: str | None = None):
     """The mode-10 (or mode-9 binary) reader of the engine library, buffer by
-    buffer (the reference's read loop, Analysis.cpp:821-833)."""
+    buffer (the reference's read loop, Analysis.cpp:821-833).  A species
+    class as particle_monval needs species_table, the path of a PDG table
+    file (the reference reads EOS/pdg.dat)."""
     from .samples import Buffer
     lib = _lib.load()
     rd = C.c_void_p()
@@ -88,8 +90,21 @@ def read_buffers(filename: str, event_buffer_size: int, particle_monval: int,
         raise HbtError(rc, f"hbt_reader_open {filename}")
     out = []
     try:
+        if species_table is not None:
+            t = C.c_void_p()
+            rc = lib.hbt_species_read(species_table.encode(), C.byref(t))
+            if rc:
+                raise HbtError(rc, f"hbt_species_read {species_table}")
+            rc = lib.hbt_reader_set_species(rd, t)
+            lib.hbt_species_free(t)
+            if rc:
+                raise HbtError(rc, "hbt_reader_set_species")
         while not lib.hbt_reader_eof(rd):
             nev = lib.hbt_reader_next(rd)
+            if nev < 0:
+                bad = lib.hbt_reader_bad_monval(rd)
+                raise HbtError(nev, f"hbt_reader_next {filename}" +
+                               (f": monval {bad} is not in the species table" if bad else ""))
             off_p = C.POINTER(C.c_int64)()
             soa = (_lib._F64P * 8)()
             mass = _lib._F64P()
@@ -478,11 +493,39 @@ def bf_config_from_params(params: dict, device: int = 0,
                             int(params["rap_type"]), int(device), int(kernel))
 
 
+def read_species_table(path: str) -> dict:
+    """Charge, baryon number and strangeness per monval of a PDG table file in
+    the reference's format (EOS/pdg.dat, particle_decay.cpp:23-129), with the
+    anti-baryon entries the reference adds; int32 arrays keyed monval,
+    charge, baryon, strange."""
+    lib = _lib.load()
+    t = C.c_void_p()
+    rc = lib.hbt_species_read(path.encode(), C.byref(t))
+    if rc:
+        raise HbtError(rc, f"hbt_species_read {path}")
+    try:
+        n = lib.hbt_species_count(t)
+        out = {k: np.zeros(n, np.int32) for k in ("monval", "charge", "baryon", "strange")}
+        v = [C.c_int32() for _ in range(4)]
+        for i in range(n):
+            rc = lib.hbt_species_get(t, i, *[C.byref(x) for x in v])
+            if rc:
+                raise HbtError(rc, "hbt_species_get")
+            for key, x in zip(("monval", "charge", "baryon", "strange"), v):
+                out[key][i] = x.value
+    finally:
+        lib.hbt_species_free(t)
+    return out
+
+
 class BalanceFunction:
     """BalanceFunction (BalanceFunction.h) on the GPU: calculate per buffer,
-    histograms and counts, the reference's output files."""
+    histograms and counts, the reference's output files.  species: a table
+    from read_species_table, needed when particle_alpha or particle_beta is
+    a species class (9999, +-9998, +-9997, +-9996)."""
 
-    def __init__(self, params: dict, device: int = 0, kernel: int = _lib.HBT_BF_KERNEL_FAST):
+    def __init__(self, params: dict, device: int = 0, kernel: int = _lib.HBT_BF_KERNEL_FAST,
+                 species: dict | None = None):
         self._lib = _lib.load()
         self.params = {k: params[k] for k in BF_KEYS}
         self.cfg = bf_config_from_params(params, device, kernel)
@@ -491,6 +534,15 @@ class BalanceFunction:
         if rc:
             raise HbtError(rc, "hbt_bf_create: " + self._lib.hbt_bf_last_error(None).decode())
         self.h = h
+        if species is not None:
+            self.set_species(species)
+
+    def set_species(self, species: dict):
+        cols = [np.ascontiguousarray(species[k], dtype=np.int32)
+                for k in ("monval", "charge", "baryon", "strange")]
+        self._check(self._lib.hbt_bf_set_species(
+            self.h, len(cols[0]), *[c.ctypes.data_as(C.POINTER(C.c_int32)) for c in cols]),
+            "hbt_bf_set_species")
 
     def _check(self, rc, what):
         if rc:

@@ -335,7 +335,12 @@ double hbt_flow_event_plane_angle(int32_t n, const double *px, const double *py,
  * species) is below event_buffer_size and returns the number of events
  * (possibly 0 at end of file); hbt_reader_buffer exposes the kept
  * particles (SoA {px,py,pz,E,x,y,z,t}, event offsets) until the next call.
- * particle_monval selects one Monte-Carlo id (99999 keeps all).  Numbers
+ * particle_monval selects one Monte-Carlo id, 99999 (all species) or a
+ * species class (9999, +-9998, +-9997, +-9996, decide_to_pick_OSCAR,
+ * particleSamples.cpp:625-678), which needs hbt_reader_set_species; then a
+ * particle whose monval is not in the table makes hbt_reader_next return
+ * HBT_ERR_ARG (the reference's lookup throws), and hbt_reader_bad_monval
+ * names it.  Numbers
  * are read as strtod reads them (bit-identical values); lines are parsed on
  * several host threads. */
 typedef struct hbt_reader hbt_reader;
@@ -354,6 +359,12 @@ int32_t hbt_reader_eof(const hbt_reader *rd);
 int hbt_reader_buffer(const hbt_reader *rd, const int64_t **ev_off, const double *soa[8],
                       const double **mass, int64_t *total_read);
 int hbt_reader_close(hbt_reader *rd);
+/* The species table (hbt_species_read, below) for a species-class
+ * particle_monval: the particle_decay lookups of particleSamples::
+ * decide_to_pick_OSCAR; copied. */
+typedef struct hbt_species_table hbt_species_table;
+int hbt_reader_set_species(hbt_reader *rd, const hbt_species_table *t);
+int64_t hbt_reader_bad_monval(const hbt_reader *rd);
 
 /* Wall time (ms, HIP events on the context's stream) of the last
  * hbt_run_same / hbt_run_mixed kernel launches. */
@@ -368,9 +379,12 @@ int hbt_slot_info(hbt_ctx *ctx, int32_t slot, int32_t *nev, int64_t *n_kept);
  * (ids -a, -b), same-event (a,b), (abar,bbar), (a,bbar), (abar,b) and the
  * four mixed-event histograms, each [Bnpts][20] over (Delta y or Delta eta,
  * Delta phi), counts in doubles as the reference keeps them.  Particles
- * outside [BpT_min, BpT_max] are skipped.  Ids name one Monte-Carlo id;
- * the species classes 9999, +-9998, +-9997, +-9996 need the PDG tables and
- * are refused. */
+ * outside [BpT_min, BpT_max] are skipped.  An id names one Monte-Carlo id,
+ * 99999 (all species) or a species class of particleSamples::
+ * decide_to_pick_OSCAR (particleSamples.cpp:625-678): 9999 charged, 9998 /
+ * -9998 positive / negative, 9997 / -9997 baryons / anti-baryons, 9996 /
+ * -9996 positive / negative strangeness; the antiparticle list of an id is
+ * that of -id.  Classes need the species table (hbt_bf_set_species). */
 typedef struct {
     int32_t particle_alpha, particle_beta;
     double BpT_min, BpT_max, Brap_max;
@@ -423,6 +437,24 @@ int hbt_bf_read(hbt_bf_ctx *ctx, double *hist, int64_t *N_b, int64_t *N_bbar);
 /* Timing and counts of the last hbt_bf_calculate (no counterpart in the
  * reference, which has no timing). */
 int hbt_bf_last_info(hbt_bf_ctx *ctx, hbt_bf_info *info);
+/* The charge, baryon number and strangeness of n species, for the species
+ * classes; replaces the particle_decay lookups of decide_to_pick_OSCAR
+ * (particle_decay.cpp:181-196).  A particle whose monval is not in the table
+ * makes hbt_bf_calculate fail when a class is evaluated for it, where the
+ * reference's map lookup throws. */
+int hbt_bf_set_species(hbt_bf_ctx *ctx, int32_t n, const int32_t *monval, const int32_t *charge,
+                       const int32_t *baryon, const int32_t *strange);
+
+/* ---------------- species table (particle_decay.cpp:23-129) ----------------
+ * A PDG table file in the reference's format (EOS/pdg.dat): per species
+ * "monval name mass width gspin baryon strange charm bottom gisospin charge
+ * decays" and one 8-number record per decay channel; every baryon (baryon
+ * number 1) gets its anti-baryon entry, as the reference adds it. */
+int hbt_species_read(const char *path, hbt_species_table **out);
+int32_t hbt_species_count(const hbt_species_table *t);
+int hbt_species_get(const hbt_species_table *t, int32_t i, int32_t *monval, int32_t *charge,
+                    int32_t *baryon, int32_t *strange);
+int hbt_species_free(hbt_species_table *t);
 /* Replaces output_balance_function (BalanceFunction.cpp:211-318): writes
  * Balance_function_<a>_<b>_Delta_y.dat, _Delta_phi.dat and
  * Correlation_function_<a>_<b>_2D.dat into path. */

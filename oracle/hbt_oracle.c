@@ -745,9 +745,51 @@ typedef struct {
     int64_t *off; /* nev+1 */
 } bf_list;
 
-static int bf_pick(int poi, int monval) { return poi == 99999 || monval == poi; }
+/* particle_decay::get_particle_charge & co. (particle_decay.cpp:181-196):
+ * the map lookup; NULL when monval is not in the table (the reference's
+ * std::map::at throws). */
+static const int *sp_find(const orc_species *sp, int monval, int *idx) {
+    if (!sp) return NULL;
+    int lo = 0, hi = sp->n - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        if (sp->monval[mid] == monval) {
+            *idx = mid;
+            return &sp->monval[mid];
+        }
+        if (sp->monval[mid] < monval)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return NULL;
+}
 
-static void bf_list_build(bf_list *L, const orc_bf_buffer *b, int poi, int rap_type) {
+/* particleSamples::decide_to_pick_OSCAR (particleSamples.cpp:625-678). */
+static int bf_pick(orc_bf_state *s, int poi, int monval) {
+    if (poi == 99999) return 1;
+    const int cls = poi == 9999 || poi == 9998 || poi == -9998 || poi == 9997 || poi == -9997 ||
+                    poi == 9996 || poi == -9996;
+    if (!cls) return monval == poi;
+    int i = 0;
+    if (!sp_find(s->sp, monval, &i)) {
+        s->bad_monval_seen = 1;
+        return 0;
+    }
+    const int charge = s->sp->charge[i], baryon = s->sp->baryon[i], strange = s->sp->strange[i];
+    switch (poi) {
+        case 9999: return charge != 0;
+        case 9998: return charge > 0;
+        case -9998: return charge < 0;
+        case 9997: return baryon > 0;
+        case -9997: return baryon < 0;
+        case 9996: return strange > 0;
+        default: return strange < 0; /* -9996 */
+    }
+}
+
+static void bf_list_build(orc_bf_state *s, bf_list *L, const orc_bf_buffer *b, int poi,
+                          int rap_type) {
     const int64_t n = b->nev > 0 ? b->ev_off[b->nev] - b->ev_off[0] : 0;
     L->pT = (double *)malloc(sizeof(double) * (size_t)(n + 1));
     L->phi = (double *)malloc(sizeof(double) * (size_t)(n + 1));
@@ -757,7 +799,7 @@ static void bf_list_build(bf_list *L, const orc_bf_buffer *b, int poi, int rap_t
     L->off[0] = 0;
     for (int e = 0; e < b->nev; e++) {
         for (int64_t i = b->ev_off[e]; i < b->ev_off[e + 1]; i++) {
-            if (!bf_pick(poi, b->monval[i])) continue;
+            if (!bf_pick(s, poi, b->monval[i])) continue;
             const double px = b->px[i], py = b->py[i], pz = b->pz[i];
             const double pT = sqrt(px * px + py * py);
             const double mT = sqrt(pT * pT + b->mass[i] * b->mass[i]);
@@ -831,12 +873,12 @@ void orc_bf_calculate(orc_bf_state *s, orc_rng *r, const orc_bf_buffer *b,
                       const orc_bf_buffer *mix) {
     const int rt = s->p.rap_type;
     bf_list a, abar, bb, bbar, bm, bbarm;
-    bf_list_build(&a, b, s->p.alpha, rt);
-    bf_list_build(&abar, b, -s->p.alpha, rt);
-    bf_list_build(&bb, b, s->p.beta, rt);
-    bf_list_build(&bbar, b, -s->p.beta, rt);
-    bf_list_build(&bm, mix, s->p.beta, rt);
-    bf_list_build(&bbarm, mix, -s->p.beta, rt);
+    bf_list_build(s, &a, b, s->p.alpha, rt);
+    bf_list_build(s, &abar, b, -s->p.alpha, rt);
+    bf_list_build(s, &bb, b, s->p.beta, rt);
+    bf_list_build(s, &bbar, b, -s->p.beta, rt);
+    bf_list_build(s, &bm, mix, s->p.beta, rt);
+    bf_list_build(s, &bbarm, mix, -s->p.beta, rt);
     s->N_b += bf_count(s, &bb);
     s->N_bbar += bf_count(s, &bbar);
     const size_t H = (size_t)s->p.Bnpts * s->Bnphi;
@@ -930,4 +972,118 @@ int orc_bf_write_outputs(const orc_bf_state *s, const char *path) {
     free(os_y); free(ss_y); free(osm_y); free(ssm_y);
     free(os_p); free(ss_p); free(osm_p); free(ssm_p);
     return 0;
+}
+
+/* ---------------- species table (particle_decay.cpp:23-129) ---------------- */
+/* read_resonances_list: per species monval name mass width gspin baryon
+ * strange charm bottom gisospin charge decays, then decays channel records
+ * (dummy, Npart, branching ratio, five products).  A species with baryon
+ * number 1 adds its anti-species (-monval, quantum numbers negated); before
+ * that, the reference's table lookup of each decay product of that baryon
+ * creates a zero entry for a product not yet in the table (:96-103). */
+typedef struct {
+    int n, cap;
+    int *m, *q; /* q: 3 ints per entry */
+} sp_build;
+
+static int spb_find(const sp_build *b, int m) {
+    for (int i = 0; i < b->n; i++)
+        if (b->m[i] == m) return i;
+    return -1;
+}
+
+static void spb_put(sp_build *b, int m, int c, int ba, int st) {
+    int i = spb_find(b, m);
+    if (i < 0) {
+        if (b->n == b->cap) {
+            b->cap = b->cap ? 2 * b->cap : 256;
+            b->m = (int *)realloc(b->m, sizeof(int) * (size_t)b->cap);
+            b->q = (int *)realloc(b->q, 3 * sizeof(int) * (size_t)b->cap);
+        }
+        i = b->n++;
+        b->m[i] = m;
+    }
+    b->q[3 * i] = c;
+    b->q[3 * i + 1] = ba;
+    b->q[3 * i + 2] = st;
+}
+
+static int cmp_int_idx(const void *x, const void *y) {
+    const int *a = (const int *)x, *b = (const int *)y;
+    return (a[0] > b[0]) - (a[0] < b[0]);
+}
+
+int orc_species_read(const char *path, orc_species *sp) {
+    memset(sp, 0, sizeof(*sp));
+    FILE *f = fopen(path, "r");
+    if (!f) return 1;
+    sp_build b = {0, 0, NULL, NULL};
+    int monval, rc = 0;
+    while (fscanf(f, "%d", &monval) == 1) {
+        char name[256];
+        double mass, width, br;
+        int gspin, baryon, strange, charm, bottom, giso, charge, decays;
+        if (fscanf(f, "%255s %lf %lf %d %d %d %d %d %d %d %d", name, &mass, &width, &gspin,
+                   &baryon, &strange, &charm, &bottom, &giso, &charge, &decays) != 11 ||
+            decays < 0) {
+            rc = 2;
+            break;
+        }
+        int *parts = (int *)malloc(sizeof(int) * 5 * (size_t)(decays + 1));
+        for (int d = 0; d < decays && !rc; d++) {
+            int dummy, npart;
+            if (fscanf(f, "%d %d %lf %d %d %d %d %d", &dummy, &npart, &br, &parts[5 * d],
+                       &parts[5 * d + 1], &parts[5 * d + 2], &parts[5 * d + 3],
+                       &parts[5 * d + 4]) != 8)
+                rc = 2;
+        }
+        if (!rc) {
+            spb_put(&b, monval, charge, baryon, strange);
+            if (baryon == 1) {
+                for (int k = 0; k < 5 * decays; k++)
+                    if (parts[k] != 0 && spb_find(&b, parts[k]) < 0) spb_put(&b, parts[k], 0, 0, 0);
+                spb_put(&b, -monval, -charge, -baryon, -strange);
+            }
+        }
+        free(parts);
+        if (rc) break;
+    }
+    if (!rc && !feof(f)) rc = 2;
+    fclose(f);
+    if (rc) {
+        free(b.m);
+        free(b.q);
+        return rc;
+    }
+    /* sort (monval, index) pairs */
+    int *pairs = (int *)malloc(sizeof(int) * 2 * (size_t)(b.n + 1));
+    for (int i = 0; i < b.n; i++) {
+        pairs[2 * i] = b.m[i];
+        pairs[2 * i + 1] = i;
+    }
+    qsort(pairs, (size_t)b.n, 2 * sizeof(int), cmp_int_idx);
+    sp->n = b.n;
+    sp->monval = (int *)malloc(sizeof(int) * (size_t)(b.n + 1));
+    sp->charge = (int *)malloc(sizeof(int) * (size_t)(b.n + 1));
+    sp->baryon = (int *)malloc(sizeof(int) * (size_t)(b.n + 1));
+    sp->strange = (int *)malloc(sizeof(int) * (size_t)(b.n + 1));
+    for (int i = 0; i < b.n; i++) {
+        const int j = pairs[2 * i + 1];
+        sp->monval[i] = b.m[j];
+        sp->charge[i] = b.q[3 * j];
+        sp->baryon[i] = b.q[3 * j + 1];
+        sp->strange[i] = b.q[3 * j + 2];
+    }
+    free(pairs);
+    free(b.m);
+    free(b.q);
+    return 0;
+}
+
+void orc_species_free(orc_species *sp) {
+    free(sp->monval);
+    free(sp->charge);
+    free(sp->baryon);
+    free(sp->strange);
+    memset(sp, 0, sizeof(*sp));
 }

@@ -137,6 +137,16 @@ const orc_buffer *orc_reader_buffer(const orc_reader *rd);
 long orc_reader_total_particles(const orc_reader *rd);
 void orc_reader_close(orc_reader *rd);
 
+/* ---------------- species table (particle_decay.cpp:23-129) ---------------- */
+/* Charge, baryon number and strangeness per monval, sorted by monval. */
+typedef struct {
+    int n;
+    int *monval, *charge, *baryon, *strange;
+} orc_species;
+/* read_resonances_list: the PDG table file, anti-baryon entries added. */
+int orc_species_read(const char *path, orc_species *sp);
+void orc_species_free(orc_species *sp);
+
 /* ---------------- balance function (BalanceFunction.cpp) ---------------- */
 typedef struct {
     int alpha, beta;             /* particle_alpha, particle_beta */
@@ -151,6 +161,8 @@ typedef struct {
     double *hist;                /* [8][Bnpts][Bnphi]: C_ab, C_abarbbar, C_abbar,
                                     C_abarb, then the four mixed-event ones */
     long N_b, N_bbar;
+    const orc_species *sp;       /* for the species classes (may be NULL) */
+    int bad_monval_seen;         /* a class was evaluated for a monval not in sp */
 } orc_bf_state;
 
 /* One buffer of events with all species (monval, mass and momenta). */

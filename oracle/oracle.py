@@ -277,10 +277,32 @@ class BfParams(C.Structure):
                 ("rap_type", C.c_int)]
 
 
+class Species(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(n, C.POINTER(C.c_int))
+                                   for n in ("monval", "charge", "baryon", "strange")]
+
+
 class BfState(C.Structure):
     _fields_ = [("p", BfParams), ("Bnphi", C.c_int), ("dphi", C.c_double),
                 ("Bphi_min", C.c_double), ("drap", C.c_double), ("Brap_min", C.c_double),
-                ("hist", _D), ("N_b", C.c_long), ("N_bbar", C.c_long)]
+                ("hist", _D), ("N_b", C.c_long), ("N_bbar", C.c_long),
+                ("sp", C.POINTER(Species)), ("bad_monval_seen", C.c_int)]
+
+
+def species_read(path: str) -> dict:
+    """orc_species_read (particle_decay.cpp:23-129): int arrays monval,
+    charge, baryon, strange, sorted by monval."""
+    L = lib()
+    L.orc_species_read.argtypes = [C.c_char_p, C.POINTER(Species)]
+    L.orc_species_free.argtypes = [C.POINTER(Species)]
+    sp = Species()
+    rc = L.orc_species_read(path.encode(), C.byref(sp))
+    if rc:
+        raise ValueError(f"orc_species_read {path}: {rc}")
+    out = {k: np.ctypeslib.as_array(getattr(sp, k), shape=(sp.n,)).astype(np.int32)
+           for k in ("monval", "charge", "baryon", "strange")}
+    L.orc_species_free(C.byref(sp))
+    return out
 
 
 class BfBuffer(C.Structure):
@@ -309,7 +331,7 @@ BF_KEYS = ("particle_alpha", "particle_beta", "BpT_min", "BpT_max", "Brap_max", 
 class BfOracle:
     """CPU restatement of BalanceFunction (BalanceFunction.cpp)."""
 
-    def __init__(self, params: dict):
+    def __init__(self, params: dict, species: dict | None = None):
         L = lib()
         if not hasattr(L, "_bf_ready"):
             L.orc_bf_init.argtypes = [C.POINTER(BfState), C.POINTER(BfParams)]
@@ -325,6 +347,14 @@ class BfOracle:
         self.s = BfState()
         if L.orc_bf_init(C.byref(self.s), C.byref(p)):
             raise ValueError("orc_bf_init")
+        if species is not None:
+            # sorted by monval (the oracle's lookup is a binary search)
+            o = np.argsort(np.asarray(species["monval"]), kind="stable")
+            self._sp_arrays = [np.ascontiguousarray(np.asarray(species[k])[o], dtype=np.intc)
+                               for k in ("monval", "charge", "baryon", "strange")]
+            self._sp = Species(len(o), *[a.ctypes.data_as(C.POINTER(C.c_int))
+                                         for a in self._sp_arrays])
+            self.s.sp = C.pointer(self._sp)
 
     def __del__(self):
         try:
@@ -336,6 +366,8 @@ class BfOracle:
         b = CBfBuffer(buf)
         m = CBfBuffer(mix) if mix is not None else b
         self.L.orc_bf_calculate(C.byref(self.s), C.byref(rng.r), C.byref(b.c), C.byref(m.c))
+        if self.s.bad_monval_seen:
+            raise KeyError("a monval is not in the species table (the reference throws)")
 
     def write_outputs(self, path: str) -> int:
         return self.L.orc_bf_write_outputs(C.byref(self.s), path.encode())

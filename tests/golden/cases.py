@@ -94,7 +94,15 @@ def ref_arrays_as_u64(ref):
     return out
 
 
-BF_CASE_NAMES = ["bf_pipi", "bf_kpi_eta", "bf_pi_antipi"]
+BF_CASE_NAMES = ["bf_pipi", "bf_kpi_eta", "bf_pi_antipi", "bf_charged", "bf_baryon_strange",
+                 "bf_allcharged_p"]
+
+
+def bf_species_table(spec):
+    """Path of the species table a balance-function case ran the reference
+    with (as its EOS/pdg.dat), or None for identified species."""
+    name = spec.get("species_table")
+    return os.path.join(HERE, name) if name else None
 
 
 def bf_input(spec, digests):

@@ -51,6 +51,30 @@ CASES = {
         Bnpts=31, Brap_max=1.0, rap_type=1, event_buffer_size=100000)),
 }
 
+# Species classes need the species table: these cases run the reference with
+# tests/golden/species_table.dat as its EOS/pdg.dat (particle_decay.cpp:28).
+SPECIES_TABLE = os.path.join(HERE, "species_table.dat")
+MIXED = PIKPM + ((111, 0.13498, 0.25, 1.5), (311, 0.49761, 0.30, 0.5),
+                 (-311, 0.49761, 0.30, 0.5), (2212, 0.93827, 0.35, 0.8),
+                 (-2212, 0.93827, 0.35, 0.6), (2112, 0.93957, 0.35, 0.5),
+                 (3122, 1.11568, 0.35, 0.4), (-3122, 1.11568, 0.35, 0.3),
+                 (7777777, 1.5, 0.3, 0.2))
+CLASS_CASES = {
+    # the reference's parameters.dat choice: positive against negative hadrons
+    "bf_charged": dict(input=(2011, 8, 150.0), species=MIXED, params=dict(
+        randomSeed=53, particle_alpha=9998, particle_beta=-9998, BpT_min=0.2, BpT_max=3.0,
+        Bnpts=21, Brap_max=2.0, rap_type=1, event_buffer_size=100000)),
+    # baryons against negative strangeness, pseudo-rapidity, several buffers
+    "bf_baryon_strange": dict(input=(2012, 10, 120.0), species=MIXED, params=dict(
+        randomSeed=59, particle_alpha=9997, particle_beta=-9996, BpT_min=0.1, BpT_max=2.5,
+        Bnpts=16, Brap_max=1.6, rap_type=0, event_buffer_size=600)),
+    # all charged against protons: the antiparticle list of 9999 is that of
+    # -9999, an identified id no particle carries
+    "bf_allcharged_p": dict(input=(2013, 6, 180.0), species=MIXED, params=dict(
+        randomSeed=61, particle_alpha=9999, particle_beta=2212, BpT_min=0.2, BpT_max=3.0,
+        Bnpts=25, Brap_max=2.0, rap_type=1, event_buffer_size=100000)),
+}
+
 
 def read_dump(path):
     with open(path, "rb") as f:
@@ -70,11 +94,16 @@ def run_case(name, spec):
     work = tempfile.mkdtemp(prefix=f"golden_{name}_")
     res = os.path.join(work, "results")
     os.makedirs(res)
-    os.symlink(os.path.join(mg.REF, "EOS"), os.path.join(work, "EOS"))
+    species = spec.get("species", PIKPM)
+    if "species" in spec:
+        os.makedirs(os.path.join(work, "EOS"))
+        shutil.copy(SPECIES_TABLE, os.path.join(work, "EOS", "pdg.dat"))
+    else:
+        os.symlink(os.path.join(mg.REF, "EOS"), os.path.join(work, "EOS"))
     shutil.copy(os.path.join(mg.REF, "parameters.dat"), os.path.join(work, "parameters.dat"))
     seed, nev, mult = spec["input"]
     samples.write_mode10(os.path.join(res, "particle_samples.gz"),
-                         samples.generate(seed, nev, mult, species=PIKPM))
+                         samples.generate(seed, nev, mult, species=species))
     with gzip.open(os.path.join(res, "particle_samples.gz"), "rb") as f:
         digest = hashlib.sha256(f.read()).hexdigest()
     params = dict(mg.BASE)
@@ -99,8 +128,11 @@ def run_case(name, spec):
     with open(os.path.join(outdir, "outputs.json"), "w") as f:
         json.dump(outs, f, indent=1, sort_keys=True)
     with open(os.path.join(outdir, "params.json"), "w") as f:
-        json.dump(dict(input=list(spec["input"]), species=[list(s) for s in PIKPM],
-                       params=params), f, indent=1)
+        meta = dict(input=list(spec["input"]), species=[list(s) for s in species],
+                    params=params)
+        if "species" in spec:
+            meta["species_table"] = os.path.basename(SPECIES_TABLE)
+        json.dump(meta, f, indent=1)
     with open(os.path.join(outdir, "input.sha256"), "w") as f:
         json.dump({"particle_samples.gz": digest}, f, indent=1)
     shutil.rmtree(work)
@@ -108,5 +140,7 @@ def run_case(name, spec):
 
 
 if __name__ == "__main__":
-    for name, spec in CASES.items():
-        run_case(name, spec)
+    only = sys.argv[1:]
+    for name, spec in {**CASES, **CLASS_CASES}.items():
+        if not only or name in only:
+            run_case(name, spec)

@@ -20,7 +20,8 @@ def bf_params(spec):
 def run_oracle(name):
     spec, ref, outs, dig = cases.load(name)
     full = cases.bf_input(spec, dig)
-    o = oracle.BfOracle(bf_params(spec))
+    table = cases.bf_species_table(spec)
+    o = oracle.BfOracle(bf_params(spec), species=oracle.species_read(table) if table else None)
     rng = oracle.OracleRng(int(spec["params"]["randomSeed"]))
     bufs = samples.buffers_by_budget(full, int(spec["params"]["event_buffer_size"]))
     assert [n for _, n in bufs] == list(ref["nev_per_buffer"])

@@ -21,7 +21,8 @@ def test_bf_gpu_equals_reference(name, tmp_path):
     full = cases.bf_input(spec, dig)
     params = {k: spec["params"][k] for k in hbt.BF_KEYS}
     print(f"balance function case {name}", flush=True)
-    bf = hbt.BalanceFunction(params)
+    table = cases.bf_species_table(spec)
+    bf = hbt.BalanceFunction(params, species=hbt.read_species_table(table) if table else None)
     rng = hbt.Random(int(spec["params"]["randomSeed"]))
     for first, count in samples.buffers_by_budget(full, int(spec["params"]["event_buffer_size"])):
         bf.calculate_balance_function(full.events(first, count), rng)
@@ -107,7 +108,9 @@ def test_bf_exact_kernel_equals_reference(name):
     full = cases.bf_input(spec, dig)
     params = {k: spec["params"][k] for k in hbt.BF_KEYS}
     print(f"balance function case {name}, exact form", flush=True)
-    bf = hbt.BalanceFunction(params, kernel=_lib.HBT_BF_KERNEL_EXACT)
+    table = cases.bf_species_table(spec)
+    bf = hbt.BalanceFunction(params, kernel=_lib.HBT_BF_KERNEL_EXACT,
+                             species=hbt.read_species_table(table) if table else None)
     rng = hbt.Random(int(spec["params"]["randomSeed"]))
     for first, count in samples.buffers_by_budget(full, int(spec["params"]["event_buffer_size"])):
         bf.calculate_balance_function(full.events(first, count), rng)
@@ -189,8 +192,7 @@ def test_bf_errors_and_empty_buffer():
     """Refused configurations (hbt_bf_create) and calls, and an empty buffer."""
     base = dict(particle_alpha=211, particle_beta=-211, BpT_min=0.2, BpT_max=3.0,
                 Brap_max=2.0, Bnpts=21, rap_type=1)
-    for bad in (dict(Bnpts=818), dict(Bnpts=1), dict(rap_type=2), dict(particle_alpha=9998),
-                dict(particle_beta=-9997)):
+    for bad in (dict(Bnpts=818), dict(Bnpts=1), dict(rap_type=2)):
         with pytest.raises(hbt.HbtError) as ei:
             hbt.BalanceFunction(dict(base, **bad))
         assert ei.value.code == -1, bad  # HBT_ERR_ARG
@@ -205,3 +207,27 @@ def test_bf_errors_and_empty_buffer():
     with pytest.raises(hbt.HbtError):  # a separate mixed buffer without events
         bf.calculate_balance_function(buf, hbt.Random(5), mix=empty)
     bf.close()
+
+
+def test_bf_species_class_errors():
+    """A species class without a species table, and a particle whose monval
+    is not in the table (the reference's lookup throws), are errors."""
+    params = dict(particle_alpha=9998, particle_beta=-9998, BpT_min=0.2, BpT_max=3.0,
+                  Brap_max=2.0, Bnpts=21, rap_type=1)
+    buf = samples.generate(2701, 4, 60.0, species=PI_K)
+    bf = hbt.BalanceFunction(params)
+    with pytest.raises(hbt.HbtError):
+        bf.calculate_balance_function(buf, hbt.Random(3))
+    table = hbt.read_species_table(os.path.join(cases.HERE, "species_table.dat"))
+    keep = table["monval"] != 321
+    bf.set_species({k: v[keep] for k, v in table.items()})
+    with pytest.raises(hbt.HbtError) as ei:
+        bf.calculate_balance_function(buf, hbt.Random(3))
+    assert "321" in str(ei.value)
+    bf.set_species(table)
+    bf.calculate_balance_function(buf, hbt.Random(3))
+    r = bf.results()
+    bf.close()
+    o = oracle.BfOracle(params, species=table)
+    o.calculate_buffer(buf, oracle.OracleRng(3))
+    assert np.array_equal(r["hist"], o.arrays()["hist"])

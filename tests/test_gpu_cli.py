@@ -134,3 +134,62 @@ def test_cli_mode9_binary(tmp_path):
                 va = np.array([float(x) for x in la.split()])
                 vb = np.array([float(x) for x in lb.split()])
                 assert (np.isclose(va, vb, rtol=2e-8, atol=0) | (np.isnan(va) & np.isnan(vb))).all(), (fn, la, lb)
+
+
+def test_cli_species_class(tmp_path):
+    """particle_monval = 9998 (positive hadrons of several masses) through the
+    CLI, which loads EOS/pdg.dat for the class as the reference's
+    particle_decay does; against the oracle on the same particles, picked
+    by charge from the same table, with the same buffering (the budget counts
+    all species).  Bar as for FP64 above."""
+    species = ((211, 0.13957, 0.25, 3.0), (-211, 0.13957, 0.25, 3.0),
+               (111, 0.13498, 0.25, 1.0), (321, 0.49368, 0.3, 1.0), (-321, 0.49368, 0.3, 1.0),
+               (2212, 0.93827, 0.35, 0.6), (-2212, 0.93827, 0.35, 0.4))
+    full = samples.generate(1021, 12, 140.0, species=species)
+    work = tmp_path / "work"
+    res = work / "results"
+    res.mkdir(parents=True)
+    (work / "EOS").mkdir()
+    table = os.path.join(cases.HERE, "species_table.dat")
+    with open(table) as f:
+        (work / "EOS" / "pdg.dat").write_text(f.read())
+    samples.write_mode10(str(res / "particle_samples.gz"), full)
+    hp = cases.hbt_params({"params": dict(qnpts=21, q_min=-0.15, q_max=0.15, n_KT=4, KT_min=0.2,
+                                          KT_max=0.8, HBTrap_min=-1.0, HBTrap_max=1.0,
+                                          needed_number_of_pairs=1e18)})
+    keys = dict(hp, analyze_HBT=1, read_in_mode=10, particle_monval=9998, event_buffer_size=700,
+                read_in_real_mixed_events=0, randomSeed=37, hbt_precision=0)
+    write_parameters(str(work / "parameters.dat"), keys)
+    print("cli species class 9998", flush=True)
+    r = subprocess.run([CLI], cwd=str(work), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    sp = oracle.species_read(table)
+    charge = dict(zip(sp["monval"].tolist(), sp["charge"].tolist()))
+    keep_all = np.array([charge[int(m)] > 0 for m in full.monval])
+    o = oracle.Oracle(hp)
+    rng = oracle.OracleRng(37)
+    for first, count in samples.buffers_by_budget(full, 700):
+        b = full.events(first, count)
+        keep = keep_all[int(full.ev_off[first]):int(full.ev_off[first + count])]
+        cnt = [int(keep[int(b.ev_off[e]):int(b.ev_off[e + 1])].sum()) for e in range(b.nev)]
+        sel = np.flatnonzero(keep)
+        kept = samples.Buffer(ev_off=np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64),
+                              px=b.px[sel], py=b.py[sel], pz=b.pz[sel], E=b.E[sel], x=b.x[sel],
+                              y=b.y[sel], z=b.z[sel], t=b.t[sel], monval=b.monval[sel],
+                              mass=b.mass[sel])
+        o.calculate_buffer(kept, rng)
+    od = tmp_path / "oracle"
+    od.mkdir()
+    o.write_outputs(str(od), 0)
+    got = sorted(f for f in os.listdir(res) if f.endswith(".dat"))
+    assert got == sorted(os.listdir(od)) and got
+    for fn in got:
+        a = (res / fn).read_text().splitlines()
+        b = (od / fn).read_text().splitlines()
+        assert len(a) == len(b)
+        for la, lb in zip(a, b):
+            if la != lb:
+                va = np.array([float(x) for x in la.split()])
+                vb = np.array([float(x) for x in lb.split()])
+                assert (np.isclose(va, vb, rtol=2e-8, atol=0)
+                        | (np.isnan(va) & np.isnan(vb))).all(), (fn, la, lb)

@@ -86,9 +86,62 @@ def test_reader_unit_test_counts():
     assert list(np.diff(b.ev_off)) == [619, 635] and total == 1524
 
 
-def test_reader_rejects_unsupported_species_codes(tmp_path):
+def test_reader_species_class_needs_table(tmp_path):
     with pytest.raises(hbt.HbtError):
         hbt.read_buffers(cases.FIXTURE_GZ, 100000, 9999)
+
+
+SPECIES_MIX = ((211, 0.13957, 0.25, 3.0), (-211, 0.13957, 0.25, 3.0), (111, 0.13498, 0.25, 1.0),
+               (321, 0.49368, 0.3, 1.0), (-321, 0.49368, 0.3, 1.0), (2212, 0.93827, 0.35, 0.6),
+               (-2212, 0.93827, 0.35, 0.4), (3122, 1.11568, 0.35, 0.3),
+               (-3122, 1.11568, 0.35, 0.2), (7777777, 1.5, 0.3, 0.1))
+
+
+@pytest.mark.parametrize("poi", [9999, 9998, -9998, 9997, -9997, 9996, -9996])
+def test_reader_species_classes(poi, tmp_path):
+    """decide_to_pick_OSCAR (particleSamples.cpp:625-678) with the species
+    table: the kept particles are those of the class, in file order, with the
+    reference's buffering (the budget counts all species)."""
+    from hadronic_afterburner_toolkit_amd import samples
+    table = os.path.join(cases.HERE, "species_table.dat")
+    sp = hbt.read_species_table(table)
+    q = {int(m): (int(c), int(b), int(st))
+         for m, c, b, st in zip(sp["monval"], sp["charge"], sp["baryon"], sp["strange"])}
+    pick = {9999: lambda c, b, st: c != 0, 9998: lambda c, b, st: c > 0,
+            -9998: lambda c, b, st: c < 0, 9997: lambda c, b, st: b > 0,
+            -9997: lambda c, b, st: b < 0, 9996: lambda c, b, st: st > 0,
+            -9996: lambda c, b, st: st < 0}[poi]
+    full = samples.generate(2801, 9, 90.0, species=SPECIES_MIX)
+    path = str(tmp_path / "particle_samples.gz")
+    samples.write_mode10(path, full)
+    got = hbt.read_buffers(path, 400, poi, species_table=table)
+    keep = np.array([pick(*q[int(m)]) for m in full.monval])
+    e0 = 0
+    for buf, total in got:
+        nev = buf.nev
+        if nev == 0:
+            continue
+        a, b = int(full.ev_off[e0]), int(full.ev_off[e0 + nev])
+        assert total == b - a
+        sel = np.flatnonzero(keep[a:b]) + a
+        assert np.array_equal(buf.px, full.px[sel]) and np.array_equal(buf.E, full.E[sel])
+        cnt = [int(keep[int(full.ev_off[e]):int(full.ev_off[e + 1])].sum())
+               for e in range(e0, e0 + nev)]
+        assert list(np.diff(buf.ev_off)) == cnt
+        e0 += nev
+    assert e0 == full.nev and keep.any() and not keep.all()
+
+
+def test_reader_species_missing_from_table(tmp_path):
+    """A monval the table does not hold fails the read, as the reference's
+    lookup throws; the reader names it."""
+    from hadronic_afterburner_toolkit_amd import samples
+    full = samples.generate(2802, 3, 40.0, species=SPECIES_MIX[:2] + ((4444, 1.0, 0.3, 1.0),))
+    path = str(tmp_path / "particle_samples.gz")
+    samples.write_mode10(path, full)
+    with pytest.raises(hbt.HbtError, match="4444"):
+        hbt.read_buffers(path, 10 ** 6, 9998,
+                         species_table=os.path.join(cases.HERE, "species_table.dat"))
 
 
 def test_psi_ref_golden():
