@@ -144,6 +144,7 @@ SIGNATURES = [
     ("hbt_reader_close", C.c_int, [C.c_void_p]),
     ("hbt_reader_set_species", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hbt_reader_bad_monval", C.c_int64, [C.c_void_p]),
+    ("hbt_reader_monval", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_int32))]),
     ("hbt_server_create", C.c_int, [C.POINTER(HbtConfig), C.POINTER(HbtServerConfig),
                                     C.POINTER(C.c_void_p)]),
     ("hbt_server_destroy", C.c_int, [C.c_void_p]),
@@ -164,6 +165,10 @@ SIGNATURES = [
                                    C.POINTER(C.c_int32), C.POINTER(_F64P), C.c_int32,
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                    C.POINTER(_F64P), C.c_void_p]),
+    ("hbt_bf_calculate_planned", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_int32), C.POINTER(_F64P), C.c_int32,
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                           C.POINTER(_F64P), C.POINTER(C.c_int32), _F64P]),
     ("hbt_bf_read", C.c_int, [C.c_void_p, _F64P, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)]),
     ("hbt_bf_write_outputs", C.c_int, [C.POINTER(HbtBfConfig), _F64P, C.c_char_p]),

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contract=
           "-Wno-unused-result", "-Wno-unused-value", "-Wno-bitwise-instead-of-logical"]
 
 HOST_SRCS = ["host/ParameterReader.cpp", "host/Random.cpp", "host/particleSamples.cpp",
-             "host/HBT_correlation.cpp"]  # the drop-in classes of the CLI
+             "host/HBT_correlation.cpp", "host/BalanceFunction.cpp"]  # the drop-in classes of the CLI
 
 
 def _run(cmd):

@@ -614,10 +614,17 @@ int hbt_bf_reset(hbt_bf_ctx *c) {
     return HBT_OK;
 }
 
-int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const int32_t *monval,
-                     const double *const cols[5], int32_t mix_nev, const int64_t *mix_off,
-                     const int32_t *mix_monval, const double *const mix_cols[5], hbt_rng *rng) {
-    if (!c || !rng || nev < 0) return HBT_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// One buffer; the mixing draws come from rng or, when rng is null, from
+// plan_ev / plan_rot ([4][nev], validated here).
+int bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const int32_t *monval,
+                 const double *const cols[5], int32_t mix_nev, const int64_t *mix_off,
+                 const int32_t *mix_monval, const double *const mix_cols[5], hbt_rng *rng,
+                 const int32_t *plan_ev, const double *plan_rot) {
+    if (!c || nev < 0) return HBT_ERR_ARG;
     if (nev > 0 && (!ev_off || !monval || !cols)) return bf_fail(c, HBT_ERR_ARG, "null buffer");
     const auto t0 = std::chrono::steady_clock::now();
     const bool alias = mix_off == nullptr;
@@ -630,6 +637,12 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     if (nev > 0 && mix_nev < 1) return bf_fail(c, HBT_ERR_ARG, "no mixed events");
     if (nev > 0 && !alias && (!mix_monval || !mix_cols))
         return bf_fail(c, HBT_ERR_ARG, "null mixed buffer");
+    if (!rng) {
+        if (nev > 0 && (!plan_ev || !plan_rot)) return bf_fail(c, HBT_ERR_ARG, "null plan");
+        for (size_t i = 0; i < (size_t)4 * nev; i++)
+            if (plan_ev[i] < 0 || plan_ev[i] >= mix_nev)
+                return bf_fail(c, HBT_ERR_ARG, "mixed partner outside the mixed buffer");
+    }
     BFCHECK(c, hipSetDevice(c->cfg.device));
     const hbt_bf_config &k = c->cfg;
     c->info = hbt_bf_info{};
@@ -698,12 +711,18 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     // partner then a rotation (BalanceFunction.cpp:165-169)
     std::vector<int> mix_ev((size_t)4 * std::max(nev, 1));
     std::vector<double> mix_rot((size_t)4 * std::max(nev, 1));
-    double max_rot = 0;
+    bool rot_in_range = true;  // the fast form's bounds assume rot in [0, 2 pi]
     for (int hh = 0; hh < 4; hh++)
         for (int e = 0; e < nev; e++) {
-            mix_ev[(size_t)hh * nev + e] = hbt_rng_int(rng) % mix_nev;
-            mix_rot[(size_t)hh * nev + e] = hbt_rng_uniform(rng) * 2. * M_PI;
-            max_rot = std::max(max_rot, mix_rot[(size_t)hh * nev + e]);
+            const size_t i = (size_t)hh * nev + e;
+            if (rng) {
+                mix_ev[i] = hbt_rng_int(rng) % mix_nev;
+                mix_rot[i] = hbt_rng_uniform(rng) * 2. * M_PI;
+            } else {
+                mix_ev[i] = plan_ev[i];
+                mix_rot[i] = plan_rot[i];
+            }
+            if (!(mix_rot[i] >= 0.0 && mix_rot[i] <= 2. * M_PI)) rot_in_range = false;
         }
     if (nev == 0) return HBT_OK;
 
@@ -731,7 +750,7 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     const double Gp = std::ldexp(maxU + maxV, -21) + 1e-6;
     const double Gy = std::ldexp(maxA + maxB, -21) + 1e-6 + 4e-10 * inv_drap;
     const bool exact = k.kernel == HBT_BF_KERNEL_EXACT || mpa > M_PI || mpb > M_PI ||
-                       max_rot > 2. * M_PI || Gp > 0.25 || Gy > 0.25;
+                       !rot_in_range || Gp > 0.25 || Gy > 0.25;
     const bool odd = (k.Bnpts & 1) != 0;
     const bool fold = !exact && lds_bytes(k.Bnpts, true) <= kMaxLds * sizeof(unsigned);
     c->info.kernel = exact ? HBT_BF_KERNEL_EXACT : HBT_BF_KERNEL_FAST;
@@ -900,6 +919,27 @@ int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const in
     c->info.fallback = exact ? 0 : fb;
     c->info.launches = (int32_t)nlaunch;
     return HBT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbt_bf_calculate(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off, const int32_t *monval,
+                     const double *const cols[5], int32_t mix_nev, const int64_t *mix_off,
+                     const int32_t *mix_monval, const double *const mix_cols[5], hbt_rng *rng) {
+    if (!rng) return HBT_ERR_ARG;
+    return bf_calculate(c, nev, ev_off, monval, cols, mix_nev, mix_off, mix_monval, mix_cols, rng,
+                        nullptr, nullptr);
+}
+
+int hbt_bf_calculate_planned(hbt_bf_ctx *c, int32_t nev, const int64_t *ev_off,
+                             const int32_t *monval, const double *const cols[5],
+                             int32_t mix_nev, const int64_t *mix_off,
+                             const int32_t *mix_monval, const double *const mix_cols[5],
+                             const int32_t *mix_ev, const double *mix_rot) {
+    return bf_calculate(c, nev, ev_off, monval, cols, mix_nev, mix_off, mix_monval, mix_cols,
+                        nullptr, mix_ev, mix_rot);
 }
 
 int hbt_bf_set_species(hbt_bf_ctx *c, int32_t n, const int32_t *monval, const int32_t *charge,

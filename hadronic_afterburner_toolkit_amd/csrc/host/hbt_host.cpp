@@ -198,6 +198,7 @@ struct hbt_reader {
     std::vector<int64_t> off;
     std::vector<double> cols[8];  // px py pz E x y z t
     std::vector<double> mass;
+    std::vector<int32_t> monv;  // Monte-Carlo id of each kept particle
     long long total = 0;
     // species classes (particle_monval 9999, +-9998, +-9997, +-9996)
     bool has_species = false;
@@ -452,6 +453,7 @@ int32_t read_text_buffer(hbt_reader *rd) {
     rd->off.assign(1, 0);
     for (auto &c : rd->cols) c.clear();
     rd->mass.clear();
+    rd->monv.clear();
     size_t k = 0;
     for (int n : nper) {
         for (int ip = 0; ip < n; ip++, k++) {
@@ -466,6 +468,7 @@ int32_t read_text_buffer(hbt_reader *rd) {
             rd->cols[6].push_back(v[4]);
             rd->cols[7].push_back(v[1]);
             rd->mass.push_back(v[0]);
+            rd->monv.push_back(mon[k]);
         }
         rd->off.push_back((int64_t)rd->cols[0].size());
     }
@@ -482,6 +485,7 @@ int32_t read_binary_buffer(hbt_reader *rd) {
     rd->off.assign(1, 0);
     for (auto &c : rd->cols) c.clear();
     rd->mass.clear();
+    rd->monv.clear();
     long long num_particles = 0;
     int nev = 0;
     std::vector<char> rec;
@@ -511,6 +515,7 @@ int32_t read_binary_buffer(hbt_reader *rd) {
             rd->cols[6].push_back(a[4]);
             rd->cols[7].push_back(a[1]);
             rd->mass.push_back(a[0]);
+            rd->monv.push_back(mon);
         }
         num_particles += std::max(0, n_particle);
         nev++;
@@ -568,6 +573,12 @@ int hbt_reader_set_species(hbt_reader *rd, const hbt_species_table *t) {
 }
 
 int64_t hbt_reader_bad_monval(const hbt_reader *rd) { return rd && rd->bad ? rd->bad_monval : 0; }
+
+int hbt_reader_monval(const hbt_reader *rd, const int32_t **monval) {
+    if (!rd || !monval) return HBT_ERR_ARG;
+    *monval = rd->monv.data();
+    return HBT_OK;
+}
 
 int32_t hbt_reader_eof(const hbt_reader *rd) { return rd && rd->eof ? 1 : 0; }
 

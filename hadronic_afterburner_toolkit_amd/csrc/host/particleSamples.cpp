@@ -81,6 +81,9 @@ void particleSamples::read(hbt_reader *rd, Events &ev) {
     const int64_t n = off[nev];
     for (int a = 0; a < 8; a++) ev.cols[a].assign(soa[a], soa[a] + n);
     ev.mass.assign(mass, mass + n);
+    const int32_t *mon = nullptr;
+    hbt_reader_monval(rd, &mon);
+    ev.monval.assign(mon, mon + n);
 }
 
 void particleSamples::read_in_particle_samples_and_filter() { read(reader, same_); }

@@ -2,7 +2,8 @@
 // HBT path uses (src/particleSamples.h): read_in_mode 10 (gzip text,
 // particleSamples.cpp:1247-1286) and 9 (binary records, :1203-1245) in
 // buffers of whole events, species filter
-// by one Monte-Carlo id, optional separate mixed-event file
+// (a Monte-Carlo id, 99999 for all, or a species class), optional separate
+// mixed-event file
 // (read_in_real_mixed_events), the same getters.  Particles are kept as
 // structure-of-arrays so that HBT_correlation can hand them to the GPU
 // engine without repacking.
@@ -51,6 +52,7 @@ class particleSamples {
         std::vector<int64_t> off{0};
         std::vector<double> cols[8];
         std::vector<double> mass;
+        std::vector<int32_t> monval;
         particle_info get(int ev, int i, int monval) const;
     };
     const Events &same_events() const { return same_; }

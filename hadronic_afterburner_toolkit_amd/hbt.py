@@ -114,7 +114,11 @@ def read_buffers(filename: str, event_buffer_size: int, particle_monval: int,
             n = int(off[-1])
             cols = [np.ctypeslib.as_array(soa[a], shape=(n,)).copy() if n else np.zeros(0)
                     for a in range(8)]
-            out.append((Buffer(off, *cols), int(total.value)))
+            mon_p = C.POINTER(C.c_int32)()
+            lib.hbt_reader_monval(rd, C.byref(mon_p))
+            mon = np.ctypeslib.as_array(mon_p, shape=(n,)).copy() if n else np.zeros(0, np.int32)
+            m = np.ctypeslib.as_array(mass, shape=(n,)).copy() if n else np.zeros(0)
+            out.append((Buffer(off, *cols, monval=mon, mass=m), int(total.value)))
     finally:
         lib.hbt_reader_close(rd)
     return out

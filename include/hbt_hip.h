@@ -365,6 +365,9 @@ int hbt_reader_close(hbt_reader *rd);
 typedef struct hbt_species_table hbt_species_table;
 int hbt_reader_set_species(hbt_reader *rd, const hbt_species_table *t);
 int64_t hbt_reader_bad_monval(const hbt_reader *rd);
+/* The Monte-Carlo id of each particle of the last buffer (particle_info::
+ * monval, particleSamples.cpp:1275), valid until the next call. */
+int hbt_reader_monval(const hbt_reader *rd, const int32_t **monval);
 
 /* Wall time (ms, HIP events on the context's stream) of the last
  * hbt_run_same / hbt_run_mixed kernel launches. */
@@ -431,6 +434,17 @@ int hbt_bf_calculate(hbt_bf_ctx *ctx, int32_t nev, const int64_t *ev_off,
                      const int32_t *monval, const double *const cols[5], int32_t mix_nev,
                      const int64_t *mix_off, const int32_t *mix_monval,
                      const double *const mix_cols[5], hbt_rng *rng);
+/* The same with the mixing draws made by the caller from its own stream (the
+ * reference's shared RandomUtil::Random): mix_ev[h * nev + e] and
+ * mix_rot[h * nev + e] for mixed histogram h = 0..3 (C_ab, C_abarbbar,
+ * C_abbar, C_abarb) and event e, drawn in that order as
+ * rand_int_uniform() % mix_nev and rand_uniform() * 2. * M_PI
+ * (BalanceFunction.cpp:165-169). */
+int hbt_bf_calculate_planned(hbt_bf_ctx *ctx, int32_t nev, const int64_t *ev_off,
+                             const int32_t *monval, const double *const cols[5],
+                             int32_t mix_nev, const int64_t *mix_off,
+                             const int32_t *mix_monval, const double *const mix_cols[5],
+                             const int32_t *mix_ev, const double *mix_rot);
 /* hist: 8 * Bnpts * 20 doubles (C_ab, C_abarbbar, C_abbar, C_abarb, then the
  * mixed four), N_b / N_bbar: b / bbar particles inside the pT cut. */
 int hbt_bf_read(hbt_bf_ctx *ctx, double *hist, int64_t *N_b, int64_t *N_bbar);

@@ -193,3 +193,42 @@ def test_cli_species_class(tmp_path):
                 vb = np.array([float(x) for x in lb.split()])
                 assert (np.isclose(va, vb, rtol=2e-8, atol=0)
                         | (np.isnan(va) & np.isnan(vb))).all(), (fn, la, lb)
+
+
+@pytest.mark.parametrize("name", cases.BF_CASE_NAMES)
+def test_cli_balance_function_files_equal_reference(name, tmp_path):
+    """analyze_balance_function = 1 through the CLI (drop-in BalanceFunction
+    and particleSamples over the C ABI, mixing draws from the shared random
+    stream): the three files byte-identical to the reference's on every
+    balance-function golden case (species classes read EOS/pdg.dat)."""
+    import hashlib
+    import io
+    import gzip
+    spec, ref, outs, dig = cases.load(name)
+    full = cases.bf_input(spec, dig)
+    work = tmp_path / "work"
+    res = work / "results"
+    res.mkdir(parents=True)
+    text = io.StringIO()
+    for e in range(full.nev):
+        a, b = int(full.ev_off[e]), int(full.ev_off[e + 1])
+        text.write(f"{b - a}\n")
+        for i in range(a, b):
+            text.write("%d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n" % (
+                full.monval[i], full.mass[i], full.t[i], full.x[i], full.y[i], full.z[i],
+                full.E[i], full.px[i], full.py[i], full.pz[i]))
+    with gzip.open(res / "particle_samples.gz", "wb") as f:
+        f.write(text.getvalue().encode())
+    table = cases.bf_species_table(spec)
+    if table:
+        (work / "EOS").mkdir()
+        with open(table) as f:
+            (work / "EOS" / "pdg.dat").write_text(f.read())
+    keys = dict(spec["params"], analyze_HBT=0, analyze_balance_function=1)
+    write_parameters(str(work / "parameters.dat"), keys)
+    print(f"cli balance function {name}", flush=True)
+    r = subprocess.run([CLI], cwd=str(work), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for fn, meta in outs.items():
+        with open(res / fn, "rb") as f:
+            assert hashlib.sha256(f.read()).hexdigest() == meta["sha256"], fn

@@ -125,6 +125,8 @@ def test_reader_species_classes(poi, tmp_path):
         assert total == b - a
         sel = np.flatnonzero(keep[a:b]) + a
         assert np.array_equal(buf.px, full.px[sel]) and np.array_equal(buf.E, full.E[sel])
+        assert np.array_equal(buf.monval, full.monval[sel])
+        assert np.array_equal(buf.mass, full.mass[sel])
         cnt = [int(keep[int(full.ev_off[e]):int(full.ev_off[e + 1])].sum())
                for e in range(e0, e0 + nev)]
         assert list(np.diff(buf.ev_off)) == cnt
