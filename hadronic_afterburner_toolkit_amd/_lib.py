@@ -29,7 +29,7 @@ class HbtConfig(C.Structure):
         ("HBTrap_min", C.c_double), ("HBTrap_max", C.c_double),
         ("needed_number_of_pairs", C.c_double), ("number_of_mixed_events", C.c_int32),
         ("precision", C.c_int32), ("kernel", C.c_int32), ("device", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("kphi_guard_prad", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -93,7 +93,7 @@ class HbtServerStats(C.Structure):
                 ("launches", C.c_int64), ("items", C.c_int64), ("pairs", C.c_double),
                 ("arena_bytes", C.c_int64), ("arena_used", C.c_int64),
                 ("arena_peak", C.c_int64), ("resident", C.c_int32),
-                ("resident_peak", C.c_int32), ("queued", C.c_int32), ("reserved", C.c_int32),
+                ("resident_peak", C.c_int32), ("queued", C.c_int32), ("deferred", C.c_int32),
                 ("busy_s", C.c_double)]
 
 
@@ -110,6 +110,7 @@ SIGNATURES = [
     ("hbt_run_mixed", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), _F64P]),
     ("hbt_read", C.c_int, [C.c_void_p, C.POINTER(HbtResult)]),
     ("hbt_stage_counts", C.c_int, [C.c_void_p, _U64P, _U64P]),
+    ("hbt_deferred_pairs", C.c_int, [C.c_void_p, _U64P]),
     ("hbt_rng_create", C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     ("hbt_rng_destroy", C.c_int, [C.c_void_p]),
     ("hbt_rng_int", C.c_int32, [C.c_void_p]),

@@ -64,6 +64,9 @@ __device__ __forceinline__ int cap_same_pair(const DevParams &P, const DevHist &
     const int st = cascade_f64<false, true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi, b,
                                             q_inv, qinv_ok);
     const bool bin = (st & ST_BIN) != 0;
+    // a pair on a K_phi bin edge cannot be placed in the cap's order here:
+    // counted, so the host reports it
+    if (st & ST_DEFER) atomicAdd(H.ndefer, 1u);
     if (COUNT) {
         if (bin) atomicAdd(&C.cnt3[(long long)key3_of(P, b) * C.nrows + gi], 1u);
         if (qinv_ok) atomicAdd(&C.cntq[(long long)b.kt * C.nrows + gi], 1u);
@@ -104,6 +107,9 @@ __device__ __forceinline__ int cap_mixed_pair(const DevParams &P, const DevHist 
     const int st = cascade_f64<true, true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi, b,
                                            q_inv, qinv_ok);
     const bool bin = (st & ST_BIN) != 0;
+    // a pair on a K_phi bin edge cannot be placed in the cap's order here:
+    // counted, so the host reports it
+    if (st & ST_DEFER) atomicAdd(H.ndefer, 1u);
     if (COUNT) {
         if (bin) atomicAdd(&C.cnt3[(long long)key3_of(P, b) * C.nrows + gi], 1u);
         if (qinv_ok) atomicAdd(&C.cntq[(long long)b.kt * C.nrows + gi], 1u);

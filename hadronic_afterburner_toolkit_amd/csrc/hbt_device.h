@@ -27,12 +27,28 @@ struct DevParams {
     float f_half, f_qthr, f_mid, f_pad;
     // precision-form exact-stage constants
     double inv_dKT, inv_dq, inv_two_pi, two_pi_hi, two_pi_lo, inv_dKphi;
+    // K_phi angles closer than this (rad) to a bin edge, 0 or 2 pi are
+    // decided on the host with glibc's atan2 (the device's may differ from it
+    // in the last bits)
+    double kphi_guard;
 };
 
 // Histogram bins, array of structures: 8 doubles (64 bytes, one aligned
 // segment) per bin, so one bin's accumulations share a cache line.
 enum { BIN_COUNT = 0, BIN_QO = 1, BIN_QS = 2, BIN_QL = 3, BIN_NUM = 4, BIN_DEN = 5,
        BIN_STRIDE = 8 };
+
+// A pair whose K_phi bin the device leaves to the host: the cascade's
+// inputs (same-event: both particles' px py pz E x y z t in the reference's
+// orientation; mixed: px py pz E of particle 1 and of the rotated partner),
+// psi_ref, and the histogram it belongs to.
+struct DeferRec {
+    double p1[8], p2[8];
+    double psi;
+    double *bins;
+    int kind;  // 0 same-event, 1 mixed
+    int pad;
+};
 
 struct DevHist {
     // [nbins][BIN_STRIDE]; COUNT holds the pair count as an FP64 integer
@@ -41,6 +57,11 @@ struct DevHist {
     unsigned long long *qinv_count, *qinv_den;
     double *qinv_sum, *qinv_num;
     unsigned long long *stage;  // [0..4] same, [5..9] mixed
+    // pairs deferred to the host (K_phi at a bin edge): records and count;
+    // a count above defer_cap is an overflow the host reports
+    DeferRec *defer;
+    unsigned *ndefer;
+    unsigned defer_cap, pad;
 };
 
 // FP64 particle arrays in HBM (structure of arrays).
@@ -71,20 +92,33 @@ __device__ __forceinline__ void atomic_add_f64(double *p, double v) {
     unsafeAtomicAdd(p, v);
 }
 
-__device__ __forceinline__ long bin_index(const DevParams &P, int kt, int kphi, int io, int is,
-                                          int il) {
+__host__ __device__ __forceinline__ long bin_index(const DevParams &P, int kt, int kphi, int io,
+                                                   int is, int il) {
     const long q = P.qnpts;
     return ((((long)kt * P.nphi + kphi) * q + io) * q + is) * q + il;
 }
 
 // K_phi bin with the reference's wrap loops (HBT_correlation.cpp:408-423).
-// Returns -1 when the reference would skip the pair.
-__device__ __forceinline__ int kphi_index(const DevParams &P, double K_x, double K_y,
-                                          double psi_ref) {
+// Returns -1 when the reference would skip the pair.  On the device, an
+// angle within kphi_guard of 0 or of a bin edge (2 pi included) returns -3:
+// the reference's glibc atan2 and the device's can differ there in the last
+// bits, so the host decides the pair with glibc's atan2 (the host build of
+// this function has no guard).
+__host__ __device__ __forceinline__ int kphi_index(const DevParams &P, double K_x, double K_y,
+                                                   double psi_ref) {
     double local_K_phi = atan2(K_y, K_x);
     double delta_phi = local_K_phi - psi_ref;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (fabs(delta_phi) < P.kphi_guard) return -3;
+#endif
     while (delta_phi < 0.) delta_phi += P.two_pi;
     while (delta_phi > P.two_pi) delta_phi -= P.two_pi;
+#if defined(__HIP_DEVICE_COMPILE__)
+    {
+        const double w = delta_phi / P.dKphi;
+        if (fabs(w - rint(w)) * P.dKphi < P.kphi_guard) return -3;
+    }
+#endif
     int idx = (int)(delta_phi / P.dKphi);
     if (idx < 0 || idx >= P.n_Kphi) return -1;
     return idx;
@@ -97,8 +131,9 @@ struct PairBins {
     double q_x, q_y, q_z, q_E;
 };
 
-// Stages reached, for the plain kernel's counters.
-enum { ST_KT = 1, ST_OUT = 2, ST_SIDE = 4, ST_BIN = 8 };
+// Stages reached, for the plain kernel's counters; ST_DEFER: every window
+// passed and the K_phi bin is left to the host (no ST_BIN).
+enum { ST_KT = 1, ST_OUT = 2, ST_SIDE = 4, ST_BIN = 8, ST_DEFER = 16 };
 
 // The K/q cascade (HBT_correlation.cpp:311-398 / :574-647).  MIXED selects
 // the mixed-event upper cuts (>=, :613, :622, :641).  Returns the stage
@@ -107,7 +142,7 @@ enum { ST_KT = 1, ST_OUT = 2, ST_SIDE = 4, ST_BIN = 8 };
 // EXT = false compiles the q_inv and K_phi branches out (the caller
 // guarantees invariant_radius_flag = azimuthal_flag = 0).
 template <bool MIXED, bool EXT = true>
-__device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, double p1y,
+__host__ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, double p1y,
                                            double p1z, double p1E, double p2x, double p2y,
                                            double p2z, double p2E, double psi_ref,
                                            PairBins &b, double &q_inv, int &qinv_ok) {
@@ -170,14 +205,32 @@ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p1x, doubl
     b.il = (int)((q_long - P.q_base) / P.delta_q);
     if (b.il >= P.qnpts || b.il < 0) return st;
     b.kphi = 0;
-    if (EXT && P.azimuthal) {
-        b.kphi = kphi_index(P, K_x, K_y, psi_ref);
-        if (b.kphi < 0) return st;
-    }
     b.q_out = q_out;
     b.q_side = q_side;
     b.q_long = q_long;
+    if (EXT && P.azimuthal) {
+        b.kphi = kphi_index(P, K_x, K_y, psi_ref);
+        if (b.kphi == -3) return st | ST_DEFER;
+        if (b.kphi < 0) return st;
+    }
     return st | ST_BIN;
+}
+
+// Records a pair for the host (a count beyond defer_cap keeps counting, so
+// the host sees the overflow).
+__device__ __forceinline__ void push_defer(const DevHist &H, const double a[8], const double c[8],
+                                           double psi, int kind) {
+    const unsigned slot = atomicAdd(H.ndefer, 1u);
+    if (slot >= H.defer_cap) return;
+    DeferRec &d = H.defer[slot];
+    for (int f = 0; f < 8; f++) {
+        d.p1[f] = a[f];
+        d.p2[f] = c[f];
+    }
+    d.psi = psi;
+    d.bins = H.bins;
+    d.kind = kind;
+    d.pad = 0;
 }
 
 // One binned same-event pair: bin and the four FP64 sums it adds to.
@@ -200,7 +253,7 @@ __device__ __forceinline__ int eval_same_f64(const DevParams &P, const DevHist &
     int qinv_ok;
     int st = cascade_f64<false, EXT>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b,
                                      q_inv, qinv_ok);
-    if (!(st & (ST_BIN)) && !qinv_ok) return st;
+    if (!(st & (ST_BIN | ST_DEFER)) && !qinv_ok) return st;
     double t_diff = t1 - t2;
     double x_diff = x1 - x2;
     double y_diff = y1 - y2;
@@ -212,6 +265,11 @@ __device__ __forceinline__ int eval_same_f64(const DevParams &P, const DevHist &
         atomic_add_u64(&H.qinv_count[k], 1ull);
         atomic_add_f64(&H.qinv_sum[k], q_inv);
         atomic_add_f64(&H.qinv_num[k], cos_qx);
+    }
+    if (EXT && (st & ST_DEFER)) {
+        const double a[8] = {p1x, p1y, p1z, p1E, x1, y1, z1, t1};
+        const double c[8] = {p2x, p2y, p2z, p2E, x2, y2, z2, t2};
+        push_defer(H, a, c, psi_ref, 0);
     }
     if (st & ST_BIN) {
         rec.k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
@@ -236,6 +294,11 @@ __device__ __forceinline__ int eval_mixed_f64(const DevParams &P, const DevHist 
     int st = cascade_f64<true, EXT>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi_ref, b, q_inv,
                                     qinv_ok);
     if (qinv_ok) atomic_add_u64(&H.qinv_den[(long)b.kt * P.qnpts + (qinv_ok - 1)], 1ull);
+    if (EXT && (st & ST_DEFER)) {
+        const double a[8] = {p1x, p1y, p1z, p1E, 0, 0, 0, 0};
+        const double c[8] = {p2x, p2y, p2z, p2E, 0, 0, 0, 0};
+        push_defer(H, a, c, psi_ref, 1);
+    }
     if (st & ST_BIN) k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
     return st;
 }

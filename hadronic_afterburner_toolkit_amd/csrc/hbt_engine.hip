@@ -239,6 +239,21 @@ __global__ __launch_bounds__(TILE) void k_mixed_cap(DevParams P, DevHist H, DevS
 
 __global__ void k_zero_int(int *p) { *p = 0; }
 
+// Adds the host-decided pairs to their histograms (a handful per call).
+__global__ void k_apply_defer(const DeferAdd *a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DeferAdd d = a[i];
+    DevHist H{};
+    H.bins = d.bins;
+    if (d.kind == 0) {
+        const SameRec r{d.k, d.v[1], d.v[2], d.v[3], d.v[4]};
+        add_same_direct(H, r);
+    } else {
+        add_mixed_direct(H, d.k);
+    }
+}
+
 __global__ void k_zero_words(unsigned long long *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -314,6 +329,10 @@ struct hbt_ctx {
     double *d_qinv_sum = nullptr, *d_qinv_num = nullptr;
     unsigned long long *d_stage = nullptr;   // [10]
     unsigned long long *d_binned = nullptr;  // [2]
+    DeferRec *d_defer = nullptr;             // [kDeferCap] pairs left to the host
+    unsigned *d_ndefer = nullptr;
+    bool batch_capped = false;               // the last hbt_launch_batch took the cap path
+    uint64_t deferred_total = 0;             // pairs decided on the host so far
     unsigned long long *d_sums = nullptr;    // scratch for counters
     Slot slot[2];
     // mixing plan (device), and its host copy for the plain kernels
@@ -418,13 +437,16 @@ void drop_graph(hbt_ctx *c) {
 }
 
 DevHist hist_view(hbt_ctx *c) {
-    DevHist h;
+    DevHist h{};
     h.bins = c->d_bins;
     h.qinv_count = c->d_qinv_count;
     h.qinv_den = c->d_qinv_den;
     h.qinv_sum = c->d_qinv_sum;
     h.qinv_num = c->d_qinv_num;
     h.stage = c->d_stage;
+    h.defer = c->d_defer;
+    h.ndefer = c->d_ndefer;
+    h.defer_cap = kDeferCap;
     return h;
 }
 
@@ -452,8 +474,49 @@ int zero_hist(hbt_ctx *c) {
     HIPCHECK(c, hipMemsetAsync(c->d_qinv_num, 0, q8, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_stage, 0, 10 * 8, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_binned, 0, 2 * 8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_ndefer, 0, sizeof(unsigned), c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->enum_same_total = c->enum_mixed_total = 0;
+    return HBT_OK;
+}
+
+// Pairs the device left to the host (K_phi within kphi_guard of a bin edge):
+// the host runs the same FP64 cascade with glibc's atan2 (kphi_index's host
+// build), as the reference does, and adds each pair it bins to the device
+// histogram.  Called after the stream is idle, before results are read.  A
+// capped run cannot place such a pair in the cap's order and fails instead.
+int resolve_deferred(hbt_ctx *c, bool capped) {
+    if (!c->P.azimuthal) return HBT_OK;  // only K_phi bins defer pairs
+    unsigned n = 0;
+    HIPCHECK(c, hipMemcpy(&n, c->d_ndefer, sizeof(n), hipMemcpyDeviceToHost));
+    if (n == 0) return HBT_OK;
+    if (capped)
+        return fail(c, HBT_ERR_STATE,
+                    "a pair at a K_phi bin edge under a binding needed_number_of_pairs");
+    if (n > kDeferCap)
+        return fail(c, HBT_ERR_CAPACITY, "more K_phi bin-edge pairs than the deferral buffer");
+    std::vector<DeferRec> rec(n);
+    HIPCHECK(c, hipMemcpy(rec.data(), c->d_defer, n * sizeof(DeferRec), hipMemcpyDeviceToHost));
+    c->deferred_total += n;
+    std::vector<DeferAdd> adds;
+    resolve_records(c->P, rec.data(), n, adds);
+    HIPCHECK(c, hipMemsetAsync(c->d_ndefer, 0, sizeof(unsigned), c->stream));
+    if (!adds.empty()) {
+        DeferAdd *d = nullptr;
+        HIPCHECK(c, hipMalloc((void **)&d, adds.size() * sizeof(DeferAdd)));
+        hipError_t e = hipMemcpyAsync(d, adds.data(), adds.size() * sizeof(DeferAdd),
+                                      hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            const int nb = (int)((adds.size() + 255) / 256);
+            hipLaunchKernelGGL(k_apply_defer, dim3(nb), dim3(256), 0, c->stream, d,
+                               (int)adds.size());
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(d);
+        HIPCHECK(c, e);
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     return HBT_OK;
 }
 
@@ -1063,6 +1126,8 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     if (!rc) rc = dev_alloc(cp, &c->d_sums, (size_t)k.n_KT * std::max(1, D.P.nphi));
     if (!rc) rc = dev_alloc(cp, &c->d_counter, 1);
     if (!rc) rc = dev_alloc(cp, &c->d_view, 1);
+    if (!rc) rc = dev_alloc(cp, &c->d_defer, kDeferCap);
+    if (!rc) rc = dev_alloc(cp, &c->d_ndefer, 1);
     if (!rc) rc = zero_hist(cp);
     if (rc) {
         g_create_err = c->err;
@@ -1082,7 +1147,7 @@ int hbt_destroy(hbt_ctx *c) {
                     c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
                     c->d_items,     c->d_counter,   c->d_psi,        c->d_view,
                     c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff,
-                    c->d_wins};
+                    c->d_wins,      c->d_defer,     c->d_ndefer};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -1152,6 +1217,8 @@ int hbt_run_same(hbt_ctx *c) {
         if (!rc) rc = timed_fast(c, &c->ms_same);
         if (rc) return rc;
     }
+    rc = resolve_deferred(c, binds);
+    if (rc) return rc;
     c->enum_same_total += es;
     return HBT_OK;
 }
@@ -1188,6 +1255,8 @@ int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const doubl
         if (!rc) rc = timed_fast(c, &c->ms_mixed);
         if (rc) return rc;
     }
+    rc = resolve_deferred(c, binds);
+    if (rc) return rc;
     c->enum_mixed_total += em;
     return HBT_OK;
 }
@@ -1223,6 +1292,7 @@ int hbt_launch_batch(hbt_ctx *c) {
     if (!rc) rc = check_cap(c, true, c->batch_enum_mixed, bm);
     if (rc) return rc;
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    c->batch_capped = bs || bm;
     if (bs || bm) {  // a cap may bind: each side in enumeration order
         rc = bs ? run_capped(c, false) : run_plain(c, true, false);
         if (!rc) rc = bm ? run_capped(c, true) : (c->plan_valid ? run_plain(c, false, true) : 0);
@@ -1252,7 +1322,7 @@ int hbt_sync(hbt_ctx *c, double *batch_ms) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->ms_batch = ms;
     if (batch_ms) *batch_ms = c->ms_batch;
-    return HBT_OK;
+    return resolve_deferred(c, c->batch_capped);
 }
 
 int hbt_batch_info(hbt_ctx *c, uint64_t *enum_same, uint64_t *enum_mixed, int32_t *nitems) {
@@ -1334,6 +1404,10 @@ int hbt_last_kernel_ms(hbt_ctx *c, double *same_ms, double *mixed_ms) {
 int hbt_read(hbt_ctx *c, hbt_result *o) {
     if (!c || !o) return HBT_ERR_ARG;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    {
+        const int rc = resolve_deferred(c, c->batch_capped);
+        if (rc) return rc;
+    }
     const size_t q8 = (size_t)c->nq * 8;
     if (o->num_count || o->den || o->q_out_sum || o->q_side_sum || o->q_long_sum || o->num) {
         std::vector<double> aos((size_t)c->nbins * BIN_STRIDE);
@@ -1384,6 +1458,12 @@ int hbt_read(hbt_ctx *c, hbt_result *o) {
             if (o->cnt_den_ktphi) o->cnt_den_ktphi[k] = sd[k];
         }
     }
+    return HBT_OK;
+}
+
+int hbt_deferred_pairs(hbt_ctx *c, uint64_t *n) {
+    if (!c || !n) return HBT_ERR_ARG;
+    *n = c->deferred_total;
     return HBT_OK;
 }
 

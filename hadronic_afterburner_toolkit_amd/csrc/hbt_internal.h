@@ -72,6 +72,7 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     P.inv_dq = 1.0 / P.delta_q;
     P.inv_two_pi = 1.0 / (2.0 * M_PI);
     P.inv_dKphi = 1.0 / P.dKphi;
+    P.kphi_guard = (k.kphi_guard_prad > 0 ? k.kphi_guard_prad : 1) * 1e-12;
     P.two_pi_hi = 6.28318530717958623199592693709e+00;  // 2*pi rounded to double
     P.two_pi_lo = 2.44929359829470635445e-16;            // 2*pi - two_pi_hi
     // Prefilter (hbt_fast.h).  T = 4 K_perp^2 must lie in
@@ -531,6 +532,54 @@ inline void items_mixed(const ItemShape &s, const std::vector<long long> &foff,
             }
         }
     });
+}
+
+// Pairs left to the host by the device (DeferRec): room per context or
+// request, and one bin addition each.
+constexpr unsigned kDeferCap = 1u << 16;     // per context
+constexpr unsigned kReqDeferCap = 1u << 12;  // per server request
+
+struct DeferAdd {
+    double *bins;  // the histogram (device address)
+    long long k;   // bin
+    int kind;      // 0: count + q_out, q_side, q_long, cos sums; 1: denominator
+    int pad;
+    double v[5];
+};
+
+// The host's decision for deferred pairs: the same FP64 cascade, whose host
+// build takes K_phi from glibc's atan2 as the reference does
+// (HBT_correlation.cpp:408-423), and the cos phase of a same-event pair
+// (:431-436).
+inline void resolve_records(const DevParams &P, const DeferRec *rec, unsigned n,
+                            std::vector<DeferAdd> &adds) {
+    for (unsigned i = 0; i < n; i++) {
+        const DeferRec &r = rec[i];
+        PairBins b;
+        double q_inv;
+        int qinv_ok;
+        if (r.kind == 0) {
+            const int st = cascade_f64<false, true>(P, r.p1[0], r.p1[1], r.p1[2], r.p1[3], r.p2[0],
+                                                    r.p2[1], r.p2[2], r.p2[3], r.psi, b, q_inv,
+                                                    qinv_ok);
+            if (!(st & ST_BIN)) continue;
+            const double t_diff = r.p1[7] - r.p2[7];
+            const double x_diff = r.p1[4] - r.p2[4];
+            const double y_diff = r.p1[5] - r.p2[5];
+            const double z_diff = r.p1[6] - r.p2[6];
+            const double cq = std::cos(P.hbarC_inv * (b.q_E * t_diff - b.q_x * x_diff -
+                                                      b.q_y * y_diff - b.q_z * z_diff));
+            adds.push_back(DeferAdd{r.bins, bin_index(P, b.kt, b.kphi, b.io, b.is, b.il), 0, 0,
+                                    {1.0, b.q_out, b.q_side, b.q_long, cq}});
+        } else {
+            const int st = cascade_f64<true, true>(P, r.p1[0], r.p1[1], r.p1[2], r.p1[3], r.p2[0],
+                                                   r.p2[1], r.p2[2], r.p2[3], r.psi, b, q_inv,
+                                                   qinv_ok);
+            if (!(st & ST_BIN)) continue;
+            adds.push_back(DeferAdd{r.bins, bin_index(P, b.kt, b.kphi, b.io, b.is, b.il), 1, 0,
+                                    {0, 0, 0, 0, 0}});
+        }
+    }
 }
 
 // hbt_result from the AoS bins and q_inv arrays in host memory; the pair

@@ -115,7 +115,7 @@ struct Request {
     // device residency
     size_t foot = 0, arena_off = 0;
     size_t o_bins = 0, o_rec = 0, o_pf = 0, o_cw = 0, o_spf = 0, o_scw = 0,
-           o_orig = 0, o_win = 0, o_cs = 0, o_psi = 0;
+           o_orig = 0, o_win = 0, o_cs = 0, o_psi = 0, o_defer = 0;
     int view = -1, rslot = -1;
     View v{};
     hipEvent_t done_ev = nullptr;
@@ -142,6 +142,7 @@ struct hbt_server {
     std::vector<char *> h_results;
     std::vector<int> free_results;
     size_t result_bytes = 0;
+    size_t defer_count_off = 0;  // byte offset of the deferred-pair count in a result
     WorkItem *d_items[2] = {};
     WorkItem *h_items[2] = {};
     hipEvent_t step_ev[2] = {};     // end of launch (ordering)
@@ -206,6 +207,8 @@ size_t layout(const hbt_server *s, Request &r) {
     o = align256(o + (size_t)std::max<long long>(rows, 1) * 16);
     r.o_psi = o;
     o = align256(o + (size_t)std::max(r.ngroups, 1) * 8);
+    r.o_defer = o;
+    o = align256(o + (size_t)kReqDeferCap * sizeof(DeferRec));
     return o;
 }
 
@@ -219,6 +222,11 @@ void make_request_view(hbt_server *s, Request &r) {
     v.H.qinv_sum = (double *)(q + 2 * s->D.nq);
     v.H.qinv_num = (double *)(q + 3 * s->D.nq);
     v.H.stage = s->d_misc;
+    // pairs left to the host: records in the request's block, their count in
+    // the zeroed result tail, copied back with the histograms
+    v.H.defer = (DeferRec *)(b + r.o_defer);
+    v.H.ndefer = (unsigned *)(b + r.o_bins + s->defer_count_off);
+    v.H.defer_cap = kReqDeferCap;
     v.A = v.B = (const PartRec *)(b + r.o_rec);
     v.pfA = v.pfB = (const void *)(b + r.o_pf);
     v.cwA = v.cwB = (const void *)(b + r.o_cw);
@@ -473,6 +481,47 @@ void launcher_loop(hbt_server *s) {
     s->cv.notify_all();
 }
 
+// Pairs the device left to the host for a finished request (K_phi at a bin
+// edge): decided with glibc's atan2 (resolve_records) and added to the
+// request's result in host memory, before it is reported done.
+hipError_t resolve_request_defer(hbt_server *s, Request &r, std::string &err) {
+    char *res = s->h_results[r.rslot];
+    unsigned n = 0;
+    std::memcpy(&n, res + s->defer_count_off, sizeof(n));
+    if (n == 0) return hipSuccess;
+    if (n > kReqDeferCap) {
+        err = "more K_phi bin-edge pairs than the request's deferral buffer";
+        return hipSuccess;
+    }
+    std::vector<DeferRec> rec(n);
+    const hipError_t e = hipMemcpy(rec.data(), s->d_arena + r.arena_off + r.o_defer,
+                                   n * sizeof(DeferRec), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    std::vector<DeferAdd> adds;
+    resolve_records(s->D.P, rec.data(), n, adds);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->st.deferred += (int32_t)n;
+    }
+    double *bins = (double *)res;
+    for (const DeferAdd &a : adds) {
+        double *b = bins + a.k * BIN_STRIDE;
+        if (a.kind == 0) {
+            b[BIN_COUNT] += 1.0;
+            b[BIN_QO] += a.v[1];
+            b[BIN_QS] += a.v[2];
+            b[BIN_QL] += a.v[3];
+            b[BIN_NUM] += a.v[4];
+        } else {
+            unsigned long long den;
+            std::memcpy(&den, b + BIN_DEN, 8);
+            den += 1;
+            std::memcpy(b + BIN_DEN, &den, 8);
+        }
+    }
+    return hipSuccess;
+}
+
 void completion_loop(hbt_server *s) {
     hipSetDevice(s->device);
     for (;;) {
@@ -483,13 +532,15 @@ void completion_loop(hbt_server *s) {
             if (s->draining.empty()) return;  // stopped and nothing left to read back
             r = s->draining.front();
         }
-        const hipError_t e = hipEventSynchronize(r->done_ev);
+        hipError_t e = hipEventSynchronize(r->done_ev);
+        std::string derr;
+        if (e == hipSuccess) e = resolve_request_defer(s, *r, derr);
         std::lock_guard<std::mutex> lk(s->mu);
         s->draining.pop_front();
         r->t_done = s->now();
-        if (e != hipSuccess) {
+        if (e != hipSuccess || !derr.empty()) {
             r->state = HBT_REQ_FAILED;
-            r->err = std::string("readback: ") + hipGetErrorString(e);
+            r->err = !derr.empty() ? derr : std::string("readback: ") + hipGetErrorString(e);
             s->st.failed++;
         } else {
             r->state = HBT_REQ_DONE;
@@ -565,7 +616,8 @@ int hbt_server_create(const hbt_config *cfg, const hbt_server_config *sc_in, hbt
     s->shape = item_shape(sc.tune);
     s->f32 = cfg->precision == HBT_PREC_FP32C;
     s->ext = s->D.P.qinv || s->D.P.azimuthal;
-    s->result_bytes = (size_t)s->D.nbins * 64 + (size_t)s->D.nq * 32;
+    s->defer_count_off = (size_t)s->D.nbins * 64 + (size_t)s->D.nq * 32;
+    s->result_bytes = s->defer_count_off + 256;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return sfail(nullptr, HBT_ERR_NODEVICE, "no HIP device visible");

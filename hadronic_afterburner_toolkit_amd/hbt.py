@@ -45,6 +45,8 @@ def config_from_params(params: dict, precision: int = _lib.HBT_PREC_FP64,
                  "needed_number_of_pairs"):
         setattr(cfg, name, float(p[name]))
     cfg.precision, cfg.kernel, cfg.device = precision, kernel, device
+    # engine key: K_phi guard band in 1e-12 rad (0: the default 1e-12 rad)
+    cfg.kphi_guard_prad = int((params or {}).get("hbt_kphi_guard_prad", 0))
     return cfg
 
 
@@ -322,6 +324,13 @@ class HBTCorrelation:
         self._check(self._lib.hbt_stage_counts(self.h, _u64p(s), _u64p(m)), "stage")
         return s, m
 
+    def deferred_pairs(self) -> int:
+        """Pairs decided on the host so far (K_phi within the guard band of a
+        bin edge)."""
+        n = np.zeros(1, np.uint64)
+        self._check(self._lib.hbt_deferred_pairs(self.h, _u64p(n)), "hbt_deferred_pairs")
+        return int(n[0])
+
     def results(self) -> Results:
         r, res = _alloc_results(self.params, self.nphi)
         self._check(self._lib.hbt_read(self.h, C.byref(res)), "hbt_read")
@@ -378,6 +387,8 @@ def _make_config(params, precision, kernel, device):
     cfg.precision = precision
     cfg.kernel = kernel
     cfg.device = device
+    # engine key: K_phi guard band in 1e-12 rad (0: the default 1e-12 rad)
+    cfg.kphi_guard_prad = int((params or {}).get("hbt_kphi_guard_prad", 0))
     return p, cfg
 
 

@@ -67,7 +67,12 @@ typedef struct {
     int32_t precision;       /* HBT_PREC_* */
     int32_t kernel;          /* HBT_KERNEL_* */
     int32_t device;          /* HIP device ordinal */
-    int32_t reserved[5];
+    /* K_phi angles within this many 1e-12 rad of a bin edge (or of 0 and
+     * 2 pi) are decided on the host with glibc's atan2, as the reference
+     * computes them; the device's atan2 may differ from it in the last bits.
+     * 0: 1 (1e-12 rad).  Larger values only move more pairs to the host. */
+    int32_t kphi_guard_prad;
+    int32_t reserved[4];
 } hbt_config;
 
 typedef struct hbt_ctx hbt_ctx;
@@ -142,6 +147,12 @@ int hbt_read(hbt_ctx *ctx, hbt_result *out);
  * of the FLOP model.  Only the plain kernel counts every stage; the fast
  * kernel reports enumerated and binned. */
 int hbt_stage_counts(hbt_ctx *ctx, uint64_t same[5], uint64_t mixed[5]);
+
+/* Pairs the context decided on the host so far: every window passed and
+ * the K_phi angle lay within kphi_guard_prad of a bin edge, where the
+ * device's atan2 may differ from the reference's glibc atan2 (no counterpart
+ * in the reference). */
+int hbt_deferred_pairs(hbt_ctx *ctx, uint64_t *n);
 
 /* Replaces RandomUtil::Random (Random.h:11-25, Random.cpp:7-14):
  * std::mt19937(seed) with the reference's two distributions.  seed -1
@@ -281,7 +292,8 @@ typedef struct {
     int64_t items;            /* work items launched */
     double pairs;             /* enumerated pairs launched */
     int64_t arena_bytes, arena_used, arena_peak;
-    int32_t resident, resident_peak, queued, reserved;
+    int32_t resident, resident_peak, queued;
+    int32_t deferred;  /* pairs decided on the host (K_phi at a bin edge) */
     double busy_s;            /* summed device time of the launches (HIP events) */
 } hbt_server_stats;
 
