@@ -31,6 +31,7 @@ HBT_correlation::HBT_correlation(ParameterReader &paraRdr, std::string path,
     c.precision = (int)paraRdr_.getVal("hbt_precision", HBT_PREC_FP64);
     c.kernel = (int)paraRdr_.getVal("hbt_kernel", HBT_KERNEL_FAST);
     c.device = (int)paraRdr_.getVal("hbt_device", 0);
+    c.kphi_guard_prad = (int)paraRdr_.getVal("hbt_kphi_guard_prad", 0);
     check(hbt_create(&c, &ctx_), "hbt_create");
 }
 

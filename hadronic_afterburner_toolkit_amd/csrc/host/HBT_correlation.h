@@ -5,6 +5,8 @@
 //   hbt_precision  0 = FP64 (bit-exact counts, default), 1 = precision form
 //   hbt_kernel     0 = plain LDS-tiled kernel, 1 = fast kernel (default)
 //   hbt_device     HIP device ordinal (default 0)
+//   hbt_kphi_guard_prad  K_phi guard band in 1e-12 rad (default 1): pairs
+//                  that close to a K_phi bin edge are binned on the host
 //   number_of_mixed_events  partners per event (default: the reference's
 //                           floor(nev/2)+1)
 #ifndef HBT_AMD_HBT_CORRELATION_H
