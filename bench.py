@@ -74,6 +74,14 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None
     g.upload_groups(groups)
     g.prepare_batch(*plan)
     upload_s = time.perf_counter() - t0
+    # again on the same context, from packed columns: the steady state of a
+    # caller that uploads buffer after buffer (device and pinned buffers kept)
+    packed = hbt.pack_groups(groups)
+    t0 = time.perf_counter()
+    g.upload_packed(packed)
+    upload_warm_s = time.perf_counter() - t0
+    g.prepare_batch(*plan)
+    prepare_warm_s = time.perf_counter() - t0 - upload_warm_s
     if tuning:
         g.set_tuning(*tuning)
     tune = g.autotune() if autotune else g.tuning()
@@ -91,7 +99,8 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None
     log(f"  prec={prec} kernel={kern} steps={steps} wall={wall:.3f}s "
         f"dev_ms/step={np.mean(dev_ms):.2f} tune={tune}")
     return g, dict(wall=wall, dev_ms=dev_ms, enum_same=es, enum_mixed=em, nitems=nitems,
-                   tune=tune, upload_s=upload_s)
+                   tune=tune, upload_s=upload_s, upload_warm_s=upload_warm_s,
+                   prepare_warm_s=prepare_warm_s)
 
 
 def serve(args, log):
@@ -609,6 +618,8 @@ def main():
         out[f"{name}_pairs_per_s"] = pairs_per_s(res[name])
         out[f"{name}_dev_ms_per_step"] = float(np.mean(res[name]["dev_ms"]))
         out[f"{name}_upload_s"] = res[name]["upload_s"]
+        out[f"{name}_upload_warm_s"] = res[name]["upload_warm_s"]
+        out[f"{name}_prepare_warm_s"] = res[name]["prepare_warm_s"]
     if "fp32c" in res and "plain" in res:
         out["speedup_vs_plain_fp64"] = out["fp32c_pairs_per_s"] / out["plain_pairs_per_s"]
     if "fp32c" in res and "fp64" in res:

@@ -322,6 +322,8 @@ struct hbt_ctx {
     DevParams P;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
+    char *h_stage = nullptr;  // pinned staging of uploads (grown on demand)
+    size_t stage_cap = 0;
     int cu_count = 0;
     long nbins = 0, nq = 0;
     double *d_bins = nullptr;  // [nbins][BIN_STRIDE]
@@ -954,6 +956,8 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     std::string err;
     int rc = gather_particles(c->D, c->cfg, ngroups, grp_ev, nev, ev_off, soa, h, err);
     if (rc) return fail(c, rc, err);
+    // a batch launched without hbt_sync may still read the slot and the staging
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     drop_graph(c);
     c->plan_valid = false;
     Slot &s = c->slot[slot];
@@ -983,34 +987,50 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         if (rc) return rc;
         s.cap_ev = (size_t)nev + 1;
     }
-    hvec<double> col((size_t)n);
-    for (int a = 0; a < 8; a++) {
-        for (long long k = 0; k < n; k++) col[k] = soa[a][h.keep[k]];
-        HIPCHECK(c, hipMemcpy(s.d_soa[a], col.data(), n * 8, hipMemcpyHostToDevice));
-    }
+    // Everything goes through the context's pinned staging buffer: the
+    // columns are gathered into it on several threads and copied while the
+    // records are staged behind them; all copies are on the context's stream.
+    std::vector<StagePiece> pieces;
+    auto add = [&](void *dst, const void *src, size_t bytes) {
+        if (dst && bytes) pieces.push_back(StagePiece{dst, src, bytes});
+    };
     if (nf > 0) {
         const std::pair<DevRecords *, const RecordSet *> sets[] = {{&s.ev, &h.fast},
                                                                    {&s.srt, &h.srt}};
         for (const auto &x : sets) {
             DevRecords &d = *x.first;
             const RecordSet &r = *x.second;
-            if (d.rec)
-                HIPCHECK(c, hipMemcpy(d.rec, r.rec.data(), nf * sizeof(PartRec), hipMemcpyHostToDevice));
+            add(d.rec, r.rec.data(), (size_t)nf * sizeof(PartRec));
             if (f32) {
-                HIPCHECK(c, hipMemcpy(d.pf32, r.pf32.data(), nf * sizeof(PfRec32), hipMemcpyHostToDevice));
-                HIPCHECK(c, hipMemcpy(d.cw32, r.cw32.data(), nf * 4, hipMemcpyHostToDevice));
+                add(d.pf32, r.pf32.data(), (size_t)nf * sizeof(PfRec32));
+                add(d.cw32, r.cw32.data(), (size_t)nf * 4);
             } else {
-                HIPCHECK(c, hipMemcpy(d.pf64, r.pf64.data(), nf * sizeof(PfRec64), hipMemcpyHostToDevice));
-                HIPCHECK(c, hipMemcpy(d.cw64, r.cw64.data(), nf * 8, hipMemcpyHostToDevice));
+                add(d.pf64, r.pf64.data(), (size_t)nf * sizeof(PfRec64));
+                add(d.cw64, r.cw64.data(), (size_t)nf * 8);
             }
         }
-        HIPCHECK(c, hipMemcpy(s.d_orig, h.orr.data(), nf * sizeof(OrigRank), hipMemcpyHostToDevice));
+        add(s.d_orig, h.orr.data(), (size_t)nf * sizeof(OrigRank));
     }
+    add(s.d_off, h.off.data(), ((size_t)nev + 1) * 8);
+    const size_t col_bytes = stage_round((size_t)n * 8);
+    HIPCHECK(c, ensure_pinned(&c->h_stage, &c->stage_cap, 8 * col_bytes + stage_bytes(pieces)));
+    char *stage = c->h_stage;
+    parallel_ranges(n, 1 << 15, [&](long long k0, long long k1) {
+        for (int a = 0; a < 8; a++) {
+            double *dst = reinterpret_cast<double *>(stage + a * col_bytes);
+            const double *src = soa[a];
+            for (long long k = k0; k < k1; k++) dst[k] = src[h.keep[k]];
+        }
+    });
+    for (int a = 0; a < 8 && n > 0; a++)
+        HIPCHECK(c, hipMemcpyAsync(s.d_soa[a], stage + a * col_bytes, (size_t)n * 8,
+                                   hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, stage_pieces(stage + 8 * col_bytes, pieces, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     s.nf = nf;
     s.foff = std::move(h.foff);
     s.f_pt = std::move(h.f_pt);
     s.srt_px = std::move(h.srt_px);
-    HIPCHECK(c, hipMemcpy(s.d_off, h.off.data(), (nev + 1) * 8, hipMemcpyHostToDevice));
     s.ngroups = ngroups;
     s.grp_ev.assign(grp_ev, grp_ev + ngroups + 1);
     s.nev = nev;
@@ -1155,6 +1175,7 @@ int hbt_destroy(hbt_ctx *c) {
     for (auto &ev : c->ev)
         if (ev) hipEventDestroy(ev);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->h_stage) hipHostFree(c->h_stage);
     delete c;
     return HBT_OK;
 }

@@ -220,6 +220,66 @@ inline void parallel_ranges(long long n, long long min_per_thread, F fn) {
     for (auto &x : th) x.join();
 }
 
+// Host-to-device copies through a pinned staging buffer: a pageable copy
+// runs at a fraction of the link rate and cannot overlap host work.
+struct StagePiece {
+    void *dst;        // device destination
+    const void *src;  // host source
+    size_t bytes;
+};
+
+inline size_t stage_round(size_t b) { return (b + 255) / 256 * 256; }
+
+inline size_t stage_bytes(const std::vector<StagePiece> &v) {
+    size_t t = 0;
+    for (const StagePiece &x : v) t += stage_round(x.bytes);
+    return t;
+}
+
+// Grows a pinned buffer to at least `bytes` (by a quarter at least). The
+// caller has waited for every copy out of the old one.
+inline hipError_t ensure_pinned(char **buf, size_t *cap, size_t bytes) {
+    if (bytes <= *cap) return hipSuccess;
+    const size_t want = std::max(bytes, *cap + *cap / 4);
+    if (*buf) {
+        const hipError_t e = hipHostFree(*buf);
+        if (e != hipSuccess) return e;
+    }
+    *buf = nullptr;
+    *cap = 0;
+    const hipError_t e = hipHostMalloc((void **)buf, want, hipHostMallocDefault);
+    if (e == hipSuccess) *cap = want;
+    return e;
+}
+
+// Copies the pieces into stage (back to back, 256-byte aligned) in 1 MiB
+// chunks on up to 16 threads, then queues their copies to the device on st.
+inline hipError_t stage_pieces(char *stage, const std::vector<StagePiece> &v, hipStream_t st) {
+    const size_t chunk = 1 << 20;
+    std::vector<size_t> off(v.size());
+    std::vector<std::pair<size_t, size_t>> parts;  // (piece, first byte)
+    size_t t = 0;
+    for (size_t q = 0; q < v.size(); q++) {
+        off[q] = t;
+        t += stage_round(v[q].bytes);
+        for (size_t b = 0; b < v[q].bytes; b += chunk) parts.emplace_back(q, b);
+    }
+    parallel_ranges((long long)parts.size(), 4, [&](long long p0, long long p1) {
+        for (long long p = p0; p < p1; p++) {
+            const StagePiece &x = v[parts[p].first];
+            const size_t b = parts[p].second, len = std::min(chunk, x.bytes - b);
+            std::memcpy(stage + off[parts[p].first] + b, static_cast<const char *>(x.src) + b, len);
+        }
+    });
+    for (size_t q = 0; q < v.size(); q++) {
+        if (!v[q].bytes) continue;
+        const hipError_t e = hipMemcpyAsync(v[q].dst, stage + off[q], v[q].bytes,
+                                            hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                             const int32_t *grp_ev, int nev, const int64_t *ev_off,
                             const double *const soa[8], HostParticles &h, std::string &err) {

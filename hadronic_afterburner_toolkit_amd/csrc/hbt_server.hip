@@ -145,6 +145,8 @@ struct hbt_server {
     size_t defer_count_off = 0;  // byte offset of the deferred-pair count in a result
     WorkItem *d_items[2] = {};
     WorkItem *h_items[2] = {};
+    char *h_stage = nullptr;  // pinned staging of uploads (admission thread only)
+    size_t stage_cap = 0;
     hipEvent_t step_ev[2] = {};     // end of launch (ordering)
     hipEvent_t step_t0[2] = {};     // start of launch (timing)
     hipEvent_t step_t1[2] = {};     // end of launch (timing)
@@ -243,9 +245,9 @@ hipError_t upload_request(hbt_server *s, Request &r) {
     char *b = s->d_arena + r.arena_off;
     const long long nf = r.hp.nf;
     hipError_t e = launch_zero_words(b + r.o_bins, s->result_bytes / 8, s->s_upload);
+    std::vector<StagePiece> pieces;
     auto cp = [&](size_t off, const void *src, size_t bytes) {
-        if (e == hipSuccess && bytes)
-            e = hipMemcpyAsync(b + off, src, bytes, hipMemcpyHostToDevice, s->s_upload);
+        if (bytes) pieces.push_back(StagePiece{b + off, src, bytes});
     };
     const std::pair<const RecordSet *, size_t> sets[] = {{&r.hp.fast, 0}, {&r.hp.srt, 1}};
     for (const auto &x : sets) {
@@ -264,7 +266,10 @@ hipError_t upload_request(hbt_server *s, Request &r) {
     cp(r.o_win, r.wins.data(), r.wins.size() * sizeof(JWin));
     cp(r.o_cs, r.plan_cs.data(), r.plan_cs.size() * 8);
     cp(r.o_psi, r.hp.psi2.data(), r.hp.psi2.size() * 8);
-    cp(0, nullptr, 0);
+    // the previous request's copies out of the staging buffer have completed
+    // (its upload ended with a synchronization of this stream)
+    if (e == hipSuccess) e = ensure_pinned(&s->h_stage, &s->stage_cap, stage_bytes(pieces));
+    if (e == hipSuccess) e = stage_pieces(s->h_stage, pieces, s->s_upload);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_views + r.view, &r.v, sizeof(View), hipMemcpyHostToDevice,
                            s->s_upload);
@@ -565,6 +570,7 @@ void destroy_resources(hbt_server *s) {
         if (kv.second->done_ev) hipEventDestroy(kv.second->done_ev);
     for (char *p : s->h_results)
         if (p) hipHostFree(p);
+    if (s->h_stage) hipHostFree(s->h_stage);
     for (int b = 0; b < 2; b++) {
         if (s->h_items[b]) hipHostFree(s->h_items[b]);
         if (s->d_items[b]) hipFree(s->d_items[b]);

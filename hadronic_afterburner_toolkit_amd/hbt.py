@@ -181,6 +181,24 @@ class Results:
     cnt_den_ktphi: np.ndarray
 
 
+def pack_groups(bufs):
+    """Group offsets, event offsets and the 8 contiguous columns of several
+    buffers, as hbt_upload_groups takes them."""
+    grp = np.zeros(len(bufs) + 1, np.int32)
+    offs, cols = [np.zeros(1, np.int64)], [[] for _ in range(8)]
+    base_p = 0
+    for g, b in enumerate(bufs):
+        grp[g + 1] = grp[g] + b.nev
+        offs.append(np.asarray(b.ev_off[1:], np.int64) - int(b.ev_off[0]) + base_p)
+        base_p += b.n
+        a0 = int(b.ev_off[0])
+        for k, a in enumerate(b.soa()):
+            cols[k].append(np.asarray(a[a0:a0 + b.n], np.float64))
+    off = np.ascontiguousarray(np.concatenate(offs))
+    arrs = [np.ascontiguousarray(np.concatenate(c)) if c else np.zeros(0) for c in cols]
+    return grp, off, arrs
+
+
 class HBTCorrelation:
     """GPU HBT correlation accumulator with the reference's parameter keys."""
 
@@ -224,21 +242,14 @@ class HBTCorrelation:
 
     def upload_groups(self, bufs, slot: int = 0):
         """Makes several buffers (groups) resident at once (batched form)."""
-        grp = np.zeros(len(bufs) + 1, np.int32)
-        offs, cols = [np.zeros(1, np.int64)], [[] for _ in range(8)]
-        base_p = 0
-        for g, b in enumerate(bufs):
-            grp[g + 1] = grp[g] + b.nev
-            offs.append(np.asarray(b.ev_off[1:], np.int64) - int(b.ev_off[0]) + base_p)
-            base_p += b.n
-            a0 = int(b.ev_off[0])
-            for k, a in enumerate(b.soa()):
-                cols[k].append(np.asarray(a[a0:a0 + b.n], np.float64))
-        off = np.ascontiguousarray(np.concatenate(offs))
-        arrs = [np.ascontiguousarray(np.concatenate(c)) if c else np.zeros(0) for c in cols]
+        self.upload_packed(pack_groups(bufs), slot)
+
+    def upload_packed(self, packed, slot: int = 0):
+        """Uploads groups already packed by pack_groups (no Python-side copies)."""
+        grp, off, arrs = packed
         ptrs = (C.POINTER(C.c_double) * 8)(*[_f64p(a) for a in arrs])
         self._check(self._lib.hbt_upload_groups(
-            self.h, slot, len(bufs), grp.ctypes.data_as(C.POINTER(C.c_int32)), int(grp[-1]),
+            self.h, slot, len(grp) - 1, grp.ctypes.data_as(C.POINTER(C.c_int32)), int(grp[-1]),
             off.ctypes.data_as(C.POINTER(C.c_int64)), ptrs), "hbt_upload_groups")
 
     def prepare_batch(self, M_per_group, ids, cs):
