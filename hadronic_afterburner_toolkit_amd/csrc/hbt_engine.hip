@@ -402,6 +402,15 @@ int fail(hbt_ctx *c, int code, const std::string &msg) {
                         std::string(#call) + ": " + hipGetErrorString(e_));              \
     } while (0)
 
+// Host-side copies into the context's device buffers are issued on the
+// null stream, which does not wait for the context's (non-blocking) stream:
+// every entry point that rewrites such a buffer first waits for the work
+// queued there, e.g. a batch launched without hbt_sync.
+int quiesce(hbt_ctx *c) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return HBT_OK;
+}
+
 template <typename T>
 int dev_alloc(hbt_ctx *c, T **p, size_t n) {
     if (*p) hipFree(*p);
@@ -957,7 +966,7 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     int rc = gather_particles(c->D, c->cfg, ngroups, grp_ev, nev, ev_off, soa, h, err);
     if (rc) return fail(c, rc, err);
     // a batch launched without hbt_sync may still read the slot and the staging
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if ((rc = quiesce(c))) return rc;
     drop_graph(c);
     c->plan_valid = false;
     Slot &s = c->slot[slot];
@@ -1187,6 +1196,7 @@ int hbt_reset(hbt_ctx *c) {
 
 int hbt_set_psi_ref(hbt_ctx *c, double psi_ref) {
     if (!c) return HBT_ERR_ARG;
+    if (int rc = quiesce(c)) return rc;
     for (auto &p : c->psi) p = psi_ref;
     if (c->psi.empty()) c->psi.push_back(psi_ref);
     c->batch_ready = false;
@@ -1216,6 +1226,7 @@ int hbt_slot_info(hbt_ctx *c, int32_t slot, int32_t *nev, int64_t *n_kept) {
 int hbt_run_same(hbt_ctx *c) {
     if (!c) return HBT_ERR_ARG;
     int rc = check_slots(c);
+    if (!rc) rc = quiesce(c);
     if (rc) return rc;
     ItemList items;
     uint64_t es = 0;
@@ -1247,6 +1258,7 @@ int hbt_run_same(hbt_ctx *c) {
 int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const double *cos_sin) {
     if (!c) return HBT_ERR_ARG;
     int rc = check_slots(c);
+    if (!rc) rc = quiesce(c);
     if (rc) return rc;
     Slot &s = c->slot[0];
     if (s.nev == 0) return HBT_OK;
@@ -1286,6 +1298,7 @@ int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *par
                       const double *cos_sin) {
     if (!c || !M_per_group) return HBT_ERR_ARG;
     int rc = check_slots(c);
+    if (!rc) rc = quiesce(c);
     if (rc) return rc;
     Slot &s = c->slot[0];
     std::vector<int> Me(s.nev, 0);
@@ -1366,6 +1379,7 @@ int hbt_set_tuning(hbt_ctx *c, const hbt_tuning *t) {
         t->kchunk < 1 || t->kchunk > 64 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64)
         return fail(c, HBT_ERR_ARG, "tuning out of range");
     c->tune = *t;
+    if (int rc = quiesce(c)) return rc;
     if (c->batch_ready) return prepare_and_capture(c);
     return HBT_OK;
 }
@@ -1375,6 +1389,7 @@ int hbt_set_tuning(hbt_ctx *c, const hbt_tuning *t) {
 int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
     if (!c) return HBT_ERR_ARG;
     if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
+    if (int rc = quiesce(c)) return rc;
     if (c->cfg.kernel != HBT_KERNEL_FAST) {
         if (best_out) *best_out = c->tune;
         return HBT_OK;
