@@ -201,7 +201,11 @@ int hbt_upload_groups(hbt_ctx *ctx, int32_t slot, int32_t ngroups, const int32_t
 int hbt_prepare_batch(hbt_ctx *ctx, const int32_t *M_per_group, const int32_t *partner_ids,
                       const double *cos_sin);
 
-/* Enqueues the prepared batch on the context's stream (asynchronous). */
+/* Enqueues the prepared batch on the context's stream (asynchronous).
+ * Launches may be queued back to back.  An upload, hbt_prepare_batch,
+ * hbt_run_same/mixed, hbt_set_psi_ref or a tuning change issued before
+ * hbt_sync first waits for the queued launches, which therefore see the
+ * data they were launched on. */
 int hbt_launch_batch(hbt_ctx *ctx);
 
 /* Waits for the context's stream; *batch_ms (may be NULL) receives the
