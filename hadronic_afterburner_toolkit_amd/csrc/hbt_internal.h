@@ -20,6 +20,7 @@
 #include "../../include/hbt_hip.h"
 #include "hbt_device.h"
 #include "hbt_fast.h"
+#include "host_sort.h"
 
 namespace hbt {
 
@@ -414,12 +415,13 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     h.srt_px.resize((size_t)nf);
     parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
         std::vector<std::pair<double, int>> idx;  // (px, fast-list index)
+        std::vector<std::pair<uint64_t, int>> ra, rb;  // radix scratch
         for (long long g = ga; g < gb; g++) {
             const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
             idx.resize((size_t)(g1 - g0));
             for (long long q = g0; q < g1; q++) idx[q - g0] = {F.rec[q].px, (int)q};
-            // distinct pairs: the order equals a stable sort by px
-            std::sort(idx.begin(), idx.end());
+            // stable by px (fast-list order among equal px)
+            radix_sort_keys(idx, ra, rb);
             for (long long q = g0; q < g1; q++) {
                 const int src = idx[q - g0].second;
                 h.srt.copy_from(F, (size_t)q, (size_t)src, f32);
