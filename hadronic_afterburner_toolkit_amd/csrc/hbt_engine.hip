@@ -970,6 +970,7 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     drop_graph(c);
     c->plan_valid = false;
     Slot &s = c->slot[slot];
+    s.valid = false;  // until every copy below has succeeded
     const long long n = h.n, nf = h.nf;
     if ((size_t)n > s.cap || !s.d_soa[0]) {
         for (auto &p : s.d_soa) {
