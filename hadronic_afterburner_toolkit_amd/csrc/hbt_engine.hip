@@ -322,8 +322,10 @@ struct hbt_ctx {
     DevParams P;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
-    char *h_stage = nullptr;  // pinned staging of uploads (grown on demand)
+    char *h_stage = nullptr;   // pinned staging of uploads: columns (grown on demand)
     size_t stage_cap = 0;
+    char *h_stage2 = nullptr;  // and fast-kernel records
+    size_t stage2_cap = 0;
     int cu_count = 0;
     long nbins = 0, nq = 0;
     double *d_bins = nullptr;  // [nbins][BIN_STRIDE]
@@ -957,20 +959,18 @@ int prepare_and_capture(hbt_ctx *c) {
     return HBT_OK;
 }
 
-int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev, int32_t nev,
-                const int64_t *ev_off, const double *const soa[8]) {
-    if (slot != 0 && slot != 1) return fail(c, HBT_ERR_ARG, "slot must be 0 or 1");
-    if (ngroups < 1 || !grp_ev) return fail(c, HBT_ERR_ARG, "bad buffer");
-    HostParticles h;
-    std::string err;
-    int rc = gather_particles(c->D, c->cfg, ngroups, grp_ev, nev, ev_off, soa, h, err);
-    if (rc) return fail(c, rc, err);
-    // a batch launched without hbt_sync may still read the slot and the staging
-    if ((rc = quiesce(c))) return rc;
+// First half of an upload, called from the gather once the kept particles
+// are known: waits for queued work, invalidates the slot, sizes its device
+// buffers, gathers the kept columns into the pinned staging buffer on
+// several threads and queues their copies (and the event offsets') on the
+// context's stream.
+int upload_columns(hbt_ctx *c, Slot &s, bool f32, int32_t nev, const HostParticles &h,
+                   const double *const soa[8]) {
+    int rc = quiesce(c);
+    if (rc) return rc;
     drop_graph(c);
     c->plan_valid = false;
-    Slot &s = c->slot[slot];
-    s.valid = false;  // until every copy below has succeeded
+    s.valid = false;  // until every copy has succeeded
     const long long n = h.n, nf = h.nf;
     if ((size_t)n > s.cap || !s.d_soa[0]) {
         for (auto &p : s.d_soa) {
@@ -979,7 +979,6 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         }
         s.cap = (size_t)n;
     }
-    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;  // prefilter records of one precision
     if ((size_t)nf > s.cap_f || !s.d_orig) {
         for (DevRecords *r : {&s.ev, &s.srt}) {
             if (!rc && r == &s.ev) rc = dev_alloc(c, &r->rec, (size_t)nf);
@@ -997,9 +996,48 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         if (rc) return rc;
         s.cap_ev = (size_t)nev + 1;
     }
-    // Everything goes through the context's pinned staging buffer: the
-    // columns are gathered into it on several threads and copied while the
-    // records are staged behind them; all copies are on the context's stream.
+    const size_t col_bytes = stage_round((size_t)n * 8);
+    const size_t off_bytes = ((size_t)nev + 1) * 8;
+    HIPCHECK(c, ensure_pinned(&c->h_stage, &c->stage_cap, 8 * col_bytes + stage_round(off_bytes)));
+    char *stage = c->h_stage;
+    parallel_ranges(n, 1 << 15, [&](long long k0, long long k1) {
+        for (int a = 0; a < 8; a++) {
+            double *dst = reinterpret_cast<double *>(stage + a * col_bytes);
+            const double *src = soa[a];
+            for (long long k = k0; k < k1; k++) dst[k] = src[h.keep[k]];
+        }
+    });
+    for (int a = 0; a < 8 && n > 0; a++)
+        HIPCHECK(c, hipMemcpyAsync(s.d_soa[a], stage + a * col_bytes, (size_t)n * 8,
+                                   hipMemcpyHostToDevice, c->stream));
+    std::memcpy(stage + 8 * col_bytes, h.off.data(), off_bytes);
+    HIPCHECK(c, hipMemcpyAsync(s.d_off, stage + 8 * col_bytes, off_bytes, hipMemcpyHostToDevice,
+                               c->stream));
+    return HBT_OK;
+}
+
+int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev, int32_t nev,
+                const int64_t *ev_off, const double *const soa[8]) {
+    if (slot != 0 && slot != 1) return fail(c, HBT_ERR_ARG, "slot must be 0 or 1");
+    if (ngroups < 1 || !grp_ev) return fail(c, HBT_ERR_ARG, "bad buffer");
+    HostParticles h;
+    std::string err;
+    Slot &s = c->slot[slot];
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;  // prefilter records of one precision
+    // Once the kept particles are known: wait for work queued on the stream
+    // (a batch launched without hbt_sync may still read the slot and the
+    // staging), size the device buffers, and queue the columns' copies, which
+    // then run while the host sorts the fast lists by px.
+    int hook_rc = HBT_OK;
+    const std::function<int()> after_keep = [&]() -> int {
+        hook_rc = upload_columns(c, s, f32, nev, h, soa);
+        return hook_rc;
+    };
+    int rc = gather_particles(c->D, c->cfg, ngroups, grp_ev, nev, ev_off, soa, h, err, &after_keep);
+    if (hook_rc) return hook_rc;
+    if (rc) return fail(c, rc, err);
+    const long long nf = h.nf;
+    // the records, through the second pinned buffer
     std::vector<StagePiece> pieces;
     auto add = [&](void *dst, const void *src, size_t bytes) {
         if (dst && bytes) pieces.push_back(StagePiece{dst, src, bytes});
@@ -1021,21 +1059,8 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         }
         add(s.d_orig, h.orr.data(), (size_t)nf * sizeof(OrigRank));
     }
-    add(s.d_off, h.off.data(), ((size_t)nev + 1) * 8);
-    const size_t col_bytes = stage_round((size_t)n * 8);
-    HIPCHECK(c, ensure_pinned(&c->h_stage, &c->stage_cap, 8 * col_bytes + stage_bytes(pieces)));
-    char *stage = c->h_stage;
-    parallel_ranges(n, 1 << 15, [&](long long k0, long long k1) {
-        for (int a = 0; a < 8; a++) {
-            double *dst = reinterpret_cast<double *>(stage + a * col_bytes);
-            const double *src = soa[a];
-            for (long long k = k0; k < k1; k++) dst[k] = src[h.keep[k]];
-        }
-    });
-    for (int a = 0; a < 8 && n > 0; a++)
-        HIPCHECK(c, hipMemcpyAsync(s.d_soa[a], stage + a * col_bytes, (size_t)n * 8,
-                                   hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, stage_pieces(stage + 8 * col_bytes, pieces, c->stream));
+    HIPCHECK(c, ensure_pinned(&c->h_stage2, &c->stage2_cap, stage_bytes(pieces)));
+    HIPCHECK(c, stage_pieces(c->h_stage2, pieces, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     s.nf = nf;
     s.foff = std::move(h.foff);
@@ -1044,7 +1069,7 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     s.ngroups = ngroups;
     s.grp_ev.assign(grp_ev, grp_ev + ngroups + 1);
     s.nev = nev;
-    s.n = n;
+    s.n = h.n;
     s.n_in = h.n_in;
     s.off = h.off;
     s.pmax2 = h.pmax2;
@@ -1186,6 +1211,7 @@ int hbt_destroy(hbt_ctx *c) {
         if (ev) hipEventDestroy(ev);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->h_stage) hipHostFree(c->h_stage);
+    if (c->h_stage2) hipHostFree(c->h_stage2);
     delete c;
     return HBT_OK;
 }

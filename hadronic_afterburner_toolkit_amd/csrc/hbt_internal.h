@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <thread>
@@ -281,9 +282,13 @@ inline hipError_t stage_pieces(char *stage, const std::vector<StagePiece> &v, hi
     return hipSuccess;
 }
 
+// after_keep (optional) runs once the kept particles (keep, off) and the
+// list sizes (n, nf) are known, before the px sort; a non-zero return ends
+// the gather with that code.
 inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                             const int32_t *grp_ev, int nev, const int64_t *ev_off,
-                            const double *const soa[8], HostParticles &h, std::string &err) {
+                            const double *const soa[8], HostParticles &h, std::string &err,
+                            const std::function<int()> *after_keep = nullptr) {
     if (nev < 0 || ngroups < 0) return err = "negative size", HBT_ERR_ARG;
     if (nev > 0 && (!ev_off || !soa)) return err = "null particle arrays", HBT_ERR_ARG;
     for (int a = 0; a < 8 && nev > 0; a++)
@@ -392,6 +397,10 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
             }
         }
     });
+    if (after_keep) {
+        const int rc = (*after_keep)();
+        if (rc) return rc;
+    }
     // psi_ref per group over all particles passed in, in order (a sum per group)
     h.psi2.assign(ngroups, 0.0);
     if (P.azimuthal) {
