@@ -100,7 +100,10 @@ int hbt_reset(hbt_ctx *ctx);
  * nev+1 event offsets, applies the tanh(HBTrap) cut on pz/E, and makes the
  * kept particles resident in HBM.  slot 0 = same-event list, slot 1 = the
  * mixed-event list when read_in_real_mixed_events=1 (otherwise slot 1
- * aliases slot 0, particleSamples.cpp:528-531).  soa = {px,py,pz,E,x,y,z,t}. */
+ * aliases slot 0, particleSamples.cpp:528-531).  soa = {px,py,pz,E,x,y,z,t}.
+ * The copies go through pinned host buffers that the context keeps (sized
+ * to its largest upload) until hbt_destroy; the call returns once they are
+ * done, so the caller may reuse its arrays at once. */
 int hbt_upload(hbt_ctx *ctx, int32_t slot, int32_t nev, const int64_t *ev_off,
                const double *const soa[8]);
 
