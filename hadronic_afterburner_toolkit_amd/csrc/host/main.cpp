@@ -30,7 +30,7 @@ int main(int argc, char *argv[]) {
     const bool bf = paraRdr.getVal("analyze_balance_function", 0) == 1;
     if (!hbt && !bf) {
         std::cout << "neither analyze_HBT nor analyze_balance_function is 1: nothing to do "
-                     "(this build runs those two analyses)"
+                     "(this executable runs those two analyses)"
                   << std::endl;
         return 0;
     }

@@ -18,7 +18,7 @@ particleSamples::particleSamples(ParameterReader &paraRdr, std::string path,
     read_mixed_events = paraRdr_.getVal("read_in_real_mixed_events") == 1;
     if (read_in_mode != 10 && read_in_mode != 9) {
         std::cout << "particleSamples:: Error: read_in_mode " << read_in_mode
-                  << " is not supported by this build (modes 10, gzip text, and 9, binary)"
+                  << " is not supported (modes 10, gzip text, and 9, binary, are)"
                   << std::endl;
         exit(1);
     }

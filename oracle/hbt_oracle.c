@@ -2,7 +2,7 @@
  * hbt_oracle.c -- CPU oracle (test infrastructure only) for the HBT
  * pair-correlation path.  Plain C restatement of the reference; every
  * function cites the reference lines it follows.  Only tests/, the smoke
- * check and bench.py's CPU-baseline leg may load it, and only as the checker.
+ * check and bench.py's CPU-baseline leg may load it, and only to check results against.
  *
  * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math: strict
  * IEEE double, no contraction, like the reference's g++ -O3 on x86-64).

@@ -6,7 +6,7 @@
  * mode-10 sample reader (src/particleSamples.cpp), its random stream
  * (src/Random.{h,cpp}, libstdc++ 11 <random>) and its text writers.  It is
  * used by tests/, by __graft_entry__.smoke() and by bench.py's CPU-baseline
- * leg as the checker; the product (libhbt_hip.so) never links or calls it.
+ * leg to check results against; the product (libhbt_hip.so) never links or calls it.
  *
  * Pinning status of each function is recorded in the header comment of
  * hbt_oracle.c.

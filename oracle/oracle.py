@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
-this module, and only as the checker.  See hbt_oracle.c for what each
+this module, and only to check results against.  See hbt_oracle.c for what each
 function restates and how it is pinned.
 """
 from __future__ import annotations
