@@ -9,23 +9,30 @@
  *
  * Pinning (what checks each part against something other than itself):
  *  - orc_combine_same / orc_combine_mixed / orc_calculate_buffer:
- *      bit-exact against the reference's own HBT_correlation compiled from
- *      its sources (oracle/_ref, tests/test_oracle_vs_ref.py) on the
- *      reference fixture and on seeded synthetic buffers (counts, sums as
- *      hex floats); committed golden dumps (tests/golden/); hand-built
- *      edge pairs (window edges +-1 ulp, K_T edges, > vs >=); closed-form
- *      pair totals.
- *  - orc_flow_event_plane_angle: HBT_unittest.cc:39,43,47 golden psi_ref.
+ *      bit-exact (every count, every FP64 sum) against golden dumps of the
+ *      reference's own HBT_correlation compiled from its sources by
+ *      oracle/Makefile (tests/golden/make_golden.py; checked by
+ *      tests/test_oracle_golden.py): the reference fixture, seeded synthetic
+ *      buffers, binding caps, and hand-placed pairs exactly on the window
+ *      edges lo+1e-8 / hi-1e-8, one ulp around them, on interior bin edges,
+ *      at K_perp = KT_min / KT_max and on K_phi bin edges
+ *      (tests/golden/edge_inputs.py).  The mixed comparisons (>= on hi),
+ *      which the reference reaches only through random rotations, are pinned
+ *      pair by pair on the same designed pairs against the reference's
+ *      rules (tests/test_edge_known_answers.py).
+ *  - orc_flow_event_plane_angle: HBT_unittest.cc:39,43,47 golden psi_ref
+ *      (tests/test_oracle_golden.py).
  *  - orc_rng_* / orc_draw_mixing_plan: first draws from seed 0 against the
  *      values the reference's libstdc++ produces (1178568022,
- *      0.71518936510962539) and against libstdc++ itself
- *      (tests/test_oracle_rng.py compares with std::mt19937 +
- *      std::uniform_*_distribution compiled by g++); whole plans pinned
- *      indirectly through the mixed-event histograms of the reference runs.
+ *      0.71518936510962539) and draw for draw against std::mt19937 +
+ *      std::uniform_*_distribution compiled by g++ (tests/test_rng_plan.py);
+ *      whole plans pinned indirectly through the mixed-event histograms of
+ *      the reference runs.
  *  - orc_reader_*: particleSamples_unittest.cc:199-206 counts (619/635 pi+)
- *      on the reference fixture; golden comparison with the reference reader.
- *  - orc_write_outputs: byte comparison with files written by the reference
- *      binary on the same inputs.
+ *      on the reference fixture, and the reference's buffer sizes on every
+ *      golden case (tests/test_oracle_golden.py).
+ *  - orc_write_outputs: sha256 of every file the reference wrote on the
+ *      golden cases (tests/test_host_io.py).
  */
 #define _GNU_SOURCE
 #include "hbt_oracle.h"

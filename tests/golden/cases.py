@@ -21,7 +21,8 @@ FIXTURE_GZ = os.path.join(HERE, "ref_fixture", "particle_samples.gz")
 CASE_NAMES = ["fixture_3d", "fixture_q11", "fixture_kphi_qinv", "fixture_budget700",
               "fixture_qinv", "synth_cfg0",
               "synth_noboost", "synth_cap", "synth_mixedfile", "synth_budget_exact",
-              "fixture_cap_kphi_qinv", "synth_cap_qinv"]
+              "fixture_cap_kphi_qinv", "synth_cap_qinv", "edge_3d", "edge_kphi",
+              "synth_cap_kphi"]
 
 HBT_KEYS = ("qnpts", "q_min", "q_max", "n_KT", "KT_min", "KT_max", "n_Kphi",
             "azimuthal_flag", "invariant_radius_flag", "long_comoving_boost", "HBTrap_min",
@@ -66,6 +67,11 @@ def write_inputs(spec, digests, dirpath):
         _, seed, nev, mult = inp
         p = os.path.join(dirpath, "particle_samples.gz")
         samples.write_mode10(p, samples.generate(seed, nev, mult))
+        paths["same"] = p
+    elif inp[0] == "edges":
+        import edge_inputs
+        p = os.path.join(dirpath, "particle_samples.gz")
+        samples.write_mode10(p, edge_inputs.build(inp[1]))
         paths["same"] = p
     else:
         _, seed, seed2, nev, mult = inp

@@ -30,6 +30,9 @@ sys.path.insert(0, REPO)
 
 from hadronic_afterburner_toolkit_amd import samples  # noqa: E402
 
+sys.path.insert(0, HERE)
+import edge_inputs  # noqa: E402
+
 REF = "/root/reference"
 HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
 FIXTURE_GZ = os.path.join(HERE, "ref_fixture", "particle_samples.gz")
@@ -75,6 +78,25 @@ CASES = {
     "synth_budget_exact": dict(input=("synth", 1006, 6, 80.0), params=dict(
         randomSeed=19, qnpts=15, n_KT=3, KT_min=0.2, KT_max=0.6, HBTrap_min=-1.0,
         HBTrap_max=1.0, needed_number_of_pairs=1e18, event_buffer_size="total")),
+    # hand-placed pairs exactly on the window edges lo+1e-8 / hi-1e-8, one ulp
+    # around them, on interior q bin edges, at K_perp = KT_min and KT_max
+    # (edge_inputs.py); the bench's grid
+    "edge_3d": dict(input=("edges", "3d"), params=dict(
+        randomSeed=31, qnpts=41, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
+        HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18)),
+    # pair axes exactly on K_phi bin edges about psi_ref = 0, and one that
+    # wraps to 2 pi (skipped by the reference)
+    "edge_kphi": dict(input=("edges", "kphi"), params=dict(
+        randomSeed=37, qnpts=41, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
+        HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18, azimuthal_flag=1,
+        n_Kphi=4)),
+    # binding per-(K_T, K_phi) caps on a larger sample: with a wide K_phi
+    # guard band the engine decides many pairs on the host, which must be
+    # admitted in the reference's enumeration order
+    "synth_cap_kphi": dict(input=("synth", 1008, 20, 120.0), params=dict(
+        randomSeed=41, qnpts=15, n_KT=3, KT_min=0.2, KT_max=0.6, HBTrap_min=-1.0,
+        HBTrap_max=1.0, azimuthal_flag=1, n_Kphi=6, needed_number_of_pairs=300,
+        event_buffer_size=700)),
 }
 
 BASE = dict(read_in_mode=10, particle_monval=211, analyze_flow=0, analyze_HBT=1,
@@ -142,6 +164,8 @@ def run_case(name, spec, keep_dir=None):
         samples.write_mode10(os.path.join(res, "particle_samples.gz"), buf)
         if params.get("event_buffer_size") == "total":
             params["event_buffer_size"] = int(buf.n)
+    elif inp[0] == "edges":
+        samples.write_mode10(os.path.join(res, "particle_samples.gz"), edge_inputs.build(inp[1]))
     elif inp[0] == "synth2":
         _, seed, seed2, nev, mult = inp
         samples.write_mode10(os.path.join(res, "particle_samples.gz"),

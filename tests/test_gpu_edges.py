@@ -111,3 +111,56 @@ def test_edge_pairs_match_oracle(mode, boost):
     cnt = ref["num_count"].astype(np.float64)
     for key, fl in SUMS:
         assert rel(getattr(r, key), ref[key], ref[fl], cnt, tol, abs_per_pair) <= tol, key
+
+
+EXACT_MODES = {"plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12, 0.0),
+               "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 0.0),
+               "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 1e-14)}
+
+
+@pytest.mark.parametrize("kind", ["3d", "kphi"])
+@pytest.mark.parametrize("mode", list(EXACT_MODES))
+def test_exact_edges_same_and_mixed(mode, kind):
+    """Pairs placed exactly on lo+1e-8 and hi-1e-8 (and one ulp around them),
+    on interior q bin edges, at K_perp = KT_min / KT_max, and (kphi) with the
+    pair axis exactly on a K_phi bin edge or wrapping to 2 pi
+    (tests/golden/edge_inputs.py).  Same-event pairs as the reference forms
+    them; mixed pairs with the explicit rotation (cos, sin) = (1, 0), under
+    which each designed pair is formed again and decided by the mixed
+    comparisons (>= on hi).  Counts bit-exact with the oracle, sums within the
+    mode's bar."""
+    import edge_inputs as E
+    prec, kern, tol, abs_per_pair = EXACT_MODES[mode]
+    params = dict(E.EDGE_PARAMS if kind == "3d" else E.KPHI_PARAMS)
+    buf = E.build(kind)
+    print(f"exact edges {kind}: {buf.n} particles, {mode}", flush=True)
+    ids = np.array([1, 0], np.int32)
+    cs = np.array([1.0, 0.0, 1.0, 0.0])
+    g = hbt.HBTCorrelation(params, precision=prec, kernel=kern)
+    g.upload(buf)
+    g.run_same()
+    g.run_mixed(1, ids, cs)
+    r = g.results()
+    deferred = g.deferred_pairs()
+    g.close()
+    o = oracle.Oracle(params)
+    if kind == "kphi":
+        psi = oracle.psi_ref(buf, 2)
+        assert psi == 0.0  # mirror images cancel sin(2 phi) exactly
+        o.set_psi_ref(psi)
+    o.combine_same(buf)
+    o.run_mixed_plan(buf, 1, ids, cs)
+    ref = o.arrays()
+    assert ref["num_count"].sum() > 0 and ref["den"].sum() > 0
+    assert np.array_equal(r.num_count, ref["num_count"].astype(np.uint64))
+    assert np.array_equal(r.den, ref["den"].astype(np.uint64))
+    assert np.array_equal(r.cnt_num_kt, ref["cnt_num_kt"])
+    assert np.array_equal(r.cnt_den_kt, ref["cnt_den_kt"])
+    if kind == "kphi":
+        assert np.array_equal(r.cnt_num_ktphi, ref["cnt_num_ktphi"])
+        assert np.array_equal(r.cnt_den_ktphi, ref["cnt_den_ktphi"])
+        # pair axes on bin edges are decided on the host
+        assert deferred > 0
+    cnt = ref["num_count"].astype(np.float64)
+    for key, fl in SUMS:
+        assert rel(getattr(r, key), ref[key], ref[fl], cnt, tol, abs_per_pair) <= tol, key
