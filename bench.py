@@ -86,6 +86,7 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None
         g.set_tuning(*tuning)
     tune = g.autotune() if autotune else g.tuning()
     es, em, nitems = g.batch_info()
+    graph = g.batch_graph()
     for _ in range(warmup):
         g.launch_batch()
     g.sync()
@@ -99,7 +100,7 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None
     log(f"  prec={prec} kernel={kern} steps={steps} wall={wall:.3f}s "
         f"dev_ms/step={np.mean(dev_ms):.2f} tune={tune}")
     return g, dict(wall=wall, dev_ms=dev_ms, enum_same=es, enum_mixed=em, nitems=nitems,
-                   tune=tune, upload_s=upload_s, upload_warm_s=upload_warm_s,
+                   tune=tune, graph=graph, upload_s=upload_s, upload_warm_s=upload_warm_s,
                    prepare_warm_s=prepare_warm_s)
 
 
@@ -620,6 +621,7 @@ def main():
         out[f"{name}_upload_s"] = res[name]["upload_s"]
         out[f"{name}_upload_warm_s"] = res[name]["upload_warm_s"]
         out[f"{name}_prepare_warm_s"] = res[name]["prepare_warm_s"]
+        out[f"{name}_batch_graph"] = res[name]["graph"]
     if "fp32c" in res and "plain" in res:
         out["speedup_vs_plain_fp64"] = out["fp32c_pairs_per_s"] / out["plain_pairs_per_s"]
     if "fp32c" in res and "fp64" in res:

@@ -18,6 +18,8 @@ HBT_KERNEL_FAST = 1
 
 ERRORS = {0: "HBT_OK", -1: "HBT_ERR_ARG", -2: "HBT_ERR_HIP", -3: "HBT_ERR_NOMEM",
           -4: "HBT_ERR_STATE", -5: "HBT_ERR_CAPACITY", -6: "HBT_ERR_NODEVICE"}
+HBT_OK, HBT_ERR_ARG, HBT_ERR_HIP, HBT_ERR_NOMEM = 0, -1, -2, -3
+HBT_ERR_STATE, HBT_ERR_CAPACITY, HBT_ERR_NODEVICE = -4, -5, -6
 
 
 class HbtConfig(C.Structure):
@@ -128,6 +130,7 @@ SIGNATURES = [
     ("hbt_launch_batch", C.c_int, [C.c_void_p]),
     ("hbt_sync", C.c_int, [C.c_void_p, _F64P]),
     ("hbt_batch_info", C.c_int, [C.c_void_p, _U64P, _U64P, C.POINTER(C.c_int32)]),
+    ("hbt_batch_graph", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     ("hbt_get_tuning", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
     ("hbt_set_tuning", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
     ("hbt_autotune", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),

@@ -20,6 +20,10 @@
 //   pos    per key: the position, in that row's j order, of the last
 //          admitted arrival
 // and the binning pass admits (row, pos) <= (cut_row, cut_pos) per key.
+// A pair whose K_phi angle lies within the guard band of a bin edge is
+// recorded by the count pass with its (row, pos); the host names its key
+// with glibc's atan2 and adds its arrival before the scan, the pos pass
+// finds its key in a sorted table, and the host bins it if admitted.
 #pragma once
 #include "hbt_device.h"
 
@@ -39,6 +43,31 @@ struct CapView {
     const CapCut *cutq;    // [n_KT]
     long long nrows;
 };
+
+// A pair whose K_phi angle lies within the guard band of a bin edge: the
+// device cannot name its 3D key, the host decides it with glibc's atan2 as
+// the reference does.  Sorted by (row, pos) for the position pass.
+struct DeferKey {
+    long long row, pos;
+    int key;  // (K_T, K_phi) key, or -1: the reference skips the pair
+    int pad;
+};
+
+// Key of the deferred pair at (row, pos), -1 if none (binary search).
+__device__ __forceinline__ int defer_key(const DeferKey *dk, int n, long long row,
+                                         long long pos) {
+    int lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const DeferKey &d = dk[mid];
+        if (d.row == row && d.pos == pos) return d.key;
+        if (d.row < row || (d.row == row && d.pos < pos))
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return -1;
+}
 
 __device__ __forceinline__ int key3_of(const DevParams &P, const PairBins &b) {
     return b.kt * P.nphi + b.kphi;
@@ -64,14 +93,19 @@ __device__ __forceinline__ int cap_same_pair(const DevParams &P, const DevHist &
     const int st = cascade_f64<false, true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi, b,
                                             q_inv, qinv_ok);
     const bool bin = (st & ST_BIN) != 0;
-    // a pair on a K_phi bin edge cannot be placed in the cap's order here:
-    // counted, so the host reports it
-    if (st & ST_DEFER) atomicAdd(H.ndefer, 1u);
     if (COUNT) {
+        // a pair on a K_phi bin edge: recorded with its place in the order;
+        // the host names its key and adds its arrival before the scan
+        if (st & ST_DEFER) {
+            const double a[8] = {p1x, p1y, p1z, p1E, x1, y1, z1, t1};
+            const double c2[8] = {p2x, p2y, p2z, p2E, x2, y2, z2, t2};
+            push_defer(H, a, c2, psi, 0, gi, pos);
+        }
         if (bin) atomicAdd(&C.cnt3[(long long)key3_of(P, b) * C.nrows + gi], 1u);
         if (qinv_ok) atomicAdd(&C.cntq[(long long)b.kt * C.nrows + gi], 1u);
         return st;
     }
+    // (deferred pairs are binned by the host once the cuts are known)
     const bool take3 = bin && admitted(C.cut3[key3_of(P, b)], gi, pos);
     const bool takeq = qinv_ok && admitted(C.cutq[b.kt], gi, pos);
     if (!take3 && !takeq) return st & ~ST_BIN;
@@ -107,10 +141,12 @@ __device__ __forceinline__ int cap_mixed_pair(const DevParams &P, const DevHist 
     const int st = cascade_f64<true, true>(P, p1x, p1y, p1z, p1E, p2x, p2y, p2z, p2E, psi, b,
                                            q_inv, qinv_ok);
     const bool bin = (st & ST_BIN) != 0;
-    // a pair on a K_phi bin edge cannot be placed in the cap's order here:
-    // counted, so the host reports it
-    if (st & ST_DEFER) atomicAdd(H.ndefer, 1u);
     if (COUNT) {
+        if (st & ST_DEFER) {  // as for same-event pairs
+            const double a[8] = {p1x, p1y, p1z, p1E, 0, 0, 0, 0};
+            const double c2[8] = {p2x, p2y, p2z, p2E, 0, 0, 0, 0};
+            push_defer(H, a, c2, psi, 1, gi, pos);
+        }
         if (bin) atomicAdd(&C.cnt3[(long long)key3_of(P, b) * C.nrows + gi], 1u);
         if (qinv_ok) atomicAdd(&C.cntq[(long long)b.kt * C.nrows + gi], 1u);
         return st;
@@ -204,7 +240,8 @@ __device__ __forceinline__ int segment_of(const long long *off, int n, long long
 // the position is the particle index j.  gpoff: particle offsets of the
 // groups (ngroups+1).
 __global__ void k_cap_pos_same(DevParams P, DevSoA A, const long long *gpoff, int ngroups,
-                               const double *psi, CapCut *cut, int nkeys, bool qinv_keys) {
+                               const double *psi, CapCut *cut, int nkeys, bool qinv_keys,
+                               const DeferKey *dk, int ndk) {
     const int key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= nkeys) return;
     CapCut c = cut[key];
@@ -225,7 +262,8 @@ __global__ void k_cap_pos_same(DevParams P, DevSoA A, const long long *gpoff, in
         const int st = cascade_f64<false, true>(P, p1x, p1y, p1z, p1E, A.px[j], A.py[j],
                                                 A.pz[j], A.E[j], psi[g], b, q_inv, qinv_ok);
         const bool hit = qinv_keys ? (qinv_ok && b.kt == key)
-                                   : ((st & ST_BIN) && key3_of(P, b) == key);
+                                   : (((st & ST_BIN) && key3_of(P, b) == key) ||
+                                      ((st & ST_DEFER) && defer_key(dk, ndk, i, j) == key));
         if (hit && --want == 0) {
             cut[key].pos = j;
             return;
@@ -241,7 +279,8 @@ __global__ void k_cap_pos_same(DevParams P, DevSoA A, const long long *gpoff, in
 __global__ void k_cap_pos_mixed(DevParams P, DevSoA A, DevSoA B, const long long *s_off, int nev,
                                 const long long *b_off, const long long *rowoff, const int *ids,
                                 const double *cs, const long long *gpoff, int ngroups,
-                                const double *psi, CapCut *cut, int nkeys, bool qinv_keys) {
+                                const double *psi, CapCut *cut, int nkeys, bool qinv_keys,
+                                const DeferKey *dk, int ndk) {
     const int key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= nkeys) return;
     CapCut c = cut[key];
@@ -267,10 +306,12 @@ __global__ void k_cap_pos_mixed(DevParams P, DevSoA A, DevSoA B, const long long
             const int st = cascade_f64<true, true>(P, p1x, p1y, p1z, p1E, px * co - py * sn,
                                                    px * sn + py * co, B.pz[j], B.E[j], psi[g], b,
                                                    q_inv, qinv_ok);
+            const long long pos = (k << 32) | (j - b_off[pid]);
             const bool hit = qinv_keys ? (qinv_ok && b.kt == key)
-                                       : ((st & ST_BIN) && key3_of(P, b) == key);
+                                       : (((st & ST_BIN) && key3_of(P, b) == key) ||
+                                          ((st & ST_DEFER) && defer_key(dk, ndk, i, pos) == key));
             if (hit && --want == 0) {
-                cut[key].pos = (k << 32) | (j - b_off[pid]);
+                cut[key].pos = pos;
                 return;
             }
         }

@@ -48,6 +48,10 @@ struct DeferRec {
     double *bins;
     int kind;  // 0 same-event, 1 mixed
     int pad;
+    // the pair's place in the reference's enumeration order, recorded by the
+    // ordered pair-cap path (row: particle of the same-event list; pos: j, or
+    // partner << 32 | j for mixed pairs); -1 elsewhere
+    long long row, pos;
 };
 
 struct DevHist {
@@ -219,7 +223,8 @@ __host__ __device__ __forceinline__ int cascade_f64(const DevParams &P, double p
 // Records a pair for the host (a count beyond defer_cap keeps counting, so
 // the host sees the overflow).
 __device__ __forceinline__ void push_defer(const DevHist &H, const double a[8], const double c[8],
-                                           double psi, int kind) {
+                                           double psi, int kind, long long row = -1,
+                                           long long pos = -1) {
     const unsigned slot = atomicAdd(H.ndefer, 1u);
     if (slot >= H.defer_cap) return;
     DeferRec &d = H.defer[slot];
@@ -231,6 +236,8 @@ __device__ __forceinline__ void push_defer(const DevHist &H, const double a[8], 
     d.bins = H.bins;
     d.kind = kind;
     d.pad = 0;
+    d.row = row;
+    d.pos = pos;
 }
 
 // One binned same-event pair: bin and the four FP64 sums it adds to.

@@ -37,64 +37,19 @@
 #include "hbt_cap.h"
 #include "hbt_fast.h"
 #include "hbt_internal.h"
+#include "hbt_plain.h"
 
 using namespace hbt;
 
 namespace {
 
-constexpr int TILE = 256;
-
 // ------------------------------------------------------------------ kernels
-
-__device__ __forceinline__ void wave_add_stage(unsigned long long *dst, unsigned v) {
-    unsigned long long x = v;
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (__lane_id() == 0 && x) atomic_add_u64(dst, x);
-}
 
 // Same-event pairs i<j of one group [g0, g0+n): block (tj, ti) of 256x256
 // tiles, j tile staged in LDS, one thread per i (HBT_correlation.cpp:291-460).
 __global__ __launch_bounds__(TILE) void k_same_plain(DevParams P, DevHist H, DevSoA A,
                                                      long long g0, int n, double psi) {
-    const int ti = blockIdx.y, tj = blockIdx.x;
-    if (tj < ti) return;
-    __shared__ double s[8][TILE];
-    const int j0 = tj * TILE;
-    const int jn = min(TILE, n - j0);
-    const int tid = threadIdx.x;
-    if (tid < jn) {
-        const long long j = g0 + j0 + tid;
-        s[0][tid] = A.px[j];
-        s[1][tid] = A.py[j];
-        s[2][tid] = A.pz[j];
-        s[3][tid] = A.E[j];
-        s[4][tid] = A.x[j];
-        s[5][tid] = A.y[j];
-        s[6][tid] = A.z[j];
-        s[7][tid] = A.t[j];
-    }
-    __syncthreads();
-    const int i = ti * TILE + tid;
-    unsigned c_kt = 0, c_out = 0, c_side = 0, c_bin = 0;
-    if (i < n) {
-        const long long gi = g0 + i;
-        const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
-        const double x1 = A.x[gi], y1 = A.y[gi], z1 = A.z[gi], t1 = A.t[gi];
-        const int jstart = (ti == tj) ? tid + 1 : 0;
-        for (int jj = jstart; jj < jn; jj++) {
-            const int st = bin_same_f64(P, H, psi, p1x, p1y, p1z, p1E, x1, y1, z1, t1, s[0][jj],
-                                        s[1][jj], s[2][jj], s[3][jj], s[4][jj], s[5][jj],
-                                        s[6][jj], s[7][jj]);
-            c_kt += st & 1;
-            c_out += (st >> 1) & 1;
-            c_side += (st >> 2) & 1;
-            c_bin += (st >> 3) & 1;
-        }
-    }
-    wave_add_stage(&H.stage[1], c_kt);
-    wave_add_stage(&H.stage[2], c_out);
-    wave_add_stage(&H.stage[3], c_side);
-    wave_add_stage(&H.stage[4], c_bin);
+    same_plain_tile(P, H, A, g0, n, psi, blockIdx.y, blockIdx.x);
 }
 
 // Mixed pairs of the events [e0, e0 + nz/M) of one group whose plan has M
@@ -106,47 +61,8 @@ __global__ __launch_bounds__(TILE) void k_mixed_plain(DevParams P, DevHist H, De
                                                       const long long *b_off, const int *ids,
                                                       const double *cs, int M, int e0,
                                                       long long row0, double psi) {
-    const long long row = row0 + blockIdx.z;
-    const int e = e0 + (int)(blockIdx.z / M);
-    const long long ib = a_off[e];
-    const int n1 = (int)(a_off[e + 1] - ib);
-    const int pid = ids[row];
-    const long long jb = b_off[pid];
-    const int n2 = (int)(b_off[pid + 1] - jb);
-    const int ti = blockIdx.y, tj = blockIdx.x;
-    if (ti * TILE >= n1 || tj * TILE >= n2) return;
-    const double c = cs[2 * row], sn = cs[2 * row + 1];
-    __shared__ double s[4][TILE];
-    const int j0 = tj * TILE;
-    const int jn = min(TILE, n2 - j0);
-    const int tid = threadIdx.x;
-    if (tid < jn) {
-        const long long j = jb + j0 + tid;
-        const double px = B.px[j], py = B.py[j];
-        s[0][tid] = px * c - py * sn;
-        s[1][tid] = px * sn + py * c;
-        s[2][tid] = B.pz[j];
-        s[3][tid] = B.E[j];
-    }
-    __syncthreads();
-    const int i = ti * TILE + tid;
-    unsigned c_kt = 0, c_out = 0, c_side = 0, c_bin = 0;
-    if (i < n1) {
-        const long long gi = ib + i;
-        const double p1x = A.px[gi], p1y = A.py[gi], p1z = A.pz[gi], p1E = A.E[gi];
-        for (int jj = 0; jj < jn; jj++) {
-            const int st = bin_mixed_f64(P, H, psi, p1x, p1y, p1z, p1E, s[0][jj], s[1][jj],
-                                         s[2][jj], s[3][jj]);
-            c_kt += st & 1;
-            c_out += (st >> 1) & 1;
-            c_side += (st >> 2) & 1;
-            c_bin += (st >> 3) & 1;
-        }
-    }
-    wave_add_stage(&H.stage[6], c_kt);
-    wave_add_stage(&H.stage[7], c_out);
-    wave_add_stage(&H.stage[8], c_side);
-    wave_add_stage(&H.stage[9], c_bin);
+    mixed_plain_tile(P, H, A, B, a_off, b_off, ids, cs, M, e0, row0, psi, blockIdx.y,
+                     blockIdx.x, blockIdx.z);
 }
 
 // Pair-cap passes (hbt_cap.h) with the plain kernels' tiling: COUNT adds
@@ -238,6 +154,12 @@ __global__ __launch_bounds__(TILE) void k_mixed_cap(DevParams P, DevHist H, DevS
 }
 
 __global__ void k_zero_int(int *p) { *p = 0; }
+
+// Arrivals of host-decided pairs (K_phi at a bin edge) at the 3D cap.
+__global__ void k_cap_add_arrivals(unsigned *cnt, const long long *idx, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[idx[i]], 1u);
+}
 
 // Adds the host-decided pairs to their histograms (a handful per call).
 __global__ void k_apply_defer(const DeferAdd *a, int n) {
@@ -366,6 +288,9 @@ struct hbt_ctx {
     size_t cut_cap = 0, lim_cap = 0;
     long long *d_gpoff = nullptr;  // group particle offsets of slot 0
     size_t gpoff_cap = 0;
+    DeferKey *d_dkey = nullptr;    // deferred pairs of a capped call, by (row, pos)
+    long long *d_darr = nullptr;   // and their arrivals (key * nrows + row)
+    size_t dkey_cap = 0, darr_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool batch_ready = false;
@@ -498,14 +423,12 @@ int zero_hist(hbt_ctx *c) {
 // build), as the reference does, and adds each pair it bins to the device
 // histogram.  Called after the stream is idle, before results are read.  A
 // capped run cannot place such a pair in the cap's order and fails instead.
-int resolve_deferred(hbt_ctx *c, bool capped) {
+int resolve_deferred(hbt_ctx *c) {
     if (!c->P.azimuthal) return HBT_OK;  // only K_phi bins defer pairs
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     unsigned n = 0;
     HIPCHECK(c, hipMemcpy(&n, c->d_ndefer, sizeof(n), hipMemcpyDeviceToHost));
     if (n == 0) return HBT_OK;
-    if (capped)
-        return fail(c, HBT_ERR_STATE,
-                    "a pair at a K_phi bin edge under a binding needed_number_of_pairs");
     if (n > kDeferCap)
         return fail(c, HBT_ERR_CAPACITY, "more K_phi bin-edge pairs than the deferral buffer");
     std::vector<DeferRec> rec(n);
@@ -708,16 +631,21 @@ int upload_plan(hbt_ctx *c, const std::vector<int> &M_per_event, const int32_t *
                 const double *cs, uint64_t &enum_mixed) {
     Slot &s = c->slot[0];
     const Slot &m = mixed_slot(c);
-    c->rowoff.assign(s.nev + 1, 0);
-    for (int e = 0; e < s.nev; e++) c->rowoff[e + 1] = c->rowoff[e] + M_per_event[e];
-    const long long rows = c->rowoff[s.nev];
+    // the previous plan (and a batch built on it) is gone from here on; the
+    // new one is committed only once every partner id has been checked and
+    // copied, so a failed call leaves no plan at all
+    drop_graph(c);
+    c->plan_valid = false;
+    std::vector<long long> rowoff(s.nev + 1, 0);
+    for (int e = 0; e < s.nev; e++) rowoff[e + 1] = rowoff[e] + M_per_event[e];
+    const long long rows = rowoff[s.nev];
     std::vector<int> gids((size_t)std::max<long long>(rows, 1));
     enum_mixed = 0;
     for (int g = 0; g < s.ngroups; g++) {
         const int mb = m.grp_ev[g], mn = m.grp_ev[g + 1] - m.grp_ev[g];
         for (int e = s.grp_ev[g]; e < s.grp_ev[g + 1]; e++) {
             const long long n1 = s.off[e + 1] - s.off[e];
-            for (long long r = c->rowoff[e]; r < c->rowoff[e + 1]; r++) {
+            for (long long r = rowoff[e]; r < rowoff[e + 1]; r++) {
                 const int p = ids[r];
                 if (p < 0 || p >= mn) return fail(c, HBT_ERR_ARG, "partner id out of range");
                 gids[r] = mb + p;
@@ -735,10 +663,11 @@ int upload_plan(hbt_ctx *c, const std::vector<int> &M_per_event, const int32_t *
     int rc = ensure(c, &c->d_rowoff, c->rowoff_cap, (size_t)s.nev + 1);
     if (rc) return rc;
     HIPCHECK(c, hipMemcpy(c->d_ids, gids.data(), rows * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_cs, cs, rows * 16, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->d_rowoff, rowoff.data(), (s.nev + 1) * 8, hipMemcpyHostToDevice));
     gids.resize((size_t)rows);
     c->plan_gids.swap(gids);
-    HIPCHECK(c, hipMemcpy(c->d_cs, cs, rows * 16, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->d_rowoff, c->rowoff.data(), (s.nev + 1) * 8, hipMemcpyHostToDevice));
+    c->rowoff.swap(rowoff);
     c->plan_valid = true;
     c->plan_enum_mixed = enum_mixed;
     return HBT_OK;
@@ -820,9 +749,101 @@ int run_plain(hbt_ctx *c, bool same, bool mixed) {
 // The ordered path for one side (same-event or mixed) of the current call:
 // arrivals per (key, row), the cut per key, then binning of the admitted
 // pairs (hbt_cap.h).  FP64 with the literal cascade in every precision.
+// Deferred pairs of a capped call (K_phi within the guard band of a bin
+// edge): the count pass recorded each with its (row, pos).  The host runs
+// the cascade with glibc's atan2 (as the reference computes K_phi) to name
+// the pair's (K_T, K_phi) key, or none when the reference skips it; adds the
+// arrivals; and uploads the keys sorted by (row, pos) for the pos pass.
+struct CapDefer {
+    std::vector<DeferRec> rec;
+    std::vector<DeferKey> keys;   // sorted by (row, pos)
+    std::vector<DeferAdd> adds;   // per record, what it adds if admitted
+    std::vector<int> key_of;      // per record, its key or -1
+};
+
+int cap_defer_keys(hbt_ctx *c, long long nrows, CapDefer &dk) {
+    dk = CapDefer();
+    if (!c->P.azimuthal) return HBT_OK;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    unsigned n = 0;
+    HIPCHECK(c, hipMemcpy(&n, c->d_ndefer, sizeof(n), hipMemcpyDeviceToHost));
+    if (n == 0) return HBT_OK;
+    if (n > kDeferCap)
+        return fail(c, HBT_ERR_CAPACITY, "more K_phi bin-edge pairs than the deferral buffer");
+    dk.rec.resize(n);
+    HIPCHECK(c, hipMemcpy(dk.rec.data(), c->d_defer, n * sizeof(DeferRec), hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemset(c->d_ndefer, 0, sizeof(unsigned)));
+    c->deferred_total += n;
+    std::vector<long long> arr;
+    dk.key_of.assign(n, -1);
+    dk.adds.resize(n);
+    for (unsigned i = 0; i < n; i++) {
+        std::vector<DeferAdd> one;
+        resolve_records(c->P, &dk.rec[i], 1, one);
+        if (one.empty()) continue;  // the reference skips it (K_phi index n_Kphi)
+        const long per = (long)c->P.qnpts * c->P.qnpts * c->P.qnpts;
+        const int key = (int)(one[0].k / per);  // bins are [kt][kphi][q^3]
+        dk.key_of[i] = key;
+        dk.adds[i] = one[0];
+        arr.push_back((long long)key * nrows + dk.rec[i].row);
+    }
+    for (unsigned i = 0; i < n; i++)
+        dk.keys.push_back(DeferKey{dk.rec[i].row, dk.rec[i].pos, dk.key_of[i], 0});
+    std::sort(dk.keys.begin(), dk.keys.end(), [](const DeferKey &a, const DeferKey &b) {
+        return a.row < b.row || (a.row == b.row && a.pos < b.pos);
+    });
+    int rc = ensure(c, &c->d_dkey, c->dkey_cap, dk.keys.size());
+    if (!rc) rc = ensure(c, &c->d_darr, c->darr_cap, std::max<size_t>(1, arr.size()));
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpy(c->d_dkey, dk.keys.data(), dk.keys.size() * sizeof(DeferKey),
+                          hipMemcpyHostToDevice));
+    if (!arr.empty()) {
+        HIPCHECK(c, hipMemcpy(c->d_darr, arr.data(), arr.size() * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_cap_add_arrivals, dim3((unsigned)((arr.size() + 255) / 256)),
+                           dim3(256), 0, c->stream, c->d_cnt, (const long long *)c->d_darr,
+                           (int)arr.size());
+        HIPCHECK(c, hipGetLastError());
+    }
+    return HBT_OK;
+}
+
+// After the binning pass: each deferred pair whose key's cut admits its
+// (row, pos) is added to its bin (HBT_correlation.cpp:402-429, :651-678).
+int cap_defer_bin(hbt_ctx *c, int nkey3, const CapDefer &dk) {
+    if (dk.rec.empty()) return HBT_OK;
+    std::vector<CapCut> cut(nkey3);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(cut.data(), c->d_cut, nkey3 * sizeof(CapCut), hipMemcpyDeviceToHost));
+    std::vector<DeferAdd> adds;
+    for (size_t i = 0; i < dk.rec.size(); i++) {
+        const int key = dk.key_of[i];
+        if (key < 0) continue;
+        const CapCut &ct = cut[key];
+        const long long row = dk.rec[i].row, pos = dk.rec[i].pos;
+        if (row < ct.row || (row == ct.row && pos <= ct.pos)) adds.push_back(dk.adds[i]);
+    }
+    if (adds.empty()) return HBT_OK;
+    DeferAdd *d = nullptr;
+    HIPCHECK(c, hipMalloc((void **)&d, adds.size() * sizeof(DeferAdd)));
+    hipError_t e = hipMemcpy(d, adds.data(), adds.size() * sizeof(DeferAdd), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_apply_defer, dim3((unsigned)((adds.size() + 255) / 256)), dim3(256),
+                           0, c->stream, d, (int)adds.size());
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    HIPCHECK(c, e);
+    return HBT_OK;
+}
+
 int run_capped(hbt_ctx *c, bool mixed) {
     const Slot &s = c->slot[0];
     const Slot &m = mixed_slot(c);
+    CapDefer dk;
+    // pairs deferred by an uncapped pass queued before this one are binned
+    // first, so the records the count pass leaves are this pass's own
+    if (int rc0 = resolve_deferred(c)) return rc0;
     const long long nrows = std::max(1ll, s.n);
     const int nkey3 = c->cfg.n_KT * c->P.nphi, nkq = c->cfg.n_KT, nkeys = nkey3 + nkq;
     int rc = ensure(c, &c->d_cnt, c->cnt_cap, (size_t)nkeys * nrows);
@@ -901,6 +922,11 @@ int run_capped(hbt_ctx *c, bool mixed) {
             }
         }
         if (!count) break;
+        // pairs on a K_phi bin edge: their keys from the host's cascade, their
+        // arrivals added before the scan
+        rc = cap_defer_keys(c, nrows, dk);
+        if (rc) return rc;
+        const int ndk = (int)dk.keys.size();
         // cut row per key, then the position inside it
         hipLaunchKernelGGL(k_cap_scan, dim3(nkeys), dim3(1024), 0, c->stream, c->d_cnt, nrows,
                            c->d_lim, c->d_cut);
@@ -908,20 +934,25 @@ int run_capped(hbt_ctx *c, bool mixed) {
         const double *psi = c->d_psi;
         if (!mixed) {
             hipLaunchKernelGGL(k_cap_pos_same, dim3((nkey3 + 63) / 64), dim3(64), 0, c->stream,
-                               c->P, A, c->d_gpoff, s.ngroups, psi, c->d_cut, nkey3, false);
+                               c->P, A, c->d_gpoff, s.ngroups, psi, c->d_cut, nkey3, false,
+                               c->d_dkey, ndk);
             hipLaunchKernelGGL(k_cap_pos_same, dim3((nkq + 63) / 64), dim3(64), 0, c->stream,
-                               c->P, A, c->d_gpoff, s.ngroups, psi, c->d_cut + nkey3, nkq, true);
+                               c->P, A, c->d_gpoff, s.ngroups, psi, c->d_cut + nkey3, nkq, true,
+                               c->d_dkey, ndk);
         } else {
             hipLaunchKernelGGL(k_cap_pos_mixed, dim3((nkey3 + 63) / 64), dim3(64), 0, c->stream,
                                c->P, A, B, s.d_off, s.nev, m.d_off, c->d_rowoff, c->d_ids,
-                               c->d_cs, c->d_gpoff, s.ngroups, psi, c->d_cut, nkey3, false);
+                               c->d_cs, c->d_gpoff, s.ngroups, psi, c->d_cut, nkey3, false,
+                               c->d_dkey, ndk);
             hipLaunchKernelGGL(k_cap_pos_mixed, dim3((nkq + 63) / 64), dim3(64), 0, c->stream,
                                c->P, A, B, s.d_off, s.nev, m.d_off, c->d_rowoff, c->d_ids,
-                               c->d_cs, c->d_gpoff, s.ngroups, psi, c->d_cut + nkey3, nkq, true);
+                               c->d_cs, c->d_gpoff, s.ngroups, psi, c->d_cut + nkey3, nkq, true,
+                               c->d_dkey, ndk);
         }
         HIPCHECK(c, hipGetLastError());
     }
-    return HBT_OK;
+    // the deferred pairs the cuts admit, binned with the host's values
+    return cap_defer_bin(c, nkey3, dk);
 }
 
 int prepare_and_capture(hbt_ctx *c) {
@@ -1202,7 +1233,8 @@ int hbt_destroy(hbt_ctx *c) {
                     c->d_sums,      c->d_ids,       c->d_cs,         c->d_rowoff,
                     c->d_items,     c->d_counter,   c->d_psi,        c->d_view,
                     c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff,
-                    c->d_wins,      c->d_defer,     c->d_ndefer};
+                    c->d_wins,      c->d_defer,     c->d_ndefer,     c->d_dkey,
+                    c->d_darr};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -1276,7 +1308,7 @@ int hbt_run_same(hbt_ctx *c) {
         if (!rc) rc = timed_fast(c, &c->ms_same);
         if (rc) return rc;
     }
-    rc = resolve_deferred(c, binds);
+    rc = resolve_deferred(c);
     if (rc) return rc;
     c->enum_same_total += es;
     return HBT_OK;
@@ -1315,7 +1347,7 @@ int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const doubl
         if (!rc) rc = timed_fast(c, &c->ms_mixed);
         if (rc) return rc;
     }
-    rc = resolve_deferred(c, binds);
+    rc = resolve_deferred(c);
     if (rc) return rc;
     c->enum_mixed_total += em;
     return HBT_OK;
@@ -1327,6 +1359,9 @@ int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *par
     int rc = check_slots(c);
     if (!rc) rc = quiesce(c);
     if (rc) return rc;
+    // a failure below leaves no prepared batch and no plan
+    drop_graph(c);
+    c->plan_valid = false;
     Slot &s = c->slot[0];
     std::vector<int> Me(s.nev, 0);
     long long rows = 0;
@@ -1383,7 +1418,7 @@ int hbt_sync(hbt_ctx *c, double *batch_ms) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->ms_batch = ms;
     if (batch_ms) *batch_ms = c->ms_batch;
-    return resolve_deferred(c, c->batch_capped);
+    return resolve_deferred(c);
 }
 
 int hbt_batch_info(hbt_ctx *c, uint64_t *enum_same, uint64_t *enum_mixed, int32_t *nitems) {
@@ -1391,6 +1426,13 @@ int hbt_batch_info(hbt_ctx *c, uint64_t *enum_same, uint64_t *enum_mixed, int32_
     if (enum_same) *enum_same = c->batch_enum_same;
     if (enum_mixed) *enum_mixed = c->batch_enum_mixed;
     if (nitems) *nitems = c->nitems;
+    return HBT_OK;
+}
+
+int hbt_batch_graph(hbt_ctx *c, int32_t *graph) {
+    if (!c || !graph) return HBT_ERR_ARG;
+    if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
+    *graph = c->graph_exec != nullptr ? 1 : 0;
     return HBT_OK;
 }
 
@@ -1468,7 +1510,7 @@ int hbt_read(hbt_ctx *c, hbt_result *o) {
     if (!c || !o) return HBT_ERR_ARG;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     {
-        const int rc = resolve_deferred(c, c->batch_capped);
+        const int rc = resolve_deferred(c);
         if (rc) return rc;
     }
     const size_t q8 = (size_t)c->nq * 8;

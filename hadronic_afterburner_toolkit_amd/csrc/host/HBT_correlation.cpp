@@ -44,26 +44,62 @@ void HBT_correlation::check(int rc, const char *what) const {
     exit(1);
 }
 
-// The events of event_list as one buffer in an engine slot.
-void HBT_correlation::upload_events(int slot, const particleSamples::Events &ev,
-                                    const std::vector<int> &event_list) {
-    std::vector<int64_t> off(1, 0);
-    std::vector<double> cols[8];
+// The mixed-event list is a separate file only with read_in_real_mixed_events
+// = 1; otherwise the reference's particleSamples aliases it to the
+// same-event list (particleSamples.cpp:528-531).
+bool HBT_correlation::mixed_is_separate() const {
+    return paraRdr_.getVal("read_in_real_mixed_events", 0) == 1;
+}
+
+// The listed events of the same-event (or mixed-event) list, gathered through
+// the reference's getters into the SoA columns px py pz E x y z t.
+void HBT_correlation::gather(bool mixed, const std::vector<int> &event_list) {
+    particleSamples &ps = *particle_list;
+    off_.assign(1, 0);
+    size_t n = 0;
+    for (int e : event_list)
+        n += (size_t)(mixed ? ps.get_number_of_particles_mixed_event(e)
+                            : ps.get_number_of_particles(e));
+    for (auto &c : cols_) c.resize(n);
+    size_t k = 0;
     for (int e : event_list) {
-        for (int64_t k = ev.off[e]; k < ev.off[e + 1]; k++)
-            for (int a = 0; a < 8; a++) cols[a].push_back(ev.cols[a][k]);
-        off.push_back((int64_t)cols[0].size());
+        const int ne = mixed ? ps.get_number_of_particles_mixed_event(e)
+                             : ps.get_number_of_particles(e);
+        for (int i = 0; i < ne; i++, k++) {
+            const particle_info p =
+                mixed ? ps.get_particle_from_mixed_event(e, i) : ps.get_particle(e, i);
+            cols_[0][k] = p.px;
+            cols_[1][k] = p.py;
+            cols_[2][k] = p.pz;
+            cols_[3][k] = p.E;
+            cols_[4][k] = p.x;
+            cols_[5][k] = p.y;
+            cols_[6][k] = p.z;
+            cols_[7][k] = p.t;
+        }
+        off_.push_back((int64_t)k);
     }
+}
+
+// The listed events as one buffer in an engine slot.
+void HBT_correlation::upload_events(int slot, bool mixed, const std::vector<int> &event_list) {
+    gather(mixed, event_list);
     const double *soa[8];
-    for (int a = 0; a < 8; a++) soa[a] = cols[a].data();
-    check(hbt_upload(ctx_, slot, (int32_t)event_list.size(), off.data(), soa), "hbt_upload");
+    for (int a = 0; a < 8; a++) soa[a] = cols_[a].data();
+    check(hbt_upload(ctx_, slot, (int32_t)event_list.size(), off_.data(), soa), "hbt_upload");
+}
+
+static std::vector<int> all_events(int nev) {
+    std::vector<int> v((size_t)nev);
+    for (int e = 0; e < nev; e++) v[(size_t)e] = e;
+    return v;
 }
 
 // :233-249, over all particles of the buffer (no rapidity cut)
 void HBT_correlation::calculate_flow_event_plane_angle(int n_order) {
-    const auto &ev = particle_list->same_events();
-    psi_ref = hbt_flow_event_plane_angle((int32_t)ev.cols[0].size(), ev.cols[0].data(),
-                                         ev.cols[1].data(), n_order);
+    gather(false, all_events(particle_list->get_number_of_events()));
+    psi_ref = hbt_flow_event_plane_angle((int32_t)cols_[0].size(), cols_[0].data(),
+                                         cols_[1].data(), n_order);
 }
 
 void HBT_correlation::calculate_HBT_correlation_function(
@@ -71,26 +107,23 @@ void HBT_correlation::calculate_HBT_correlation_function(
     set_particle_list(particle_list_in);
     const int nev = particle_list->get_number_of_events();
     number_of_oversample_events_ = nev;
-    const auto &same = particle_list->same_events();
-    const double *soa[8];
-    for (int a = 0; a < 8; a++) soa[a] = same.cols[a].data();
-    check(hbt_upload(ctx_, 0, nev, same.off.data(), soa), "hbt_upload");
-    if (particle_list->mixed_is_separate()) {
-        const auto &mix = particle_list->mixed_events();
-        const double *soam[8];
-        for (int a = 0; a < 8; a++) soam[a] = mix.cols[a].data();
-        check(hbt_upload(ctx_, 1, particle_list->get_number_of_mixed_events(), mix.off.data(),
-                         soam),
-              "hbt_upload");
+    const int mixed_nev = particle_list->get_number_of_mixed_events();
+    // slot 1 is left empty when the mixed list aliases the same-event list;
+    // one an earlier combine_and_bin_particle_pairs_mixed_events call loaded
+    // is overwritten with the mixed list the getters give
+    if (mixed_is_separate() || slot1_loaded_) {
+        upload_events(1, true, all_events(mixed_nev));
+        slot1_loaded_ = true;
     }
-    if (cfg_.azimuthal_flag == 1) {  // :181-183
-        calculate_flow_event_plane_angle(2);
+    upload_events(0, false, all_events(nev));
+    if (cfg_.azimuthal_flag == 1) {  // :181-183, on the columns just gathered
+        psi_ref = hbt_flow_event_plane_angle((int32_t)cols_[0].size(), cols_[0].data(),
+                                             cols_[1].data(), 2);
         check(hbt_set_psi_ref(ctx_, psi_ref), "hbt_set_psi_ref");
     }
     check(hbt_run_same(ctx_), "hbt_run_same");  // :190-194
     if (nev == 0) return;
     // :197-216: M partners per event, then inside the mixed call M angles
-    const int mixed_nev = particle_list->get_number_of_mixed_events();
     const int M = cfg_.number_of_mixed_events > 0 ? cfg_.number_of_mixed_events
                                                   : mixed_nev / 2 + 1;
     std::vector<int32_t> ids((size_t)nev * M);
@@ -114,7 +147,7 @@ void HBT_correlation::calculate_HBT_correlation_function(
 
 // :251-462 on the listed events of the current particle list
 void HBT_correlation::combine_and_bin_particle_pairs(std::vector<int> event_list) {
-    upload_events(0, particle_list->same_events(), event_list);
+    upload_events(0, false, event_list);
     check(hbt_run_same(ctx_), "hbt_run_same");
 }
 
@@ -124,10 +157,11 @@ void HBT_correlation::combine_and_bin_particle_pairs_mixed_events(
     int event_id, std::vector<int> mixed_event_list) {
     const int M = (int)mixed_event_list.size();
     if (M == 0) return;
-    upload_events(0, particle_list->same_events(), std::vector<int>{event_id});
-    std::vector<int> all;
-    for (int e = 0; e < particle_list->get_number_of_mixed_events(); e++) all.push_back(e);
-    upload_events(1, particle_list->mixed_events(), all);
+    // the mixed list goes to slot 1 first: an upload of slot 1 alone never
+    // aliases slot 0
+    upload_events(1, true, all_events(particle_list->get_number_of_mixed_events()));
+    slot1_loaded_ = true;
+    upload_events(0, false, std::vector<int>{event_id});
     std::vector<double> cs(2 * (size_t)M);
     for (int k = 0; k < M; k++) {
         const double random_rotation = ran_gen_ptr_->rand_uniform() * 2 * M_PI;

@@ -1,7 +1,12 @@
 // HBT_correlation -- drop-in replacement of the reference's class
 // (src/HBT_correlation.h:14-104): same constructor, methods and output
 // files; the pair loops run on the MI355X through the C ABI
-// (include/hbt_hip.h).  Engine keys read from the parameters (optional):
+// (include/hbt_hip.h).  It reads the particle container only through the
+// reference's particleSamples getters (src/particleSamples.h:213-253:
+// get_number_of_events, get_number_of_particles, get_particle and their
+// mixed-event counterparts), so it compiles against the reference's
+// particleSamples as well as against this repository's.  Engine keys read
+// from the parameters (optional):
 //   hbt_precision  0 = FP64 (bit-exact counts, default), 1 = precision form
 //   hbt_kernel     0 = plain LDS-tiled kernel, 1 = fast kernel (default)
 //   hbt_device     HIP device ordinal (default 0)
@@ -53,9 +58,16 @@ class HBT_correlation {
     double psi_ref = 0.0;
     int number_of_oversample_events_ = 0;
 
+    // SoA staging of the buffer handed to the engine (reused across calls)
+    std::vector<int64_t> off_;
+    std::vector<double> cols_[8];
+    // engine slot 1 holds a mixed list uploaded by an earlier call
+    bool slot1_loaded_ = false;
+
     void check(int rc, const char *what) const;
-    void upload_events(int slot, const particleSamples::Events &ev,
-                       const std::vector<int> &event_list);
+    bool mixed_is_separate() const;
+    void gather(bool mixed, const std::vector<int> &event_list);
+    void upload_events(int slot, bool mixed, const std::vector<int> &event_list);
     void write(int which);
 };
 

@@ -277,6 +277,12 @@ class HBTCorrelation:
                     "hbt_batch_info")
         return a.value, b.value, n.value
 
+    def batch_graph(self) -> bool:
+        """True when hbt_launch_batch replays the captured hipGraph."""
+        v = C.c_int32()
+        self._check(self._lib.hbt_batch_graph(self.h, C.byref(v)), "hbt_batch_graph")
+        return bool(v.value)
+
     def tuning(self) -> dict:
         t = HbtTuning()
         self._check(self._lib.hbt_get_tuning(self.h, C.byref(t)), "hbt_get_tuning")

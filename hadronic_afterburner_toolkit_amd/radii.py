@@ -8,7 +8,9 @@ Files are the writer's output (HBT_correlation.cpp:726-783): one line per
 q_out q_side q_long num den_norm, where num is the sum of cos(q.dx / hbar c)
 over the bin's same-event pairs and den_norm the bin's mixed-event pair count
 times the K_T bin's numerator/denominator pair ratio.  The correlation
-function is C = num / den_norm.
+function is C = num / den_norm; averaged over events it is <num> /
+<den_norm>, with the error the reference's h5 averager computes from the
+event-by-event spread (event_average), which the fit takes as sigma.
 
     python -m hadronic_afterburner_toolkit_amd.radii average <event folders...> <out folder>
     python -m hadronic_afterburner_toolkit_amd.radii fit <results folder>
@@ -45,40 +47,128 @@ def average_files(paths) -> np.ndarray:
     return acc / len(paths)
 
 
+def event_average(tables) -> np.ndarray:
+    """Event average of one K_T file over events with the correlation
+    function and its error from the event-by-event spread, as the
+    reference's averager computes them
+    (average_event_HBT_correlation_function_h5.py:155-215).
+
+    Legacy 6-column tables (q_out q_side q_long npair num den: pairs in the
+    bin, sum of cos, mixed pairs in the bin) give the reference's 8 columns:
+    the event means of the six, then C = num / (Npair_num / Npair_den * den)
+    and its error, where Npair_* are the tables' column sums (the K_T bin's
+    pair totals) and the error combines the event variances of num, den and
+    of the totals with their covariances, over sqrt(nev).
+
+    The writer's 5-column tables (q_out q_side q_long num den_norm,
+    HBT_correlation.cpp:767-776) already carry the pair-total ratio inside
+    den_norm (:728-730, :762-765), so the totals are 1 with no spread and the
+    same formula gives C = <num> / <den_norm> with the error from the event
+    variances of num and den_norm; returned as 7 columns: the five means, C,
+    C_err."""
+    tabs = [np.asarray(t, dtype=np.float64) for t in tables]
+    if not tabs:
+        raise ValueError("no events to average")
+    ncol = tabs[0].shape[1]
+    if ncol not in (5, 6) or any(t.shape != tabs[0].shape for t in tabs):
+        raise ValueError("expected equally shaped 5- or 6-column tables")
+    nev = len(tabs)
+    legacy = ncol == 6
+    ci, di = (4, 5) if legacy else (3, 4)
+    rows = tabs[0].shape[0]
+    num = np.zeros(rows)
+    s_num = np.zeros(rows)
+    s_num_np = np.zeros(rows)
+    den = np.zeros(rows)
+    s_den = np.zeros(rows)
+    s_den_np = np.zeros(rows)
+    np_num = s_np_num = np_den = s_np_den = 0.0
+    avg = np.zeros_like(tabs[0])
+    for t in tabs:
+        a_num = float(t[:, 3].sum()) if legacy else 1.0
+        a_den = float(t[:, 5].sum()) if legacy else 1.0
+        num += t[:, ci]
+        s_num += t[:, ci] ** 2
+        np_num += a_num
+        s_np_num += a_num ** 2
+        s_num_np += t[:, ci] * a_num
+        den += t[:, di]
+        s_den += t[:, di] ** 2
+        np_den += a_den
+        s_np_den += a_den ** 2
+        s_den_np += t[:, di] * a_den
+        avg += t
+    avg /= nev
+    num /= nev
+    den /= nev
+    np_num /= nev
+    np_den /= nev
+    v_num = s_num / nev - num ** 2
+    v_den = s_den / nev - den ** 2
+    v_np_num = s_np_num / nev - np_num ** 2
+    v_np_den = s_np_den / nev - np_den ** 2
+    c_num = s_num_np / nev - num * np_num
+    c_den = s_den_np / nev - den * np_den
+    with np.errstate(divide="ignore", invalid="ignore"):
+        corr = num / (np_num / np_den * den)
+        # relative variances; the pair-total terms vanish for 5-column tables
+        r_num = v_num / num ** 2 + v_np_num / np_num ** 2 - 2.0 * c_num / (num * np_num)
+        r_den = v_den / den ** 2 + v_np_den / np_den ** 2 - 2.0 * c_den / (den * np_den)
+        # rounding can leave a zero spread slightly negative
+        err_num = num / np_num * np.sqrt(np.maximum(r_num, 0.0)) / np.sqrt(nev)
+        err_den = den / np_den * np.sqrt(np.maximum(r_den, 0.0)) / np.sqrt(nev)
+        corr_err = np.sqrt((err_num / (den / np_den)) ** 2
+                           + ((num / np_num) * err_den / ((den / np_den) ** 2)) ** 2)
+    out = np.zeros((rows, ncol + 2))
+    out[:, :ncol] = avg
+    out[:, ncol], out[:, ncol + 1] = corr, corr_err
+    return out
+
+
 def write_table(path: str, data: np.ndarray) -> None:
     """The averaged table in the reference's format (:45-46)."""
     np.savetxt(path, data, fmt="%.10e", delimiter="  ")
 
 
 def average_folders(folders, out_folder: str, pattern: str = "HBT*") -> list[str]:
-    """For every file matching pattern in the first folder, the mean over
-    all folders, written under the same name into out_folder."""
+    """For every file matching pattern in the first folder, the event average
+    over all folders with C and its event-spread error (event_average),
+    written under the same name into out_folder."""
     folders = list(folders)
     os.makedirs(out_folder, exist_ok=True)
     names = sorted(os.path.basename(p) for p in glob.glob(os.path.join(folders[0], pattern)))
     for name in names:
-        write_table(os.path.join(out_folder, name),
-                    average_files(os.path.join(f, name) for f in folders))
+        tabs = [np.loadtxt(os.path.join(f, name)) for f in folders]
+        out = event_average(tabs) if tabs[0].ndim == 2 and tabs[0].shape[1] in (5, 6) \
+            else sum(tabs) / len(tabs)
+        write_table(os.path.join(out_folder, name), out)
     return names
 
 
 def correlation(table: np.ndarray):
-    """(q_out, q_side, q_long, C, C_err) of a 5-column table, each shaped
-    (nq, nq, nq) as [q_long][q_out][q_side].  C = num / den_norm (NaN where
-    den_norm is 0).  The files carry no variance, so C_err is an estimate:
-    (1 + |C|) / sqrt(den_norm), the counting error of den_norm pairs whose
-    cos terms have unit spread."""
+    """(q_out, q_side, q_long, C, C_err) of a table, each shaped (nq, nq, nq)
+    as [q_long][q_out][q_side]:
+      - an averaged table (7 columns from the writer's files, 8 in the
+        reference's layout): C and C_err as event_average computed them;
+      - one writer file (5 columns): C = num / den_norm (NaN where den_norm
+        is 0); one event carries no spread, so C_err is NaN and the fit
+        weighs every bin alike."""
     table = np.asarray(table, dtype=np.float64)
-    if table.ndim != 2 or table.shape[1] != 5:
-        raise ValueError("expected the writer's 5-column table (ecoOutput = 0)")
+    if table.ndim != 2 or table.shape[1] not in (5, 7, 8):
+        raise ValueError("expected a writer table (5 columns, ecoOutput = 0) or an averaged "
+                         "table (7 or 8 columns)")
     nq = int(round(table.shape[0] ** (1.0 / 3.0)))
     if nq ** 3 != table.shape[0]:
         raise ValueError(f"{table.shape[0]} lines is not a cube of q bins")
     shp = (nq, nq, nq)
-    num, den = table[:, 3], table[:, 4]
-    with np.errstate(divide="ignore", invalid="ignore"):
-        c = np.where(den > 0, num / den, np.nan)
-        err = np.where(den > 0, (1.0 + np.abs(c)) / np.sqrt(den), np.nan)
+    if table.shape[1] == 5:
+        num, den = table[:, 3], table[:, 4]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            c = np.where(den > 0, num / den, np.nan)
+        err = np.full_like(c, np.nan)
+    else:
+        k = table.shape[1] - 2
+        c, err = table[:, k].copy(), table[:, k + 1].copy()
     return (table[:, 0].reshape(shp), table[:, 1].reshape(shp), table[:, 2].reshape(shp),
             c.reshape(shp), err.reshape(shp))
 
@@ -98,12 +188,17 @@ def gaussian_3d(q, lam, r_out, r_side, r_long, r_os, r_ol):
 def fit_radii(q_out, q_side, q_long, c, c_err, q_cut: float, q_cut_min: float = 0.0,
               guess=GUESS):
     """Weighted least-squares fit of gaussian_3d to the bins with
-    q_cut_min < |q| < q_cut and a finite C (fit_HBT_radii.py:70-88).
+    q_cut_min < |q| < q_cut and a finite C (fit_HBT_radii.py:70-88), with
+    sigma = C_err + eps; unweighted when no bin has a finite C_err (a single
+    event's file).
     Returns (params, errors, R^2) with params (lambda, R_out, R_side,
     R_long, R_os, R_ol)."""
     from scipy.optimize import curve_fit
     qn = np.sqrt(q_out ** 2 + q_side ** 2 + q_long ** 2)
-    sel = (qn > q_cut_min) & (qn < q_cut) & np.isfinite(c) & np.isfinite(c_err)
+    weighted = bool(np.isfinite(c_err).any())
+    sel = (qn > q_cut_min) & (qn < q_cut) & np.isfinite(c)
+    if weighted:
+        sel &= np.isfinite(c_err)
     if sel.sum() < len(guess):
         raise ValueError("too few bins inside the q cut")
     q = [q_out[sel], q_side[sel], q_long[sel]]
@@ -112,8 +207,11 @@ def fit_radii(q_out, q_side, q_long, c, c_err, q_cut: float, q_cut_min: float = 
         # a cross term fitted at 0 has no gradient (it enters as R^2): its
         # error comes back infinite, as with the reference's script
         warnings.simplefilter("ignore")
-        params, cov = curve_fit(gaussian_3d, q, np.ravel(c[sel]), p0=list(guess),
-                                sigma=np.ravel(c_err[sel]) + EPS, absolute_sigma=True)
+        if weighted:  # the averaged error column + eps (fit_HBT_radii.py:61, :74-76)
+            params, cov = curve_fit(gaussian_3d, q, np.ravel(c[sel]), p0=list(guess),
+                                    sigma=np.ravel(c_err[sel]) + EPS, absolute_sigma=True)
+        else:
+            params, cov = curve_fit(gaussian_3d, q, np.ravel(c[sel]), p0=list(guess))
     errors = np.sqrt(np.diag(cov))
     resid = c[sel] - gaussian_3d(q, *params)
     r2 = 1.0 - np.var(resid) / np.var(c[sel])

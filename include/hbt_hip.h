@@ -219,6 +219,14 @@ int hbt_sync(hbt_ctx *ctx, double *batch_ms);
  * sum N_g(N_g-1)/2, mixed sum N_e * sum N_partner) and its work items. */
 int hbt_batch_info(hbt_ctx *ctx, uint64_t *enum_same, uint64_t *enum_mixed, int32_t *nitems);
 
+/* How hbt_launch_batch issues the prepared batch: *graph = 1 when it
+ * replays the hipGraph captured by hbt_prepare_batch (counter fill +
+ * persistent fast kernel), 0 when it launches directly (plain kernel, or a
+ * runtime that refused the capture, e.g. under a counter-collecting
+ * profiler).  HBT_ERR_STATE without a prepared batch.  No counterpart in the
+ * reference (its loops run on the host). */
+int hbt_batch_graph(hbt_ctx *ctx, int32_t *graph);
+
 /* Launch shape of the fast kernel: R i-particles per lane (1, 2, 4), j
  * chunk of same-event tiles, partner windows per mixed work item (one per
  * plan row whose partner has particles in the tile's |pT| window),

@@ -76,3 +76,25 @@ def test_server_create_without_gpu_or_bad_args_fails_cleanly():
         lib.hbt_server_destroy(h)
     else:
         assert rc == -6 and h.value is None
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/src"),
+                    reason="needs the reference's sources (build container only)")
+def test_dropin_class_links_into_the_reference_program():
+    """The drop-in HBT_correlation compiles against the reference's own
+    particleSamples, ParameterReader and Random headers and links into the
+    reference's program (main.cpp, Analysis.cpp, ...) in place of the
+    reference's class (oracle/Makefile target dropin)."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "dropin"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    exe = os.path.join(REPO, "oracle", "_ref", "dropin", "hadronic_afterburner_tools.e")
+    syms = subprocess.run(["nm", "-C", exe], capture_output=True, text=True).stdout
+    # this repository's class (its gather through the reference's getters),
+    # the reference's Analysis driving it, and the C ABI it calls
+    assert "HBT_correlation::gather(bool" in syms
+    assert "Analysis::HBTAnalysis()" in syms
+    assert "particleSamples::read_in_particle_samples_and_filter()" in syms
+    assert " U hbt_run_same" in syms
+

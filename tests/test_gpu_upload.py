@@ -87,3 +87,74 @@ def test_upload_waits_for_running_batch():
         g.sync()
         same(g.results(), ref[nxt])
     g.close()
+
+
+def test_bad_plan_leaves_no_batch():
+    """A plan with a partner id out of range fails hbt_prepare_batch and
+    leaves neither a plan nor a prepared batch: launching, autotuning and
+    reading the graph state report HBT_ERR_STATE, a tuning change succeeds
+    without preparing anything, and a valid plan afterwards gives the same
+    results as a fresh context."""
+    groups = workload(321)
+    Mg, ids, cs = plans(groups, 9, 10)
+    g = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP64, kernel=_lib.HBT_KERNEL_FAST)
+    g.upload_groups(groups)
+    g.prepare_batch(Mg, ids, cs)
+    assert g.batch_graph() in (True, False)
+    bad = ids.copy()
+    bad[len(bad) // 2] = 10 ** 6
+    with pytest.raises(_lib.HbtError) as e:
+        g.prepare_batch(Mg, bad, cs)
+    assert e.value.code == _lib.HBT_ERR_ARG
+    for call in (g.launch_batch, g.batch_graph, g.autotune):
+        with pytest.raises(_lib.HbtError) as e:
+            call()
+        assert e.value.code == _lib.HBT_ERR_STATE, call
+    g.set_tuning(1, 1024, 4, 8)
+    with pytest.raises(_lib.HbtError) as e:
+        g.launch_batch()
+    assert e.value.code == _lib.HBT_ERR_STATE
+    g.prepare_batch(Mg, ids, cs)
+    g.launch_batch()
+    g.sync()
+    r = g.results()
+    g.close()
+    f = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP64, kernel=_lib.HBT_KERNEL_FAST)
+    f.upload_groups(groups)
+    f.prepare_batch(Mg, ids, cs)
+    f.launch_batch()
+    f.sync()
+    ref = f.results()
+    f.close()
+    same(r, ref)
+
+
+def test_batch_is_a_captured_graph():
+    """hbt_prepare_batch captures the fast kernel's batch in a hipGraph on
+    this runtime, and hbt_launch_batch replays it; the plain kernel's batch
+    runs launch by launch.  Replays give the results of direct launches."""
+    groups = workload(99)
+    Mg, ids, cs = plans(groups, 4, 10)
+    g = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP32C, kernel=_lib.HBT_KERNEL_FAST)
+    g.upload_groups(groups)
+    g.prepare_batch(Mg, ids, cs)
+    assert g.batch_graph() is True
+    for _ in range(3):
+        g.launch_batch()
+        g.sync()
+    r3 = g.results()
+    g.close()
+    one = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP32C, kernel=_lib.HBT_KERNEL_FAST)
+    one.upload_groups(groups)
+    one.prepare_batch(Mg, ids, cs)
+    one.launch_batch()
+    one.sync()
+    r1 = one.results()
+    one.close()
+    assert np.array_equal(r3.num_count, 3 * r1.num_count)
+    assert np.array_equal(r3.den, 3 * r1.den)
+    p = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP64, kernel=_lib.HBT_KERNEL_PLAIN)
+    p.upload_groups(groups)
+    p.prepare_batch(Mg, ids, cs)
+    assert p.batch_graph() is False
+    p.close()
