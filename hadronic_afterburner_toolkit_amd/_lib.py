@@ -131,6 +131,11 @@ SIGNATURES = [
     ("hbt_sync", C.c_int, [C.c_void_p, _F64P]),
     ("hbt_batch_info", C.c_int, [C.c_void_p, _U64P, _U64P, C.POINTER(C.c_int32)]),
     ("hbt_batch_graph", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    ("hbt_calculate_noalloc", C.c_int, [C.POINTER(HbtConfig), C.c_int32, C.c_int64, C.c_int64,
+                                        C.c_void_p,
+                                        C.POINTER(C.c_void_p), C.c_int32, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("hbt_noalloc_last_error", C.c_char_p, []),
     ("hbt_get_tuning", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
     ("hbt_set_tuning", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),
     ("hbt_autotune", C.c_int, [C.c_void_p, C.POINTER(HbtTuning)]),

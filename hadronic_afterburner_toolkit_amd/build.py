@@ -45,6 +45,7 @@ def build_engine() -> str:
     _run([HIPCC] + COMMON + ["-shared", "-o", tmp, os.path.join(CSRC, "hbt_engine.hip"),
                              os.path.join(CSRC, "hbt_server.hip"),
                              os.path.join(CSRC, "hbt_bf.hip"),
+                             os.path.join(CSRC, "hbt_noalloc.hip"),
                              os.path.join(CSRC, "host", "hbt_host.cpp"), "-lz", "-lpthread"])
     os.replace(tmp, out)
     return out

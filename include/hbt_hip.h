@@ -227,6 +227,44 @@ int hbt_batch_info(hbt_ctx *ctx, uint64_t *enum_same, uint64_t *enum_mixed, int3
  * reference (its loops run on the host). */
 int hbt_batch_graph(hbt_ctx *ctx, int32_t *graph);
 
+/* ---- allocation-free entry point ------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same-event
+ * and mixed-event pairs of one buffer with the plain FP64 kernel -- what
+ * hbt_upload + hbt_run_same + hbt_run_mixed do for a context created with
+ * HBT_KERNEL_PLAIN, HBT_PREC_FP64 -- with every array in device memory owned
+ * by the caller, launched on the caller's stream, and no allocation:
+ *   ev_off[nev+1] (int64, device): event offsets into the soa columns;
+ *   soa[8] (host array of device pointers): px py pz E x y z t of the
+ *     buffer, already filtered to one species; the tanh(HBTrap) rapidity cut
+ *     is applied here, keeping the reference's gather order;
+ *   n_particles = ev_off[nev] - ev_off[0], max_event >= the largest event
+ *     (they size the grids);
+ *   partner_ids[nev*M] (int32, device, ids of events of this buffer, which
+ *     is also the mixed-event list), cos_sin[2*nev*M] (FP64, device): the
+ *     mixing plan as hbt_rng_draw_plan returns it;
+ *   bins (device): the histogram, n_KT * qnpts^3 bins of 8 doubles
+ *     {count (an exact FP64 integer), sum q_out, sum q_side, sum q_long,
+ *     sum cos, den (u64 bit pattern), -, -}, bin [kt][iq_out][iq_side]
+ *     [iq_long] (iq_long fastest); pairs are added to what it holds;
+ *   stream: a hipStream_t (NULL: the null stream).
+ * Limits: azimuthal_flag = invariant_radius_flag = 0, the pair cap off
+ * (needed_number_of_pairs >= 1e18), kernel HBT_KERNEL_PLAIN, precision
+ * HBT_PREC_FP64; at most HBT_NOALLOC_MAX_PARTICLES particles and
+ * HBT_NOALLOC_MAX_EVENTS events (HBT_ERR_CAPACITY beyond).  The kept
+ * particles, their offsets and the stage counters live in a fixed device
+ * array of about 16 MB per GPU, held until the process ends and shared by
+ * every call on that GPU: calls that overlap on one GPU race.  Returns once
+ * the work is queued; the histogram is complete when the stream is.  Errors:
+ * hbt_noalloc_last_error(). */
+#define HBT_NOALLOC_MAX_PARTICLES (1 << 18)
+#define HBT_NOALLOC_MAX_EVENTS (1 << 14)
+int hbt_calculate_noalloc(const hbt_config *cfg, int32_t nev, int64_t n_particles,
+                          int64_t max_event, const int64_t *ev_off, const double *const soa[8],
+                          int32_t M, const int32_t *partner_ids, const double *cos_sin,
+                          double *bins, void *stream);
+const char *hbt_noalloc_last_error(void);
+
 /* Launch shape of the fast kernel: R i-particles per lane (1, 2, 4), j
  * chunk of same-event tiles, partner windows per mixed work item (one per
  * plan row whose partner has particles in the tile's |pT| window),

@@ -98,3 +98,30 @@ def test_dropin_class_links_into_the_reference_program():
     assert "particleSamples::read_in_particle_samples_and_filter()" in syms
     assert " U hbt_run_same" in syms
 
+
+def test_noalloc_rejects_sizes_beyond_its_scratch():
+    """hbt_calculate_noalloc checks its limits on the host, before any device
+    work: more events or particles than its fixed scratch array holds, a
+    configuration it does not cover, a binding pair cap."""
+    lib = _lib.load()
+    cfg = hbt.config_from_params(dict(qnpts=21, n_KT=4, KT_min=0.2, KT_max=0.8,
+                                      needed_number_of_pairs=1e18),
+                                 precision=_lib.HBT_PREC_FP64, kernel=_lib.HBT_KERNEL_PLAIN)
+    dummy = C.c_void_p(1 << 20)  # never dereferenced: every call below fails its checks
+    soa = (C.c_void_p * 8)(*([dummy] * 8))
+
+    def call(c, nev, n, mx):
+        return lib.hbt_calculate_noalloc(C.byref(c), nev, n, mx, dummy, soa, 1, dummy, dummy,
+                                         dummy, None)
+
+    assert call(cfg, (1 << 14) + 1, 100, 10) == _lib.HBT_ERR_CAPACITY
+    assert b"EVENTS" in lib.hbt_noalloc_last_error()
+    assert call(cfg, 10, (1 << 18) + 1, 10) == _lib.HBT_ERR_CAPACITY
+    assert b"PARTICLES" in lib.hbt_noalloc_last_error()
+    assert call(cfg, 10, 100, 101) == _lib.HBT_ERR_ARG
+    capped = hbt.config_from_params(dict(qnpts=21, needed_number_of_pairs=3e7),
+                                    precision=_lib.HBT_PREC_FP64, kernel=_lib.HBT_KERNEL_PLAIN)
+    assert call(capped, 10, 100, 10) == _lib.HBT_ERR_ARG
+    fast = hbt.config_from_params(dict(qnpts=21, needed_number_of_pairs=1e18),
+                                  precision=_lib.HBT_PREC_FP64, kernel=_lib.HBT_KERNEL_FAST)
+    assert call(fast, 10, 100, 10) == _lib.HBT_ERR_ARG
