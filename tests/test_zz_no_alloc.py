@@ -6,8 +6,8 @@ Two calls in a row on fresh histograms, the second with a smaller buffer (the
 scratch array keeps the first call's contents).  Bars: every count (bin
 counts, denominators) bit-exact; every FP64 sum within 1e-12 under
 err = |S - S_ref| / max(|S_ref|, sum|.|_bin), the floor sum|cos| / sum|q|
-of each bin from the oracle.  Device memory for the caller's arrays comes
-from torch (plumbing only).
+of each bin from the oracle.  The caller's device arrays and stream come
+from the HIP runtime the engine library is linked with.
 """
 import ctypes as C
 
@@ -36,27 +36,66 @@ def existing_entry(buf, M, ids, cs):
     return r
 
 
-def noalloc_entry(buf, M, ids, cs, stream):
-    import torch
+class Hip:
+    """The caller's side of the entry point: device arrays and a stream from
+    the HIP runtime the engine library runs on (loaded with it)."""
+
+    def __init__(self):
+        _lib.load()
+        self.h = C.CDLL("libamdhip64.so.7")
+        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipFree.argtypes = [C.c_void_p]
+        self.h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        self.h.hipStreamDestroy.argtypes = [C.c_void_p]
+        self.ptrs = []
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.h.hipMalloc(C.byref(p), max(1, a.nbytes)) == 0
+        self.ptrs.append(p)
+        if a.nbytes:
+            assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return p
+
+    def to_host(self, p, like):
+        out = np.empty_like(like)
+        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def stream(self):
+        s = C.c_void_p()
+        assert self.h.hipStreamCreate(C.byref(s)) == 0
+        return s
+
+    def free(self):
+        for p in self.ptrs:
+            self.h.hipFree(p)
+        self.ptrs = []
+
+
+def noalloc_entry(hip, buf, M, ids, cs, stream):
     lib = _lib.load()
     cfg = hbt.config_from_params(PARAMS, precision=_lib.HBT_PREC_FP64,
                                  kernel=_lib.HBT_KERNEL_PLAIN)
-    dev = torch.device("cuda:0")
-    cols = [torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev) for a in buf.soa()]
-    off = torch.from_numpy(np.ascontiguousarray(buf.ev_off, np.int64)).to(dev)
-    d_ids = torch.from_numpy(np.ascontiguousarray(ids, np.int32)).to(dev)
-    d_cs = torch.from_numpy(np.ascontiguousarray(cs, np.float64)).to(dev)
+    cols = [hip.to_device(np.asarray(a, np.float64)) for a in buf.soa()]
+    off = hip.to_device(np.asarray(buf.ev_off, np.int64))
+    d_ids = hip.to_device(np.asarray(ids, np.int32))
+    d_cs = hip.to_device(np.asarray(cs, np.float64))
     nbins = PARAMS["n_KT"] * PARAMS["qnpts"] ** 3
-    bins = torch.zeros(nbins * 8, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    soa = (C.c_void_p * 8)(*[c.data_ptr() for c in cols])
+    zeros = np.zeros(nbins * 8)
+    bins = hip.to_device(zeros)
+    soa = (C.c_void_p * 8)(*[c.value for c in cols])
     lens = np.diff(buf.ev_off)
     rc = lib.hbt_calculate_noalloc(C.byref(cfg), buf.nev, int(buf.n), int(lens.max()),
-                                   off.data_ptr(), soa, M, d_ids.data_ptr(), d_cs.data_ptr(),
-                                   bins.data_ptr(), stream.cuda_stream)
+                                   off, soa, M, d_ids, d_cs, bins, stream)
     assert rc == 0, lib.hbt_noalloc_last_error()
-    stream.synchronize()
-    return bins.cpu().numpy().reshape(nbins, 8)
+    assert hip.h.hipStreamSynchronize(stream) == 0
+    out = hip.to_host(bins, zeros).reshape(nbins, 8)
+    hip.free()
+    return out
 
 
 def check(bins, r, floors):
@@ -73,16 +112,17 @@ def check(bins, r, floors):
 
 
 def test_noalloc_equals_existing_entry_point(oracle_mod):
-    import torch
     full = samples.generate(777, 14, 220.0)
-    stream = torch.cuda.Stream()
+    hip = Hip()
+    stream = hip.stream()
     for k, (first, count) in enumerate(((0, 14), (3, 5))):
         buf = full.events(first, count)
         M, ids, cs = hbt.Random(40 + k).draw_plan(buf.nev, buf.nev, 0)
         print(f"no-alloc call {k}: {buf.nev} events, {buf.n} particles, M={M}", flush=True)
         r = existing_entry(buf, M, ids, cs)
-        bins = noalloc_entry(buf, M, ids, cs, stream)
+        bins = noalloc_entry(hip, buf, M, ids, cs, stream)
         o = oracle_mod.Oracle(PARAMS)
         o.combine_same(buf)
         o.run_mixed_plan(buf, M, ids, cs)
         check(bins, r, o.arrays())
+    hip.h.hipStreamDestroy(stream)
