@@ -67,9 +67,9 @@ def flops(stages_same, stages_mixed):
             + 7 * m[0] + 12 * m[1] + 5 * m[2] + 17 * m[3] + 1 * m[4])
 
 
-def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None):
+def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None, binning=0):
     from hadronic_afterburner_toolkit_amd import hbt
-    g = hbt.HBTCorrelation(PARAMS, precision=prec, kernel=kern)
+    g = hbt.HBTCorrelation(dict(PARAMS, hbt_binning=binning), precision=prec, kernel=kern)
     t0 = time.perf_counter()
     g.upload_groups(groups)
     g.prepare_batch(*plan)
@@ -97,9 +97,10 @@ def run_mode(groups, plan, prec, kern, steps, warmup, autotune, log, tuning=None
         g.launch_batch()
         dev_ms.append(g.sync())
     wall = time.perf_counter() - t0
-    log(f"  prec={prec} kernel={kern} steps={steps} wall={wall:.3f}s "
-        f"dev_ms/step={np.mean(dev_ms):.2f} tune={tune}")
-    return g, dict(wall=wall, dev_ms=dev_ms, enum_same=es, enum_mixed=em, nitems=nitems,
+    slab = g.slab_info()
+    log(f"  prec={prec} kernel={kern} binning={binning} steps={steps} wall={wall:.3f}s "
+        f"dev_ms/step={np.mean(dev_ms):.2f} tune={tune} slab={slab}")
+    return g, dict(wall=wall, dev_ms=dev_ms, enum_same=es, enum_mixed=em, nitems=nitems, slab=slab,
                    tune=tune, graph=graph, upload_s=upload_s, upload_warm_s=upload_warm_s,
                    prepare_warm_s=prepare_warm_s)
 
@@ -351,6 +352,35 @@ def balance_function(args, log):
     return out
 
 
+def dropin_line(groups, seed, log):
+    """The drop-in path as Analysis::HBTAnalysis drives it
+    (src/Analysis.cpp:821-833): buffer after buffer,
+    calculate_HBT_correlation_function (upload, psi_ref, same-event pairs,
+    mixing plan from the shared random stream, mixed pairs), then the
+    histograms read back once.  The 40 bench groups as 40 buffers, precision
+    form; cold = a fresh context (allocations included), warm = the same 40
+    buffers again on that context."""
+    from hadronic_afterburner_toolkit_amd import _lib, hbt
+    out = {"buffers": len(groups)}
+    g = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP32C, kernel=_lib.HBT_KERNEL_FAST)
+    for label in ("cold", "warm"):
+        rng = hbt.Random(seed)
+        s0, m0 = g.stage_counts()
+        t0 = time.perf_counter()
+        for b in groups:
+            g.calculate_HBT_correlation_function(b, rng)
+        g.results()
+        dt = time.perf_counter() - t0
+        s1, m1 = g.stage_counts()
+        pairs = float(int(s1[0]) - int(s0[0]) + int(m1[0]) - int(m0[0]))
+        out[label] = {"s": dt, "ms_per_buffer": 1e3 * dt / len(groups), "pairs": pairs,
+                      "pairs_per_s": pairs / dt}
+        log(f"  drop-in {label}: {dt:.3f} s for {len(groups)} buffers "
+            f"({1e3 * dt / len(groups):.2f} ms per buffer)")
+    g.close()
+    return out
+
+
 def cpu_baseline(group, plan_g, log):
     """The CPU oracle on group 0 (its same-event pairs and its events' mixed
     pairs with the same plan), one thread."""
@@ -513,19 +543,38 @@ def atomic_ceiling(log):
     return rates
 
 
+def kernel_build_hash():
+    """sha256 (16 hex digits) of the sources the fast kernel and its slab
+    binning are built from (csrc/*.h, csrc/hbt_engine.hip)."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(REPO, "hadronic_afterburner_toolkit_amd", "csrc")
+    for p in sorted(glob.glob(os.path.join(csrc, "*.h")) + [os.path.join(csrc, "hbt_engine.hip")]):
+        with open(p, "rb") as f:
+            h.update(os.path.basename(p).encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel_prefix):
     """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (lines 'traffic_bytes_per_launch <kernel> N')."""
+    rocprofv3 counter summary (lines 'traffic_bytes_per_launch <kernel> N
+    <build hash>'), with the build hash of the sources it was measured on
+    and whether it matches the current sources; the value is None when it
+    does not (the kernel changed since the counters were read)."""
+    cur = kernel_build_hash()
     try:
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" \
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" \
                         and parts[1] == kernel_prefix:
-                    return float(parts[2])
+                    built = parts[3] if len(parts) > 3 else None
+                    ok = built == cur
+                    return (float(parts[2]) if ok else None), built, cur
     except OSError:
         pass
-    return None
+    return None, None, cur
 
 
 def main():
@@ -539,7 +588,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--only", choices=["fp32c", "fp64", "plain"], default=None)
+    ap.add_argument("--only", choices=["fp32c", "fp64", "fp32c_slab", "fp64_slab", "plain"],
+                    default=None)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving form")
     ap.add_argument("--no-extra", action="store_true",
@@ -578,17 +628,22 @@ def main():
     groups = make_workload(args.events, args.mult, args.per_group, args.seed)
     plan = draw_plans(groups, args.seed + 1, PARAMS["number_of_mixed_events"])
     res = {}
-    modes = {"fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps),
-             "fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps),
-             "plain": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, max(1, min(2, args.steps)))}
+    # fp32c / fp64: the fast kernel, binned pairs added with device-scope
+    # atomics; the *_slab modes sum them per (block, q_out plane) in LDS
+    # instead (slab binning), for comparison
+    modes = {"fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps, 0),
+             "fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps, 0),
+             "fp32c_slab": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps, 1),
+             "fp64_slab": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps, 1),
+             "plain": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, max(1, min(2, args.steps)), 0)}
     ctx = {}
-    for name, (prec, kern, k) in modes.items():
+    for name, (prec, kern, k, binning) in modes.items():
         if args.only and name != args.only:
             continue
         log(f"mode {name}")
         ctx[name], res[name] = run_mode(groups, plan, prec, kern, k,
                                         args.warmup if name != "plain" else 1,
-                                        not (args.no_autotune or tuning), log, tuning)
+                                        not (args.no_autotune or tuning), log, tuning, binning)
 
     def pairs_per_s(r):
         return (r["enum_same"] + r["enum_mixed"]) * len(r["dev_ms"]) / r["wall"]
@@ -622,8 +677,13 @@ def main():
         out[f"{name}_upload_warm_s"] = res[name]["upload_warm_s"]
         out[f"{name}_prepare_warm_s"] = res[name]["prepare_warm_s"]
         out[f"{name}_batch_graph"] = res[name]["graph"]
+        out[f"{name}_slab"] = res[name]["slab"]
     if "fp32c" in res and "plain" in res:
         out["speedup_vs_plain_fp64"] = out["fp32c_pairs_per_s"] / out["plain_pairs_per_s"]
+    for form in ("fp32c", "fp64"):
+        if form in res and form + "_slab" in res:
+            out[f"speedup_slab_vs_atomics_{form}"] = (out[f"{form}_slab_pairs_per_s"]
+                                                     / out[f"{form}_pairs_per_s"])
     if "fp32c" in res and "fp64" in res:
         out["speedup_vs_fast_fp64"] = out["fp32c_pairs_per_s"] / out["fp64_pairs_per_s"]
         a, b = ctx["fp32c"].results(), ctx["fp64"].results()
@@ -646,8 +706,11 @@ def main():
         ach = fl / dev_s / 1e12
         peak = FP32_PEAK_TFLOPS if head == "fp32c" else FP64_PEAK_TFLOPS
         kname = "k_fast<float>" if head == "fp32c" else "k_fast<double>"
+        traffic, t_build, cur_build = measured_traffic(kname)
         out["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                           "frac": ach / peak, "traffic": measured_traffic(kname),
+                           "frac": ach / peak, "traffic": traffic,
+                           "traffic_source": "from_profile (profiles/k_fast_traffic.md)",
+                           "traffic_build": t_build, "kernel_build": cur_build,
                            "kernel": kname,
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
         # the bound that holds: every binned pair is one atomic cache-line
@@ -656,14 +719,23 @@ def main():
         # rates of those two patterns; the ceiling is the blended rate
         acc = s[4] + m[4]
         ceil = acc / (s[4] / atom_ceil["same"] + m[4] / atom_ceil["mixed"]) if atom_ceil else None
+        # the floor the device-scope atomics set; slab binning (the *_slab
+        # modes) adds the binned pairs without them
+        smode = head + "_slab"
+        s_dev = float(np.mean(res[smode]["dev_ms"])) * 1e-3 if smode in res else None
         out["atomic_roofline"] = {
-            "bound": "memory_side_atomics", "achieved": acc / dev_s, "ceiling": ceil,
-            "unit": "atomic line accesses/s",
+            "bound": "memory_side_atomics", "achieved": acc / dev_s,
+            "ceiling": ceil, "unit": "atomic line accesses/s",
             "frac": (acc / dev_s / ceil) if ceil else None,
+            "slab_mode_ms": s_dev * 1e3 if s_dev else None,
+            "slab_mode_below_floor": bool(ceil and s_dev and s_dev < acc / ceil),
             "accesses_per_launch": acc, "accesses_same": s[4], "accesses_mixed": m[4],
             "floor_ms": 1e3 * acc / ceil if ceil else None, "kernel": kname,
             "ceiling_source": "hist_atomics_bench.e f64coop5 (same-event) and "
                               "u32carry_1_per_lane (mixed), measured in this run"}
+    if not args.no_extra and not args.only:
+        out["dropin"] = dropin_line(groups, args.seed + 1, log)
+        out["dropin"]["kernel_ms_per_buffer"] = float(np.mean(res[head]["dev_ms"])) / len(groups)
     if not args.no_serve:
         out["serving"] = serve(args, log)
     if not args.no_extra:

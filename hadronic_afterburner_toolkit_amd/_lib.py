@@ -31,7 +31,7 @@ class HbtConfig(C.Structure):
         ("HBTrap_min", C.c_double), ("HBTrap_max", C.c_double),
         ("needed_number_of_pairs", C.c_double), ("number_of_mixed_events", C.c_int32),
         ("precision", C.c_int32), ("kernel", C.c_int32), ("device", C.c_int32),
-        ("kphi_guard_prad", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("kphi_guard_prad", C.c_int32), ("binning", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -131,6 +131,7 @@ SIGNATURES = [
     ("hbt_sync", C.c_int, [C.c_void_p, _F64P]),
     ("hbt_batch_info", C.c_int, [C.c_void_p, _U64P, _U64P, C.POINTER(C.c_int32)]),
     ("hbt_batch_graph", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    ("hbt_slab_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _U64P, _U64P]),
     ("hbt_calculate_noalloc", C.c_int, [C.POINTER(HbtConfig), C.c_int32, C.c_int64, C.c_int64,
                                         C.c_void_p,
                                         C.POINTER(C.c_void_p), C.c_int32, C.c_void_p,

@@ -288,6 +288,31 @@ struct hbt_ctx {
     size_t cut_cap = 0, lim_cap = 0;
     long long *d_gpoff = nullptr;  // group particle offsets of slot 0
     size_t gpoff_cap = 0;
+    // slab binning of the fast kernel (hbt_slab.h)
+    struct {
+        bool on = false;            // the next fast launch bins through slabs
+        int S = 0, SB = 0, parts = 1, nblk = 0;
+        std::vector<int> scap, mcap;
+        std::vector<long long> sbase, mbase;
+        // demand measured on the last launch: per slab total and largest
+        // per block (same, mixed), with that launch's pairs and blocks
+        std::vector<double> tot_s, tot_m, max_s, max_m;
+        double hist_enum_s = 0, hist_enum_m = 0;
+        int hist_nblk = 0;
+        bool have_hist = false, pending = false, resize = false;
+        uint64_t direct = 0;        // binned pairs of the last launch added directly
+        char *d_vals = nullptr;
+        unsigned short *d_sidx = nullptr, *d_midx = nullptr;
+        long long *d_sbase = nullptr, *d_mbase = nullptr;
+        int *d_scap = nullptr, *d_mcap = nullptr;
+        unsigned *d_sfill = nullptr, *d_mfill = nullptr;
+        unsigned long long *d_stats = nullptr;
+        SlabPart *d_parts = nullptr;
+        size_t vals_cap = 0, sidx_cap = 0, midx_cap = 0, base_cap = 0, fill_cap = 0,
+               stats_cap = 0, parts_cap = 0, scap_cap = 0;
+    } slab;
+    uint64_t launch_enum_same = 0, launch_enum_mixed = 0;  // pairs of the items uploaded last
+    FastArgs batch_args{};  // arguments of the prepared batch's fast launch
     DeferKey *d_dkey = nullptr;    // deferred pairs of a capped call, by (row, pos)
     long long *d_darr = nullptr;   // and their arrivals (key * nrows + row)
     size_t dkey_cap = 0, darr_cap = 0;
@@ -555,6 +580,130 @@ View make_view(hbt_ctx *c) {
 
 // Kernel arguments of the context's own launches: one view (index 0),
 // copied to the device before the launch (and before a graph capture).
+long fast_blocks(const hbt_ctx *c, int nitems) {
+    return std::max(1l, std::min<long>((long)c->cu_count * c->tune.blocks_per_cu,
+                                       ((long)nitems + 3) / 4));
+}
+
+// Reads the demand the last slab-binned launch measured (after the stream
+// has drained); a launch that had to add more than 0.2 % of its pairs
+// directly marks the regions for resizing.
+int slab_read_stats(hbt_ctx *c) {
+    auto &sl = c->slab;
+    if (!sl.pending) return HBT_OK;
+    sl.pending = false;
+    const int S = sl.S;
+    std::vector<unsigned long long> st((size_t)4 * S + 1);
+    HIPCHECK(c, hipMemcpy(st.data(), sl.d_stats, st.size() * 8, hipMemcpyDeviceToHost));
+    sl.tot_s.assign(S, 0);
+    sl.tot_m.assign(S, 0);
+    sl.max_s.assign(S, 0);
+    sl.max_m.assign(S, 0);
+    double tot = 0;
+    for (int k = 0; k < S; k++) {
+        sl.tot_s[k] = (double)st[k];
+        sl.tot_m[k] = (double)st[S + k];
+        sl.max_s[k] = (double)st[2 * S + 1 + k];
+        sl.max_m[k] = (double)st[3 * S + 1 + k];
+        tot += sl.tot_s[k] + sl.tot_m[k];
+    }
+    sl.direct = st[2 * S];
+    sl.hist_enum_s = (double)c->launch_enum_same;
+    sl.hist_enum_m = (double)c->launch_enum_mixed;
+    sl.hist_nblk = sl.nblk;
+    sl.have_hist = true;
+    if ((double)sl.direct > 0.002 * tot) sl.resize = true;
+    return HBT_OK;
+}
+
+// Sizes the (block, slab) regions of the next fast launch, on the host and
+// outside any graph capture.  Capacity per region: the largest demand one
+// block had for the slab on the last launch, scaled to this launch's pairs
+// and blocks, with a margin; with no history, a uniform share of a generous
+// binned fraction.  The record buffers are capped at kSlabBudget bytes:
+// beyond, regions shrink and more pairs take the direct path.
+constexpr double kSlabBudget = 16e9;
+
+int slab_prepare(hbt_ctx *c, int nitems) {
+    auto &sl = c->slab;
+    const int q = c->P.qnpts;
+    const int S = c->cfg.n_KT * c->P.nphi * q, SB = q * q;
+    sl.on = c->cfg.binning == 1 && c->cfg.kernel == HBT_KERNEL_FAST && !c->P.qinv &&
+            !c->P.azimuthal && S <= kMaxSlabs && SB <= kMaxSlabBins && nitems > 0;
+    if (!sl.on) return HBT_OK;
+    const int nblk = (int)fast_blocks(c, nitems);
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
+    const size_t vsz = f32 ? 16 : 32;
+    if (sl.S != S) sl.have_hist = false;
+    sl.S = S;
+    sl.SB = SB;
+    sl.nblk = nblk;
+    sl.resize = false;
+    sl.scap.assign(S, 0);
+    sl.mcap.assign(S, 0);
+    const double es = (double)c->launch_enum_same, em = (double)c->launch_enum_mixed;
+    for (int k = 0; k < S; k++) {
+        double xs, xm;
+        if (sl.have_hist) {
+            const double bs = (double)sl.hist_nblk / nblk;
+            xs = sl.hist_enum_s > 0 ? sl.max_s[k] * es / sl.hist_enum_s * bs : 0.0;
+            xm = sl.hist_enum_m > 0 ? sl.max_m[k] * em / sl.hist_enum_m * bs : 0.0;
+            // a slab the last launch never reached may still get pairs now
+            xs = std::max(xs, 0.002 * es / S / nblk);
+            xm = std::max(xm, 0.002 * em / S / nblk);
+        } else {
+            xs = 3.0 * 0.03 * es / S / nblk;
+            xm = 3.0 * 0.03 * em / S / nblk;
+        }
+        sl.scap[k] = (int)std::min(1e9, std::ceil(1.25 * xs) + 32);
+        sl.mcap[k] = (int)std::min(1e9, std::ceil(1.25 * xm) + 32);
+    }
+    double rs = 0, rm = 0;
+    for (int k = 0; k < S; k++) {
+        rs += (double)sl.scap[k] * nblk;
+        rm += (double)sl.mcap[k] * nblk;
+    }
+    const double bytes = rs * (vsz + 2) + rm * 2;
+    if (bytes > kSlabBudget) {
+        const double f = kSlabBudget / bytes;
+        rs = rm = 0;
+        for (int k = 0; k < S; k++) {
+            sl.scap[k] = std::max(1, (int)(sl.scap[k] * f));
+            sl.mcap[k] = std::max(1, (int)(sl.mcap[k] * f));
+            rs += (double)sl.scap[k] * nblk;
+            rm += (double)sl.mcap[k] * nblk;
+        }
+    }
+    sl.sbase.assign(S, 0);
+    sl.mbase.assign(S, 0);
+    long long as = 0, am = 0;
+    for (int k = 0; k < S; k++) {
+        sl.sbase[k] = as;
+        sl.mbase[k] = am;
+        as += (long long)sl.scap[k] * nblk;
+        am += (long long)sl.mcap[k] * nblk;
+    }
+    // parts per slab: about four k_slab_bin blocks per CU in all
+    sl.parts = std::max(1, std::min(64, (4 * c->cu_count + S - 1) / S));
+    int rc = ensure(c, &sl.d_vals, sl.vals_cap, (size_t)std::max(1ll, as) * vsz);
+    if (!rc) rc = ensure(c, &sl.d_sidx, sl.sidx_cap, (size_t)std::max(1ll, as));
+    if (!rc) rc = ensure(c, &sl.d_midx, sl.midx_cap, (size_t)std::max(1ll, am));
+    if (!rc) rc = ensure(c, &sl.d_sbase, sl.base_cap, (size_t)2 * S);
+    if (!rc) rc = ensure(c, &sl.d_scap, sl.scap_cap, (size_t)2 * S);
+    if (!rc) rc = ensure(c, &sl.d_sfill, sl.fill_cap, (size_t)2 * nblk * S);
+    if (!rc) rc = ensure(c, &sl.d_stats, sl.stats_cap, (size_t)4 * S + 1);
+    if (!rc) rc = ensure(c, &sl.d_parts, sl.parts_cap, (size_t)S * sl.parts * SB);
+    if (rc) return rc;
+    sl.d_mbase = sl.d_sbase + S;
+    sl.d_mcap = sl.d_scap + S;
+    sl.d_mfill = sl.d_sfill + (size_t)nblk * S;
+    HIPCHECK(c, hipMemcpy(sl.d_sbase, sl.sbase.data(), S * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(sl.d_mbase, sl.mbase.data(), S * 8, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(sl.d_scap, sl.scap.data(), S * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(sl.d_mcap, sl.mcap.data(), S * 4, hipMemcpyHostToDevice));
+    return HBT_OK;
+}
+
 int fast_args(hbt_ctx *c, FastArgs &a) {
     set_fp32_margins(c);
     const View v = make_view(c);
@@ -565,14 +714,55 @@ int fast_args(hbt_ctx *c, FastArgs &a) {
     a.nitems = c->nitems;
     a.counter = c->d_counter;
     a.binned = c->d_binned;
+    a.slab = SlabArgs{};
+    int rc = slab_prepare(c, c->nitems);
+    if (rc) return rc;
+    const auto &sl = c->slab;
+    if (sl.on) {
+        a.slab.vals = sl.d_vals;
+        a.slab.sidx = sl.d_sidx;
+        a.slab.midx = sl.d_midx;
+        a.slab.sbase = sl.d_sbase;
+        a.slab.mbase = sl.d_mbase;
+        a.slab.scap = sl.d_scap;
+        a.slab.mcap = sl.d_mcap;
+        a.slab.sfill = sl.d_sfill;
+        a.slab.mfill = sl.d_mfill;
+        a.slab.stats = sl.d_stats;
+        a.slab.S = sl.S;
+        a.slab.SB = sl.SB;
+    }
     return HBT_OK;
 }
 
+// The fast kernel, and with slab binning its two binning kernels, queued on
+// st (kernel launches only: this runs inside a graph capture).
 int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
-    long blocks = std::min<long>((long)c->cu_count * c->tune.blocks_per_cu,
-                                 ((long)a.nitems + 3) / 4);
-    HIPCHECK(c, launch_fast_kernel(a, c->cfg.precision == HBT_PREC_FP32C, c->tune.R,
-                                   c->P.qinv || c->P.azimuthal, std::max(1l, blocks), st));
+    const long blocks = fast_blocks(c, a.nitems);
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
+    const bool slab = a.slab.vals != nullptr;
+    if (slab) {
+        if (blocks != c->slab.nblk)
+            return fail(c, HBT_ERR_STATE, "slab regions sized for another launch shape");
+        HIPCHECK(c, launch_zero_words(a.slab.stats, (size_t)4 * a.slab.S + 1, st));
+    }
+    HIPCHECK(c, launch_fast_kernel(a, f32, c->tune.R, c->P.qinv || c->P.azimuthal, blocks, st));
+    if (slab) {
+        const int parts = c->slab.parts;
+        if (f32)
+            hipLaunchKernelGGL(k_slab_bin<float>, dim3(a.slab.S, parts), dim3(1024), 0, st,
+                               a.slab, (int)blocks, parts, c->slab.d_parts);
+        else
+            hipLaunchKernelGGL(k_slab_bin<double>, dim3(a.slab.S, parts), dim3(1024), 0, st,
+                               a.slab, (int)blocks, parts, c->slab.d_parts);
+        HIPCHECK(c, hipGetLastError());
+        const long long nb = (long long)a.slab.S * a.slab.SB;
+        hipLaunchKernelGGL(k_slab_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
+                           (const SlabPart *)c->slab.d_parts, a.slab.S, a.slab.SB, parts,
+                           c->d_bins);
+        HIPCHECK(c, hipGetLastError());
+        c->slab.pending = true;
+    }
     return HBT_OK;
 }
 
@@ -701,7 +891,7 @@ int timed_fast(hbt_ctx *c, float *ms) {
     HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(c, hipEventSynchronize(c->ev[1]));
     HIPCHECK(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
-    return HBT_OK;
+    return slab_read_stats(c);
 }
 
 // Plain kernels: one launch per group (same) and per group (mixed).
@@ -968,9 +1158,12 @@ int prepare_and_capture(hbt_ctx *c) {
         c->batch_ready = true;
         return HBT_OK;
     }
+    c->launch_enum_same = c->batch_enum_same;
+    c->launch_enum_mixed = c->batch_enum_mixed;
     FastArgs a;
     rc = fast_args(c, a);
     if (rc) return rc;
+    c->batch_args = a;
     // The batch is one graph (counter fill + persistent kernel).  A runtime
     // that refuses the capture (a counter-collecting profiler does) gets the
     // same launches issued directly by hbt_launch_batch.
@@ -1234,7 +1427,9 @@ int hbt_destroy(hbt_ctx *c) {
                     c->d_items,     c->d_counter,   c->d_psi,        c->d_view,
                     c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff,
                     c->d_wins,      c->d_defer,     c->d_ndefer,     c->d_dkey,
-                    c->d_darr};
+                    c->d_darr,      c->slab.d_vals, c->slab.d_sidx,  c->slab.d_midx,
+                    c->slab.d_sbase, c->slab.d_scap, c->slab.d_sfill, c->slab.d_stats,
+                    c->slab.d_parts};  // d_mbase, d_mcap, d_mfill point into these
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -1304,6 +1499,8 @@ int hbt_run_same(hbt_ctx *c) {
         HIPCHECK(c, hipEventElapsedTime(&c->ms_same, c->ev[0], c->ev[1]));
     } else {
         c->batch_ready = false;
+        c->launch_enum_same = es;
+        c->launch_enum_mixed = 0;
         rc = upload_items(c, items);
         if (!rc) rc = timed_fast(c, &c->ms_same);
         if (rc) return rc;
@@ -1343,6 +1540,8 @@ int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const doubl
     } else {
         ItemList items;
         build_items(c, false, true, items, es);
+        c->launch_enum_same = 0;
+        c->launch_enum_mixed = em;
         rc = upload_items(c, items);
         if (!rc) rc = timed_fast(c, &c->ms_mixed);
         if (rc) return rc;
@@ -1383,6 +1582,13 @@ int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *par
 int hbt_launch_batch(hbt_ctx *c) {
     if (!c) return HBT_ERR_ARG;
     if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
+    if (c->slab.resize && c->cfg.kernel == HBT_KERNEL_FAST) {
+        // the last launch outgrew its slab regions: size them again from its
+        // demand (and capture the batch again) before this launch
+        int rc0 = quiesce(c);
+        if (!rc0) rc0 = prepare_and_capture(c);
+        if (rc0) return rc0;
+    }
     bool bs = false, bm = false;
     int rc = check_cap(c, false, c->batch_enum_same, bs);
     if (!rc) rc = check_cap(c, true, c->batch_enum_mixed, bm);
@@ -1397,9 +1603,7 @@ int hbt_launch_batch(hbt_ctx *c) {
         if (c->nitems > 0 && c->graph_exec) {
             HIPCHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
         } else if (c->nitems > 0) {
-            FastArgs a;
-            rc = fast_args(c, a);
-            if (!rc) rc = launch_fast(c, a, c->stream);
+            rc = launch_fast(c, c->batch_args, c->stream);
             if (rc) return rc;
         }
     } else {
@@ -1418,6 +1622,7 @@ int hbt_sync(hbt_ctx *c, double *batch_ms) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->ms_batch = ms;
     if (batch_ms) *batch_ms = c->ms_batch;
+    if (int rc = slab_read_stats(c)) return rc;
     return resolve_deferred(c);
 }
 
@@ -1433,6 +1638,20 @@ int hbt_batch_graph(hbt_ctx *c, int32_t *graph) {
     if (!c || !graph) return HBT_ERR_ARG;
     if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
     *graph = c->graph_exec != nullptr ? 1 : 0;
+    return HBT_OK;
+}
+
+int hbt_slab_info(hbt_ctx *c, int32_t *on, uint64_t *direct, uint64_t *capacity) {
+    if (!c) return HBT_ERR_ARG;
+    if (int rc = quiesce(c)) return rc;
+    if (int rc = slab_read_stats(c)) return rc;
+    const auto &sl = c->slab;
+    uint64_t cap = 0;
+    for (size_t k = 0; k < sl.scap.size(); k++)
+        cap += ((uint64_t)sl.scap[k] + (uint64_t)sl.mcap[k]) * (uint64_t)sl.nblk;
+    if (on) *on = sl.on ? 1 : 0;
+    if (direct) *direct = sl.on ? sl.direct : 0;
+    if (capacity) *capacity = sl.on ? cap : 0;
     return HBT_OK;
 }
 
@@ -1480,6 +1699,8 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
                     ItemList items;
                     uint64_t es;
                     build_items(c, true, true, items, es);
+                    c->launch_enum_same = es;
+                    c->launch_enum_mixed = c->plan_valid ? c->plan_enum_mixed : 0;
                     int rc = upload_items(c, items);
                     if (rc) return rc;
                     float ms = 0.f;

@@ -23,6 +23,7 @@
 #pragma once
 #include "hbt_device.h"
 #include "hbt_f32.h"
+#include "hbt_slab.h"
 
 namespace hbt {
 
@@ -91,6 +92,9 @@ struct FastArgs {
     int nitems;
     int *counter;              // work counter, zeroed before launch
     unsigned long long *binned;  // [2]: binned same, mixed (for stats)
+    // slab binning (hbt_slab.h) of the common-case instances; vals == null:
+    // every binned pair is added with device-scope atomics
+    SlabArgs slab;
 };
 
 // The view table does not change during a launch.  Reading it through the
@@ -183,6 +187,13 @@ struct Exact<float, EXT> {
     }
 };
 
+// A binned pair's value as the form accumulates it: FP64 as computed, or
+// rounded to FP32 in the precision form.
+template <typename Prec>
+__device__ __forceinline__ double fp_pair(double v) {
+    return std::is_same<Prec, float>::value ? (double)(float)v : v;
+}
+
 // Per-wave LDS staging of binned same-event pairs for the cooperative
 // scatter: bin index and the four sums of up to 64 pairs.
 struct Stage {
@@ -204,7 +215,8 @@ struct Stage {
 template <typename Prec, bool EXT, int QCAP>
 __device__ __forceinline__ int drain(const DevParams &P, const View &v, const unsigned *q,
                                      Stage &stg,
-                                     unsigned head, int cnt, const WorkItem &w, double psi) {
+                                     unsigned head, int cnt, const WorkItem &w, double psi,
+                                     const SlabArgs &sl, unsigned *fill, unsigned &direct_n) {
     const int lane = __lane_id();
     int st = 0;
     SameRec rec;
@@ -225,21 +237,35 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
         }
     }
     const bool binned = (st & ST_BIN) != 0;
+    // slab binning: the pair becomes a record of its (block, slab) region;
+    // a full region sends it down the direct path
+    const bool slab = !EXT && sl.vals != nullptr;
     if (w.kind != 0) {
-        if (binned) add_mixed_direct(v.H, kmix);
+        if (binned && !(slab && slab_put_mixed(sl, fill, kmix))) {
+            add_mixed_direct(v.H, kmix);
+            direct_n += slab ? 1u : 0u;
+        }
         return binned ? 1 : 0;
     }
-    const unsigned long long m = __ballot(binned);
+    bool direct = binned;
+    if (slab && binned) {
+        direct = !slab_put_same<Prec>(sl, fill, rec.k, rec.qo, rec.qs, rec.ql, rec.c);
+        direct_n += direct ? 1u : 0u;
+    }
+    const unsigned long long m = __ballot(direct);
     const int nb = __popcll(m);
-    if (nb == 0) return 0;
-    if (binned) {
+    if (nb == 0) return binned ? 1 : 0;
+    if (direct) {
         const unsigned idx = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         stg.k[idx] = rec.k;
-        stg.v[idx][0] = rec.qo;
-        stg.v[idx][1] = rec.qs;
-        stg.v[idx][2] = rec.ql;
-        stg.v[idx][3] = rec.c;
+        // the precision form's per-pair values are FP32 on either path (slab
+        // records hold them as FP32), so the path a pair takes does not
+        // change what it adds
+        stg.v[idx][0] = fp_pair<Prec>(rec.qo);
+        stg.v[idx][1] = fp_pair<Prec>(rec.qs);
+        stg.v[idx][2] = fp_pair<Prec>(rec.ql);
+        stg.v[idx][3] = fp_pair<Prec>(rec.c);
     }
     __builtin_amdgcn_wave_barrier();
     // lane sub of a group adds field sub of its pair's bin: the count (an
@@ -303,6 +329,14 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<R, EXT>()))) void k_fast(Fa
     constexpr int QCAP = Ring<R>::CAP;
     __shared__ unsigned ring[4][QCAP];
     __shared__ Stage stage[4];
+    // slab binning: records per slab of this block (same, then mixed)
+    __shared__ unsigned slab_fill[EXT ? 1 : 2 * kMaxSlabs];
+    const bool slab = !EXT && a.slab.vals != nullptr;
+    if (slab) {
+        for (int k = threadIdx.x; k < 2 * kMaxSlabs; k += blockDim.x) slab_fill[k] = 0u;
+        __syncthreads();
+    }
+    unsigned direct_n = 0;
     const int lane = __lane_id();
     unsigned *q = ring[threadIdx.x >> 6];
     Stage &stg = stage[threadIdx.x >> 6];
@@ -381,7 +415,8 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<R, EXT>()))) void k_fast(Fa
                     if (tail - head >= 64 || (last && t >= cnt && tail != head)) {
                         const int n = (int)min(64u, tail - head);
                         __builtin_amdgcn_wave_barrier();
-                        const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, psi);
+                        const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, psi,
+                                                              a.slab, slab_fill, direct_n);
                         if (mixed)
                             nb_mixed += nb;
                         else
@@ -446,6 +481,18 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<R, EXT>()))) void k_fast(Fa
     if (lane == 0) {
         if (nb_same) atomic_add_u64(&a.binned[0], nb_same);
         if (nb_mixed) atomic_add_u64(&a.binned[1], nb_mixed);
+    }
+    if (slab) {
+        // the block's demand per slab, for k_slab_bin (which clamps it to the
+        // regions' capacity) and for sizing later launches
+        unsigned long long dn = direct_n;
+        for (int off = 32; off > 0; off >>= 1) dn += __shfl_down(dn, off);
+        if (lane == 0 && dn) atomic_add_u64(&a.slab.stats[2 * a.slab.S], dn);
+        __syncthreads();
+        for (int k = threadIdx.x; k < a.slab.S; k += blockDim.x) {
+            a.slab.sfill[(long long)blockIdx.x * a.slab.S + k] = slab_fill[k];
+            a.slab.mfill[(long long)blockIdx.x * a.slab.S + k] = slab_fill[kMaxSlabs + k];
+        }
     }
 }
 

@@ -427,6 +427,7 @@ void launcher_loop(hbt_server *s) {
                                hipMemcpyHostToDevice, s->s_compute);
             if (e == hipSuccess) e = hipEventRecord(s->step_t0[buf], s->s_compute);
             FastArgs a;
+            a.slab = SlabArgs{};  // the server adds binned pairs with atomics
             a.P = s->D.P;
             if (s->f32) fp32_margins(s->D, pmax2, a.P);
             a.views = s->d_views;

@@ -32,6 +32,7 @@ HBT_correlation::HBT_correlation(ParameterReader &paraRdr, std::string path,
     c.kernel = (int)paraRdr_.getVal("hbt_kernel", HBT_KERNEL_FAST);
     c.device = (int)paraRdr_.getVal("hbt_device", 0);
     c.kphi_guard_prad = (int)paraRdr_.getVal("hbt_kphi_guard_prad", 0);
+    c.binning = (int)paraRdr_.getVal("hbt_binning", 0);
     check(hbt_create(&c, &ctx_), "hbt_create");
 }
 

@@ -12,6 +12,7 @@
 //   hbt_device     HIP device ordinal (default 0)
 //   hbt_kphi_guard_prad  K_phi guard band in 1e-12 rad (default 1): pairs
 //                  that close to a K_phi bin edge are binned on the host
+//   hbt_binning    0 = device-scope atomics (default), 1 = slab binning
 //   number_of_mixed_events  partners per event (default: the reference's
 //                           floor(nev/2)+1)
 #ifndef HBT_AMD_HBT_CORRELATION_H

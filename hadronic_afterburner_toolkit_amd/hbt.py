@@ -47,6 +47,8 @@ def config_from_params(params: dict, precision: int = _lib.HBT_PREC_FP64,
     cfg.precision, cfg.kernel, cfg.device = precision, kernel, device
     # engine key: K_phi guard band in 1e-12 rad (0: the default 1e-12 rad)
     cfg.kphi_guard_prad = int((params or {}).get("hbt_kphi_guard_prad", 0))
+    # engine key: 0 device-scope atomics (default), 1 slab binning where it applies
+    cfg.binning = int((params or {}).get("hbt_binning", 0))
     return cfg
 
 
@@ -283,6 +285,15 @@ class HBTCorrelation:
         self._check(self._lib.hbt_batch_graph(self.h, C.byref(v)), "hbt_batch_graph")
         return bool(v.value)
 
+    def slab_info(self) -> dict:
+        """Slab binning of the last fast launch: on, pairs added directly
+        because their region was full, region capacity in records."""
+        on = C.c_int32()
+        d, cap = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+        self._check(self._lib.hbt_slab_info(self.h, C.byref(on), _u64p(d), _u64p(cap)),
+                    "hbt_slab_info")
+        return {"on": bool(on.value), "direct": int(d[0]), "capacity": int(cap[0])}
+
     def tuning(self) -> dict:
         t = HbtTuning()
         self._check(self._lib.hbt_get_tuning(self.h, C.byref(t)), "hbt_get_tuning")
@@ -406,6 +417,8 @@ def _make_config(params, precision, kernel, device):
     cfg.device = device
     # engine key: K_phi guard band in 1e-12 rad (0: the default 1e-12 rad)
     cfg.kphi_guard_prad = int((params or {}).get("hbt_kphi_guard_prad", 0))
+    # engine key: 0 device-scope atomics (default), 1 slab binning where it applies
+    cfg.binning = int((params or {}).get("hbt_binning", 0))
     return p, cfg
 
 

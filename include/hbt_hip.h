@@ -72,7 +72,14 @@ typedef struct {
      * computes them; the device's atan2 may differ from it in the last bits.
      * 0: 1 (1e-12 rad).  Larger values only move more pairs to the host. */
     int32_t kphi_guard_prad;
-    int32_t reserved[4];
+    /* histogram accumulation of the fast kernel: 0 = every binned pair added
+     * to the histogram with device-scope atomics (the faster on MI355X);
+     * 1 = slab binning where it applies (no K_phi, no q_inv,
+     * qnpts^2 <= 2048, n_KT * qnpts <= 512): binned pairs written as records
+     * per (block, q_out plane) and summed in LDS, no device-scope atomics.
+     * Counts are the same; sums agree within each form's tolerance. */
+    int32_t binning;
+    int32_t reserved[3];
 } hbt_config;
 
 typedef struct hbt_ctx hbt_ctx;
@@ -226,6 +233,13 @@ int hbt_batch_info(hbt_ctx *ctx, uint64_t *enum_same, uint64_t *enum_mixed, int3
  * profiler).  HBT_ERR_STATE without a prepared batch.  No counterpart in the
  * reference (its loops run on the host). */
 int hbt_batch_graph(hbt_ctx *ctx, int32_t *graph);
+
+/* Slab binning of the last fast launch (hbt_config.binning): *on = 1 when
+ * its binned pairs went through slab records, *direct = how many of them
+ * took the direct atomic path because their (block, slab) region was full,
+ * *capacity = records the regions held (same-event + mixed).  No
+ * counterpart in the reference. */
+int hbt_slab_info(hbt_ctx *ctx, int32_t *on, uint64_t *direct, uint64_t *capacity);
 
 /* ---- allocation-free entry point ------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
