@@ -319,6 +319,7 @@ struct hbt_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool batch_ready = false;
+    bool batch_capture = true;   // the prepared batch is captured as a graph (launched more than once)
     std::string err;
     uint64_t needed = 0, needed_qinv = 0;
     uint64_t enum_same_total = 0, enum_mixed_total = 0;  // since reset
@@ -778,40 +779,57 @@ struct ItemList {
 // count is known from the plan (upload_plan).
 void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_same) {
     const Slot &s = c->slot[0];
-    const ItemShape sh = item_shape(c->tune);
+    ItemShape sh = item_shape(c->tune);
     const double D = px_window(c->D);
-    L.items.clear();
-    L.wins.clear();
-    enum_same = 0;
+    // a launch with fewer items than kItemsPerWave per resident wave (one
+    // buffer of the per-buffer path, a small request) leaves most waves idle
+    // and ends on its largest item: its j chunks and partner windows per item
+    // shrink (4x per round, to 256 j's and one window) until it has enough
+    const size_t waves = (size_t)std::max(1, c->cu_count) * (size_t)c->tune.blocks_per_cu * 4;
     std::vector<uint64_t> pairs;
-    if (same)
-        items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px.data(), D, 0, L.items, &pairs,
-                   enum_same);
-    if (mixed && c->plan_valid) {
-        const Slot &m = mixed_slot(c);
-        items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt.data(),
-                    m.f_pt.data(), D, 0, L.items, L.wins, &pairs);
+    for (;;) {
+        L.items.clear();
+        L.wins.clear();
+        pairs.clear();
+        enum_same = 0;
+        if (same)
+            items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px.data(), D, 0, L.items, &pairs,
+                       enum_same);
+        if (mixed && c->plan_valid) {
+            const Slot &m = mixed_slot(c);
+            items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt.data(),
+                        m.f_pt.data(), D, 0, L.items, L.wins, &pairs);
+        }
+        const size_t want = kItemsPerWave * waves;
+        if (L.items.size() >= want || (sh.JC <= 256 && sh.KC <= 1)) break;
+        // items grow about inversely with the chunk sizes: one rebuild with
+        // both cut by the missing factor (rounded up to a power of two)
+        int f = 2;
+        while ((size_t)f * L.items.size() < want && f < 64) f *= 2;
+        sh.JC = std::max(256, sh.JC / f);
+        sh.KC = std::max(1, sh.KC / f);
     }
     // larger items first, by factor-4 size classes (stable inside a class,
     // which keeps a group's items together): the persistent waves pull items
     // in queue order, so the launch ends on small items instead of one wave
     // finishing a large one (uneven multiplicities make items differ by 10x
     // and more)
-    auto size_class = [&](uint32_t k) {
+    // (a counting sort over the 33 classes of a 64-bit count)
+    auto size_class = [&](size_t k) {
         int c = 0;
         for (uint64_t v = pairs[k]; v >= 4; v >>= 2) c++;
         return c;
     };
-    std::vector<uint32_t> order(L.items.size());
-    std::vector<int> cls(L.items.size());
-    for (size_t k = 0; k < order.size(); k++) {
-        order[k] = (uint32_t)k;
-        cls[k] = size_class((uint32_t)k);
+    const size_t n = L.items.size();
+    std::vector<uint8_t> cls(n);
+    size_t start[34] = {};
+    for (size_t k = 0; k < n; k++) {
+        cls[k] = (uint8_t)size_class(k);
+        start[32 - cls[k] + 1]++;
     }
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return cls[a] > cls[b]; });
-    std::vector<WorkItem> sorted(L.items.size());
-    for (size_t k = 0; k < order.size(); k++) sorted[k] = L.items[order[k]];
+    for (int c = 1; c < 34; c++) start[c] += start[c - 1];
+    std::vector<WorkItem> sorted(n);
+    for (size_t k = 0; k < n; k++) sorted[start[32 - cls[k]]++] = L.items[k];
     L.items.swap(sorted);
 }
 
@@ -1166,9 +1184,12 @@ int prepare_and_capture(hbt_ctx *c) {
     c->batch_args = a;
     // The batch is one graph (counter fill + persistent kernel).  A runtime
     // that refuses the capture (a counter-collecting profiler does) gets the
-    // same launches issued directly by hbt_launch_batch.
+    // same launches issued directly by hbt_launch_batch, and so does a batch
+    // launched once (the per-buffer path), for which instantiating a graph
+    // costs more than it saves.
     hipGraph_t g = nullptr;
-    bool captured = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    bool captured = c->batch_capture &&
+                    hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (captured) {
         rc = launch_fast(c, a, c->stream);
         captured = hipStreamEndCapture(c->stream, &g) == hipSuccess && !rc && g;
@@ -1552,8 +1573,8 @@ int hbt_run_mixed(hbt_ctx *c, int32_t M, const int32_t *partner_ids, const doubl
     return HBT_OK;
 }
 
-int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *partner_ids,
-                      const double *cos_sin) {
+static int prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *partner_ids,
+                  const double *cos_sin, bool capture) {
     if (!c || !M_per_group) return HBT_ERR_ARG;
     int rc = check_slots(c);
     if (!rc) rc = quiesce(c);
@@ -1576,7 +1597,13 @@ int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *par
     uint64_t em = 0;
     rc = upload_plan(c, Me, partner_ids, cos_sin, em);
     if (rc) return rc;
+    c->batch_capture = capture;
     return prepare_and_capture(c);
+}
+
+int hbt_prepare_batch(hbt_ctx *c, const int32_t *M_per_group, const int32_t *partner_ids,
+                      const double *cos_sin) {
+    return prepare_batch(c, M_per_group, partner_ids, cos_sin, true);
 }
 
 int hbt_launch_batch(hbt_ctx *c) {
@@ -1620,7 +1647,10 @@ int hbt_sync(hbt_ctx *c, double *batch_ms) {
     if (!c) return HBT_ERR_ARG;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->ms_batch = ms;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess)
+        c->ms_batch = ms;
+    else
+        (void)hipGetLastError();  // no batch timed yet: not an error of a later call
     if (batch_ms) *batch_ms = c->ms_batch;
     if (int rc = slab_read_stats(c)) return rc;
     return resolve_deferred(c);
@@ -1789,6 +1819,8 @@ int hbt_read(hbt_ctx *c, hbt_result *o) {
 
 int hbt_deferred_pairs(hbt_ctx *c, uint64_t *n) {
     if (!c || !n) return HBT_ERR_ARG;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (int rc = resolve_deferred(c)) return rc;
     *n = c->deferred_total;
     return HBT_OK;
 }
@@ -1867,6 +1899,11 @@ int hbt_calculate_buffer(hbt_ctx *c, hbt_rng *rng) {
     if (!c || !rng) return HBT_ERR_ARG;
     int rc = check_slots(c);
     if (rc) return rc;
+    // the previous buffer's launch (already waited for by this buffer's
+    // upload): its pairs left to the host are decided before this launch
+    // adds its own
+    rc = hbt_sync(c, nullptr);
+    if (rc) return rc;
     Slot &s = c->slot[0];
     const Slot &m = mixed_slot(c);
     std::vector<int32_t> Mg(s.ngroups, 0);
@@ -1886,9 +1923,10 @@ int hbt_calculate_buffer(hbt_ctx *c, hbt_rng *rng) {
         hbt_rng_draw_plan(rng, nev, mixed_nev, c->cfg.number_of_mixed_events, ids.data() + base,
                           cs.data() + 2 * base);
     }
-    rc = hbt_prepare_batch(c, Mg.data(), ids.data(), cs.data());
+    rc = prepare_batch(c, Mg.data(), ids.data(), cs.data(), false);
+    // not waited for: the caller's next upload does its host work while the
+    // launch runs (hbt_read, hbt_sync and the next upload wait for it)
     if (!rc) rc = hbt_launch_batch(c);
-    if (!rc) rc = hbt_sync(c, nullptr);
     return rc;
 }
 

@@ -187,11 +187,12 @@ struct Exact<float, EXT> {
     }
 };
 
-// A binned pair's value as the form accumulates it: FP64 as computed, or
-// rounded to FP32 in the precision form.
+// A binned pair's value as the form accumulates it: FP64 as computed, or,
+// in the precision form with slab binning (whose records hold FP32 values),
+// rounded to FP32 on the direct path too.
 template <typename Prec>
-__device__ __forceinline__ double fp_pair(double v) {
-    return std::is_same<Prec, float>::value ? (double)(float)v : v;
+__device__ __forceinline__ double fp_pair(double v, bool slab) {
+    return std::is_same<Prec, float>::value && slab ? (double)(float)v : v;
 }
 
 // Per-wave LDS staging of binned same-event pairs for the cooperative
@@ -259,13 +260,13 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
         const unsigned idx = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         stg.k[idx] = rec.k;
-        // the precision form's per-pair values are FP32 on either path (slab
-        // records hold them as FP32), so the path a pair takes does not
-        // change what it adds
-        stg.v[idx][0] = fp_pair<Prec>(rec.qo);
-        stg.v[idx][1] = fp_pair<Prec>(rec.qs);
-        stg.v[idx][2] = fp_pair<Prec>(rec.ql);
-        stg.v[idx][3] = fp_pair<Prec>(rec.c);
+        // with slab binning the precision form's per-pair values are FP32 on
+        // either path (its records hold them as FP32), so the path a pair
+        // takes does not change what it adds; without it they stay FP64
+        stg.v[idx][0] = fp_pair<Prec>(rec.qo, slab);
+        stg.v[idx][1] = fp_pair<Prec>(rec.qs, slab);
+        stg.v[idx][2] = fp_pair<Prec>(rec.ql, slab);
+        stg.v[idx][3] = fp_pair<Prec>(rec.c, slab);
     }
     __builtin_amdgcn_wave_barrier();
     // lane sub of a group adds field sub of its pair's bin: the count (an
@@ -313,17 +314,21 @@ struct Ring {
 // Occupancy: the exact stage is latency-bound (dependent loads of the two
 // particle records per lane), so the kernels trade a few spilled registers
 // for more waves per SIMD: common case 5 at R=1 (96 VGPRs) and 4 at R=2
-// (128 VGPRs), 5 % faster on the bench workload; K_phi / q_inv variants 3
-// at R=1, 2 (168 VGPRs), 11-16 % faster on the K_phi workloads.  R=4 keeps
-// the compiler's choice.
-template <int R, bool EXT>
+// (128 VGPRs), 5 % faster on the bench workload.  The K_phi / q_inv
+// instances carry the literal cascade with atan2 and the host-defer path
+// inline (168 VGPRs unconstrained): the precision form's, which reaches it
+// only for guard-band pairs, runs at 4 (128 VGPRs, spills in that rare
+// path), 10 % faster than at 3 on the batched K_phi workload; the FP64
+// form's, where it is the main path, at 3.  R=4 keeps the compiler's choice.
+template <typename Prec, int R, bool EXT>
 constexpr int fast_waves_per_eu() {
-    return R == 4 ? 1 : (EXT ? 3 : (R == 1 ? 5 : 4));
+    return R == 4 ? 1
+                  : (EXT ? (std::is_same<Prec, float>::value ? 4 : 3) : (R == 1 ? 5 : 4));
 }
 
 template <typename Prec, int R, bool EXT>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<R, EXT>()))) void k_fast(FastArgs a) {
+__attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_fast(FastArgs a) {
     using Rec = typename std::conditional<std::is_same<Prec, double>::value, PfRec64,
                                           PfRec32>::type;
     constexpr int QCAP = Ring<R>::CAP;

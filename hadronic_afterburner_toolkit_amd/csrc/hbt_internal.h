@@ -6,12 +6,16 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <unistd.h>
+
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -202,24 +206,105 @@ inline double px_window(const Derived &d) {
     return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
 }
 
-// Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476),
-// psi_ref per group (:181-183, :233-249; over all particles passed in), and
-// the prefilter records.  Events keep < 2^18 particles (the fast kernel's
-// survivor encoding).
+// Host threads of the upload path, started once per process: a per-buffer
+// upload is a few milliseconds of host work, and starting threads for each
+// of its passes would cost as much as the passes themselves.  run() hands
+// tasks 0..nt-1 to the workers and the caller; while the pool is busy (a
+// nested call, another thread's upload) or in a forked child, the caller
+// runs the tasks itself.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool pool;
+        return pool;
+    }
+    int threads() const { return (int)workers_.size() + 1; }
+    void run(int nt, const std::function<void(int)> &fn) {
+        std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
+        if (!busy.owns_lock() || getpid() != pid_ || workers_.empty()) {
+            for (int t = 0; t < nt; t++) fn(t);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = &fn;
+            ntask_ = nt;
+            next_ = 0;
+            left_ = nt;
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return left_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    HostPool() : pid_(getpid()) {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        for (unsigned t = 1; t < hw; t++) workers_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+            gen_++;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_)
+            if (t.joinable()) t.join();
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    // takes tasks of the current run until none is left
+    void work() {
+        for (;;) {
+            const std::function<void(int)> *f;
+            int t;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (!fn_ || next_ >= ntask_) return;
+                t = next_++;
+                f = fn_;
+            }
+            (*f)(t);
+            std::lock_guard<std::mutex> g(mu_);
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> workers_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int ntask_ = 0, next_ = 0, left_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    const pid_t pid_;
+};
+
 // Runs fn(begin, end) over [0, n) split into contiguous ranges on up to 16
 // host threads (one when n is small).
 template <typename F>
 inline void parallel_ranges(long long n, long long min_per_thread, F fn) {
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const long long nt = std::max(1ll, std::min<long long>(hw, n / std::max(1ll, min_per_thread)));
+    HostPool &pool = HostPool::get();
+    const long long nt =
+        std::max(1ll, std::min<long long>(pool.threads(), n / std::max(1ll, min_per_thread)));
     if (nt <= 1) {
         fn(0ll, n);
         return;
     }
-    std::vector<std::thread> th;
-    for (long long t = 0; t < nt; t++)
-        th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
-    for (auto &x : th) x.join();
+    pool.run((int)nt, [&](int t) { fn(n * t / nt, n * (t + 1) / nt); });
 }
 
 // Host-to-device copies through a pinned staging buffer: a pageable copy
@@ -285,6 +370,10 @@ inline hipError_t stage_pieces(char *stage, const std::vector<StagePiece> &v, hi
 // after_keep (optional) runs once the kept particles (keep, off) and the
 // list sizes (n, nf) are known, before the px sort; a non-zero return ends
 // the gather with that code.
+// Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476),
+// psi_ref per group (:181-183, :233-249; over all particles passed in), and
+// the prefilter records.  Events keep < 2^18 particles (the fast kernel's
+// survivor encoding).
 inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                             const int32_t *grp_ev, int nev, const int64_t *ev_off,
                             const double *const soa[8], HostParticles &h, std::string &err,
@@ -322,7 +411,9 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     h.off.assign(nev + 1, 0);
     h.foff.assign(nev + 1, 0);
     std::vector<double> pmax((size_t)nev + 1, 0.0), bmax((size_t)nev + 1, 0.0);
-    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
+    // events per host thread: about 2048 particles' worth at least
+    const long long ev_grain = std::max(1ll, 2048ll * nev / std::max(1ll, h.n_in));
+    parallel_ranges(nev, ev_grain, [&](long long e0, long long e1) {
         for (long long e = e0; e < e1; e++) {
             long long c = 0, cf = 0;
             for (int64_t i = ev_off[e]; i < ev_off[e + 1]; i++) {
@@ -359,7 +450,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     const double eps32 = 5.9604644775390625e-8;
     // pass 2, per event: gather order, then the fast list sorted by |pT|
     // (stable, so equal |pT| keep gather order)
-    parallel_ranges(nev, 64, [&](long long e0, long long e1) {
+    parallel_ranges(nev, ev_grain, [&](long long e0, long long e1) {
         std::vector<std::pair<double, long long>> ev;  // |pT|, kept index
         std::vector<int64_t> src;
         for (long long e = e0; e < e1; e++) {
@@ -449,6 +540,9 @@ struct ItemShape {
     int KC;  // mixed partner windows per item
 };
 
+// Work items per resident wave below which a launch's items are made smaller.
+constexpr size_t kItemsPerWave = 4;
+
 inline ItemShape item_shape(const hbt_tuning &t) {
     return ItemShape{64 * t.R, std::max(64, t.jchunk), std::max(1, std::min(t.kchunk, 64))};
 }
@@ -495,6 +589,21 @@ inline void items_by_group_chunks(int ngroups, std::vector<WorkItem> &items,
 // pairs (optional) receives each item's evaluated pair count.  enum_same
 // receives the reference's enumerated pair count sum n(n-1)/2 over the kept
 // lists (off).
+// Pairs (i, j) with i in [a, a + ni), j in [jb, j1) and i < j, in closed
+// form: i < jb pairs with all of [jb, j1); i >= jb with (i, j1).
+inline uint64_t tile_pairs(long long a, long long ni, long long jb, long long j1) {
+    uint64_t cnt = 0;
+    const long long ie = a + ni;
+    const long long e1 = std::min(ie, jb);
+    if (e1 > a && j1 > jb) cnt += (uint64_t)(e1 - a) * (uint64_t)(j1 - jb);
+    const long long b2 = std::max(a, jb), e2 = std::min(ie, j1 - 1);
+    if (e2 > b2) {  // i = b2 .. e2-1 contribute j1-1-i: the integers lo..hi
+        const long long n = e2 - b2, hi = j1 - 1 - b2, lo = j1 - e2;
+        cnt += (uint64_t)((lo + hi) * n / 2);
+    }
+    return cnt;
+}
+
 inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                        const std::vector<long long> &foff, const std::vector<int> &grp_ev,
                        const double *srt_px, double dpx, int view,
@@ -527,16 +636,7 @@ inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                     w.grp = g;
                     w.view = view;
                     out.items.push_back(w);
-                    if (pairs) {
-                        // pairs (i, j) with i in [a, a+ni), j in [jb, jb+nj), i < j
-                        uint64_t cnt = 0;
-                        const long long j1 = jb + w.nj;
-                        for (long long i = a; i < a + ni; i++) {
-                            const long long lo = std::max(jb, i + 1);
-                            if (j1 > lo) cnt += (uint64_t)(j1 - lo);
-                        }
-                        out.pairs.push_back(cnt);
-                    }
+                    if (pairs) out.pairs.push_back(tile_pairs(a, ni, jb, jb + w.nj));
                 }
             }
         }

@@ -25,8 +25,9 @@
 //     histogram (count, sums, den), plain read-modify-write: each bin has
 //     one owner.
 //
-// The precision form's per-pair values are FP32, accumulated in FP64: the
-// per-bin reduction is the only FP64 part of its histogram path.
+// In slab mode the precision form's per-pair values are FP32 (records and
+// the direct path alike), accumulated in FP64; without slab binning they
+// stay FP64.
 #pragma once
 #include <type_traits>
 

@@ -186,7 +186,9 @@ int32_t hbt_rng_draw_plan(hbt_rng *rng, int32_t nev, int32_t mixed_nev,
  * computed at upload over all its particles (:181-183), the mixing plan of
  * each group drawn from rng in group order (:200-216, :495-497), then all
  * same-event and mixed pairs as one batch.  Histograms accumulate across
- * calls like the reference's. */
+ * calls like the reference's.  Returns once the batch is launched: the next
+ * hbt_upload does its host work while it runs, and hbt_read, hbt_sync,
+ * hbt_stage_counts and the next upload wait for it. */
 int hbt_calculate_buffer(hbt_ctx *ctx, hbt_rng *rng);
 
 /* ---------------- batched form ----------------

@@ -4,7 +4,7 @@
 and with device-scope atomics (hbt_binning = 0, the default) beside it:
   - counts (bins, denominators, K_T counters): bit-exact;
   - sums: err = |S - S_ref| / max(|S_ref|, sum|.|_bin) within 1e-12 (FP64)
-    and 1e-6 (precision form: its per-pair values are stored in FP32).
+    and 1e-6 (precision form: in slab mode its per-pair values are FP32).
 A sample whose pairs all fall into a few bins overflows the regions on the
 first launch (those pairs take the direct path, results unchanged); the
 next launch is sized from the measured demand."""
