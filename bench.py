@@ -39,6 +39,9 @@ PARAMS = dict(qnpts=41, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
               number_of_mixed_events=10, long_comoving_boost=1)
 
 
+BENCH_BINS = 41 ** 3 * 5  # histogram bins of the headline workload (q 41^3, 5 K_T bins)
+
+
 def make_workload(nev, mult, per_group, seed):
     from hadronic_afterburner_toolkit_amd import samples
     full = samples.generate(seed, nev, mult)
@@ -228,11 +231,21 @@ def batch_line(groups, params, seed, prec, steps, tuning, autotune, log, label):
                 binned_same=int(sc[4]) // launches, binned_mixed=int(mc[4]) // launches), res
 
 
-def add_atomic_floor(line, rates):
-    """The launch's atomic floor: binned pairs over the measured rates."""
-    if rates:
-        line["atomic_floor_ms"] = 1e3 * (line["binned_same"] / rates["same"]
-                                         + line["binned_mixed"] / rates["mixed"])
+def add_atomic_floor(line, rates, nbins=None):
+    """The launch's atomic floor: binned pairs over the measured rates, those
+    of the micro-benchmark's histogram size nearest to nbins (in ratio) when
+    it measured several."""
+    if not rates:
+        return
+    same, mixed, at = rates["same"], rates["mixed"], BENCH_BINS
+    if nbins and rates.get("by_bins"):
+        at = min(rates["by_bins"], key=lambda n: abs(np.log(n / nbins)))
+        if abs(np.log(at / nbins)) < abs(np.log(BENCH_BINS / nbins)):
+            same, mixed = rates["by_bins"][at]
+        else:
+            at = BENCH_BINS
+    line["atomic_floor_ms"] = 1e3 * (line["binned_same"] / same + line["binned_mixed"] / mixed)
+    line["atomic_floor_bins"] = at
 
 
 def batched_uneven(args, log):
@@ -529,7 +542,7 @@ def atomic_ceiling(log):
     except (OSError, subprocess.TimeoutExpired) as e:
         log(f"atomics micro-benchmark failed: {e}")
         return None
-    rates = {}
+    rates, same_n, mixed_n = {}, {}, {}
     for line in r.stdout.splitlines():
         parts = line.split()
         if r.returncode == 0 and len(parts) >= 4:
@@ -537,9 +550,15 @@ def atomic_ceiling(log):
                 rates["same"] = float(parts[3])
             elif parts[0] == "u32carry_1_per_lane":
                 rates["mixed"] = float(parts[3])
+            elif parts[0].startswith("f64coop5_bins_"):
+                same_n[int(parts[0].rsplit("_", 1)[1])] = float(parts[3])
+            elif parts[0].startswith("u32carry_bins_"):
+                mixed_n[int(parts[0].rsplit("_", 1)[1])] = float(parts[3])
     if len(rates) != 2:
         log("atomics micro-benchmark printed no f64coop5 / u32carry_1_per_lane line")
         return None
+    # the same two patterns at the K_phi workloads' histogram sizes
+    rates["by_bins"] = {n: (same_n[n], mixed_n[n]) for n in same_n if n in mixed_n}
     return rates
 
 
@@ -743,9 +762,10 @@ def main():
         out["batched_uneven"] = batched_uneven(args, log)
         out["large_histogram"] = large_histogram(
             args, log, (t["R"], t["jchunk"], t["kchunk"], t["blocks_per_cu"]))
-        for line in (out["batched_uneven"]["pi+"], out["batched_uneven"]["K+"],
-                     out["large_histogram"]):
-            add_atomic_floor(line, atom_ceil)
+        for line, nb in ((out["batched_uneven"]["pi+"], BENCH_BINS * 16),
+                         (out["batched_uneven"]["K+"], BENCH_BINS * 16),
+                         (out["large_histogram"], out["large_histogram"]["bins"])):
+            add_atomic_floor(line, atom_ceil, nb)
     if not args.no_extra:
         out["ingest"] = ingest(args, log)
         out["balance_function"] = balance_function(args, log)

@@ -210,8 +210,8 @@ inline double px_window(const Derived &d) {
 // upload is a few milliseconds of host work, and starting threads for each
 // of its passes would cost as much as the passes themselves.  run() hands
 // tasks 0..nt-1 to the workers and the caller; while the pool is busy (a
-// nested call, another thread's upload) or in a forked child, the caller
-// runs the tasks itself.
+// nested call, another thread's upload: the server's concurrent requests)
+// or in a forked child, the call starts threads of its own instead.
 class HostPool {
   public:
     static HostPool &get() {
@@ -222,7 +222,10 @@ class HostPool {
     void run(int nt, const std::function<void(int)> &fn) {
         std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
         if (!busy.owns_lock() || getpid() != pid_ || workers_.empty()) {
-            for (int t = 0; t < nt; t++) fn(t);
+            std::vector<std::thread> th;
+            for (int t = 1; t < nt; t++) th.emplace_back(fn, t);
+            fn(0);
+            for (auto &x : th) x.join();
             return;
         }
         {

@@ -27,9 +27,15 @@
 //   coop5_xcd_part     one histogram, each XCD touching only the bins
 //                      b = xcc (mod 8)
 //
+// The fast kernel's two patterns at the histogram sizes of the K_phi
+// workloads (f64coop5_bins_N, u32carry_bins_N): 41^3 x 5 x 16 bins (353 MB,
+// the batched-uneven form) and 61^3 x 9 x 16 bins (2.1 GB, the
+// large-histogram form).
+//
 // Prints one line per pattern: ms per launch and binned pairs/s.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 
 #define CK(x)                                                                       \
@@ -177,7 +183,10 @@ __global__ void k_lds(double *h, unsigned nbins, unsigned per) {
 
 int main() {
     const unsigned nbins = 41u * 41u * 41u * 5u;
-    const size_t bytes = (size_t)nbins * 8 * sizeof(double) * 8;  // room for 8 copies
+    const unsigned nbins_kphi = nbins * 16u, nbins_large = 61u * 61u * 61u * 9u * 16u;
+    // room for 8 copies of the bench histogram, and for the largest one
+    const size_t bytes = std::max((size_t)nbins * 8 * sizeof(double) * 8,
+                                  (size_t)nbins_large * 8 * sizeof(double));
     double *h = nullptr;
     CK(hipMalloc(&h, bytes));
     CK(hipMemset(h, 0, bytes));
@@ -262,6 +271,31 @@ int main() {
         const double n = (double)p.blocks * threads * per * p.pairs_per_lane_iter;
         std::printf("%-18s %8.3f ms/launch  %.3e binned pairs/s\n", p.name, per_launch,
                     n / (per_launch * 1e-3));
+        std::fflush(stdout);
+    }
+    const struct {
+        const char *name;
+        void (*k)(double *, unsigned, unsigned);
+        unsigned nbins;
+        double pairs_per_lane_iter;
+    } sized[] = {{"f64coop5_bins_5513680", k_f64coop5, nbins_kphi, 12.0 / 64.0},
+                 {"u32carry_bins_5513680", k_u32carry, nbins_kphi, 1.0},
+                 {"f64coop5_bins_32685264", k_f64coop5, nbins_large, 12.0 / 64.0},
+                 {"u32carry_bins_32685264", k_u32carry, nbins_large, 1.0}};
+    for (const auto &p : sized) {
+        hipLaunchKernelGGL(p.k, dim3(blocks), dim3(threads), 0, 0, h, p.nbins, per);
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+        CK(hipEventRecord(e0));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(p.k, dim3(blocks), dim3(threads), 0, 0, h, p.nbins, per);
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        const double per_launch = ms / reps;
+        std::printf("%-18s %8.3f ms/launch  %.3e binned pairs/s\n", p.name, per_launch,
+                    lanes * p.pairs_per_lane_iter / (per_launch * 1e-3));
         std::fflush(stdout);
     }
     CK(hipFree(h));
