@@ -153,7 +153,10 @@ __global__ __launch_bounds__(TILE) void k_mixed_cap(DevParams P, DevHist H, DevS
     if (!COUNT) wave_add_stage(&H.stage[9], c_bin);
 }
 
-__global__ void k_zero_int(int *p) { *p = 0; }
+// The work counters of a launch's queues (one per 64-byte line).
+__global__ void k_zero_counters(int *p, int nq) {
+    if ((int)threadIdx.x < nq) p[hbt::kQueueStride * threadIdx.x] = 0;
+}
 
 // Arrivals of host-decided pairs (K_phi at a bin edge) at the 3D cap.
 __global__ void k_cap_add_arrivals(unsigned *cnt, const long long *idx, int n) {
@@ -278,6 +281,8 @@ struct hbt_ctx {
     JWin *d_wins = nullptr;  // partner windows of the mixed items
     size_t wins_cap = 0;
     int nitems = 0;
+    int nqueues = 1;                          // work queues of the uploaded items
+    std::vector<int> qoff = std::vector<int>(2, 0);
     int *d_counter = nullptr;
     View *d_view = nullptr;  // the context's one analysis view
     // ordered pair-cap path (allocated when a cap binds)
@@ -713,6 +718,9 @@ int fast_args(hbt_ctx *c, FastArgs &a) {
     a.views = c->d_view;
     a.items = c->d_items;
     a.nitems = c->nitems;
+    a.nqueues = c->nqueues;
+    for (int q = 0; q <= kMaxQueues; q++)
+        a.qoff[q] = q <= c->nqueues ? c->qoff[(size_t)q] : c->nitems;
     a.counter = c->d_counter;
     a.binned = c->d_binned;
     a.slab = SlabArgs{};
@@ -770,6 +778,8 @@ int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
 struct ItemList {
     std::vector<WorkItem> items;
     std::vector<JWin> wins;
+    int nqueues = 1;                          // work queues (one per XCD at most)
+    std::vector<int> qoff = std::vector<int>(2, 0);
 };
 
 // Work items: same-event tiles of every group, then mixed tiles of every
@@ -777,6 +787,44 @@ struct ItemList {
 // Mixed: i tile x up to kchunk partner windows of the event's plan rows.
 // enum_same receives the reference's same-event pair count; the mixed
 // count is known from the plan (upload_plan).
+// Work queues, one per XCD: every item of a group goes to one queue, so the
+// waves of one XCD work on one group at a time, whose particle records stay
+// in that XCD's L2.  Groups are dealt to queues largest first, each to the
+// least loaded queue; the queues keep the items' size-class order.
+// nqueues is a power of two up to max_queues and at most the number of
+// groups with items.
+void assign_queues(ItemList &L, const std::vector<uint64_t> &pairs, int ngroups,
+                   int max_queues) {
+    const size_t n = L.items.size();
+    std::vector<uint64_t> load((size_t)std::max(1, ngroups), 0);
+    for (size_t k = 0; k < n; k++) load[(size_t)L.items[k].grp] += pairs[k] + 1;
+    int busy = 0;
+    for (uint64_t x : load) busy += x > 0;
+    int nq = 1;
+    while (nq * 2 <= std::min(max_queues, busy)) nq *= 2;
+    std::vector<int> gq(load.size(), 0);
+    if (nq > 1) {
+        std::vector<int> order(load.size());
+        for (size_t g = 0; g < order.size(); g++) order[g] = (int)g;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return load[(size_t)a] > load[(size_t)b]; });
+        std::vector<uint64_t> qload((size_t)nq, 0);
+        for (int g : order) {
+            const int q = (int)(std::min_element(qload.begin(), qload.end()) - qload.begin());
+            gq[(size_t)g] = q;
+            qload[(size_t)q] += load[(size_t)g];
+        }
+    }
+    L.nqueues = nq;
+    L.qoff.assign((size_t)nq + 1, 0);
+    for (size_t k = 0; k < n; k++) L.qoff[(size_t)gq[(size_t)L.items[k].grp] + 1]++;
+    for (int q = 0; q < nq; q++) L.qoff[(size_t)q + 1] += L.qoff[(size_t)q];
+    std::vector<int> at(L.qoff.begin(), L.qoff.end() - 1);
+    std::vector<WorkItem> byq(n);
+    for (size_t k = 0; k < n; k++) byq[(size_t)at[(size_t)gq[(size_t)L.items[k].grp]]++] = L.items[k];
+    L.items.swap(byq);
+}
+
 void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_same) {
     const Slot &s = c->slot[0];
     ItemShape sh = item_shape(c->tune);
@@ -829,8 +877,27 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
     }
     for (int c = 1; c < 34; c++) start[c] += start[c - 1];
     std::vector<WorkItem> sorted(n);
-    for (size_t k = 0; k < n; k++) sorted[start[32 - cls[k]]++] = L.items[k];
+    std::vector<uint64_t> spairs(n);
+    for (size_t k = 0; k < n; k++) {
+        const size_t at = start[32 - cls[k]]++;
+        sorted[at] = L.items[k];
+        spairs[at] = pairs[k];
+    }
     L.items.swap(sorted);
+    // per-XCD queues where a group's records take at most 3/4 of one XCD's
+    // L2, which also holds the mixed lists and the histogram lines (on the
+    // bench workloads: 1 % on the headline's precision form, 3 % on the
+    // batched K_phi line, 18 % on its K+ launch; the headline's FP64 form
+    // loses 1 %); larger groups (the
+    // large-histogram form's 4e5 particles: 7 %) run faster with every XCD
+    // on the same group
+    long long maxg = 0;
+    for (int g = 0; g < s.ngroups; g++)
+        maxg = std::max(maxg, s.foff[(size_t)s.grp_ev[g + 1]] - s.foff[(size_t)s.grp_ev[g]]);
+    const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
+    const size_t rec = sizeof(PartRec) + (f32 ? sizeof(PfRec32) + 4 : sizeof(PfRec64) + 8);
+    assign_queues(L, spairs, s.ngroups,
+                  (size_t)maxg * rec * 4 <= kL2BytesPerXcd * 3 ? kMaxQueues : 1);
 }
 
 // Uploads the plan: ids are relative to the event's own group and are
@@ -894,6 +961,8 @@ int upload_items(hbt_ctx *c, const ItemList &L) {
         HIPCHECK(c, hipMemcpy(c->d_wins, L.wins.data(), L.wins.size() * sizeof(JWin),
                               hipMemcpyHostToDevice));
     c->nitems = (int)items.size();
+    c->nqueues = L.nqueues;
+    c->qoff = L.qoff;
     return HBT_OK;
 }
 
@@ -1337,7 +1406,7 @@ namespace hbt {
 
 hipError_t launch_fast_kernel(const FastArgs &a, bool f32, int R, bool ext, long blocks,
                               hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, a.counter);
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, a.counter, a.nqueues);
 #define HBT_LAUNCH(T, RR, X) \
     hipLaunchKernelGGL((k_fast<T, RR, X>), dim3(blocks), dim3(256), 0, st, a)
 #define HBT_LAUNCH_R(T, X)                    \
@@ -1424,7 +1493,7 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
     if (!rc) rc = dev_alloc(cp, &c->d_stage, 10);
     if (!rc) rc = dev_alloc(cp, &c->d_binned, 2);
     if (!rc) rc = dev_alloc(cp, &c->d_sums, (size_t)k.n_KT * std::max(1, D.P.nphi));
-    if (!rc) rc = dev_alloc(cp, &c->d_counter, 1);
+    if (!rc) rc = dev_alloc(cp, &c->d_counter, (size_t)kQueueStride * kMaxQueues);
     if (!rc) rc = dev_alloc(cp, &c->d_view, 1);
     if (!rc) rc = dev_alloc(cp, &c->d_defer, kDeferCap);
     if (!rc) rc = dev_alloc(cp, &c->d_ndefer, 1);

@@ -85,12 +85,19 @@ struct View {
     const double *psi;         // psi_ref per group
 };
 
+// Work queues of a launch: at most one per XCD.  Queue q holds items
+// [qoff[q], qoff[q+1]); its counter is counter[kQueueStride * q].
+constexpr int kMaxQueues = 8;
+constexpr int kQueueStride = 16;  // ints: one 64-byte line per counter
+
 struct FastArgs {
     DevParams P;
     const View *views;
     const WorkItem *items;
     int nitems;
-    int *counter;              // work counter, zeroed before launch
+    int nqueues;                   // 1, 2, 4 or 8
+    int qoff[kMaxQueues + 1];
+    int *counter;              // work counters, zeroed before launch
     unsigned long long *binned;  // [2]: binned same, mixed (for stats)
     // slab binning (hbt_slab.h) of the common-case instances; vals == null:
     // every binned pair is added with device-scope atomics
@@ -283,6 +290,11 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
     return binned ? 1 : 0;
 }
 
+// XCD of the executing wave (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcc_id() {
+    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u);
+}
+
 // j records are read with scalar loads (uniform address, read-only data
 // seen through the constant address space), so the j values arrive in SGPRs
 // and cost no vector instructions to broadcast.
@@ -302,8 +314,9 @@ struct Ring {
 };
 
 // ---------------------------------------------------------------- kernel
-// Persistent: each wave pulls work items from a global counter until none
-// are left.  Per item the wave holds 64R i particles (R per lane) and walks
+// Persistent: each wave pulls work items from the queue of its XCD (the host
+// puts all items of a group in one queue, so a group's particles stay in one
+// XCD's L2), then from the next queues in turn, until all are empty.  Per item the wave holds 64R i particles (R per lane) and walks
 // its j segments (same-event: one range with j > i in the px-sorted list;
 // mixed: one partner window per segment with the i particles rotated by
 // -theta),
@@ -350,12 +363,19 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     const Prec q_thr = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_qthr : a.P.f_qthr);
     unsigned long long nb_same = 0, nb_mixed = 0;
 
+    const int nq = a.nqueues;
+    int wq = xcc_id() & (nq - 1);  // nq is a power of two
+    int queues_left = nq;
     for (;;) {
         int it = 0;
-        if (lane == 0) it = atomicAdd(a.counter, 1);
+        if (lane == 0) it = atomicAdd(a.counter + kQueueStride * wq, 1);
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-        if (it >= a.nitems) break;
-        const WorkItem w = a.items[it];
+        if (it >= a.qoff[wq + 1] - a.qoff[wq]) {  // this queue is empty: the next one
+            if (--queues_left == 0) break;
+            wq = (wq + 1) & (nq - 1);
+            continue;
+        }
+        const WorkItem w = a.items[a.qoff[wq] + it];
         const View v = HBT_LOAD_VIEW(a.views, w.view);
         const double psi = v.psi[w.grp];
         const bool mixed = w.kind != 0;

@@ -545,6 +545,8 @@ struct ItemShape {
 
 // Work items per resident wave below which a launch's items are made smaller.
 constexpr size_t kItemsPerWave = 4;
+// L2 of one XCD (MI355X: 4 MB).
+constexpr size_t kL2BytesPerXcd = (size_t)4 << 20;
 
 inline ItemShape item_shape(const hbt_tuning &t) {
     return ItemShape{64 * t.R, std::max(64, t.jchunk), std::max(1, std::min(t.kchunk, 64))};

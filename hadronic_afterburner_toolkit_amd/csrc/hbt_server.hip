@@ -433,6 +433,8 @@ void launcher_loop(hbt_server *s) {
             a.views = s->d_views;
             a.items = s->d_items[buf];
             a.nitems = (int)items.size();
+            a.nqueues = 1;  // one queue: a step mixes the items of several requests
+            for (int q = 0; q <= kMaxQueues; q++) a.qoff[q] = q == 0 ? 0 : a.nitems;
             a.counter = s->d_counter;
             a.binned = s->d_misc + 16;
             const long blocks = std::max(
