@@ -647,11 +647,13 @@ def main():
     groups = make_workload(args.events, args.mult, args.per_group, args.seed)
     plan = draw_plans(groups, args.seed + 1, PARAMS["number_of_mixed_events"])
     res = {}
-    # fp32c / fp64: the fast kernel, binned pairs added with device-scope
-    # atomics; the *_slab modes sum them per (block, q_out plane) in LDS
-    # instead (slab binning), for comparison
-    modes = {"fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps, 0),
-             "fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps, 0),
+    # fp32c / fp64: the fast kernel with the histogram privatisation the
+    # autotuner measures faster (hbt_binning 2: device-scope atomics or slab
+    # binning; atomics when the launch shape is given with --tuning); the
+    # *_slab modes force slab binning (binned pairs summed per (block, q_out
+    # plane) in LDS), for comparison
+    modes = {"fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps, 2),
+             "fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps, 2),
              "fp32c_slab": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, args.steps, 1),
              "fp64_slab": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, args.steps, 1),
              "plain": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, max(1, min(2, args.steps)), 0)}

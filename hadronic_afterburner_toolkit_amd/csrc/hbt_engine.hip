@@ -325,6 +325,9 @@ struct hbt_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool batch_ready = false;
     bool batch_capture = true;   // the prepared batch is captured as a graph (launched more than once)
+    // histogram privatisation of the fast kernel: slab binning (true) or
+    // device-scope atomics; cfg.binning 1 / 0, or hbt_autotune's pick for 2
+    bool slab_binning = false;
     std::string err;
     uint64_t needed = 0, needed_qinv = 0;
     uint64_t enum_same_total = 0, enum_mixed_total = 0;  // since reset
@@ -630,12 +633,19 @@ int slab_read_stats(hbt_ctx *c) {
 // beyond, regions shrink and more pairs take the direct path.
 constexpr double kSlabBudget = 16e9;
 
+// Slab binning needs a 3D histogram (no K_phi, no q_inv) whose slabs fit
+// the fast kernel's LDS counters and k_slab_bin's LDS.
+bool slab_applies(const hbt_ctx *c) {
+    const int q = c->P.qnpts;
+    return c->cfg.kernel == HBT_KERNEL_FAST && !c->P.qinv && !c->P.azimuthal &&
+           c->cfg.n_KT * c->P.nphi * q <= kMaxSlabs && q * q <= kMaxSlabBins;
+}
+
 int slab_prepare(hbt_ctx *c, int nitems) {
     auto &sl = c->slab;
     const int q = c->P.qnpts;
     const int S = c->cfg.n_KT * c->P.nphi * q, SB = q * q;
-    sl.on = c->cfg.binning == 1 && c->cfg.kernel == HBT_KERNEL_FAST && !c->P.qinv &&
-            !c->P.azimuthal && S <= kMaxSlabs && SB <= kMaxSlabBins && nitems > 0;
+    sl.on = c->slab_binning && slab_applies(c) && nitems > 0;
     if (!sl.on) return HBT_OK;
     const int nblk = (int)fast_blocks(c, nitems);
     const bool f32 = c->cfg.precision == HBT_PREC_FP32C;
@@ -1468,6 +1478,7 @@ int hbt_create(const hbt_config *cfg, hbt_ctx **out) {
 
     std::unique_ptr<hbt_ctx> c(new hbt_ctx());
     c->cfg = k;
+    c->slab_binning = k.binning == 1;  // 2: atomics until hbt_autotune measures both
     c->cu_count = prop.multiProcessorCount;
     c->D = D;
     c->P = D.P;
@@ -1781,6 +1792,7 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
         if (best_out) *best_out = c->tune;
         return HBT_OK;
     }
+    if (c->cfg.binning == 2) c->slab_binning = false;  // the shapes are timed with atomics
     // small j chunks and one window per mixed item give small batches
     // enough items to fill the chip; large ones amortize the per-item setup
     const int Rs[] = {1, 2, 4};
@@ -1812,6 +1824,28 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
                     }
                 }
     c->tune = best;
+    // histogram privatisation (binning 2): the best launch shape again with
+    // slab binning; kept where it beats device-scope atomics.  Its regions
+    // are sized from the first launch's demand, so the later launches count.
+    if (c->cfg.binning == 2 && slab_applies(c)) {
+        c->slab_binning = true;
+        ItemList items;
+        uint64_t es;
+        build_items(c, true, true, items, es);
+        c->launch_enum_same = es;
+        c->launch_enum_mixed = c->plan_valid ? c->plan_enum_mixed : 0;
+        int rc = upload_items(c, items);
+        float slab_ms = 1e30f;
+        for (int k = 0; k < 3 && !rc; k++) {
+            float ms = 0.f;
+            rc = timed_fast(c, &ms);
+            if (k > 0) slab_ms = std::min(slab_ms, ms);
+        }
+        if (rc) return rc;
+        c->slab_binning = slab_ms < best.ms;
+        if (c->slab_binning) best.ms = slab_ms;
+        c->tune = best;
+    }
     int rc = prepare_and_capture(c);
     if (!rc) rc = zero_hist(c);
     if (rc) return rc;

@@ -54,6 +54,9 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
         return err = "unknown kernel", HBT_ERR_ARG;
     if (k.precision == HBT_PREC_FP32C && k.kernel != HBT_KERNEL_FAST)
         return err = "the precision form runs on the fast kernel only", HBT_ERR_ARG;
+    if (k.binning < 0 || k.binning > 2)
+        return err = "binning must be 0 (atomics), 1 (slab) or 2 (chosen by hbt_autotune)",
+               HBT_ERR_ARG;
     DevParams &P = d.P;
     std::memset(&P, 0, sizeof(P));
     P.qnpts = k.qnpts;

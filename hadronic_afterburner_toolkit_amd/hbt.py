@@ -47,7 +47,8 @@ def config_from_params(params: dict, precision: int = _lib.HBT_PREC_FP64,
     cfg.precision, cfg.kernel, cfg.device = precision, kernel, device
     # engine key: K_phi guard band in 1e-12 rad (0: the default 1e-12 rad)
     cfg.kphi_guard_prad = int((params or {}).get("hbt_kphi_guard_prad", 0))
-    # engine key: 0 device-scope atomics (default), 1 slab binning where it applies
+    # engine key: 0 device-scope atomics (default), 1 slab binning where it
+    # applies, 2 the faster of the two as hbt_autotune measures it
     cfg.binning = int((params or {}).get("hbt_binning", 0))
     return cfg
 
@@ -417,7 +418,8 @@ def _make_config(params, precision, kernel, device):
     cfg.device = device
     # engine key: K_phi guard band in 1e-12 rad (0: the default 1e-12 rad)
     cfg.kphi_guard_prad = int((params or {}).get("hbt_kphi_guard_prad", 0))
-    # engine key: 0 device-scope atomics (default), 1 slab binning where it applies
+    # engine key: 0 device-scope atomics (default), 1 slab binning where it
+    # applies, 2 the faster of the two as hbt_autotune measures it
     cfg.binning = int((params or {}).get("hbt_binning", 0))
     return p, cfg
 

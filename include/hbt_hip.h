@@ -76,8 +76,11 @@ typedef struct {
      * to the histogram with device-scope atomics (the faster on MI355X);
      * 1 = slab binning where it applies (no K_phi, no q_inv,
      * qnpts^2 <= 2048, n_KT * qnpts <= 512): binned pairs written as records
-     * per (block, q_out plane) and summed in LDS, no device-scope atomics.
-     * Counts are the same; sums agree within each form's tolerance. */
+     * per (block, q_out plane) and summed in LDS, no device-scope atomics;
+     * 2 = the one hbt_autotune measures faster on the prepared batch
+     * (atomics until then).  Counts are the same; sums agree within each
+     * form's tolerance.  hbt_slab_info reports which one the last launch
+     * used. */
     int32_t binning;
     int32_t reserved[3];
 } hbt_config;

@@ -44,6 +44,24 @@ def test_create_without_gpu_or_bad_args_fails_cleanly():
     assert b"qnpts" in lib.hbt_last_error(None)
 
 
+def test_binning_must_be_atomics_slab_or_autotuned():
+    """hbt_config.binning is validated before any device is touched: 0, 1, 2
+    get past it (to the missing device here), anything else is HBT_ERR_ARG."""
+    from hadronic_afterburner_toolkit_amd import hbt
+    lib = _lib.load()
+    h = C.c_void_p()
+    for b, want in ((3, _lib.HBT_ERR_ARG), (-1, _lib.HBT_ERR_ARG)):
+        _, cfg = hbt._make_config({"hbt_binning": b}, _lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 0)
+        assert lib.hbt_create(C.byref(cfg), C.byref(h)) == want
+        assert b"binning" in lib.hbt_last_error(None)
+    for b in (0, 1, 2):
+        _, cfg = hbt._make_config({"hbt_binning": b}, _lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 0)
+        rc = lib.hbt_create(C.byref(cfg), C.byref(h))
+        assert rc != _lib.HBT_ERR_ARG
+        if rc == _lib.HBT_OK:  # a device is visible: the context is real
+            lib.hbt_destroy(h)
+
+
 def test_rng_works_without_gpu():
     r = hbt.Random(0)
     assert r.rand_int_uniform() == 1178568022

@@ -115,3 +115,30 @@ def test_overflowing_regions_take_the_direct_path(form):
     assert infos[0]["on"] and infos[0]["direct"] > 0
     assert infos[1]["direct"] < infos[0]["direct"]
     check(rs, a, tol, scale=2)
+
+
+@pytest.mark.parametrize("form", list(FORMS))
+def test_autotuner_picks_the_histogram_privatisation(form):
+    """hbt_binning = 2: hbt_autotune times the launch shapes with device-scope
+    atomics, then the best one with slab binning, and keeps the faster; the
+    results after the tuned launch are the oracle's either way."""
+    prec, tol = FORMS[form]
+    full = samples.generate(608, 120, 400.0)
+    groups = [full.events(20 * k, 20) for k in range(6)]
+    plan = plans(groups, 63, 10)
+    print(f"autotuned privatisation, {form}", flush=True)
+    a = oracle_batch(groups, plan)
+    g = hbt.HBTCorrelation(dict(PARAMS, hbt_binning=2), precision=prec,
+                           kernel=_lib.HBT_KERNEL_FAST)
+    g.upload_groups(groups)
+    g.prepare_batch(*plan)
+    tune = g.autotune()
+    g.launch_batch()
+    g.sync()
+    info = g.slab_info()
+    r = g.results()
+    g.close()
+    print(f"  tuning {tune}, slab binning {info['on']}", flush=True)
+    assert info["on"] in (True, False)
+    check(r, a, tol)
+
