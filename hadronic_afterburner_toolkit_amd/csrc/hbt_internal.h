@@ -6,16 +6,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <unistd.h>
-
 #include <algorithm>
 #include <cmath>
-#include <condition_variable>
 #include <cstdint>
 #include <cstring>
 #include <functional>
 #include <memory>
-#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -25,7 +21,9 @@
 #include "../../include/hbt_hip.h"
 #include "hbt_device.h"
 #include "hbt_fast.h"
+#include "host_pool.h"
 #include "host_sort.h"
+#include "host_tiles.h"
 
 namespace hbt {
 
@@ -207,110 +205,6 @@ inline double px_window(const Derived &d) {
     if (d.P.qinv) return INFINITY;
     const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
     return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
-}
-
-// Host threads of the upload path, started once per process: a per-buffer
-// upload is a few milliseconds of host work, and starting threads for each
-// of its passes would cost as much as the passes themselves.  run() hands
-// tasks 0..nt-1 to the workers and the caller; while the pool is busy (a
-// nested call, another thread's upload: the server's concurrent requests)
-// or in a forked child, the call starts threads of its own instead.
-class HostPool {
-  public:
-    static HostPool &get() {
-        static HostPool pool;
-        return pool;
-    }
-    int threads() const { return (int)workers_.size() + 1; }
-    void run(int nt, const std::function<void(int)> &fn) {
-        std::unique_lock<std::mutex> busy(run_mu_, std::try_to_lock);
-        if (!busy.owns_lock() || getpid() != pid_ || workers_.empty()) {
-            std::vector<std::thread> th;
-            for (int t = 1; t < nt; t++) th.emplace_back(fn, t);
-            fn(0);
-            for (auto &x : th) x.join();
-            return;
-        }
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            fn_ = &fn;
-            ntask_ = nt;
-            next_ = 0;
-            left_ = nt;
-            gen_++;
-        }
-        cv_.notify_all();
-        work();
-        std::unique_lock<std::mutex> g(mu_);
-        done_.wait(g, [&] { return left_ == 0; });
-        fn_ = nullptr;
-    }
-
-  private:
-    HostPool() : pid_(getpid()) {
-        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        for (unsigned t = 1; t < hw; t++) workers_.emplace_back([this] { loop(); });
-    }
-    ~HostPool() {
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            stop_ = true;
-            gen_++;
-        }
-        cv_.notify_all();
-        for (auto &t : workers_)
-            if (t.joinable()) t.join();
-    }
-    void loop() {
-        uint64_t seen = 0;
-        for (;;) {
-            {
-                std::unique_lock<std::mutex> g(mu_);
-                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-            }
-            work();
-        }
-    }
-    // takes tasks of the current run until none is left
-    void work() {
-        for (;;) {
-            const std::function<void(int)> *f;
-            int t;
-            {
-                std::lock_guard<std::mutex> g(mu_);
-                if (!fn_ || next_ >= ntask_) return;
-                t = next_++;
-                f = fn_;
-            }
-            (*f)(t);
-            std::lock_guard<std::mutex> g(mu_);
-            if (--left_ == 0) done_.notify_all();
-        }
-    }
-    std::mutex run_mu_, mu_;
-    std::condition_variable cv_, done_;
-    std::vector<std::thread> workers_;
-    const std::function<void(int)> *fn_ = nullptr;
-    int ntask_ = 0, next_ = 0, left_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-    const pid_t pid_;
-};
-
-// Runs fn(begin, end) over [0, n) split into contiguous ranges on up to 16
-// host threads (one when n is small).
-template <typename F>
-inline void parallel_ranges(long long n, long long min_per_thread, F fn) {
-    HostPool &pool = HostPool::get();
-    const long long nt =
-        std::max(1ll, std::min<long long>(pool.threads(), n / std::max(1ll, min_per_thread)));
-    if (nt <= 1) {
-        fn(0ll, n);
-        return;
-    }
-    pool.run((int)nt, [&](int t) { fn(n * t / nt, n * (t + 1) / nt); });
 }
 
 // Host-to-device copies through a pinned staging buffer: a pageable copy
@@ -597,21 +491,6 @@ inline void items_by_group_chunks(int ngroups, std::vector<WorkItem> &items,
 // pairs (optional) receives each item's evaluated pair count.  enum_same
 // receives the reference's enumerated pair count sum n(n-1)/2 over the kept
 // lists (off).
-// Pairs (i, j) with i in [a, a + ni), j in [jb, j1) and i < j, in closed
-// form: i < jb pairs with all of [jb, j1); i >= jb with (i, j1).
-inline uint64_t tile_pairs(long long a, long long ni, long long jb, long long j1) {
-    uint64_t cnt = 0;
-    const long long ie = a + ni;
-    const long long e1 = std::min(ie, jb);
-    if (e1 > a && j1 > jb) cnt += (uint64_t)(e1 - a) * (uint64_t)(j1 - jb);
-    const long long b2 = std::max(a, jb), e2 = std::min(ie, j1 - 1);
-    if (e2 > b2) {  // i = b2 .. e2-1 contribute j1-1-i: the integers lo..hi
-        const long long n = e2 - b2, hi = j1 - 1 - b2, lo = j1 - e2;
-        cnt += (uint64_t)((lo + hi) * n / 2);
-    }
-    return cnt;
-}
-
 inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                        const std::vector<long long> &foff, const std::vector<int> &grp_ev,
                        const double *srt_px, double dpx, int view,
