@@ -74,7 +74,7 @@ class HbtBfInfo(C.Structure):
 class HbtServerConfig(C.Structure):
     _fields_ = [("hbm_bytes", C.c_int64), ("max_resident", C.c_int32),
                 ("max_step_items", C.c_int32), ("step_pairs", C.c_double),
-                ("tune", HbtTuning), ("reserved", C.c_int32 * 8)]
+                ("tune", HbtTuning), ("part_particles", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 HBT_REQ_STATES = {0: "queued", 1: "loading", 2: "active", 3: "draining", 4: "done",

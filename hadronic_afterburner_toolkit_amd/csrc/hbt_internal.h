@@ -270,6 +270,13 @@ inline hipError_t stage_pieces(char *stage, const std::vector<StagePiece> &v, hi
 // after_keep (optional) runs once the kept particles (keep, off) and the
 // list sizes (n, nf) are known, before the px sort; a non-zero return ends
 // the gather with that code.
+// The rapidity cut of the gathers: tanh(HBTrap_min) < pz/E < tanh(HBTrap_max)
+// (HBT_correlation.cpp:255-266, :468-476; cut_* = tanh of the bounds).
+inline bool rapidity_kept(double pz, double E, double cut_min, double cut_max) {
+    const double ratio = pz / E;
+    return ratio > cut_min && ratio < cut_max;
+}
+
 // Rapidity cut of the gathers (HBT_correlation.cpp:255-266, :468-476),
 // psi_ref per group (:181-183, :233-249; over all particles passed in), and
 // the prefilter records.  Events keep < 2^18 particles (the fast kernel's
@@ -294,10 +301,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     h.n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
     const double cut_max = tanh(k.HBTrap_max);
     const double cut_min = tanh(k.HBTrap_min);
-    auto kept = [&](int64_t i) {
-        const double ratio = soa[2][i] / soa[3][i];
-        return ratio > cut_min && ratio < cut_max;
-    };
+    auto kept = [&](int64_t i) { return rapidity_kept(soa[2][i], soa[3][i], cut_min, cut_max); };
     // the fast lists' |pT| range: particles a binnable pair can contain
     const double D = px_window(d);
     const double r_lo = std::isfinite(D) ? (P.KT_min - 0.5 * D) * (1.0 - 1e-9) : -INFINITY;

@@ -455,7 +455,7 @@ class HBTServer:
     def __init__(self, params: dict | None = None, precision: int = _lib.HBT_PREC_FP64,
                  hbm_bytes: int = 16 << 30, max_resident: int = 16,
                  max_step_items: int = 1 << 18, step_pairs: float = 2e9,
-                 tuning=(2, 1024, 4, 8), device: int = 0):
+                 tuning=(2, 1024, 4, 8), device: int = 0, part_particles: int = 0):
         self.params, cfg = _make_config(params, precision, _lib.HBT_KERNEL_FAST, device)
         self.nphi = cfg.n_Kphi if cfg.azimuthal_flag else 1
         self._lib = _lib.load()
@@ -465,6 +465,7 @@ class HBTServer:
         sc.max_step_items = int(max_step_items)
         sc.step_pairs = float(step_pairs)
         sc.tune = HbtTuning(*[int(v) for v in tuning], 0.0)
+        sc.part_particles = int(part_particles)  # 0: the server's default (1e6)
         h = C.c_void_p()
         rc = self._lib.hbt_server_create(C.byref(cfg), C.byref(sc), C.byref(h))
         if rc:

@@ -319,7 +319,10 @@ int hbt_autotune(hbt_ctx *ctx, hbt_tuning *best);
  * Requests are admitted, in arrival order among those that fit, when their
  * HBM footprint (particles, plan, histograms) fits the device arena and a
  * resident slot is free; their particle records and histograms then stay in
- * HBM until the result is read back.  Every launch of the fast kernel
+ * HBM until the result is read back.  A large request goes in parts of whole
+ * groups (part_particles): each part is admitted, uploaded and launched as
+ * soon as it is prepared, while the next one is prepared, all adding to the
+ * request's histograms.  Every launch of the fast kernel
  * carries work items of all resident requests (each with its own view of
  * particles and histograms), a bounded number of pairs per launch, shared
  * among them smallest-remaining-first, so requests that arrive while others
@@ -336,7 +339,10 @@ typedef struct {
     int32_t max_step_items;  /* work items per launch */
     double step_pairs;       /* enumerated pairs per launch (scheduling quantum) */
     hbt_tuning tune;         /* launch shape of the fast kernel */
-    int32_t reserved[8];
+    /* a request is prepared, uploaded and launched in parts of whole groups
+     * of about this many input particles, which overlap (0: 1,000,000) */
+    int32_t part_particles;
+    int32_t reserved[7];
 } hbt_server_config;
 
 enum { HBT_REQ_QUEUED = 0, HBT_REQ_LOADING = 1, HBT_REQ_ACTIVE = 2, HBT_REQ_DRAINING = 3,

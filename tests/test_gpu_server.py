@@ -148,3 +148,68 @@ def test_footprint_admission():
     with pytest.raises(hbt.HbtError):
         srv.submit(request(999, 200, 2000.0), 10, 1)  # needs more than the arena
     srv.close()
+
+
+@pytest.mark.parametrize("prec", [_lib.HBT_PREC_FP64, _lib.HBT_PREC_FP32C])
+def test_request_in_parts_equals_oracle_and_whole(prec):
+    """part_particles small enough that every group of the request is a part
+    of its own: six parts prepared, uploaded and launched one after the
+    other, all adding to one histogram block.  Against the oracle (counts
+    bit-exact, FP64 sums within 1e-12, the precision form's within 1e-6) and
+    against the same request in one part (counts identical, sums within
+    1e-12 of each other), also with another request in parts running
+    beside it."""
+    buf = request(41, 60, 300.0, sigma=3.5)
+    other = request(42, 40, 500.0, sigma=5.0)
+    print(f"request in parts, precision {prec}", flush=True)
+    whole_srv = hbt.HBTServer(PARAMS, precision=prec, hbm_bytes=1 << 30)
+    rid = whole_srv.submit(buf, 10, 9)
+    assert whole_srv.wait(rid, 60.0)
+    whole = whole_srv.result(rid)
+    whole_srv.release(rid)
+    whole_srv.close()
+    srv = hbt.HBTServer(PARAMS, precision=prec, hbm_bytes=1 << 30, step_pairs=2e7,
+                        part_particles=2000)
+    oid = srv.submit(other, 10, 10)
+    rid = srv.submit(buf, 10, 9)
+    assert srv.wait(rid, 60.0) and srv.wait(oid, 60.0)
+    parts = srv.result(rid)
+    info = srv.info(rid)
+    st = srv.stats()
+    srv.release(rid)
+    srv.release(oid)
+    srv.close()
+    ref = oracle_request(buf, 10, 9)
+    assert info["state"] == "done"
+    assert info["enum_same"] == int(ref["same_st"][0])
+    assert info["enum_mixed"] == int(ref["mixed_st"][0])
+    assert st["resident_peak"] >= 2
+    for k in COUNTS:
+        assert np.array_equal(getattr(parts, k), ref[k].astype(np.uint64)), k
+    tol = 1e-12 if prec == _lib.HBT_PREC_FP64 else 1e-6
+    for key, fl in SUMS:
+        assert rel(getattr(parts, key), ref[key], ref[fl]) <= tol, key
+    same_result(parts, whole, 1e-12)
+
+
+def test_kphi_request_in_parts_decides_edge_pairs_on_the_host():
+    """16 K_phi bins with a 1e-3 rad guard band: many pairs of every part go
+    to the request's shared deferral buffer and are decided on the host once
+    the last part has run.  Counts bit-exact against the oracle."""
+    params = dict(PARAMS, azimuthal_flag=1, n_Kphi=16, hbt_kphi_guard_prad=10 ** 9)
+    buf = request(43, 40, 250.0, sigma=4.0)
+    print("K_phi request in parts, guard 1e-3 rad", flush=True)
+    srv = hbt.HBTServer(params, precision=_lib.HBT_PREC_FP64, hbm_bytes=1 << 30,
+                        part_particles=2000)
+    rid = srv.submit(buf, 10, 12)
+    assert srv.wait(rid, 60.0)
+    r = srv.result(rid)
+    st = srv.stats()
+    srv.release(rid)
+    srv.close()
+    ref = oracle_request(buf, 10, 12, params)
+    assert st["deferred"] > 0
+    for k in ("num_count", "den"):
+        assert np.array_equal(getattr(r, k), ref[k].astype(np.uint64)), k
+    for key, fl in SUMS:
+        assert rel(getattr(r, key), ref[key], ref[fl]) <= 1e-12, key
