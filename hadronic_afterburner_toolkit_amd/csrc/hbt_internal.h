@@ -356,7 +356,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     // (stable, so equal |pT| keep gather order)
     parallel_ranges(nev, ev_grain, [&](long long e0, long long e1) {
         std::vector<std::pair<double, long long>> ev;  // |pT|, kept index
-        std::vector<int64_t> src;
+        std::vector<std::pair<uint64_t, long long>> ra, rb;  // radix scratch
         for (long long e = e0; e < e1; e++) {
             ev.clear();
             long long q = h.off[e];
@@ -367,8 +367,8 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                 if (fast(pt)) ev.emplace_back(pt, q);
                 q++;
             }
-            // (|pT|, gather index) pairs are distinct: the order equals a stable sort by |pT|
-            std::sort(ev.begin(), ev.end());
+            // stable by |pT| (gather order among equal |pT|)
+            radix_sort_keys(ev, ra, rb);
             long long f = h.foff[e];
             for (const auto &x : ev) {
                 const int64_t i = h.keep[x.second];
