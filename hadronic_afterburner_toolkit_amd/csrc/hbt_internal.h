@@ -196,6 +196,14 @@ struct HostParticles {
     hvec<OrigRank> orr;           // srt position -> fast-list index, gather rank
     hvec<double> srt_px;          // px in srt order (same-event windows)
     double pmax2 = 0.0;  // max px^2+py^2 over the fast list (FP32 prefilter margins)
+    // Empties the lists for the next gather and keeps their storage, so a
+    // caller that reuses one (the request server's pool) writes into pages
+    // already faulted in: fresh buffers cost a third of a gather in page
+    // faults.
+    void reset() {
+        n = n_in = nf = 0;
+        pmax2 = 0.0;
+    }
 };
 
 // Half-width D of the px and |pT| windows of a binnable pair: a pair inside
@@ -297,7 +305,7 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
         if (grp_ev[g + 1] < grp_ev[g]) return err = "group offsets decrease", HBT_ERR_ARG;
     const DevParams &P = d.P;
     const bool f32 = k.precision == HBT_PREC_FP32C;
-    h = HostParticles();
+    h.reset();  // every list below is sized and written in full
     h.n_in = nev > 0 ? ev_off[nev] - ev_off[0] : 0;
     const double cut_max = tanh(k.HBTrap_max);
     const double cut_min = tanh(k.HBTrap_min);

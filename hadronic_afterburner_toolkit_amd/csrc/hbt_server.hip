@@ -58,6 +58,8 @@ using Clock = std::chrono::steady_clock;
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+constexpr size_t kHostPool = 4;  // host lists kept for reuse
+
 // First-fit allocator over the device arena.
 class Arena {
   public:
@@ -175,6 +177,9 @@ struct hbt_server {
     size_t result_bytes = 0;
     size_t defer_count_off = 0;  // byte offset of the deferred-pair count in a result
     long long part_particles = 0;  // input particles per part of a request
+    // host lists of uploaded parts, kept for the next parts' gathers (their
+    // pages are faulted in already); at most kHostPool
+    std::vector<HostParticles> host_pool;
     WorkItem *d_items[2] = {};
     WorkItem *h_items[2] = {};
     char *h_stage = nullptr;  // pinned staging of uploads (admission thread only)
@@ -397,10 +402,11 @@ void admission_loop(hbt_server *s) {
             r.state = HBT_REQ_ACTIVE;
             r.t_active = s->now();
         }
-        // host copies are on the device now
+        // host copies are on the device now: the lists go back to the pool
         HostParticles keep_meta;
         keep_meta.n = p->hp.n;
         keep_meta.n_in = p->hp.n_in;
+        if (s->host_pool.size() < kHostPool) s->host_pool.push_back(std::move(p->hp));
         p->hp = std::move(keep_meta);
         std::vector<double>().swap(p->plan_cs);
         std::vector<JWin>().swap(p->wins);
@@ -938,6 +944,13 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
         p.ngroups = g1 - g0;
         p.grp_ev.resize((size_t)p.ngroups + 1);
         for (int g = 0; g <= p.ngroups; g++) p.grp_ev[(size_t)g] = grp_ev[g0 + g] - p.e0;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (!s->host_pool.empty()) {
+                p.hp = std::move(s->host_pool.back());
+                s->host_pool.pop_back();
+            }
+        }
         std::string err;
         int rc = gather_particles(s->D, s->cfg, p.ngroups, p.grp_ev.data(), p.nev,
                                   ev_off + p.e0, soa, p.hp, err);
