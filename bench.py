@@ -523,6 +523,7 @@ def copy_calibration(nbytes, reps):
 
 
 TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "k_fast_traffic.md")
+SQ_SUMMARY = os.path.join(REPO, "profiles", "k_fast_sq_counters.md")
 ATOMICS_BENCH = os.path.join(REPO, "hadronic_afterburner_toolkit_amd", "hist_atomics_bench.e")
 
 
@@ -575,25 +576,28 @@ def kernel_build_hash():
     return h.hexdigest()[:16]
 
 
-def measured_traffic(kernel_prefix):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary (lines 'traffic_bytes_per_launch <kernel> N
-    <build hash>'), with the build hash of the sources it was measured on
-    and whether it matches the current sources; the value is None when it
-    does not (the kernel changed since the counters were read)."""
+def profile_value(path, key, kernel_prefix):
+    """A per-launch figure of the dominant kernel from a committed rocprofv3
+    counter summary (lines '<key> <kernel> N <build hash>'), with the build
+    hash of the sources it was measured on and the current one; the value is
+    None when they differ (the kernel changed since the counters were
+    read)."""
     cur = kernel_build_hash()
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" \
-                        and parts[1] == kernel_prefix:
+                if len(parts) >= 3 and parts[0] == key and parts[1] == kernel_prefix:
                     built = parts[3] if len(parts) > 3 else None
-                    ok = built == cur
-                    return (float(parts[2]) if ok else None), built, cur
+                    return (float(parts[2]) if built == cur else None), built, cur
     except OSError:
         pass
     return None, None, cur
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of the dominant kernel (profiles/k_fast_traffic.md)."""
+    return profile_value(TRAFFIC_SUMMARY, "traffic_bytes_per_launch", kernel_prefix)
 
 
 def main():
@@ -734,6 +738,14 @@ def main():
                            "traffic_build": t_build, "kernel_build": cur_build,
                            "kernel": kname,
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
+        # issue-side view of the same kernel: wave64 VALU instructions per
+        # launch (SQ_INSTS_VALU, committed counter summary of this build),
+        # each holding its SIMD 4 cycles, over the launch on 1024 SIMDs at
+        # the 2.4 GHz peak clock
+        insts, _, _ = profile_value(SQ_SUMMARY, "valu_insts_per_launch", kname)
+        out["roofline"]["valu_issue_frac"] = (
+            insts * 4 / (MI355X_CUS * 4 * MI355X_GHZ * 1e9 * dev_s) if insts else None)
+        out["roofline"]["valu_insts_per_launch"] = insts
         # the bound that holds: every binned pair is one atomic cache-line
         # access (same-event: count + four sums in one FP64 instruction;
         # mixed: the denominator's u32 add), against the measured chip-wide
