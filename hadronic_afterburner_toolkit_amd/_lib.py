@@ -131,6 +131,7 @@ SIGNATURES = [
     ("hbt_sync", C.c_int, [C.c_void_p, _F64P]),
     ("hbt_batch_info", C.c_int, [C.c_void_p, _U64P, _U64P, C.POINTER(C.c_int32)]),
     ("hbt_batch_graph", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    ("hbt_batch_queues", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("hbt_slab_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _U64P, _U64P]),
     ("hbt_calculate_noalloc", C.c_int, [C.POINTER(HbtConfig), C.c_int32, C.c_int64, C.c_int64,
                                         C.c_void_p,

@@ -1751,6 +1751,15 @@ int hbt_batch_graph(hbt_ctx *c, int32_t *graph) {
     return HBT_OK;
 }
 
+int hbt_batch_queues(hbt_ctx *c, int32_t *nqueues, int32_t *qoff) {
+    if (!c || !nqueues) return HBT_ERR_ARG;
+    if (!c->batch_ready) return fail(c, HBT_ERR_STATE, "no prepared batch (hbt_prepare_batch)");
+    *nqueues = c->nqueues;
+    if (qoff)
+        for (int q = 0; q <= c->nqueues; q++) qoff[q] = c->qoff[(size_t)q];
+    return HBT_OK;
+}
+
 int hbt_slab_info(hbt_ctx *c, int32_t *on, uint64_t *direct, uint64_t *capacity) {
     if (!c) return HBT_ERR_ARG;
     if (int rc = quiesce(c)) return rc;

@@ -286,6 +286,14 @@ class HBTCorrelation:
         self._check(self._lib.hbt_batch_graph(self.h, C.byref(v)), "hbt_batch_graph")
         return bool(v.value)
 
+    def batch_queues(self) -> list:
+        """First item of each work queue of the prepared batch, and the end
+        (one queue per XCD at most, every item of a group in one queue)."""
+        n = C.c_int32()
+        q = (C.c_int32 * 9)()
+        self._check(self._lib.hbt_batch_queues(self.h, C.byref(n), q), "hbt_batch_queues")
+        return [q[k] for k in range(n.value + 1)]
+
     def slab_info(self) -> dict:
         """Slab binning of the last fast launch: on, pairs added directly
         because their region was full, region capacity in records."""

@@ -239,6 +239,12 @@ int hbt_batch_info(hbt_ctx *ctx, uint64_t *enum_same, uint64_t *enum_mixed, int3
  * reference (its loops run on the host). */
 int hbt_batch_graph(hbt_ctx *ctx, int32_t *graph);
 
+/* Work queues of the prepared batch: *nqueues = 1, 2, 4 or 8 (one per XCD at
+ * most; every item of a group in one queue), qoff[0..*nqueues] (when not
+ * NULL, room for 9) the queues' first items.  HBT_ERR_STATE without a
+ * prepared batch.  No counterpart in the reference. */
+int hbt_batch_queues(hbt_ctx *ctx, int32_t *nqueues, int32_t *qoff);
+
 /* Slab binning of the last fast launch (hbt_config.binning): *on = 1 when
  * its binned pairs went through slab records, *direct = how many of them
  * took the direct atomic path because their (block, slab) region was full,
