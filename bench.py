@@ -423,6 +423,7 @@ def cpu_baseline_multicore(groups, plan, log):
     Mg, ids, cs = plan
     offs = np.concatenate([[0], np.cumsum([int(Mg[g]) * groups[g].nev for g in range(len(groups))])])
     pairs = [0.0] * nthreads
+    arrays = [None] * nthreads
 
     def work(g):
         M = int(Mg[g])
@@ -431,6 +432,7 @@ def cpu_baseline_multicore(groups, plan, log):
         o.run_mixed_plan(groups[g], M, ids[offs[g]:offs[g + 1]], cs[2 * offs[g]:2 * offs[g + 1]])
         a = o.arrays()
         pairs[g] = float(a["same_st"][0] + a["mixed_st"][0])
+        arrays[g] = a
 
     th = [threading.Thread(target=work, args=(g,)) for g in range(nthreads)]
     t0 = time.perf_counter()
@@ -446,10 +448,40 @@ def cpu_baseline_multicore(groups, plan, log):
     except OSError:
         pass
     log(f"  cpu oracle, {nthreads} threads: {sum(pairs):.3e} pairs in {dt:.2f}s")
+    # the groups' histograms add up (counts, sums and the per-bin floors)
+    total = {k: np.sum([np.asarray(a[k]) for a in arrays], axis=0) for k in arrays[0]}
     return {"value": sum(pairs) / dt, "unit": "pairs/s", "cores": nthreads, "kind": "port",
             "nproc": os.cpu_count(), "cpu_model": model,
             "sample": f"groups 0..{nthreads - 1} of {len(groups)}, one thread each: "
-                      f"{sum(pairs):.3e} same+mixed pairs in {dt:.1f}s, oracle/hbt_oracle.c -O2"}
+                      f"{sum(pairs):.3e} same+mixed pairs in {dt:.1f}s, oracle/hbt_oracle.c -O2"}, total
+
+
+def precision_vs_oracle(groups, plan, a):
+    """The precision form on these groups as one batch against the FP64
+    oracle's arrays a, per bin: max and RMS of |S - S_ref| / max(|S_ref|,
+    sum|.|_bin) over the bins with pairs (num and the three q sums), and
+    whether the counts are identical."""
+    from hadronic_afterburner_toolkit_amd import _lib, hbt
+    gs = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP32C, kernel=_lib.HBT_KERNEL_FAST)
+    gs.upload_groups(groups)
+    gs.prepare_batch(*plan)
+    gs.launch_batch()
+    gs.sync()
+    rr = gs.results()
+    gs.close()
+    errs = []
+    for key, fl_key in (("num", "abs_num"), ("q_out_sum", "abs_q_out"),
+                        ("q_side_sum", "abs_q_side"), ("q_long_sum", "abs_q_long")):
+        ref = a[key]
+        den = np.maximum(np.abs(ref), a[fl_key])
+        sel = den > 0
+        errs.append(np.abs(getattr(rr, key) - ref)[sel] / den[sel])
+    e = np.concatenate(errs)
+    return {"max_rel_bin_err": float(e.max()), "rms_rel_bin_err": float(np.sqrt(np.mean(e * e))),
+            "counts_identical": bool(np.array_equal(rr.num_count, a["num_count"].astype(np.uint64))
+                                     and np.array_equal(rr.den, a["den"].astype(np.uint64))),
+            "groups": len(groups),
+            "metric": "|S-S_ref|/max(|S_ref|,sum|.|_bin), bins with pairs, num and q sums"}
 
 
 def ingest(args, log):
@@ -792,30 +824,17 @@ def main():
                                "sample": f"group 0 of {len(groups)} ({g0.nev} events, "
                                          f"{g0.n} particles): {pairs:.3e} same+mixed pairs in "
                                          f"{dt:.1f}s, oracle/hbt_oracle.c -O2"}
-        out["cpu_baseline_multicore"] = cpu_baseline_multicore(groups, plan, log)
-        # precision form vs the FP64 oracle on that sample, per bin
+        out["cpu_baseline_multicore"], multi = cpu_baseline_multicore(groups, plan, log)
+        # precision form vs the FP64 oracle per bin: on group 0, and as one
+        # batch on the groups the multicore oracle ran (its histograms summed)
         if "fp32c" in res:
-            from hadronic_afterburner_toolkit_amd import hbt
-            gs = hbt.HBTCorrelation(PARAMS, precision=_lib.HBT_PREC_FP32C,
-                                    kernel=_lib.HBT_KERNEL_FAST)
-            gs.upload_groups([g0])
-            gs.prepare_batch(np.array([M], np.int32), plan_g[1], plan_g[2])
-            gs.launch_batch()
-            gs.sync()
-            rr = gs.results()
-            errs = []
-            for key, fl_key in (("num", "abs_num"), ("q_out_sum", "abs_q_out"),
-                                ("q_side_sum", "abs_q_side"), ("q_long_sum", "abs_q_long")):
-                ref = a[key]
-                den = np.maximum(np.abs(ref), a[fl_key])
-                sel = den > 0
-                errs.append(np.abs(getattr(rr, key) - ref)[sel] / den[sel])
-            e = np.concatenate(errs)
-            out["fp32c_vs_fp64_oracle"] = {
-                "max_rel_bin_err": float(e.max()), "rms_rel_bin_err": float(np.sqrt(np.mean(e * e))),
-                "counts_identical": bool(np.array_equal(rr.num_count, a["num_count"].astype(np.uint64))
-                                         and np.array_equal(rr.den, a["den"].astype(np.uint64))),
-                "metric": "|S-S_ref|/max(|S_ref|,sum|.|_bin), bins with pairs, num and q sums"}
+            n_multi = int(out["cpu_baseline_multicore"]["cores"])
+            rows = sum(int(plan[0][g]) * groups[g].nev for g in range(n_multi))
+            for label, gsel, pl, ref_a in (
+                    ("fp32c_vs_fp64_oracle", [g0], (np.array([M], np.int32), plan_g[1], plan_g[2]), a),
+                    (f"fp32c_vs_fp64_oracle_{n_multi}_groups", groups[:n_multi],
+                     (plan[0][:n_multi], plan[1][:rows], plan[2][:2 * rows]), multi)):
+                out[label] = precision_vs_oracle(gsel, pl, ref_a)
     print(json.dumps(out), flush=True)
 
 
