@@ -296,25 +296,26 @@ struct hbt_ctx {
     // slab binning of the fast kernel (hbt_slab.h)
     struct {
         bool on = false;            // the next fast launch bins through slabs
-        int S = 0, SB = 0, parts = 1, nblk = 0;
-        std::vector<int> scap, mcap;
-        std::vector<long long> sbase, mbase;
+        int S = 0, SB = 0, nunits = 0, nblk = 0;
+        std::vector<int> scap, uoff;
+        std::vector<long long> sbase;
+        std::vector<int4> units;    // k_slab_bin blocks: (slab, first region, end region)
         // demand measured on the last launch: per slab total and largest
-        // per block (same, mixed), with that launch's pairs and blocks
-        std::vector<double> tot_s, tot_m, max_s, max_m;
-        double hist_enum_s = 0, hist_enum_m = 0;
+        // per block, with that launch's same-event pairs and blocks
+        std::vector<double> tot_s, max_s;
+        double hist_enum_s = 0;
         int hist_nblk = 0;
         bool have_hist = false, pending = false, resize = false;
-        uint64_t direct = 0;        // binned pairs of the last launch added directly
+        uint64_t direct = 0;        // binned same-event pairs of the last launch added directly
         char *d_vals = nullptr;
-        unsigned short *d_sidx = nullptr, *d_midx = nullptr;
-        long long *d_sbase = nullptr, *d_mbase = nullptr;
-        int *d_scap = nullptr, *d_mcap = nullptr;
-        unsigned *d_sfill = nullptr, *d_mfill = nullptr;
+        long long *d_sbase = nullptr;
+        int *d_scap = nullptr, *d_uoff = nullptr;
+        int4 *d_units = nullptr;
+        unsigned *d_sfill = nullptr;
         unsigned long long *d_stats = nullptr;
         SlabPart *d_parts = nullptr;
-        size_t vals_cap = 0, sidx_cap = 0, midx_cap = 0, base_cap = 0, fill_cap = 0,
-               stats_cap = 0, parts_cap = 0, scap_cap = 0;
+        size_t vals_cap = 0, base_cap = 0, fill_cap = 0, stats_cap = 0, parts_cap = 0,
+               scap_cap = 0, units_cap = 0, uoff_cap = 0;
     } slab;
     uint64_t launch_enum_same = 0, launch_enum_mixed = 0;  // pairs of the items uploaded last
     FastArgs batch_args{};  // arguments of the prepared batch's fast launch
@@ -602,23 +603,18 @@ int slab_read_stats(hbt_ctx *c) {
     if (!sl.pending) return HBT_OK;
     sl.pending = false;
     const int S = sl.S;
-    std::vector<unsigned long long> st((size_t)4 * S + 1);
+    std::vector<unsigned long long> st((size_t)2 * S + 1);
     HIPCHECK(c, hipMemcpy(st.data(), sl.d_stats, st.size() * 8, hipMemcpyDeviceToHost));
     sl.tot_s.assign(S, 0);
-    sl.tot_m.assign(S, 0);
     sl.max_s.assign(S, 0);
-    sl.max_m.assign(S, 0);
     double tot = 0;
     for (int k = 0; k < S; k++) {
         sl.tot_s[k] = (double)st[k];
-        sl.tot_m[k] = (double)st[S + k];
-        sl.max_s[k] = (double)st[2 * S + 1 + k];
-        sl.max_m[k] = (double)st[3 * S + 1 + k];
-        tot += sl.tot_s[k] + sl.tot_m[k];
+        sl.max_s[k] = (double)st[S + 1 + k];
+        tot += sl.tot_s[k];
     }
-    sl.direct = st[2 * S];
+    sl.direct = st[S];
     sl.hist_enum_s = (double)c->launch_enum_same;
-    sl.hist_enum_m = (double)c->launch_enum_mixed;
     sl.hist_nblk = sl.nblk;
     sl.have_hist = true;
     if ((double)sl.direct > 0.002 * tot) sl.resize = true;
@@ -629,7 +625,7 @@ int slab_read_stats(hbt_ctx *c) {
 // outside any graph capture.  Capacity per region: the largest demand one
 // block had for the slab on the last launch, scaled to this launch's pairs
 // and blocks, with a margin; with no history, a uniform share of a generous
-// binned fraction.  The record buffers are capped at kSlabBudget bytes:
+// binned fraction.  The record buffer is capped at kSlabBudget bytes:
 // beyond, regions shrink and more pairs take the direct path.
 constexpr double kSlabBudget = 16e9;
 
@@ -639,6 +635,32 @@ bool slab_applies(const hbt_ctx *c) {
     const int q = c->P.qnpts;
     return c->cfg.kernel == HBT_KERNEL_FAST && !c->P.qinv && !c->P.azimuthal &&
            c->cfg.n_KT * c->P.nphi * q <= kMaxSlabs && q * q <= kMaxSlabBins;
+}
+
+// k_slab_bin's units: about four per resident block (two 72 KB blocks per
+// CU), dealt to slabs in proportion to their demand on the last launch (a
+// uniform share without one), each slab's regions split evenly among its
+// units, and at most kMaxUnitRecords records of capacity in a unit.
+void slab_units(hbt_ctx *c) {
+    auto &sl = c->slab;
+    const int S = sl.S, nblk = sl.nblk;
+    const double target = 4.0 * 2 * std::max(1, c->cu_count);
+    double tot = 0;
+    if (sl.have_hist)
+        for (int k = 0; k < S; k++) tot += sl.tot_s[k];
+    sl.units.clear();
+    sl.uoff.assign(S + 1, 0);
+    for (int k = 0; k < S; k++) {
+        const double w = tot > 0 ? sl.tot_s[k] / tot : 1.0 / S;
+        const double need = std::ceil((double)nblk * sl.scap[k] / kMaxUnitRecords);
+        const int n = (int)std::min((double)nblk, std::max({1.0, need, std::round(target * w)}));
+        sl.uoff[k] = (int)sl.units.size();
+        for (int u = 0; u < n; u++)
+            sl.units.push_back(int4{k, (int)((long long)u * nblk / n),
+                                    (int)((long long)(u + 1) * nblk / n), 0});
+    }
+    sl.uoff[S] = (int)sl.units.size();
+    sl.nunits = (int)sl.units.size();
 }
 
 int slab_prepare(hbt_ctx *c, int nitems) {
@@ -656,67 +678,46 @@ int slab_prepare(hbt_ctx *c, int nitems) {
     sl.nblk = nblk;
     sl.resize = false;
     sl.scap.assign(S, 0);
-    sl.mcap.assign(S, 0);
-    const double es = (double)c->launch_enum_same, em = (double)c->launch_enum_mixed;
+    const double es = (double)c->launch_enum_same;
     for (int k = 0; k < S; k++) {
-        double xs, xm;
+        double xs;
         if (sl.have_hist) {
             const double bs = (double)sl.hist_nblk / nblk;
             xs = sl.hist_enum_s > 0 ? sl.max_s[k] * es / sl.hist_enum_s * bs : 0.0;
-            xm = sl.hist_enum_m > 0 ? sl.max_m[k] * em / sl.hist_enum_m * bs : 0.0;
             // a slab the last launch never reached may still get pairs now
             xs = std::max(xs, 0.002 * es / S / nblk);
-            xm = std::max(xm, 0.002 * em / S / nblk);
         } else {
             xs = 3.0 * 0.03 * es / S / nblk;
-            xm = 3.0 * 0.03 * em / S / nblk;
         }
-        sl.scap[k] = (int)std::min(1e9, std::ceil(1.25 * xs) + 32);
-        sl.mcap[k] = (int)std::min(1e9, std::ceil(1.25 * xm) + 32);
+        sl.scap[k] = (int)std::min((double)kMaxUnitRecords, std::ceil(1.25 * xs) + 32);
     }
-    double rs = 0, rm = 0;
-    for (int k = 0; k < S; k++) {
-        rs += (double)sl.scap[k] * nblk;
-        rm += (double)sl.mcap[k] * nblk;
-    }
-    const double bytes = rs * (vsz + 2) + rm * 2;
-    if (bytes > kSlabBudget) {
-        const double f = kSlabBudget / bytes;
-        rs = rm = 0;
-        for (int k = 0; k < S; k++) {
-            sl.scap[k] = std::max(1, (int)(sl.scap[k] * f));
-            sl.mcap[k] = std::max(1, (int)(sl.mcap[k] * f));
-            rs += (double)sl.scap[k] * nblk;
-            rm += (double)sl.mcap[k] * nblk;
-        }
+    double rs = 0;
+    for (int k = 0; k < S; k++) rs += (double)sl.scap[k] * nblk;
+    if (rs * vsz > kSlabBudget) {
+        const double f = kSlabBudget / (rs * vsz);
+        for (int k = 0; k < S; k++) sl.scap[k] = std::max(1, (int)(sl.scap[k] * f));
     }
     sl.sbase.assign(S, 0);
-    sl.mbase.assign(S, 0);
-    long long as = 0, am = 0;
+    long long as = 0;
     for (int k = 0; k < S; k++) {
         sl.sbase[k] = as;
-        sl.mbase[k] = am;
         as += (long long)sl.scap[k] * nblk;
-        am += (long long)sl.mcap[k] * nblk;
     }
-    // parts per slab: about four k_slab_bin blocks per CU in all
-    sl.parts = std::max(1, std::min(64, (4 * c->cu_count + S - 1) / S));
+    slab_units(c);
     int rc = ensure(c, &sl.d_vals, sl.vals_cap, (size_t)std::max(1ll, as) * vsz);
-    if (!rc) rc = ensure(c, &sl.d_sidx, sl.sidx_cap, (size_t)std::max(1ll, as));
-    if (!rc) rc = ensure(c, &sl.d_midx, sl.midx_cap, (size_t)std::max(1ll, am));
-    if (!rc) rc = ensure(c, &sl.d_sbase, sl.base_cap, (size_t)2 * S);
-    if (!rc) rc = ensure(c, &sl.d_scap, sl.scap_cap, (size_t)2 * S);
-    if (!rc) rc = ensure(c, &sl.d_sfill, sl.fill_cap, (size_t)2 * nblk * S);
-    if (!rc) rc = ensure(c, &sl.d_stats, sl.stats_cap, (size_t)4 * S + 1);
-    if (!rc) rc = ensure(c, &sl.d_parts, sl.parts_cap, (size_t)S * sl.parts * SB);
+    if (!rc) rc = ensure(c, &sl.d_sbase, sl.base_cap, (size_t)S);
+    if (!rc) rc = ensure(c, &sl.d_scap, sl.scap_cap, (size_t)S);
+    if (!rc) rc = ensure(c, &sl.d_uoff, sl.uoff_cap, (size_t)S + 1);
+    if (!rc) rc = ensure(c, &sl.d_units, sl.units_cap, (size_t)sl.nunits);
+    if (!rc) rc = ensure(c, &sl.d_sfill, sl.fill_cap, (size_t)nblk * S);
+    if (!rc) rc = ensure(c, &sl.d_stats, sl.stats_cap, (size_t)2 * S + 1);
+    if (!rc) rc = ensure(c, &sl.d_parts, sl.parts_cap, (size_t)sl.nunits * SB);
     if (rc) return rc;
-    sl.d_mbase = sl.d_sbase + S;
-    sl.d_mcap = sl.d_scap + S;
-    sl.d_mfill = sl.d_sfill + (size_t)nblk * S;
     HIPCHECK(c, hipMemcpy(sl.d_sbase, sl.sbase.data(), S * 8, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(sl.d_mbase, sl.mbase.data(), S * 8, hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(sl.d_scap, sl.scap.data(), S * 4, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(sl.d_mcap, sl.mcap.data(), S * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(sl.d_uoff, sl.uoff.data(), (S + 1) * 4, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(sl.d_units, sl.units.data(), sl.units.size() * sizeof(int4),
+                          hipMemcpyHostToDevice));
     return HBT_OK;
 }
 
@@ -739,17 +740,17 @@ int fast_args(hbt_ctx *c, FastArgs &a) {
     const auto &sl = c->slab;
     if (sl.on) {
         a.slab.vals = sl.d_vals;
-        a.slab.sidx = sl.d_sidx;
-        a.slab.midx = sl.d_midx;
         a.slab.sbase = sl.d_sbase;
-        a.slab.mbase = sl.d_mbase;
         a.slab.scap = sl.d_scap;
-        a.slab.mcap = sl.d_mcap;
         a.slab.sfill = sl.d_sfill;
-        a.slab.mfill = sl.d_mfill;
         a.slab.stats = sl.d_stats;
+        a.slab.units = sl.d_units;
+        a.slab.uoff = sl.d_uoff;
         a.slab.S = sl.S;
         a.slab.SB = sl.SB;
+        a.slab.q = c->P.qnpts;
+        a.slab.q_base = c->P.q_base;
+        a.slab.inv_dq = 1.0 / c->P.delta_q;
     }
     return HBT_OK;
 }
@@ -763,21 +764,21 @@ int launch_fast(hbt_ctx *c, FastArgs &a, hipStream_t st) {
     if (slab) {
         if (blocks != c->slab.nblk)
             return fail(c, HBT_ERR_STATE, "slab regions sized for another launch shape");
-        HIPCHECK(c, launch_zero_words(a.slab.stats, (size_t)4 * a.slab.S + 1, st));
+        HIPCHECK(c, launch_zero_words(a.slab.stats, (size_t)2 * a.slab.S + 1, st));
     }
     HIPCHECK(c, launch_fast_kernel(a, f32, c->tune.R, c->P.qinv || c->P.azimuthal, blocks, st));
     if (slab) {
-        const int parts = c->slab.parts;
+        const int nu = c->slab.nunits;
         if (f32)
-            hipLaunchKernelGGL(k_slab_bin<float>, dim3(a.slab.S, parts), dim3(1024), 0, st,
-                               a.slab, (int)blocks, parts, c->slab.d_parts);
+            hipLaunchKernelGGL(k_slab_bin<float>, dim3(nu), dim3(1024), 0, st, a.slab,
+                               c->slab.d_parts);
         else
-            hipLaunchKernelGGL(k_slab_bin<double>, dim3(a.slab.S, parts), dim3(1024), 0, st,
-                               a.slab, (int)blocks, parts, c->slab.d_parts);
+            hipLaunchKernelGGL(k_slab_bin<double>, dim3(nu), dim3(1024), 0, st, a.slab,
+                               c->slab.d_parts);
         HIPCHECK(c, hipGetLastError());
         const long long nb = (long long)a.slab.S * a.slab.SB;
         hipLaunchKernelGGL(k_slab_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st,
-                           (const SlabPart *)c->slab.d_parts, a.slab.S, a.slab.SB, parts,
+                           (const SlabPart *)c->slab.d_parts, a.slab.uoff, a.slab.S, a.slab.SB,
                            c->d_bins);
         HIPCHECK(c, hipGetLastError());
         c->slab.pending = true;
@@ -1528,9 +1529,9 @@ int hbt_destroy(hbt_ctx *c) {
                     c->d_items,     c->d_counter,   c->d_psi,        c->d_view,
                     c->d_cnt,       c->d_cut,       c->d_lim,        c->d_gpoff,
                     c->d_wins,      c->d_defer,     c->d_ndefer,     c->d_dkey,
-                    c->d_darr,      c->slab.d_vals, c->slab.d_sidx,  c->slab.d_midx,
+                    c->d_darr,      c->slab.d_vals, c->slab.d_units, c->slab.d_uoff,
                     c->slab.d_sbase, c->slab.d_scap, c->slab.d_sfill, c->slab.d_stats,
-                    c->slab.d_parts};  // d_mbase, d_mcap, d_mfill point into these
+                    c->slab.d_parts};
     for (void *p : ptrs)
         if (p) hipFree(p);
     free_slot(c->slot[0]);
@@ -1766,8 +1767,7 @@ int hbt_slab_info(hbt_ctx *c, int32_t *on, uint64_t *direct, uint64_t *capacity)
     if (int rc = slab_read_stats(c)) return rc;
     const auto &sl = c->slab;
     uint64_t cap = 0;
-    for (size_t k = 0; k < sl.scap.size(); k++)
-        cap += ((uint64_t)sl.scap[k] + (uint64_t)sl.mcap[k]) * (uint64_t)sl.nblk;
+    for (size_t k = 0; k < sl.scap.size(); k++) cap += (uint64_t)sl.scap[k] * (uint64_t)sl.nblk;
     if (on) *on = sl.on ? 1 : 0;
     if (direct) *direct = sl.on ? sl.direct : 0;
     if (capacity) *capacity = sl.on ? cap : 0;

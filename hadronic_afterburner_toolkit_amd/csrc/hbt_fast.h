@@ -245,14 +245,12 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
         }
     }
     const bool binned = (st & ST_BIN) != 0;
-    // slab binning: the pair becomes a record of its (block, slab) region;
-    // a full region sends it down the direct path
+    // slab binning: a same-event pair becomes a record of its (block, slab)
+    // region; a full region, or stored values that would land in another
+    // bin, send it down the direct path.  Mixed pairs are always direct.
     const bool slab = !EXT && sl.vals != nullptr;
     if (w.kind != 0) {
-        if (binned && !(slab && slab_put_mixed(sl, fill, kmix))) {
-            add_mixed_direct(v.H, kmix);
-            direct_n += slab ? 1u : 0u;
-        }
+        if (binned) add_mixed_direct(v.H, kmix);
         return binned ? 1 : 0;
     }
     bool direct = binned;
@@ -347,11 +345,11 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     constexpr int QCAP = Ring<R>::CAP;
     __shared__ unsigned ring[4][QCAP];
     __shared__ Stage stage[4];
-    // slab binning: records per slab of this block (same, then mixed)
-    __shared__ unsigned slab_fill[EXT ? 1 : 2 * kMaxSlabs];
+    // slab binning: records per slab of this block
+    __shared__ unsigned slab_fill[EXT ? 1 : kMaxSlabs];
     const bool slab = !EXT && a.slab.vals != nullptr;
     if (slab) {
-        for (int k = threadIdx.x; k < 2 * kMaxSlabs; k += blockDim.x) slab_fill[k] = 0u;
+        for (int k = threadIdx.x; k < kMaxSlabs; k += blockDim.x) slab_fill[k] = 0u;
         __syncthreads();
     }
     unsigned direct_n = 0;
@@ -512,12 +510,10 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
         // regions' capacity) and for sizing later launches
         unsigned long long dn = direct_n;
         for (int off = 32; off > 0; off >>= 1) dn += __shfl_down(dn, off);
-        if (lane == 0 && dn) atomic_add_u64(&a.slab.stats[2 * a.slab.S], dn);
+        if (lane == 0 && dn) atomic_add_u64(&a.slab.stats[a.slab.S], dn);
         __syncthreads();
-        for (int k = threadIdx.x; k < a.slab.S; k += blockDim.x) {
+        for (int k = threadIdx.x; k < a.slab.S; k += blockDim.x)
             a.slab.sfill[(long long)blockIdx.x * a.slab.S + k] = slab_fill[k];
-            a.slab.mfill[(long long)blockIdx.x * a.slab.S + k] = slab_fill[kMaxSlabs + k];
-        }
     }
 }
 

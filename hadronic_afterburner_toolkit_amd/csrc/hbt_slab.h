@@ -1,29 +1,35 @@
-// hbt_slab.h -- slab binning: the histogram accumulation of the fast kernel
-// without device-scope atomics.
+// hbt_slab.h -- slab binning: the same-event histogram accumulation of the
+// fast kernel without device-scope atomics.
 //
 // Every binned pair is one read-modify-write of a random 64-byte bin.  Done
 // as a device-scope atomic, each such access is executed at the memory side
 // of this part at about 2.35e10 per second for the whole chip, whatever the
-// histogram's size or placement: that rate, not arithmetic, bounded the
-// fast kernel.  The histogram (qnpts^3 bins per K_T bin, tens of MB) cannot
-// live in LDS, and a block's binned pairs land in random bins, so there is
-// nothing to combine inside one block.  Instead:
+// histogram's size or placement: that rate, not arithmetic, bounds the fast
+// kernel.  The histogram (qnpts^3 bins per K_T bin, tens of MB) cannot live
+// in LDS, and a block's binned pairs land in random bins, so there is
+// nothing to combine inside one block.  Instead, for same-event pairs:
 //
 //  1. k_fast writes each binned pair as a record into the region of its
-//     (block, slab): a slab is one (K_T, K_phi, q_out) plane of qnpts^2
-//     bins; the block's LDS fill counter of the slab gives the record's
-//     place, so no global atomic is involved.  Same-event record: q_out,
-//     q_side, q_long, cos (FP32 in the precision form, FP64 in the FP64
-//     form) + the bin inside the slab (u16); mixed record: the bin (u16).
-//     A record whose region is full takes the direct atomic path instead
-//     (always correct; the host sizes the regions from the demand of earlier
-//     launches so that this is rare).
-//  2. k_slab_bin: block (slab, part) adds the records of a range of regions
-//     of its slab into LDS (u32 counts, FP64 sums) and stores the part's
-//     totals.
-//  3. k_slab_reduce: one thread per bin adds the parts of its bin to the
-//     histogram (count, sums, den), plain read-modify-write: each bin has
-//     one owner.
+//     (block, slab): a slab is one (K_T, q_out) plane of qnpts^2 bins; the
+//     block's LDS fill counter of the slab gives the record's place, so no
+//     global atomic is involved.  A record is the pair's q_out, q_side,
+//     q_long and cos: one 16-byte store (FP32, the precision form) or 32
+//     bytes (FP64).  It holds no bin index: k_slab_bin recovers the bin
+//     inside the slab from the stored q_side and q_long (slab_coord), and
+//     k_fast writes the record only when that gives the pair's bin, so a
+//     pair whose stored values round across a bin edge takes the direct
+//     atomic path.  So does a record whose region is full (the host sizes
+//     the regions from the demand of earlier launches so that this is rare).
+//  2. k_slab_bin: one block per unit (slab, range of regions) adds its
+//     records into LDS (u32 counts; sums as 64-bit fixed point in the
+//     precision form, FP64 in the FP64 form) and stores the unit's totals.  Slabs get units in proportion to their demand on the last
+//     launch, so the units carry similar work.
+//  3. k_slab_reduce: one thread per bin adds the units of its slab to the
+//     histogram, plain read-modify-write: each bin has one owner.
+//
+// Mixed pairs add one u32 to the denominator directly: their atomic is the
+// cheaper pattern, and a 2-byte record store per pair measured costlier than
+// it.
 //
 // In slab mode the precision form's per-pair values are FP32 (records and
 // the direct path alike), accumulated in FP64; without slab binning they
@@ -37,148 +43,179 @@ namespace hbt {
 
 constexpr int kMaxSlabs = 512;     // slabs per analysis (LDS fill counters of k_fast)
 constexpr int kMaxSlabBins = 2048;  // bins per slab (LDS of k_slab_bin)
+constexpr int kMaxUnitRecords = (1 << 23) - 1;  // records of one k_slab_bin unit
 
 struct SlabArgs {
-    void *vals;                // same-event values: float4 (precision form) or 2 x double2
-    unsigned short *sidx;      // same-event records' bin inside the slab
-    unsigned short *midx;      // mixed records' bin inside the slab
+    void *vals;                // records: float4 (precision form) or 2 x double2
     const long long *sbase;    // [S] first record of slab s's regions (block 0)
-    const long long *mbase;
     const int *scap;           // [S] records per (block, slab) region
-    const int *mcap;
     unsigned *sfill;           // [blocks][S] records each block had for the slab
-    unsigned *mfill;           //   (counted past the capacity: the demand)
-    unsigned long long *stats; // [4S + 1]: demand per slab (same, mixed), direct adds,
-                               // largest demand of one block per slab (same, mixed)
-    int S, SB;                 // slabs, bins per slab
+                               //   (counted past the capacity: the demand)
+    unsigned long long *stats; // [2S + 1]: demand per slab, direct adds,
+                               // largest demand of one block per slab
+    const int4 *units;         // k_slab_bin blocks: (slab, first region, end region, 0)
+    const int *uoff;           // [S + 1] first unit of each slab
+    int S, SB, q;              // slabs, bins per slab, q bins per axis
+    double q_base, inv_dq;     // q_side / q_long bin of a stored value
 };
 
-// Partial totals of one slab part, per bin.
+// Totals of one unit, per bin of its slab.
 struct SlabPart {
     double cnt, qo, qs, ql, c;
-    unsigned long long den;
 };
+
+// The q_side or q_long bin of a record's stored value.  k_fast and
+// k_slab_bin evaluate it on the same stored value with the same two
+// rounded operations (the build contracts nothing), so they agree bit for
+// bit; k_fast writes a record only where it equals the pair's bin.
+__device__ __forceinline__ int slab_coord(const SlabArgs &a, double v) {
+    return (int)((v - a.q_base) * a.inv_dq);
+}
 
 template <typename Prec>
 __device__ __forceinline__ bool slab_put_same(const SlabArgs &a, unsigned *fill, long k,
                                               double qo, double qs, double ql, double c) {
-    const int s = (int)(k / a.SB);
-    const int bi = (int)(k - (long)s * a.SB);
+    constexpr bool f32 = std::is_same<Prec, float>::value;
+    const int s = (int)k / a.SB;
+    const int bi = (int)k - s * a.SB;
+    const int is = bi / a.q, il = bi - is * a.q;
+    if (slab_coord(a, f32 ? (double)(float)qs : qs) != is ||
+        slab_coord(a, f32 ? (double)(float)ql : ql) != il)
+        return false;
     const unsigned pos = atomicAdd(&fill[s], 1u);
     const int cap = a.scap[s];
     if (pos >= (unsigned)cap) return false;
     const long long e = a.sbase[s] + (long long)blockIdx.x * cap + pos;
-    if (std::is_same<Prec, float>::value) {
+    if (f32) {
         ((float4 *)a.vals)[e] = make_float4((float)qo, (float)qs, (float)ql, (float)c);
     } else {
         double2 *d = (double2 *)a.vals + 2 * e;
         d[0] = make_double2(qo, qs);
         d[1] = make_double2(ql, c);
     }
-    a.sidx[e] = (unsigned short)bi;
     return true;
 }
 
-__device__ __forceinline__ bool slab_put_mixed(const SlabArgs &a, unsigned *fill, long k) {
-    const int s = (int)(k / a.SB);
-    const int bi = (int)(k - (long)s * a.SB);
-    const unsigned pos = atomicAdd(&fill[kMaxSlabs + s], 1u);
-    const int cap = a.mcap[s];
-    if (pos >= (unsigned)cap) return false;
-    a.midx[a.mbase[s] + (long long)blockIdx.x * cap + pos] = (unsigned short)bi;
-    return true;
-}
-
-// Block (slab s, part p) of 1024 threads: the records of regions
-// [p * nblk / parts, (p + 1) * nblk / parts) of slab s into LDS, then the
-// part's totals to out[(s * parts + p) * SB + bin].  Each part adds its
-// regions' demand (records wanted, also past their capacity) to stats.
 template <typename Prec>
-__global__ __launch_bounds__(1024) void k_slab_bin(SlabArgs a, int nblk, int parts,
-                                                   SlabPart *out) {
-    __shared__ unsigned cnt[kMaxSlabBins], den[kMaxSlabBins];
+__device__ __forceinline__ void slab_load(const SlabArgs &a, long long e, double &v0, double &v1,
+                                          double &v2, double &v3) {
+    if (std::is_same<Prec, float>::value) {
+        const float4 f = ((const float4 *)a.vals)[e];
+        v0 = f.x, v1 = f.y, v2 = f.z, v3 = f.w;
+    } else {
+        const double2 *d = (const double2 *)a.vals + 2 * e;
+        const double2 x = d[0], y = d[1];
+        v0 = x.x, v1 = x.y, v2 = y.x, v3 = y.y;
+    }
+}
+
+// Fixed-point scale of the precision form's LDS sums: an FP32 record value
+// v with |v| >= 2^-17 is a multiple of 2^-40, so v * 2^40 is an exact
+// integer (smaller values round by at most 2^-41); |v| <= 1 and at most
+// kMaxUnitRecords records per unit keep every sum below 2^63.  Integer sums are exact,
+// so a unit's totals do not depend on the order of its LDS adds, and the
+// 64-bit integer LDS add is cheaper than the FP64 one.
+constexpr double kFixScale = 1099511627776.0;  // 2^40
+
+template <typename Prec>
+__device__ __forceinline__ void slab_add(const SlabArgs &a, unsigned *cnt, double *sum,
+                                         double v0, double v1, double v2, double v3) {
+    // clamped for safety only: a stored record's coordinates are in range
+    const int is = min(max(slab_coord(a, v1), 0), a.q - 1);
+    const int il = min(max(slab_coord(a, v2), 0), a.q - 1);
+    const int bi = is * a.q + il;
+    atomicAdd(&cnt[bi], 1u);
+    if (std::is_same<Prec, float>::value) {
+        unsigned long long *fix = (unsigned long long *)sum;
+        atomicAdd(&fix[4 * bi], (unsigned long long)__double2ll_rn(v0 * kFixScale));
+        atomicAdd(&fix[4 * bi + 1], (unsigned long long)__double2ll_rn(v1 * kFixScale));
+        atomicAdd(&fix[4 * bi + 2], (unsigned long long)__double2ll_rn(v2 * kFixScale));
+        atomicAdd(&fix[4 * bi + 3], (unsigned long long)__double2ll_rn(v3 * kFixScale));
+    } else {
+        atomicAdd(&sum[4 * bi], v0);
+        atomicAdd(&sum[4 * bi + 1], v1);
+        atomicAdd(&sum[4 * bi + 2], v2);
+        atomicAdd(&sum[4 * bi + 3], v3);
+    }
+}
+
+// Block u of 1024 threads: unit (slab s, regions [r0, r1)).  Each wave takes
+// every 16th region of the range and its lanes the region's records, two
+// loads in flight per lane; the totals go to out[u * SB + bin].  LDS sums:
+// fixed point (kFixScale) in the precision form, FP64 in the FP64 form.  Each unit
+// adds its regions' demand (records wanted, also past their capacity) to
+// stats.
+template <typename Prec>
+__global__ __launch_bounds__(1024) void k_slab_bin(SlabArgs a, SlabPart *out) {
+    __shared__ unsigned cnt[kMaxSlabBins];
     __shared__ double sum[4 * kMaxSlabBins];
-    const int s = blockIdx.x, p = blockIdx.y;
-    const int SB = a.SB;
+    const int4 u = a.units[blockIdx.x];
+    const int s = u.x, SB = a.SB;
     for (int i = threadIdx.x; i < SB; i += blockDim.x) {
         cnt[i] = 0u;
-        den[i] = 0u;
-        sum[4 * i] = sum[4 * i + 1] = sum[4 * i + 2] = sum[4 * i + 3] = 0.0;
+        sum[4 * i] = sum[4 * i + 1] = sum[4 * i + 2] = sum[4 * i + 3] = 0.0;  // also integer 0
     }
     __syncthreads();
-    const int b0 = (int)((long long)p * nblk / parts), b1 = (int)((long long)(p + 1) * nblk / parts);
-    const int scap = a.scap[s], mcap = a.mcap[s];
-    unsigned long long ds = 0, dm = 0;
-    unsigned xs = 0, xm = 0;
-    for (int b = b0; b < b1; b++) {
-        const unsigned fs = a.sfill[(long long)b * a.S + s], fm = a.mfill[(long long)b * a.S + s];
-        ds += fs;
-        dm += fm;
-        xs = max(xs, fs);
-        xm = max(xm, fm);
-        const int ns = (int)min(fs, (unsigned)scap), nm = (int)min(fm, (unsigned)mcap);
-        const long long es = a.sbase[s] + (long long)b * scap;
-        for (int r = threadIdx.x; r < ns; r += blockDim.x) {
-            const int bi = a.sidx[es + r];
-            double v0, v1, v2, v3;
-            if (std::is_same<Prec, float>::value) {
-                const float4 f = ((const float4 *)a.vals)[es + r];
-                v0 = f.x, v1 = f.y, v2 = f.z, v3 = f.w;
-            } else {
-                const double2 *d = (const double2 *)a.vals + 2 * (es + r);
-                const double2 x = d[0], y = d[1];
-                v0 = x.x, v1 = x.y, v2 = y.x, v3 = y.y;
-            }
-            atomicAdd(&cnt[bi], 1u);
-            atomicAdd(&sum[4 * bi], v0);
-            atomicAdd(&sum[4 * bi + 1], v1);
-            atomicAdd(&sum[4 * bi + 2], v2);
-            atomicAdd(&sum[4 * bi + 3], v3);
+    const int scap = a.scap[s];
+    const long long base = a.sbase[s];
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    unsigned long long demand = 0;  // wave-uniform
+    unsigned most = 0;
+    for (int b = u.y + (int)(threadIdx.x >> 6); b < u.z; b += nw) {
+        const unsigned f = a.sfill[(long long)b * a.S + s];
+        demand += f;
+        most = max(most, f);
+        const int n = (int)min(f, (unsigned)scap);
+        const long long e0 = base + (long long)b * scap;
+        for (int r = lane; r < n; r += 128) {
+            double x0, x1, x2, x3, y0 = 0, y1 = 0, y2 = 0, y3 = 0;
+            const bool two = r + 64 < n;
+            slab_load<Prec>(a, e0 + r, x0, x1, x2, x3);
+            if (two) slab_load<Prec>(a, e0 + r + 64, y0, y1, y2, y3);
+            slab_add<Prec>(a, cnt, sum, x0, x1, x2, x3);
+            if (two) slab_add<Prec>(a, cnt, sum, y0, y1, y2, y3);
         }
-        const long long em = a.mbase[s] + (long long)b * mcap;
-        for (int r = threadIdx.x; r < nm; r += blockDim.x) atomicAdd(&den[a.midx[em + r]], 1u);
     }
-    if (threadIdx.x == 0) {  // demand of this part's regions (thread-uniform values)
-        atomicAdd(&a.stats[s], ds);
-        atomicAdd(&a.stats[a.S + s], dm);
-        atomicMax(&a.stats[2 * a.S + 1 + s], (unsigned long long)xs);
-        atomicMax(&a.stats[3 * a.S + 1 + s], (unsigned long long)xm);
+    if (lane == 0 && demand) {
+        atomicAdd(&a.stats[s], demand);
+        atomicMax(&a.stats[a.S + 1 + s], (unsigned long long)most);
     }
     __syncthreads();
-    SlabPart *o = out + ((long long)s * parts + p) * SB;
-    for (int i = threadIdx.x; i < SB; i += blockDim.x)
-        o[i] = SlabPart{(double)cnt[i], sum[4 * i], sum[4 * i + 1], sum[4 * i + 2],
-                        sum[4 * i + 3], (unsigned long long)den[i]};
+    SlabPart *o = out + (long long)blockIdx.x * SB;
+    for (int i = threadIdx.x; i < SB; i += blockDim.x) {
+        double t[4];
+        for (int f = 0; f < 4; f++)
+            t[f] = std::is_same<Prec, float>::value
+                       ? (double)((const long long *)sum)[4 * i + f] / kFixScale
+                       : sum[4 * i + f];
+        o[i] = SlabPart{(double)cnt[i], t[0], t[1], t[2], t[3]};
+    }
 }
 
-// One thread per bin of the slabs: the parts' totals into the histogram.
-// Slab s covers bins [s * SB, (s + 1) * SB) of the [kt][kphi][q_out][q_side]
+// One thread per bin of the slabs: the units' totals into the histogram.
+// Slab s covers bins [s * SB, (s + 1) * SB) of the [kt][q_out][q_side]
 // [q_long] layout.
-static __global__ void k_slab_reduce(const SlabPart *parts_out, int S, int SB, int parts, double *bins) {
+static __global__ void k_slab_reduce(const SlabPart *parts, const int *uoff, int S, int SB,
+                                     double *bins) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)S * SB) return;
     const int s = (int)(k / SB), bi = (int)(k - (long long)s * SB);
     double c = 0, qo = 0, qs = 0, ql = 0, cs = 0;
-    unsigned long long d = 0;
-    for (int p = 0; p < parts; p++) {
-        const SlabPart &x = parts_out[((long long)s * parts + p) * SB + bi];
+    for (int u = uoff[s]; u < uoff[s + 1]; u++) {
+        const SlabPart &x = parts[(long long)u * SB + bi];
         c += x.cnt;
         qo += x.qo;
         qs += x.qs;
         ql += x.ql;
         cs += x.c;
-        d += x.den;
     }
-    if (c == 0.0 && d == 0) return;
+    if (c == 0.0) return;
     double *b = bins + k * BIN_STRIDE;
-    if (c != 0.0) {
-        b[BIN_COUNT] += c;
-        b[BIN_QO] += qo;
-        b[BIN_QS] += qs;
-        b[BIN_QL] += ql;
-        b[BIN_NUM] += cs;
-    }
-    if (d) ((unsigned long long *)b)[BIN_DEN] += d;
+    b[BIN_COUNT] += c;
+    b[BIN_QO] += qo;
+    b[BIN_QS] += qs;
+    b[BIN_QL] += ql;
+    b[BIN_NUM] += cs;
 }
 
 }  // namespace hbt

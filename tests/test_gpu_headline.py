@@ -6,7 +6,9 @@ Random(2026), the first draws of the bench's plan stream), q 41^3 on
 
 It runs through the batched path the bench times (upload_groups,
 prepare_batch, one launch) in the plain FP64, fast FP64 and precision forms,
-against the oracle on the same inputs and plan:
+the fast forms also with slab binning (hbt_binning = 1: same-event pairs as
+records summed per q_out plane in LDS), against the oracle on the same
+inputs and plan:
   - counts (numerator, denominator, K_T counters): bit-exact;
   - sums: err = |S - S_ref| / max(|S_ref|, sum|.|_bin) within 1e-12 (FP64)
     and 1e-6 (precision form); the max and RMS of err over the bins with
@@ -23,9 +25,11 @@ pytestmark = pytest.mark.gpu
 PARAMS = dict(qnpts=41, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
               HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18,
               number_of_mixed_events=10, long_comoving_boost=1)
-MODES = {"plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12),
-         "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12),
-         "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6)}
+MODES = {"plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12, 0),
+         "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 0),
+         "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 0),
+         "fast_fp64_slab": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 1),
+         "fast_fp32c_slab": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 1)}
 SUMS = (("q_out_sum", "abs_q_out"), ("q_side_sum", "abs_q_side"), ("q_long_sum", "abs_q_long"),
         ("num", "abs_num"))
 
@@ -48,16 +52,20 @@ def headline_group(oracle_mod):
 @pytest.mark.parametrize("mode", list(MODES))
 def test_headline_group_vs_oracle(headline_group, mode):
     g0, (M, ids, cs), ref = headline_group
-    prec, kern, tol = MODES[mode]
+    prec, kern, tol, binning = MODES[mode]
     print(f"headline group, {mode}", flush=True)
-    g = hbt.HBTCorrelation(PARAMS, precision=prec, kernel=kern)
+    g = hbt.HBTCorrelation(dict(PARAMS, hbt_binning=binning), precision=prec, kernel=kern)
     g.upload_groups([g0])
     g.prepare_batch(np.array([M], np.int32), ids, cs)
     g.launch_batch()
     g.sync()
     r = g.results()
     es, em, _ = g.batch_info()
+    slab = g.slab_info()
     g.close()
+    assert slab["on"] == (binning == 1)
+    if binning:
+        print(f"  slab binning: {slab['direct']} same-event pairs added directly", flush=True)
     assert es == ref["same_st"][0] and em == ref["mixed_st"][0]
     assert np.array_equal(r.num_count, ref["num_count"].astype(np.uint64))
     assert np.array_equal(r.den, ref["den"].astype(np.uint64))
