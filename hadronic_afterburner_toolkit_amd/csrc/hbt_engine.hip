@@ -227,7 +227,7 @@ struct Slot {
     double *d_soa[8] = {};   // plain kernels (structure of arrays)
     long long *d_off = nullptr;
     // fast kernel: the fast list (particles whose |pT| can reach a K_T bin,
-    // |pT|-sorted per event) and its px-sorted copy per group
+    // |pT|-sorted per event) and its (px cell, py)-sorted copy per group
     long long nf = 0;
     std::vector<long long> foff;  // nev+1 fast-list offsets
     hvec<double> f_pt;            // |pT| of the fast list (mixed windows)
@@ -235,7 +235,8 @@ struct Slot {
     OrigRank *d_orig = nullptr;   // srt position -> fast-list index, gather rank
     size_t cap = 0, cap_f = 0, cap_ev = 0;
     double pmax2 = 0.0;     // max px^2+py^2 of kept particles
-    hvec<double> srt_px;          // px of the px-sorted list (work-item windows)
+    hvec<int> srt_cell;           // px cell and py of the sorted copy (work-item windows)
+    hvec<double> srt_py;
     bool valid = false;
 };
 
@@ -547,8 +548,8 @@ int ensure(hbt_ctx *c, T **p, size_t &cap, size_t n) {
     return HBT_OK;
 }
 
-void set_fp32_margins(hbt_ctx *c) {
-    fp32_margins(c->D, std::max(c->slot[0].pmax2, mixed_slot(c).pmax2), c->P);
+void set_prefilter_margins(hbt_ctx *c) {
+    prefilter_margins(c->D, std::max(c->slot[0].pmax2, mixed_slot(c).pmax2), c->P);
 }
 
 int upload_psi(hbt_ctx *c) {
@@ -722,7 +723,7 @@ int slab_prepare(hbt_ctx *c, int nitems) {
 }
 
 int fast_args(hbt_ctx *c, FastArgs &a) {
-    set_fp32_margins(c);
+    set_prefilter_margins(c);
     const View v = make_view(c);
     HIPCHECK(c, hipMemcpy(c->d_view, &v, sizeof(View), hipMemcpyHostToDevice));
     a.P = c->P;
@@ -852,8 +853,8 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
         pairs.clear();
         enum_same = 0;
         if (same)
-            items_same(sh, s.off, s.foff, s.grp_ev, s.srt_px.data(), D, 0, L.items, &pairs,
-                       enum_same);
+            items_same(sh, s.off, s.foff, s.grp_ev, s.srt_cell.data(), s.srt_py.data(), D, 0,
+                       L.items, &pairs, enum_same);
         if (mixed && c->plan_valid) {
             const Slot &m = mixed_slot(c);
             items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt.data(),
@@ -1390,7 +1391,8 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     s.nf = nf;
     s.foff = std::move(h.foff);
     s.f_pt = std::move(h.f_pt);
-    s.srt_px = std::move(h.srt_px);
+    s.srt_cell = std::move(h.srt_cell);
+    s.srt_py = std::move(h.srt_py);
     s.ngroups = ngroups;
     s.grp_ev.assign(grp_ev, grp_ev + ngroups + 1);
     s.nev = nev;
@@ -1801,7 +1803,12 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
         if (best_out) *best_out = c->tune;
         return HBT_OK;
     }
-    if (c->cfg.binning == 2) c->slab_binning = false;  // the shapes are timed with atomics
+    // histogram privatisation (binning 2): every launch shape is timed with
+    // device-scope atomics and with slab binning, and the fastest pair wins.
+    // Each is launched twice and the second launch counts: slab regions are
+    // sized from the demand of the launch before.
+    const bool both = c->cfg.binning == 2 && slab_applies(c);
+    if (c->cfg.binning == 2) c->slab_binning = false;
     // small j chunks and one window per mixed item give small batches
     // enough items to fill the chip; large ones amortize the per-item setup
     const int Rs[] = {1, 2, 4};
@@ -1810,6 +1817,7 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
     const int BPCs[] = {4, 8, 16};
     hbt_tuning best = c->tune;
     best.ms = 1e30;
+    bool best_slab = c->slab_binning;
     for (int R : Rs)
         for (int jc : JCs)
             for (int kc : KCs)
@@ -1823,38 +1831,21 @@ int hbt_autotune(hbt_ctx *c, hbt_tuning *best_out) {
                     c->launch_enum_mixed = c->plan_valid ? c->plan_enum_mixed : 0;
                     int rc = upload_items(c, items);
                     if (rc) return rc;
-                    float ms = 0.f;
-                    rc = timed_fast(c, &ms);
-                    if (!rc) rc = timed_fast(c, &ms);
-                    if (rc) return rc;
-                    if (ms < best.ms) {
-                        best = t;
-                        best.ms = ms;
+                    for (int m = 0; m < (both ? 2 : 1); m++) {
+                        if (both) c->slab_binning = m == 1;
+                        float ms = 0.f;
+                        rc = timed_fast(c, &ms);
+                        if (!rc) rc = timed_fast(c, &ms);
+                        if (rc) return rc;
+                        if (ms < best.ms) {
+                            best = t;
+                            best.ms = ms;
+                            best_slab = c->slab_binning;
+                        }
                     }
                 }
     c->tune = best;
-    // histogram privatisation (binning 2): the best launch shape again with
-    // slab binning; kept where it beats device-scope atomics.  Its regions
-    // are sized from the first launch's demand, so the later launches count.
-    if (c->cfg.binning == 2 && slab_applies(c)) {
-        c->slab_binning = true;
-        ItemList items;
-        uint64_t es;
-        build_items(c, true, true, items, es);
-        c->launch_enum_same = es;
-        c->launch_enum_mixed = c->plan_valid ? c->plan_enum_mixed : 0;
-        int rc = upload_items(c, items);
-        float slab_ms = 1e30f;
-        for (int k = 0; k < 3 && !rc; k++) {
-            float ms = 0.f;
-            rc = timed_fast(c, &ms);
-            if (k > 0) slab_ms = std::min(slab_ms, ms);
-        }
-        if (rc) return rc;
-        c->slab_binning = slab_ms < best.ms;
-        if (c->slab_binning) best.ms = slab_ms;
-        c->tune = best;
-    }
+    c->slab_binning = best_slab;
     int rc = prepare_and_capture(c);
     if (!rc) rc = zero_hist(c);
     if (rc) return rc;

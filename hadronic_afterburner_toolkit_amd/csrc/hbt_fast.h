@@ -12,13 +12,16 @@
 // Prefilter (all conservative: a pair the exact cascade accepts always
 // passes; the margins exceed the rounding error of both computations):
 //   T = |p1T + p2T|^2 = 4 K_perp^2           in [4 KT_min^2, 4 KT_max^2]
-//   Q = |p1T - p2T|^2 = q_out^2 + q_side^2   <= 2 H^2
+//   q_out^2 = dn^2 / T <= H^2                (q_out = (p1T - p2T).(p1T + p2T)
+//        / |p1T + p2T| = dn / sqrt(T), dn = n1 - n2)
+//   q_side^2 = Q - q_out^2 <= H^2            (Q = |p1T - p2T|^2)
 //   |beta1 - beta2| <= (H/2) (1/E1 + 1/E2)   (from |q_long| <= H, M_t <= K_E,
 //        q_long M_t = E2 pz1 - E1 pz2; beta = pz/E)
-// with H = max(|q_lo_cut|, |q_hi_cut|).  T and Q come from one dot product:
-// T = n1 + n2 + 2 p1.p2, Q = n1 + n2 - 2 p1.p2 (n = |pT|^2).
+// with H = max(|q_lo_cut|, |q_hi_cut|), the q windows tested without a
+// division as dn^2 <= H^2 T and Q T - dn^2 <= H^2 T.  T and Q come from one
+// dot product: T = n1 + n2 + 2 p1.p2, Q = n1 + n2 - 2 p1.p2 (n = |pT|^2).
 // For mixed pairs the i particle is rotated by -theta instead of rotating
-// every j by +theta (T, Q and beta are rotation invariant); the exact
+// every j by +theta (T, Q, dn and beta are rotation invariant); the exact
 // cascade then uses the reference's rotated j (HBT_correlation.cpp:522-523).
 #pragma once
 #include "hbt_device.h"
@@ -358,7 +361,10 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     Stage &stg = stage[threadIdx.x >> 6];
     const Prec two = (Prec)2;
     const Prec half_t = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_half : a.P.f_half);
-    const Prec q_thr = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_qthr : a.P.f_qthr);
+    constexpr bool f64 = std::is_same<Prec, double>::value;
+    const Prec h2 = (Prec)(f64 ? a.P.pf_h2 : a.P.f_h2);
+    const Prec h2c = (Prec)(f64 ? a.P.pf_h2c : a.P.f_h2c);
+    const Prec mid = (Prec)(f64 ? a.P.pf_mid : a.P.f_mid);
     unsigned long long nb_same = 0, nb_mixed = 0;
 
     const int nq = a.nqueues;
@@ -458,8 +464,14 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                             const Prec vv = fma(-two, d, sn);
                             const Prec db = ib[r] - jr.beta;
                             const Prec rw = icw[r] + jcw;
+                            // |q_out|, |q_side| <= H: dn^2 <= H^2 T, Q T - dn^2 <= H^2 T
+                            const Prec dn = in[r] - jr.nrm;
+                            const Prec dd = dn * dn;
+                            const Prec h2t = fma(h2, uu, h2c);
+                            const Prec qt = (vv + mid) * (uu + mid);
                             unsigned long long m = le_mask(absv(uu), half_t) &
-                                                   le_mask(vv, q_thr) & le_mask(absv(db), rw);
+                                                   le_mask(dd, h2t) & le_mask(qt - dd, h2t) &
+                                                   le_mask(absv(db), rw);
                             if (partial) m &= vmask[r];
                             if (diag) m &= low_lanes32(tri + u - r * 64);
                             if (m) {

@@ -32,7 +32,7 @@ constexpr double kHbarC = 0.197327053;  // parameters.h:4
 // Everything the kernels and the host need from an hbt_config.
 struct Derived {
     DevParams P;
-    double mid = 0.0, H = 0.0, c_beta = 0.0;  // prefilter constants
+    double mid = 0.0, H = 0.0, h2 = 0.0, c_beta = 0.0;  // prefilter constants
     uint64_t needed = 0, needed_qinv = 0;     // pair caps
     long nbins = 0, nq = 0;                   // histogram sizes
 };
@@ -83,19 +83,24 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     P.two_pi_hi = 6.28318530717958623199592693709e+00;  // 2*pi rounded to double
     P.two_pi_lo = 2.44929359829470635445e-16;            // 2*pi - two_pi_hi
     // Prefilter (hbt_fast.h).  T = 4 K_perp^2 must lie in
-    // [4 KT_min_sq, 4 KT_max_sq]; Q <= 2 H^2.  Margins of 1e-9 relative
-    // (+1e-12 absolute) dominate the ~1e-15 rounding of either side.
+    // [4 KT_min_sq, 4 KT_max_sq]; q_out^2, q_side^2 <= H^2.  Margins of 1e-9
+    // relative (+1e-12 absolute) dominate the ~1e-15 rounding of the
+    // reference's values; prefilter_margins adds those of the prefilter's
+    // own arithmetic, which scale with the data.
     const double t_lo = 4.0 * P.KT_min_sq, t_hi = 4.0 * P.KT_max_sq;
     d.mid = 0.5 * (t_lo + t_hi);
     P.pf_mid = d.mid;
     P.pf_half = 0.5 * (t_hi - t_lo) * (1.0 + 1e-9) + 1e-9 * t_hi + 1e-12;
     d.H = std::max(std::fabs(P.q_lo_cut), std::fabs(P.q_hi_cut));
     const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
-    P.pf_qthr = P.qinv ? INFINITY : (2.0 * Hm * Hm * (1.0 + 1e-9) + 1e-12 - d.mid);
+    d.h2 = Hm * Hm * (1.0 + 1e-9) + 1e-12;
+    P.pf_h2 = P.qinv ? 0.0 : d.h2;
+    P.pf_h2c = P.qinv ? INFINITY : d.h2 * d.mid;
     d.c_beta = 0.5 * Hm;
     P.f_mid = (float)d.mid;
     P.f_half = (float)P.pf_half;
-    P.f_qthr = P.qinv ? INFINITY : (float)P.pf_qthr;
+    P.f_h2 = (float)P.pf_h2;
+    P.f_h2c = (float)P.pf_h2c;
     d.needed = (uint64_t)(unsigned long long)k.needed_number_of_pairs;
     d.needed_qinv = (uint64_t)50 * d.needed;
     const long q = k.qnpts;
@@ -104,15 +109,23 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     return HBT_OK;
 }
 
-// Prefilter margins for the FP32 records depend on the data's momentum
-// scale (|pT|^2 up to pmax2): FP32 rounding of px, py and of the dot
-// product stays below 8 ulp of that scale; 64 ulp are allowed.
-inline void fp32_margins(const Derived &d, double pmax2, DevParams &P) {
+// Prefilter margins that depend on the data's momentum scale (|pT|^2 up to
+// pmax2), for the records' precision (FP32 in the precision form).  With
+// S = max(1, pmax2) + |mid|: the rounding of the records (px, py, nrm) and
+// of u, v, dn stays below 8 ulp of S, so that of the products dn^2, Q T and
+// H^2 T below 150 ulp of max(1, H^2) S^2 (|dn|, T, Q <= 4 S); the T window
+// allows 64 ulp of 2 S, the q windows 256 ulp of max(1, H^2) S^2.
+inline void prefilter_margins(const Derived &d, double pmax2, DevParams &P) {
     const double pm = std::max(1.0, pmax2);
-    const double eps32 = 5.9604644775390625e-8;
+    const double eps32 = 5.9604644775390625e-8, eps64 = 1.1102230246251565e-16;
     const double m = 64.0 * eps32 * (2.0 * pm + std::fabs(d.mid));
     P.f_half = (float)(d.P.pf_half + m);
-    P.f_qthr = std::isfinite(d.P.pf_qthr) ? (float)(d.P.pf_qthr + m) : INFINITY;
+    if (P.qinv) return;  // pf_h2 = 0, pf_h2c = +inf: the q windows pass every pair
+    const double S = pm + std::fabs(d.mid);
+    const double s2 = std::max(1.0, d.h2) * S * S;
+    P.pf_h2c = d.h2 * d.mid + 256.0 * eps64 * s2;
+    P.f_h2 = (float)d.h2;
+    P.f_h2c = (float)(d.h2 * d.mid + 256.0 * eps32 * s2);
 }
 
 // Allocator whose construct() without arguments default-initializes: a
@@ -179,8 +192,10 @@ struct RecordSet {
 //    (|p1T| <= K_T + |q_T|/2 <= KT_max + D/2, >= KT_min - D/2 with
 //    D = sqrt(2) H, as |q_out|, |q_side| <= H), sorted by |pT|, so a mixed
 //    tile's partners are limited to a |pT| window (|pT1 - pT2| <= D);
-//  - srt: the fast list of each group sorted by px, so a same-event tile's
-//    j range is limited to a px window (|px1 - px2| <= D).
+//  - srt: the fast list of each group sorted by px cell (cells of width
+//    srt_cell_width(D) from the group's smallest px), then by py, so a
+//    same-event tile's j's lie in its own cell and the next one (|px1 -
+//    px2| <= D) and in a py band (|py1 - py2| <= D).
 // With the q_inv branch on (it bins pairs outside the 3D windows) neither
 // cut applies and D is infinite.
 struct HostParticles {
@@ -192,9 +207,10 @@ struct HostParticles {
     std::vector<long long> foff;  // nev+1 fast-list offsets
     RecordSet fast;               // fast list: records
     hvec<double> f_pt;            // |pT| of the fast list (mixed windows)
-    RecordSet srt;                // px-sorted per group (prefilter records only)
+    RecordSet srt;                // (px cell, py)-sorted per group (prefilter records only)
     hvec<OrigRank> orr;           // srt position -> fast-list index, gather rank
-    hvec<double> srt_px;          // px in srt order (same-event windows)
+    hvec<int> srt_cell;           // px cell in srt order (same-event windows)
+    hvec<double> srt_py;          // py in srt order
     double pmax2 = 0.0;  // max px^2+py^2 over the fast list (FP32 prefilter margins)
     // Empties the lists for the next gather and keeps their storage, so a
     // caller that reuses one (the request server's pool) writes into pages
@@ -213,6 +229,14 @@ inline double px_window(const Derived &d) {
     if (d.P.qinv) return INFINITY;
     const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
     return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
+}
+
+// Width of the px cells of the same-event lists: particles two or more
+// cells apart differ in px by more than D (the margin covers the rounding
+// of the cell index), and a group has at most 2^16 cells.
+inline double srt_cell_width(double D, double px_span) {
+    if (!std::isfinite(D)) return INFINITY;
+    return std::max(D * (1.0 + 1e-6), px_span / 65536.0);
 }
 
 // Host-to-device copies through a pinned staging buffer: a pageable copy
@@ -421,24 +445,47 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
             }
         });
     }
-    // px-sorted prefilter records per group (stable)
+    // prefilter records per group sorted by px cell, then py (stable)
     h.srt.resize_prefilter((size_t)nf, f32);
     h.orr.resize((size_t)nf);
-    h.srt_px.resize((size_t)nf);
+    h.srt_cell.resize((size_t)nf);
+    h.srt_py.resize((size_t)nf);
     parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
-        std::vector<std::pair<double, int>> idx;  // (px, fast-list index)
+        std::vector<std::pair<double, int>> idx;  // (py, fast-list index)
         std::vector<std::pair<uint64_t, int>> ra, rb;  // radix scratch
+        std::vector<int> cell, start;
         for (long long g = ga; g < gb; g++) {
             const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
-            idx.resize((size_t)(g1 - g0));
-            for (long long q = g0; q < g1; q++) idx[q - g0] = {F.rec[q].px, (int)q};
-            // stable by px (fast-list order among equal px)
-            radix_sort_keys(idx, ra, rb);
+            if (g1 == g0) continue;
+            double lo = INFINITY, hi = -INFINITY;
             for (long long q = g0; q < g1; q++) {
-                const int src = idx[q - g0].second;
+                lo = std::min(lo, F.rec[q].px);
+                hi = std::max(hi, F.rec[q].px);
+            }
+            const double w = srt_cell_width(D, hi - lo);
+            idx.resize((size_t)(g1 - g0));
+            for (long long q = g0; q < g1; q++) idx[q - g0] = {F.rec[q].py, (int)q};
+            // stable by py (fast-list order among equal py), then a stable
+            // counting sort by cell
+            radix_sort_keys(idx, ra, rb);
+            cell.resize(idx.size());
+            int ncell = 1;
+            for (size_t k = 0; k < idx.size(); k++) {
+                cell[k] = std::isfinite(w)
+                              ? std::min(65536, (int)((F.rec[idx[k].second].px - lo) / w))
+                              : 0;
+                ncell = std::max(ncell, cell[k] + 1);
+            }
+            start.assign((size_t)ncell + 1, 0);
+            for (int c : cell) start[(size_t)c + 1]++;
+            for (int c = 0; c < ncell; c++) start[(size_t)c + 1] += start[(size_t)c];
+            for (size_t k = 0; k < idx.size(); k++) {
+                const long long q = g0 + start[(size_t)cell[k]]++;
+                const int src = idx[k].second;
                 h.srt.copy_from(F, (size_t)q, (size_t)src, f32);
                 h.orr[q] = OrigRank{src, frank[src]};
-                h.srt_px[q] = F.rec[src].px;
+                h.srt_cell[q] = cell[k];
+                h.srt_py[q] = F.rec[src].py;
             }
         }
     });
@@ -496,16 +543,20 @@ inline void items_by_group_chunks(int ngroups, std::vector<WorkItem> &items,
     }
 }
 
-// Same-event items of the groups over the px-sorted fast lists (group g is
-// [foff[grp_ev[g]], foff[grp_ev[g+1]])): i tile of TI consecutive particles x
-// its j window (particles after the tile start whose px lies within dpx of
-// the tile's px range), cut into chunks of JC.  srt_px: px in that order.
-// pairs (optional) receives each item's evaluated pair count.  enum_same
-// receives the reference's enumerated pair count sum n(n-1)/2 over the kept
-// lists (off).
+// Same-event items of the groups over the (px cell, py)-sorted fast lists
+// (group g is [foff[grp_ev[g]], foff[grp_ev[g+1]]); srt_cell / srt_py: the
+// cell and py of each position): i tiles of up to TI consecutive particles
+// inside one cell, each with two j windows: the particles after the tile
+// start in its own cell whose py is within dpx above the tile's, and the
+// particles of the next cell whose py is within dpx of the tile's py range.
+// Particles two or more cells apart differ in px by more than dpx, so no
+// binnable pair is left out.  Windows are cut into chunks of JC.  pairs
+// (optional) receives each item's evaluated pair count.  enum_same receives
+// the reference's enumerated pair count sum n(n-1)/2 over the kept lists
+// (off).
 inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                        const std::vector<long long> &foff, const std::vector<int> &grp_ev,
-                       const double *srt_px, double dpx, int view,
+                       const int *srt_cell, const double *srt_py, double dpx, int view,
                        std::vector<WorkItem> &items, std::vector<uint64_t> *pairs,
                        uint64_t &enum_same) {
     const int ngroups = (int)grp_ev.size() - 1;
@@ -516,27 +567,36 @@ inline void items_same(const ItemShape &s, const std::vector<long long> &off,
             if (nk >= 2) out.enum_same += (uint64_t)(nk * (nk - 1) / 2);
             const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
             if (n < 2) continue;
-            const double *px = srt_px + g0;
-            for (long long a = 0; a + 1 < n; a += s.TI) {
-                const int ni = (int)std::min<long long>(s.TI, n - a);
-                long long jhi = n;
-                if (std::isfinite(dpx)) {
-                    // px ascending inside the group: the tile spans [px[a], px[a+ni-1]]
-                    const double top = px[a + ni - 1] + dpx;
-                    jhi = std::upper_bound(px + a, px + n, top) - px;
-                }
-                for (long long jb = a + 1; jb < jhi; jb += s.JC) {
+            const int *cell = srt_cell + g0;
+            const double *py = srt_py + g0;
+            auto emit = [&](long long a, int ni, long long jb, long long je) {
+                for (; jb < je; jb += s.JC) {
                     WorkItem w{};
                     w.i0 = g0 + a;
                     w.j0 = g0 + jb;
                     w.ni = ni;
-                    w.nj = (int)std::min<long long>(s.JC, jhi - jb);
+                    w.nj = (int)std::min<long long>(s.JC, je - jb);
                     w.kind = 0;
                     w.grp = g;
                     w.view = view;
                     out.items.push_back(w);
                     if (pairs) out.pairs.push_back(tile_pairs(a, ni, jb, jb + w.nj));
                 }
+            };
+            for (long long cb = 0; cb < n;) {
+                const long long ce = std::upper_bound(cell + cb, cell + n, cell[cb]) - cell;
+                const bool next = ce < n && cell[ce] == cell[cb] + 1;
+                const long long ne = next ? std::upper_bound(cell + ce, cell + n, cell[ce]) - cell : ce;
+                for (long long a = cb; a < ce; a += s.TI) {
+                    const int ni = (int)std::min<long long>(s.TI, ce - a);
+                    const double ylo = py[a] - dpx, yhi = py[a + ni - 1] + dpx;
+                    emit(a, ni, a + 1, std::upper_bound(py + a, py + ce, yhi) - py);
+                    if (next) {
+                        const long long lb = std::lower_bound(py + ce, py + ne, ylo) - py;
+                        emit(a, ni, lb, std::upper_bound(py + lb, py + ne, yhi) - py);
+                    }
+                }
+                cb = ce;
             }
         }
     });

@@ -506,7 +506,7 @@ void launcher_loop(hbt_server *s) {
             FastArgs a;
             a.slab = SlabArgs{};  // the server adds binned pairs with atomics
             a.P = s->D.P;
-            if (s->f32) fp32_margins(s->D, pmax2, a.P);
+            prefilter_margins(s->D, pmax2, a.P);
             a.views = s->d_views;
             a.items = s->d_items[buf];
             a.nitems = (int)items.size();
@@ -965,8 +965,8 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
             p.plan_cs.assign(plan_cs.begin() + 2 * r0, plan_cs.begin() + 2 * (r0 + rows));
             uint64_t es = 0;
             const HostParticles &hp = p.hp;
-            items_same(s->shape, hp.off, hp.foff, p.grp_ev, hp.srt_px.data(), D, 0, p.items,
-                       &p.item_pairs, es);
+            items_same(s->shape, hp.off, hp.foff, p.grp_ev, hp.srt_cell.data(), hp.srt_py.data(),
+                       D, 0, p.items, &p.item_pairs, es);
             items_mixed(s->shape, hp.foff, p.grp_ev, p.rowoff, p.plan_ids, hp.foff,
                         hp.f_pt.data(), hp.f_pt.data(), D, 0, p.items, p.wins, &p.item_pairs);
             for (uint64_t q : p.item_pairs) p.pairs_left += q;

@@ -295,8 +295,9 @@ class HBTCorrelation:
         return [q[k] for k in range(n.value + 1)]
 
     def slab_info(self) -> dict:
-        """Slab binning of the last fast launch: on, pairs added directly
-        because their region was full, region capacity in records."""
+        """Slab binning of the last fast launch: on, same-event pairs added
+        directly (region full, or stored values rounding across a bin edge),
+        region capacity in records."""
         on = C.c_int32()
         d, cap = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
         self._check(self._lib.hbt_slab_info(self.h, C.byref(on), _u64p(d), _u64p(cap)),

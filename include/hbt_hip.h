@@ -73,14 +73,14 @@ typedef struct {
      * 0: 1 (1e-12 rad).  Larger values only move more pairs to the host. */
     int32_t kphi_guard_prad;
     /* histogram accumulation of the fast kernel: 0 = every binned pair added
-     * to the histogram with device-scope atomics (the faster on MI355X);
-     * 1 = slab binning where it applies (no K_phi, no q_inv,
-     * qnpts^2 <= 2048, n_KT * qnpts <= 512): binned pairs written as records
-     * per (block, q_out plane) and summed in LDS, no device-scope atomics;
-     * 2 = the one hbt_autotune measures faster on the prepared batch
-     * (atomics until then).  Counts are the same; sums agree within each
-     * form's tolerance.  hbt_slab_info reports which one the last launch
-     * used. */
+     * to the histogram with device-scope atomics; 1 = slab binning where it
+     * applies (no K_phi, no q_inv, qnpts^2 <= 2048, n_KT * qnpts <= 512):
+     * binned same-event pairs written as records per (block, q_out plane)
+     * and summed in LDS, without device-scope atomics (mixed pairs still add
+     * their u32 directly); 2 = the one hbt_autotune measures faster on the
+     * prepared batch, timing every launch shape both ways (atomics until
+     * then).  Counts are the same; sums agree within each form's tolerance.
+     * hbt_slab_info reports which one the last launch used. */
     int32_t binning;
     int32_t reserved[3];
 } hbt_config;
@@ -246,10 +246,10 @@ int hbt_batch_graph(hbt_ctx *ctx, int32_t *graph);
 int hbt_batch_queues(hbt_ctx *ctx, int32_t *nqueues, int32_t *qoff);
 
 /* Slab binning of the last fast launch (hbt_config.binning): *on = 1 when
- * its binned pairs went through slab records, *direct = how many of them
- * took the direct atomic path because their (block, slab) region was full,
- * *capacity = records the regions held (same-event + mixed).  No
- * counterpart in the reference. */
+ * its binned same-event pairs went through slab records, *direct = how many
+ * of them took the direct atomic path instead (their (block, slab) region
+ * was full, or their stored values round across a bin edge), *capacity =
+ * records the regions held.  No counterpart in the reference. */
 int hbt_slab_info(hbt_ctx *ctx, int32_t *on, uint64_t *direct, uint64_t *capacity);
 
 /* ---- allocation-free entry point ------------------------------------------
