@@ -27,6 +27,7 @@
 #include "hbt_device.h"
 #include "hbt_f32.h"
 #include "hbt_slab.h"
+#include "host_items.h"
 
 namespace hbt {
 
@@ -39,32 +40,41 @@ struct PfRec32 {
     float px, py, nrm, beta;
 };
 
-// Partner window of a mixed work item: the j range [b, e) of the mixed
-// list (the particles of the partner event whose |pT| can pair with the
-// item's i tile) and the plan row that gives the partner's rotation.
-struct JWin {
-    int b, e;
-    int row;
-    int pad;
+// The prefilter's terms of one pair (for a mixed pair, i rotated by -theta):
+// u = T - mid, dd = dn^2, h2t = H^2 T (with margin), qtd = Q T - dn^2, and
+// the beta difference db against its bound rw.  The pair passes when
+// |u| <= half, dd <= h2t, qtd <= h2t and |db| <= rw.  The kernel and the
+// host's check of the margins evaluate this same function.
+template <typename Prec>
+struct PfTerms {
+    Prec u, dd, h2t, qtd, db, rw;
 };
 
-// Same-event pair orientation: for a position in the px-sorted list, the
+template <typename Prec>
+__host__ __device__ __forceinline__ PfTerms<Prec> pf_terms(Prec ix, Prec iy, Prec in, Prec ib,
+                                                           Prec icw, Prec jx, Prec jy, Prec jn,
+                                                           Prec jb, Prec jcw, Prec h2, Prec h2c,
+                                                           Prec mid) {
+    PfTerms<Prec> t;
+    const Prec d = fma(ix, jx, iy * jy);
+    const Prec sn = in + jn;
+    t.u = fma((Prec)2, d, sn);
+    const Prec v = fma((Prec)-2, d, sn);
+    const Prec dn = in - jn;
+    t.dd = dn * dn;
+    t.h2t = fma(h2, t.u, h2c);
+    t.qtd = (v + mid) * (t.u + mid) - t.dd;
+    t.db = ib - jb;
+    t.rw = icw + jcw;
+    return t;
+}
+
+// Same-event pair orientation: for a position in the cell-sorted list, the
 // particle's index in the fast list and its rank in gather order (the
 // reference's particle 1 is the earlier of the two in gather order).
 struct OrigRank {
     int idx;
     int rank;
-};
-
-// One unit of work for one wave.
-struct WorkItem {
-    long long i0;   // first i particle (same-event: px-sorted list, mixed: fast list)
-    long long j0;   // same-event: first j particle; mixed: first window (View::mwin)
-    int ni;         // number of i particles (<= 64*R)
-    int nj;         // same-event: number of j particles; mixed: number of windows
-    int kind;       // 0 same-event, 1 mixed
-    int grp;        // group (buffer) of the pair, selects psi_ref
-    int view;       // analysis the item belongs to (index into FastArgs::views)
 };
 
 // One analysis resident in HBM: its particles, mixing plan and histograms.
@@ -78,11 +88,11 @@ struct View {
     const PartRec *B;    // mixed fast list (separate, or an alias of A)
     const void *pfA;     // PfRec64 (FP64 kernel) or PfRec32 (precision form) of A
     const void *pfB;     // of B
-    const void *pfS;     // of the same-event fast list sorted by px inside each group
+    const void *pfS;     // of the same-event fast list sorted by (px cell, py) per group
     const void *cwA;     // per-particle c/E bound of A, double or float like pfA
     const void *cwB;
     const void *cwS;
-    const OrigRank *origS;     // px-sorted position -> fast-list index and gather rank
+    const OrigRank *origS;     // cell-sorted position -> fast-list index and gather rank
     const JWin *mwin;          // partner windows of the mixed items
     const double *plan_cs;     // cos, sin of each plan row's rotation
     const double *psi;         // psi_ref per group
@@ -155,7 +165,7 @@ struct Exact;
 
 template <bool EXT>
 struct Exact<double, EXT> {
-    // i, j are positions in the px-sorted list; origS maps them to the
+    // i, j are positions in the cell-sorted list; origS maps them to the
     // fast list, and the pair is evaluated in the reference's orientation
     // (particle 1 = the earlier in gather order).
     static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
@@ -318,7 +328,7 @@ struct Ring {
 // Persistent: each wave pulls work items from the queue of its XCD (the host
 // puts all items of a group in one queue, so a group's particles stay in one
 // XCD's L2), then from the next queues in turn, until all are empty.  Per item the wave holds 64R i particles (R per lane) and walks
-// its j segments (same-event: one range with j > i in the px-sorted list;
+// its j segments (same-event: one range with j > i in the cell-sorted list;
 // mixed: one partner window per segment with the i particles rotated by
 // -theta),
 // JU j's per step.  Survivors of the prefilter go to the wave's LDS ring;
@@ -359,7 +369,6 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     const int lane = __lane_id();
     unsigned *q = ring[threadIdx.x >> 6];
     Stage &stg = stage[threadIdx.x >> 6];
-    const Prec two = (Prec)2;
     const Prec half_t = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_half : a.P.f_half);
     constexpr bool f64 = std::is_same<Prec, double>::value;
     const Prec h2 = (Prec)(f64 ? a.P.pf_h2 : a.P.f_h2);
@@ -383,7 +392,7 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
         const View v = HBT_LOAD_VIEW(a.views, w.view);
         const double psi = v.psi[w.grp];
         const bool mixed = w.kind != 0;
-        // same-event tiles run over the px-sorted list, mixed ones over the
+        // same-event tiles run over the cell-sorted list, mixed ones over the
         // |pT|-sorted fast lists
         const Rec *__restrict__ pfi = (const Rec *)(mixed ? v.pfA : v.pfS);
         const Prec *__restrict__ cwi = (const Prec *)(mixed ? v.cwA : v.cwS);
@@ -458,20 +467,13 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                     auto pair_step = [&](int u, const Rec &jr, Prec jcw) {
 #pragma unroll
                         for (int r = 0; r < R; r++) {
-                            const Prec d = fma(rx[r], jr.px, ry[r] * jr.py);
-                            const Prec sn = in[r] + jr.nrm;
-                            const Prec uu = fma(two, d, sn);
-                            const Prec vv = fma(-two, d, sn);
-                            const Prec db = ib[r] - jr.beta;
-                            const Prec rw = icw[r] + jcw;
-                            // |q_out|, |q_side| <= H: dn^2 <= H^2 T, Q T - dn^2 <= H^2 T
-                            const Prec dn = in[r] - jr.nrm;
-                            const Prec dd = dn * dn;
-                            const Prec h2t = fma(h2, uu, h2c);
-                            const Prec qt = (vv + mid) * (uu + mid);
-                            unsigned long long m = le_mask(absv(uu), half_t) &
-                                                   le_mask(dd, h2t) & le_mask(qt - dd, h2t) &
-                                                   le_mask(absv(db), rw);
+                            const PfTerms<Prec> pt =
+                                pf_terms(rx[r], ry[r], in[r], ib[r], icw[r], jr.px, jr.py,
+                                         jr.nrm, jr.beta, jcw, h2, h2c, mid);
+                            unsigned long long m = le_mask(absv(pt.u), half_t) &
+                                                   le_mask(pt.dd, pt.h2t) &
+                                                   le_mask(pt.qtd, pt.h2t) &
+                                                   le_mask(absv(pt.db), pt.rw);
                             if (partial) m &= vmask[r];
                             if (diag) m &= low_lanes32(tri + u - r * 64);
                             if (m) {

@@ -23,6 +23,7 @@
 #include "hbt_fast.h"
 #include "host_pool.h"
 #include "host_sort.h"
+#include "host_items.h"
 #include "host_tiles.h"
 
 namespace hbt {
@@ -107,6 +108,28 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     d.nbins = (long)k.n_KT * P.nphi * q * q * q;
     d.nq = (long)k.n_KT * q;
     return HBT_OK;
+}
+
+// Prefilter record of one particle (px, py, nrm = |pT|^2 - mid/2, beta)
+// and its beta bound cw = (H/2) / E (or H/2 without the LCMS boost; +inf
+// with the q_inv branch on).  FP32 records: cw covers the rounding of beta
+// (|beta| <= bm over the list) and of cw itself.
+inline void pf_record(const Derived &d, double px, double py, double pz, double E, double bm,
+                      PfRec64 &r, double &cw) {
+    const DevParams &P = d.P;
+    r = PfRec64{px, py, px * px + py * py - 0.5 * d.mid, P.boost ? pz / E : pz};
+    cw = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
+}
+
+inline void pf_record(const Derived &d, double px, double py, double pz, double E, double bm,
+                      PfRec32 &r, float &cw) {
+    const double eps32 = 5.9604644775390625e-8;
+    PfRec64 r64;
+    double cw64;
+    pf_record(d, px, py, pz, E, bm, r64, cw64);
+    r = PfRec32{(float)r64.px, (float)r64.py, (float)r64.nrm, (float)r64.beta};
+    cw = d.P.qinv ? INFINITY
+                  : (float)(cw64 * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bm));
 }
 
 // Prefilter margins that depend on the data's momentum scale (|pT|^2 up to
@@ -229,14 +252,6 @@ inline double px_window(const Derived &d) {
     if (d.P.qinv) return INFINITY;
     const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
     return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
-}
-
-// Width of the px cells of the same-event lists: particles two or more
-// cells apart differ in px by more than D (the margin covers the rounding
-// of the cell index), and a group has at most 2^16 cells.
-inline double srt_cell_width(double D, double px_span) {
-    if (!std::isfinite(D)) return INFINITY;
-    return std::max(D * (1.0 + 1e-6), px_span / 65536.0);
 }
 
 // Host-to-device copies through a pinned staging buffer: a pageable copy
@@ -383,7 +398,6 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     F.resize((size_t)nf, f32);
     h.f_pt.resize((size_t)nf);
     hvec<int> frank((size_t)nf);
-    const double eps32 = 5.9604644775390625e-8;
     // pass 2, per event: gather order, then the fast list sorted by |pT|
     // (stable, so equal |pT| keep gather order)
     parallel_ranges(nev, ev_grain, [&](long long e0, long long e1) {
@@ -406,18 +420,10 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
                 const int64_t i = h.keep[x.second];
                 const double px = soa[0][i], py = soa[1][i], pz = soa[2][i], E = soa[3][i];
                 F.rec[f] = PartRec{px, py, pz, E, soa[4][i], soa[5][i], soa[6][i], soa[7][i]};
-                const double nrm = px * px + py * py - 0.5 * d.mid;
-                const double beta = P.boost ? pz / E : pz;
-                const double cw = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
-                if (f32) {
-                    F.pf32[f] = PfRec32{(float)px, (float)py, (float)nrm, (float)beta};
-                    // covers the FP32 rounding of beta (|beta| <= bm) and of cw itself
-                    F.cw32[f] = P.qinv ? INFINITY
-                                       : (float)(cw * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bm));
-                } else {
-                    F.pf64[f] = PfRec64{px, py, nrm, beta};
-                    F.cw64[f] = cw;
-                }
+                if (f32)
+                    pf_record(d, px, py, pz, E, bm, F.pf32[f], F.cw32[f]);
+                else
+                    pf_record(d, px, py, pz, E, bm, F.pf64[f], F.cw64[f]);
                 h.f_pt[f] = x.first;
                 frank[f] = (int)x.second;
                 f++;
@@ -451,53 +457,24 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     h.srt_cell.resize((size_t)nf);
     h.srt_py.resize((size_t)nf);
     parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
-        std::vector<std::pair<double, int>> idx;  // (py, fast-list index)
-        std::vector<std::pair<uint64_t, int>> ra, rb;  // radix scratch
-        std::vector<int> cell, start;
+        CellOrderScratch sc;
+        std::vector<int> order;
         for (long long g = ga; g < gb; g++) {
             const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
-            if (g1 == g0) continue;
-            double lo = INFINITY, hi = -INFINITY;
+            order.resize((size_t)(g1 - g0));
+            cell_order((int)(g1 - g0), [&](int k) { return F.rec[g0 + k].px; },
+                       [&](int k) { return F.rec[g0 + k].py; }, D, sc, order.data(),
+                       h.srt_cell.data() + g0);
             for (long long q = g0; q < g1; q++) {
-                lo = std::min(lo, F.rec[q].px);
-                hi = std::max(hi, F.rec[q].px);
-            }
-            const double w = srt_cell_width(D, hi - lo);
-            idx.resize((size_t)(g1 - g0));
-            for (long long q = g0; q < g1; q++) idx[q - g0] = {F.rec[q].py, (int)q};
-            // stable by py (fast-list order among equal py), then a stable
-            // counting sort by cell
-            radix_sort_keys(idx, ra, rb);
-            cell.resize(idx.size());
-            int ncell = 1;
-            for (size_t k = 0; k < idx.size(); k++) {
-                cell[k] = std::isfinite(w)
-                              ? std::min(65536, (int)((F.rec[idx[k].second].px - lo) / w))
-                              : 0;
-                ncell = std::max(ncell, cell[k] + 1);
-            }
-            start.assign((size_t)ncell + 1, 0);
-            for (int c : cell) start[(size_t)c + 1]++;
-            for (int c = 0; c < ncell; c++) start[(size_t)c + 1] += start[(size_t)c];
-            for (size_t k = 0; k < idx.size(); k++) {
-                const long long q = g0 + start[(size_t)cell[k]]++;
-                const int src = idx[k].second;
+                const int src = (int)g0 + order[(size_t)(q - g0)];
                 h.srt.copy_from(F, (size_t)q, (size_t)src, f32);
                 h.orr[q] = OrigRank{src, frank[src]};
-                h.srt_cell[q] = cell[k];
                 h.srt_py[q] = F.rec[src].py;
             }
         }
     });
     return HBT_OK;
 }
-
-// Launch shape of the fast kernel's work items.
-struct ItemShape {
-    int TI;  // i particles per item (64 R)
-    int JC;  // same-event j chunk
-    int KC;  // mixed partner windows per item
-};
 
 // Work items per resident wave below which a launch's items are made smaller.
 constexpr size_t kItemsPerWave = 4;
@@ -506,162 +483,6 @@ constexpr size_t kL2BytesPerXcd = (size_t)4 << 20;
 
 inline ItemShape item_shape(const hbt_tuning &t) {
     return ItemShape{64 * t.R, std::max(64, t.jchunk), std::max(1, std::min(t.kchunk, 64))};
-}
-
-// Work items of consecutive chunks of groups, built on several host threads
-// and concatenated in group order (the result is the serial one).
-struct ItemChunk {
-    std::vector<WorkItem> items;
-    std::vector<JWin> wins;
-    std::vector<uint64_t> pairs;
-    uint64_t enum_same = 0;
-};
-
-template <typename F>
-inline void items_by_group_chunks(int ngroups, std::vector<WorkItem> &items,
-                                  std::vector<JWin> *wins, std::vector<uint64_t> *pairs,
-                                  uint64_t *enum_same, F build) {
-    const int nchunks = std::max(1, std::min(ngroups, 64));
-    std::vector<ItemChunk> ch((size_t)nchunks);
-    parallel_ranges(nchunks, 1, [&](long long c0, long long c1) {
-        for (long long c = c0; c < c1; c++)
-            build((int)((long long)ngroups * c / nchunks),
-                  (int)((long long)ngroups * (c + 1) / nchunks), ch[(size_t)c]);
-    });
-    size_t nw = wins ? wins->size() : 0;
-    for (const ItemChunk &c : ch) {
-        for (WorkItem w : c.items) {
-            if (w.kind != 0) w.j0 += (long long)nw;  // window index in the joint table
-            items.push_back(w);
-        }
-        if (wins) {
-            wins->insert(wins->end(), c.wins.begin(), c.wins.end());
-            nw = wins->size();
-        }
-        if (pairs) pairs->insert(pairs->end(), c.pairs.begin(), c.pairs.end());
-        if (enum_same) *enum_same += c.enum_same;
-    }
-}
-
-// Same-event items of the groups over the (px cell, py)-sorted fast lists
-// (group g is [foff[grp_ev[g]], foff[grp_ev[g+1]]); srt_cell / srt_py: the
-// cell and py of each position): i tiles of up to TI consecutive particles
-// inside one cell, each with two j windows: the particles after the tile
-// start in its own cell whose py is within dpx above the tile's, and the
-// particles of the next cell whose py is within dpx of the tile's py range.
-// Particles two or more cells apart differ in px by more than dpx, so no
-// binnable pair is left out.  Windows are cut into chunks of JC.  pairs
-// (optional) receives each item's evaluated pair count.  enum_same receives
-// the reference's enumerated pair count sum n(n-1)/2 over the kept lists
-// (off).
-inline void items_same(const ItemShape &s, const std::vector<long long> &off,
-                       const std::vector<long long> &foff, const std::vector<int> &grp_ev,
-                       const int *srt_cell, const double *srt_py, double dpx, int view,
-                       std::vector<WorkItem> &items, std::vector<uint64_t> *pairs,
-                       uint64_t &enum_same) {
-    const int ngroups = (int)grp_ev.size() - 1;
-    items_by_group_chunks(ngroups, items, nullptr, pairs, &enum_same,
-                          [&](int ga, int gb, ItemChunk &out) {
-        for (int g = ga; g < gb; g++) {
-            const long long nk = off[grp_ev[g + 1]] - off[grp_ev[g]];
-            if (nk >= 2) out.enum_same += (uint64_t)(nk * (nk - 1) / 2);
-            const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
-            if (n < 2) continue;
-            const int *cell = srt_cell + g0;
-            const double *py = srt_py + g0;
-            auto emit = [&](long long a, int ni, long long jb, long long je) {
-                for (; jb < je; jb += s.JC) {
-                    WorkItem w{};
-                    w.i0 = g0 + a;
-                    w.j0 = g0 + jb;
-                    w.ni = ni;
-                    w.nj = (int)std::min<long long>(s.JC, je - jb);
-                    w.kind = 0;
-                    w.grp = g;
-                    w.view = view;
-                    out.items.push_back(w);
-                    if (pairs) out.pairs.push_back(tile_pairs(a, ni, jb, jb + w.nj));
-                }
-            };
-            for (long long cb = 0; cb < n;) {
-                const long long ce = std::upper_bound(cell + cb, cell + n, cell[cb]) - cell;
-                const bool next = ce < n && cell[ce] == cell[cb] + 1;
-                const long long ne = next ? std::upper_bound(cell + ce, cell + n, cell[ce]) - cell : ce;
-                for (long long a = cb; a < ce; a += s.TI) {
-                    const int ni = (int)std::min<long long>(s.TI, ce - a);
-                    const double ylo = py[a] - dpx, yhi = py[a + ni - 1] + dpx;
-                    emit(a, ni, a + 1, std::upper_bound(py + a, py + ce, yhi) - py);
-                    if (next) {
-                        const long long lb = std::lower_bound(py + ce, py + ne, ylo) - py;
-                        emit(a, ni, lb, std::upper_bound(py + lb, py + ne, yhi) - py);
-                    }
-                }
-                cb = ce;
-            }
-        }
-    });
-}
-
-// Mixed items over the fast lists (|pT|-sorted per event): i tile of TI
-// particles of event e x up to KC partner windows, one per plan row whose
-// partner has particles within dpt of the tile's |pT| range (rows with an
-// empty window are skipped).  Windows are appended to wins (absolute
-// indices into the mixed fast list, whose event offsets are mfoff); the
-// item's j0 / nj give its first window and their number.  plan_ids: the
-// partner of each plan row; f_pt / mf_pt: |pT| of the two fast lists.
-// pairs (optional): evaluated pairs per item.
-inline void items_mixed(const ItemShape &s, const std::vector<long long> &foff,
-                        const std::vector<int> &grp_ev, const std::vector<long long> &rowoff,
-                        const std::vector<int> &plan_ids, const std::vector<long long> &mfoff,
-                        const double *f_pt, const double *mf_pt, double dpt, int view,
-                        std::vector<WorkItem> &items, std::vector<JWin> &wins,
-                        std::vector<uint64_t> *pairs) {
-    const int ngroups = (int)grp_ev.size() - 1;
-    const double *pt = mf_pt;
-    items_by_group_chunks(ngroups, items, &wins, pairs, nullptr,
-                          [&](int ga, int gb, ItemChunk &out) {
-        for (int g = ga; g < gb; g++) {
-            for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
-                const long long n1 = foff[e + 1] - foff[e];
-                for (long long a = 0; a < n1; a += s.TI) {
-                    const int ni = (int)std::min<long long>(s.TI, n1 - a);
-                    const double lo = f_pt[foff[e] + a] - dpt;
-                    const double hi = f_pt[foff[e] + a + ni - 1] + dpt;
-                    WorkItem w{};
-                    uint64_t cnt = 0;
-                    auto flush = [&]() {
-                        if (w.nj == 0) return;
-                        out.items.push_back(w);
-                        if (pairs) out.pairs.push_back(cnt);
-                        w.nj = 0;
-                        cnt = 0;
-                    };
-                    for (long long row = rowoff[e]; row < rowoff[e + 1]; row++) {
-                        const int p = plan_ids[row];
-                        long long jb = mfoff[p], je = mfoff[p + 1];
-                        if (std::isfinite(dpt)) {
-                            jb = std::lower_bound(pt + jb, pt + je, lo) - pt;
-                            je = std::upper_bound(pt + jb, pt + je, hi) - pt;
-                        }
-                        if (je <= jb) continue;
-                        if (w.nj == 0) {
-                            w.i0 = foff[e] + a;
-                            w.ni = ni;
-                            w.j0 = (long long)out.wins.size();  // chunk-local until joined
-                            w.kind = 1;
-                            w.grp = g;
-                            w.view = view;
-                        }
-                        out.wins.push_back(JWin{(int)jb, (int)je, (int)row, 0});
-                        w.nj++;
-                        cnt += (uint64_t)ni * (uint64_t)(je - jb);
-                        if (w.nj == s.KC) flush();
-                    }
-                    flush();
-                }
-            }
-        }
-    });
 }
 
 // Pairs left to the host by the device (DeferRec): room per context or
