@@ -844,7 +844,7 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
     // a launch with fewer items than kItemsPerWave per resident wave (one
     // buffer of the per-buffer path, a small request) leaves most waves idle
     // and ends on its largest item: its j chunks and partner windows per item
-    // shrink (4x per round, to 256 j's and one window) until it has enough
+    // shrink (to 64 j's and one window at most) until it has enough
     const size_t waves = (size_t)std::max(1, c->cu_count) * (size_t)c->tune.blocks_per_cu * 4;
     std::vector<uint64_t> pairs;
     for (;;) {
@@ -861,12 +861,12 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
                         m.f_pt.data(), D, 0, L.items, L.wins, &pairs);
         }
         const size_t want = kItemsPerWave * waves;
-        if (L.items.size() >= want || (sh.JC <= 256 && sh.KC <= 1)) break;
+        if (L.items.size() >= want || (sh.JC <= 64 && sh.KC <= 1)) break;
         // items grow about inversely with the chunk sizes: one rebuild with
         // both cut by the missing factor (rounded up to a power of two)
         int f = 2;
         while ((size_t)f * L.items.size() < want && f < 64) f *= 2;
-        sh.JC = std::max(256, sh.JC / f);
+        sh.JC = std::max(64, sh.JC / f);
         sh.KC = std::max(1, sh.KC / f);
     }
     // larger items first, by factor-4 size classes (stable inside a class,
