@@ -249,6 +249,7 @@ __device__ __forceinline__ void push_defer(const DevHist &H, const double a[8], 
 struct SameRec {
     long k;
     double qo, qs, ql, c;
+    int s, is, il;  // slab (K_T, K_phi, q_out plane) and q_side, q_long bins of k
 };
 
 // Same-event pair: cascade and cos (HBT_correlation.cpp:339-356, :431-446).
@@ -285,6 +286,9 @@ __device__ __forceinline__ int eval_same_f64(const DevParams &P, const DevHist &
     }
     if (st & ST_BIN) {
         rec.k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
+        rec.s = (b.kt * P.nphi + b.kphi) * P.qnpts + b.io;
+        rec.is = b.is;
+        rec.il = b.il;
         rec.qo = b.q_out;
         rec.qs = b.q_side;
         rec.ql = b.q_long;

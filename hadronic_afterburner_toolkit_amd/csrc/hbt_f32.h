@@ -176,6 +176,9 @@ __device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist 
     const double phase =
         P.hbarC_inv * (b.q_E * t_diff - b.q_x * x_diff - b.q_y * y_diff - b.q_z * z_diff);
     rec.k = bin_index(P, b.kt, b.kphi, b.io, b.is, b.il);
+    rec.s = (b.kt * P.nphi + b.kphi) * P.qnpts + b.io;
+    rec.is = b.is;
+    rec.il = b.il;
     rec.qo = b.q_out;
     rec.qs = b.q_side;
     rec.ql = b.q_long;

@@ -237,7 +237,8 @@ template <typename Prec, bool EXT, int QCAP>
 __device__ __forceinline__ int drain(const DevParams &P, const View &v, const unsigned *q,
                                      Stage &stg,
                                      unsigned head, int cnt, const WorkItem &w, double psi,
-                                     const SlabArgs &sl, unsigned *fill, unsigned &direct_n) {
+                                     const SlabArgs &sl, unsigned *fill, const int *scap,
+                                     const long long *sbase, unsigned &direct_n) {
     const int lane = __lane_id();
     int st = 0;
     SameRec rec;
@@ -268,7 +269,7 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
     }
     bool direct = binned;
     if (slab && binned) {
-        direct = !slab_put_same<Prec>(sl, fill, rec.k, rec.qo, rec.qs, rec.ql, rec.c);
+        direct = !slab_put_same<Prec>(sl, fill, scap, sbase, rec);
         direct_n += direct ? 1u : 0u;
     }
     const unsigned long long m = __ballot(direct);
@@ -358,11 +359,18 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     constexpr int QCAP = Ring<R>::CAP;
     __shared__ unsigned ring[4][QCAP];
     __shared__ Stage stage[4];
-    // slab binning: records per slab of this block
+    // slab binning: records per slab of this block, and the regions'
+    // capacity and first record per slab
     __shared__ unsigned slab_fill[EXT ? 1 : kMaxSlabs];
+    __shared__ int slab_cap[EXT ? 1 : kMaxSlabs];
+    __shared__ long long slab_base[EXT ? 1 : kMaxSlabs];
     const bool slab = !EXT && a.slab.vals != nullptr;
     if (slab) {
-        for (int k = threadIdx.x; k < kMaxSlabs; k += blockDim.x) slab_fill[k] = 0u;
+        for (int k = threadIdx.x; k < a.slab.S; k += blockDim.x) {
+            slab_fill[k] = 0u;
+            slab_cap[k] = a.slab.scap[k];
+            slab_base[k] = a.slab.sbase[k];
+        }
         __syncthreads();
     }
     unsigned direct_n = 0;
@@ -454,7 +462,8 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                         const int n = (int)min(64u, tail - head);
                         __builtin_amdgcn_wave_barrier();
                         const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, psi,
-                                                              a.slab, slab_fill, direct_n);
+                                                              a.slab, slab_fill, slab_cap,
+                                                              slab_base, direct_n);
                         if (mixed)
                             nb_mixed += nb;
                         else

@@ -72,26 +72,27 @@ __device__ __forceinline__ int slab_coord(const SlabArgs &a, double v) {
     return (int)((v - a.q_base) * a.inv_dq);
 }
 
+// A binned same-event pair r as a record of this block's region of slab
+// r.s; false: the pair takes the direct path.  fill, cap and base are the
+// block's LDS copies of its fill counters and of the regions' capacity and
+// first record (cap and base as in SlabArgs).
 template <typename Prec>
-__device__ __forceinline__ bool slab_put_same(const SlabArgs &a, unsigned *fill, long k,
-                                              double qo, double qs, double ql, double c) {
+__device__ __forceinline__ bool slab_put_same(const SlabArgs &a, unsigned *fill, const int *cap,
+                                              const long long *base, const SameRec &r) {
     constexpr bool f32 = std::is_same<Prec, float>::value;
-    const int s = (int)k / a.SB;
-    const int bi = (int)k - s * a.SB;
-    const int is = bi / a.q, il = bi - is * a.q;
-    if (slab_coord(a, f32 ? (double)(float)qs : qs) != is ||
-        slab_coord(a, f32 ? (double)(float)ql : ql) != il)
+    if (slab_coord(a, f32 ? (double)(float)r.qs : r.qs) != r.is ||
+        slab_coord(a, f32 ? (double)(float)r.ql : r.ql) != r.il)
         return false;
-    const unsigned pos = atomicAdd(&fill[s], 1u);
-    const int cap = a.scap[s];
-    if (pos >= (unsigned)cap) return false;
-    const long long e = a.sbase[s] + (long long)blockIdx.x * cap + pos;
+    const unsigned pos = atomicAdd(&fill[r.s], 1u);
+    const int c = cap[r.s];
+    if (pos >= (unsigned)c) return false;
+    const long long e = base[r.s] + (long long)blockIdx.x * c + pos;
     if (f32) {
-        ((float4 *)a.vals)[e] = make_float4((float)qo, (float)qs, (float)ql, (float)c);
+        ((float4 *)a.vals)[e] = make_float4((float)r.qo, (float)r.qs, (float)r.ql, (float)r.c);
     } else {
         double2 *d = (double2 *)a.vals + 2 * e;
-        d[0] = make_double2(qo, qs);
-        d[1] = make_double2(ql, c);
+        d[0] = make_double2(r.qo, r.qs);
+        d[1] = make_double2(r.ql, r.c);
     }
     return true;
 }
