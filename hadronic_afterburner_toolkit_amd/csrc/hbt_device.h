@@ -24,12 +24,13 @@ struct DevParams {
     // (T = |p1T+p2T|^2 = 4 K_perp^2, Q = |p1T-p2T|^2) and dn = |p1T|^2 -
     // |p2T|^2 from the two records.  As q_out = dn / sqrt(T) and q_side^2 =
     // Q - q_out^2, the windows |q_out|, |q_side| <= H read dn^2 <= H^2 T and
-    // Q T - dn^2 <= H^2 T, with H^2 T = pf_h2 u + pf_h2c.
+    // (Q - H^2) T <= dn^2; each side carries the rounding margin c.
     double pf_half;   // |T - mid| <= pf_half
     double pf_h2;     // H^2 with margin (0 with the q_inv branch on)
-    double pf_h2c;    // pf_h2 mid + rounding margin (+inf with the q_inv branch on)
+    double pf_midh2;  // mid - pf_h2
+    double pf_negc;   // -c (-inf with the q_inv branch on: the q tests pass every pair)
     double pf_mid;
-    float f_half, f_h2, f_h2c, f_mid;
+    float f_half, f_h2, f_midh2, f_negc, f_mid, f_pad;
     // precision-form exact-stage constants
     double inv_dKT, inv_dq, inv_two_pi, two_pi_hi, two_pi_lo, inv_dKphi;
     // K_phi angles closer than this (rad) to a bin edge, 0 or 2 pi are

@@ -18,7 +18,7 @@
 //   |beta1 - beta2| <= (H/2) (1/E1 + 1/E2)   (from |q_long| <= H, M_t <= K_E,
 //        q_long M_t = E2 pz1 - E1 pz2; beta = pz/E)
 // with H = max(|q_lo_cut|, |q_hi_cut|), the q windows tested without a
-// division as dn^2 <= H^2 T and Q T - dn^2 <= H^2 T.  T and Q come from one
+// division as dn^2 <= H^2 T and (Q - H^2) T <= dn^2.  T and Q come from one
 // dot product: T = n1 + n2 + 2 p1.p2, Q = n1 + n2 - 2 p1.p2 (n = |pT|^2).
 // For mixed pairs the i particle is rotated by -theta instead of rotating
 // every j by +theta (T, Q, dn and beta are rotation invariant); the exact
@@ -40,33 +40,65 @@ struct PfRec32 {
     float px, py, nrm, beta;
 };
 
-// The prefilter's terms of one pair (for a mixed pair, i rotated by -theta):
-// u = T - mid, dd = dn^2, h2t = H^2 T (with margin), qtd = Q T - dn^2, and
-// the beta difference db against its bound rw.  The pair passes when
-// |u| <= half, dd <= h2t, qtd <= h2t and |db| <= rw.  The kernel and the
-// host's check of the margins evaluate this same function.
+// Prefilter constants in the records' precision (FP64 or FP32 set of
+// DevParams): the T window's centre and half-width, H^2, mid - H^2, and the
+// q windows' margin c as -c and -2c.
+template <typename Prec>
+struct PfConst {
+    Prec half, mid, h2, midh2, negc, neg2c;
+};
+
+template <typename Prec>
+__host__ __device__ __forceinline__ PfConst<Prec> pf_const(const DevParams &P) {
+    constexpr bool f64 = std::is_same<Prec, double>::value;
+    PfConst<Prec> k;
+    k.half = (Prec)(f64 ? P.pf_half : P.f_half);
+    k.mid = (Prec)(f64 ? P.pf_mid : P.f_mid);
+    k.h2 = (Prec)(f64 ? P.pf_h2 : P.f_h2);
+    k.midh2 = (Prec)(f64 ? P.pf_midh2 : P.f_midh2);
+    k.negc = (Prec)(f64 ? P.pf_negc : P.f_negc);
+    k.neg2c = k.negc * (Prec)2;
+    return k;
+}
+
+// The prefilter's tests of one pair (for a mixed pair, i rotated by
+// -theta), as four comparisons lhs[k] <= rhs[k]: the K_T window |T - mid|
+// <= half; q_out: dn^2 - c <= H^2 T; q_side: (Q - H^2) T - 2c <= dn^2 - c;
+// beta: |beta_i - beta_j| <= cw_i + cw_j.  The kernel and the host's check
+// of the margins evaluate this same function.
 template <typename Prec>
 struct PfTerms {
-    Prec u, dd, h2t, qtd, db, rw;
+    Prec lhs[4], rhs[4];
 };
 
 template <typename Prec>
 __host__ __device__ __forceinline__ PfTerms<Prec> pf_terms(Prec ix, Prec iy, Prec in, Prec ib,
                                                            Prec icw, Prec jx, Prec jy, Prec jn,
-                                                           Prec jb, Prec jcw, Prec h2, Prec h2c,
-                                                           Prec mid) {
+                                                           Prec jb, Prec jcw,
+                                                           const PfConst<Prec> &k) {
     PfTerms<Prec> t;
     const Prec d = fma(ix, jx, iy * jy);
     const Prec sn = in + jn;
-    t.u = fma((Prec)2, d, sn);
-    const Prec v = fma((Prec)-2, d, sn);
+    const Prec u = fma((Prec)2, d, sn);   // T - mid
+    const Prec v = fma((Prec)-2, d, sn);  // Q - mid
     const Prec dn = in - jn;
-    t.dd = dn * dn;
-    t.h2t = fma(h2, t.u, h2c);
-    t.qtd = (v + mid) * (t.u + mid) - t.dd;
-    t.db = ib - jb;
-    t.rw = icw + jcw;
+    const Prec T = u + k.mid;
+    const Prec ddc = fma(dn, dn, k.negc);
+    t.lhs[0] = fabs(u);
+    t.rhs[0] = k.half;
+    t.lhs[1] = ddc;
+    t.rhs[1] = T * k.h2;
+    t.lhs[2] = fma(v + k.midh2, T, k.neg2c);
+    t.rhs[2] = ddc;
+    t.lhs[3] = fabs(ib - jb);
+    t.rhs[3] = icw + jcw;
     return t;
+}
+
+template <typename Prec>
+__host__ __device__ __forceinline__ bool pf_pass(const PfTerms<Prec> &t) {
+    return t.lhs[0] <= t.rhs[0] && t.lhs[1] <= t.rhs[1] && t.lhs[2] <= t.rhs[2] &&
+           t.lhs[3] <= t.rhs[3];
 }
 
 // Same-event pair orientation: for a position in the cell-sorted list, the
@@ -377,11 +409,7 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     const int lane = __lane_id();
     unsigned *q = ring[threadIdx.x >> 6];
     Stage &stg = stage[threadIdx.x >> 6];
-    const Prec half_t = (Prec)(std::is_same<Prec, double>::value ? a.P.pf_half : a.P.f_half);
-    constexpr bool f64 = std::is_same<Prec, double>::value;
-    const Prec h2 = (Prec)(f64 ? a.P.pf_h2 : a.P.f_h2);
-    const Prec h2c = (Prec)(f64 ? a.P.pf_h2c : a.P.f_h2c);
-    const Prec mid = (Prec)(f64 ? a.P.pf_mid : a.P.f_mid);
+    const PfConst<Prec> pk = pf_const<Prec>(a.P);
     unsigned long long nb_same = 0, nb_mixed = 0;
 
     const int nq = a.nqueues;
@@ -478,11 +506,11 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                         for (int r = 0; r < R; r++) {
                             const PfTerms<Prec> pt =
                                 pf_terms(rx[r], ry[r], in[r], ib[r], icw[r], jr.px, jr.py,
-                                         jr.nrm, jr.beta, jcw, h2, h2c, mid);
-                            unsigned long long m = le_mask(absv(pt.u), half_t) &
-                                                   le_mask(pt.dd, pt.h2t) &
-                                                   le_mask(pt.qtd, pt.h2t) &
-                                                   le_mask(absv(pt.db), pt.rw);
+                                         jr.nrm, jr.beta, jcw, pk);
+                            unsigned long long m = le_mask(pt.lhs[0], pt.rhs[0]) &
+                                                   le_mask(pt.lhs[1], pt.rhs[1]) &
+                                                   le_mask(pt.lhs[2], pt.rhs[2]) &
+                                                   le_mask(pt.lhs[3], pt.rhs[3]);
                             if (partial) m &= vmask[r];
                             if (diag) m &= low_lanes32(tri + u - r * 64);
                             if (m) {

@@ -96,12 +96,14 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     const double Hm = d.H * (1.0 + 1e-9) + 1e-12;
     d.h2 = Hm * Hm * (1.0 + 1e-9) + 1e-12;
     P.pf_h2 = P.qinv ? 0.0 : d.h2;
-    P.pf_h2c = P.qinv ? INFINITY : d.h2 * d.mid;
+    P.pf_midh2 = d.mid - P.pf_h2;
+    P.pf_negc = P.qinv ? -INFINITY : 0.0;
     d.c_beta = 0.5 * Hm;
     P.f_mid = (float)d.mid;
     P.f_half = (float)P.pf_half;
     P.f_h2 = (float)P.pf_h2;
-    P.f_h2c = (float)P.pf_h2c;
+    P.f_midh2 = (float)P.pf_midh2;
+    P.f_negc = (float)P.pf_negc;
     d.needed = (uint64_t)(unsigned long long)k.needed_number_of_pairs;
     d.needed_qinv = (uint64_t)50 * d.needed;
     const long q = k.qnpts;
@@ -135,20 +137,20 @@ inline void pf_record(const Derived &d, double px, double py, double pz, double 
 // Prefilter margins that depend on the data's momentum scale (|pT|^2 up to
 // pmax2), for the records' precision (FP32 in the precision form).  With
 // S = max(1, pmax2) + |mid|: the rounding of the records (px, py, nrm) and
-// of u, v, dn stays below 8 ulp of S, so that of the products dn^2, Q T and
-// H^2 T below 150 ulp of max(1, H^2) S^2 (|dn|, T, Q <= 4 S); the T window
-// allows 64 ulp of 2 S, the q windows 256 ulp of max(1, H^2) S^2.
+// of u, v, dn, T, Q - H^2 stays below 8 ulp of S, so that of the products
+// dn^2, (Q - H^2) T and H^2 T below 150 ulp of max(1, H^2) S^2 (|dn|, T,
+// Q <= 4 S); the T window allows 64 ulp of 2 S, each q window a margin c
+// of 256 ulp of max(1, H^2) S^2.
 inline void prefilter_margins(const Derived &d, double pmax2, DevParams &P) {
     const double pm = std::max(1.0, pmax2);
     const double eps32 = 5.9604644775390625e-8, eps64 = 1.1102230246251565e-16;
     const double m = 64.0 * eps32 * (2.0 * pm + std::fabs(d.mid));
     P.f_half = (float)(d.P.pf_half + m);
-    if (P.qinv) return;  // pf_h2 = 0, pf_h2c = +inf: the q windows pass every pair
+    if (P.qinv) return;  // pf_negc = -inf: the q windows pass every pair
     const double S = pm + std::fabs(d.mid);
     const double s2 = std::max(1.0, d.h2) * S * S;
-    P.pf_h2c = d.h2 * d.mid + 256.0 * eps64 * s2;
-    P.f_h2 = (float)d.h2;
-    P.f_h2c = (float)(d.h2 * d.mid + 256.0 * eps32 * s2);
+    P.pf_negc = -256.0 * eps64 * s2;
+    P.f_negc = (float)(-256.0 * eps32 * s2);
 }
 
 // Allocator whose construct() without arguments default-initializes: a

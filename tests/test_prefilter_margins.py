@@ -1,4 +1,4 @@
-"""Margins of the fast kernel's prefilter (pf_terms in csrc/hbt_fast.h, the
+"""Margins of the fast kernel's prefilter (pf_terms and pf_pass in csrc/hbt_fast.h, the
 constants of derive and prefilter_margins in csrc/hbt_internal.h), checked
 on the host against the reference's FP64 cascade (cascade_f64, the same
 function the kernels' exact stage and the oracle follow,
@@ -42,18 +42,13 @@ struct P4 { double px, py, pz, E; };
 template <typename Rec, typename CW, typename Prec>
 static bool passes(const DevParams &P, const Rec &a, CW acw, const Rec &b, CW bcw, double c,
                    double s, bool mixed) {
-    constexpr bool f64 = std::is_same<Prec, double>::value;
     Prec ix = a.px, iy = a.py;
     if (mixed) {  // the kernel rotates i by -theta
         ix = (Prec)((double)a.px * c + (double)a.py * s);
         iy = (Prec)((double)a.py * c - (double)a.px * s);
     }
-    const Prec half = (Prec)(f64 ? P.pf_half : P.f_half);
-    const PfTerms<Prec> t = pf_terms<Prec>(ix, iy, a.nrm, a.beta, acw, b.px, b.py, b.nrm, b.beta,
-                                           bcw, (Prec)(f64 ? P.pf_h2 : P.f_h2),
-                                           (Prec)(f64 ? P.pf_h2c : P.f_h2c),
-                                           (Prec)(f64 ? P.pf_mid : P.f_mid));
-    return std::fabs(t.u) <= half && t.dd <= t.h2t && t.qtd <= t.h2t && std::fabs(t.db) <= t.rw;
+    return pf_pass(pf_terms<Prec>(ix, iy, a.nrm, a.beta, acw, b.px, b.py, b.nrm, b.beta, bcw,
+                                  pf_const<Prec>(P)));
 }
 
 int main(int argc, char **argv) {
@@ -114,8 +109,8 @@ int main(int argc, char **argv) {
             DevParams P = d.P;
             prefilter_margins(d, pmax2, P);
             if (plant) {  // without the data-scaled margins
-                P.f_h2c = (float)(d.h2 * d.mid);
-                P.pf_h2c = d.h2 * d.mid;
+                P.f_negc = 0.0f;
+                P.pf_negc = 0.0;
             }
             for (auto &pr : prs) {
                 const P4 &a = pr.first, &b = pr.second;
