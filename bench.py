@@ -609,8 +609,8 @@ def kernel_build_hash():
 
 
 def profile_value(path, key, kernel_prefix):
-    """A per-launch figure of the dominant kernel from a committed rocprofv3
-    counter summary (lines '<key> <kernel> N <build hash>'), with the build
+    """A per-step figure of the headline mode from a committed rocprofv3
+    counter summary (lines '<key> <mode> N <build hash>'), with the build
     hash of the sources it was measured on and the current one; the value is
     None when they differ (the kernel changed since the counters were
     read)."""
@@ -627,9 +627,11 @@ def profile_value(path, key, kernel_prefix):
     return None, None, cur
 
 
-def measured_traffic(kernel_prefix):
-    """HBM bytes per launch of the dominant kernel (profiles/k_fast_traffic.md)."""
-    return profile_value(TRAFFIC_SUMMARY, "traffic_bytes_per_launch", kernel_prefix)
+def measured_traffic(mode):
+    """HBM bytes per step of the headline mode (fp32c, fp64, or with _slab:
+    the step's k_fast, k_slab_bin and k_slab_reduce) from
+    profiles/k_fast_traffic.md."""
+    return profile_value(TRAFFIC_SUMMARY, "traffic_bytes_per_step", mode)
 
 
 def main():
@@ -762,40 +764,51 @@ def main():
         dev_s = float(np.mean(r["dev_ms"])) * 1e-3
         ach = fl / dev_s / 1e12
         peak = FP32_PEAK_TFLOPS if head == "fp32c" else FP64_PEAK_TFLOPS
-        kname = "k_fast<float>" if head == "fp32c" else "k_fast<double>"
-        traffic, t_build, cur_build = measured_traffic(kname)
+        # the step's kernels: k_fast alone with device-scope atomic binning;
+        # with slab binning also k_slab_bin and k_slab_reduce (the mode the
+        # autotuner chose, in the profiles' keys)
+        slab_on = bool(r["slab"]["on"])
+        pn = "float" if head == "fp32c" else "double"
+        pmode = head + ("_slab" if slab_on else "")
+        kname = (f"k_fast<{pn}> + k_slab_bin<{pn}> + k_slab_reduce" if slab_on
+                 else f"k_fast<{pn}>")
+        traffic, t_build, cur_build = measured_traffic(pmode)
         out["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
                            "frac": ach / peak, "traffic": traffic,
                            "traffic_source": "from_profile (profiles/k_fast_traffic.md)",
                            "traffic_build": t_build, "kernel_build": cur_build,
-                           "kernel": kname,
+                           "kernel": kname, "scope": "one step: every kernel of the batch launch",
+                           "profile_mode": pmode,
                            "flop_per_launch": fl, "stages_same": s, "stages_mixed": m}
-        # issue-side view of the same kernel: wave64 VALU instructions per
-        # launch (SQ_INSTS_VALU, committed counter summary of this build),
-        # each holding its SIMD 4 cycles, over the launch on 1024 SIMDs at
-        # the 2.4 GHz peak clock
-        insts, _, _ = profile_value(SQ_SUMMARY, "valu_insts_per_launch", kname)
+        # issue-side view of the same step: wave64 VALU instructions per step
+        # (SQ_INSTS_VALU summed over its kernels, committed counter summary of
+        # this build), each holding its SIMD 4 cycles, over the step on 1024
+        # SIMDs at the 2.4 GHz peak clock
+        insts, _, _ = profile_value(SQ_SUMMARY, "valu_insts_per_step", pmode)
         out["roofline"]["valu_issue_frac"] = (
             insts * 4 / (MI355X_CUS * 4 * MI355X_GHZ * 1e9 * dev_s) if insts else None)
         out["roofline"]["valu_insts_per_launch"] = insts
-        # the bound that holds: every binned pair is one atomic cache-line
-        # access (same-event: count + four sums in one FP64 instruction;
-        # mixed: the denominator's u32 add), against the measured chip-wide
-        # rates of those two patterns; the ceiling is the blended rate
-        acc = s[4] + m[4]
-        ceil = acc / (s[4] / atom_ceil["same"] + m[4] / atom_ceil["mixed"]) if atom_ceil else None
-        # the floor the device-scope atomics set; slab binning (the *_slab
-        # modes) adds the binned pairs without them
-        smode = head + "_slab"
-        s_dev = float(np.mean(res[smode]["dev_ms"])) * 1e-3 if smode in res else None
+        # device-scope atomics: with atomic binning every binned pair is one
+        # atomic cache-line access (same-event: count + four sums in one FP64
+        # instruction; mixed: the denominator's u32 add); with slab binning
+        # only the mixed pairs' adds are.  Floors from the chip-wide rates of
+        # those patterns measured in this run.
+        acc_same = 0 if slab_on else s[4]
+        floor = ((acc_same / atom_ceil["same"] + m[4] / atom_ceil["mixed"]) if atom_ceil else None)
+        floor_all = ((s[4] / atom_ceil["same"] + m[4] / atom_ceil["mixed"]) if atom_ceil else None)
+        amode = head if slab_on else head + "_slab"
+        a_dev = float(np.mean(res[amode]["dev_ms"])) * 1e-3 if amode in res else None
         out["atomic_roofline"] = {
-            "bound": "memory_side_atomics", "achieved": acc / dev_s,
-            "ceiling": ceil, "unit": "atomic line accesses/s",
-            "frac": (acc / dev_s / ceil) if ceil else None,
-            "slab_mode_ms": s_dev * 1e3 if s_dev else None,
-            "slab_mode_below_floor": bool(ceil and s_dev and s_dev < acc / ceil),
-            "accesses_per_launch": acc, "accesses_same": s[4], "accesses_mixed": m[4],
-            "floor_ms": 1e3 * acc / ceil if ceil else None, "kernel": kname,
+            "bound": "memory_side_atomics", "binning": "slab" if slab_on else "atomics",
+            "accesses_per_launch": acc_same + m[4], "accesses_same": acc_same,
+            "accesses_mixed": m[4], "unit": "atomic line accesses/s",
+            "achieved": (acc_same + m[4]) / dev_s,
+            "floor_ms": 1e3 * floor if floor else None,
+            "frac": floor / dev_s if floor else None,
+            "atomic_binning_floor_ms": 1e3 * floor_all if floor_all else None,
+            "step_below_atomic_binning_floor": bool(floor_all and dev_s < floor_all),
+            "other_binning_mode": amode, "other_binning_ms": a_dev * 1e3 if a_dev else None,
+            "kernel": f"k_fast<{pn}>",
             "ceiling_source": "hist_atomics_bench.e f64coop5 (same-event) and "
                               "u32carry_1_per_lane (mixed), measured in this run"}
     if not args.no_extra and not args.only:
