@@ -458,23 +458,39 @@ inline int gather_particles(const Derived &d, const hbt_config &k, int ngroups,
     h.orr.resize((size_t)nf);
     h.srt_cell.resize((size_t)nf);
     h.srt_py.resize((size_t)nf);
-    parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
-        CellOrderScratch sc;
-        std::vector<int> order;
-        for (long long g = ga; g < gb; g++) {
-            const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
-            order.resize((size_t)(g1 - g0));
-            cell_order((int)(g1 - g0), [&](int k) { return F.rec[g0 + k].px; },
-                       [&](int k) { return F.rec[g0 + k].py; }, D, sc, order.data(),
-                       h.srt_cell.data() + g0);
-            for (long long q = g0; q < g1; q++) {
+    // groups on the host threads, or, with fewer groups than threads (one
+    // buffer of the per-buffer path), each group's cells and copies
+    const bool lone = ngroups < HostPool::get().threads();
+    auto sort_group = [&](long long g, CellOrderScratch &sc, std::vector<int> &order) {
+        const long long g0 = h.foff[grp_ev[g]], g1 = h.foff[grp_ev[g + 1]];
+        order.resize((size_t)(g1 - g0));
+        cell_order((int)(g1 - g0), [&](int k) { return F.rec[g0 + k].px; },
+                   [&](int k) { return F.rec[g0 + k].py; }, D, sc, order.data(),
+                   h.srt_cell.data() + g0, lone);
+        auto copy = [&](long long q0, long long q1) {
+            for (long long q = g0 + q0; q < g0 + q1; q++) {
                 const int src = (int)g0 + order[(size_t)(q - g0)];
                 h.srt.copy_from(F, (size_t)q, (size_t)src, f32);
                 h.orr[q] = OrigRank{src, frank[src]};
                 h.srt_py[q] = F.rec[src].py;
             }
-        }
-    });
+        };
+        if (lone)
+            parallel_ranges(g1 - g0, 2048, copy);
+        else
+            copy(0, g1 - g0);
+    };
+    if (lone) {
+        CellOrderScratch sc;
+        std::vector<int> order;
+        for (long long g = 0; g < ngroups; g++) sort_group(g, sc, order);
+    } else {
+        parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
+            CellOrderScratch sc;
+            std::vector<int> order;
+            for (long long g = ga; g < gb; g++) sort_group(g, sc, order);
+        });
+    }
     return HBT_OK;
 }
 

@@ -53,7 +53,10 @@ inline double srt_cell_width(double D, double px_span) {
 // srt_cell_width(D, px span) from the group's smallest px), then by py;
 // stable (input order among equal keys).  px(k), py(k): the momentum of
 // input particle k < n.  order[p] receives the input particle at position
-// p, cell[p] its cell.
+// p, cell[p] its cell.  One sort by py, then a stable counting sort by
+// cell; or, with par (a lone group: a caller looping over many groups in
+// parallel passes false), a stable counting sort by cell, then each cell's
+// particles sorted by py on the host pool.  Both give the same order.
 struct CellOrderScratch {
     std::vector<std::pair<double, int>> idx;       // (py, input particle)
     std::vector<std::pair<uint64_t, int>> ra, rb;  // radix scratch
@@ -61,8 +64,8 @@ struct CellOrderScratch {
 };
 
 template <typename PX, typename PY>
-inline void cell_order(int n, PX px, PY py, double D, CellOrderScratch &s, int *order,
-                       int *cell) {
+inline void cell_order(int n, PX px, PY py, double D, CellOrderScratch &s, int *order, int *cell,
+                       bool par = false) {
     if (n <= 0) return;
     double lo = INFINITY, hi = -INFINITY;
     for (int k = 0; k < n; k++) {
@@ -70,28 +73,58 @@ inline void cell_order(int n, PX px, PY py, double D, CellOrderScratch &s, int *
         hi = std::max(hi, (double)px(k));
     }
     const double w = srt_cell_width(D, hi - lo);
-    s.idx.resize((size_t)n);
-    for (int k = 0; k < n; k++) s.idx[(size_t)k] = {(double)py(k), k};
-    radix_sort_keys(s.idx, s.ra, s.rb);  // by py
     s.cell.resize((size_t)n);
     int ncell = 1;
     for (int k = 0; k < n; k++) {
-        const double x = (double)px(s.idx[(size_t)k].second);
-        s.cell[(size_t)k] = std::isfinite(w) ? std::min(65536, (int)((x - lo) / w)) : 0;
+        s.cell[(size_t)k] = std::isfinite(w) ? std::min(65536, (int)(((double)px(k) - lo) / w)) : 0;
         ncell = std::max(ncell, s.cell[(size_t)k] + 1);
     }
-    s.start.assign((size_t)ncell + 1, 0);  // then a stable counting sort by cell
+    s.start.assign((size_t)ncell + 1, 0);
     for (int c : s.cell) s.start[(size_t)c + 1]++;
     for (int c = 0; c < ncell; c++) s.start[(size_t)c + 1] += s.start[(size_t)c];
-    for (int k = 0; k < n; k++) {
-        const int at = s.start[(size_t)s.cell[(size_t)k]]++;
-        order[at] = s.idx[(size_t)k].second;
-        cell[at] = s.cell[(size_t)k];
+    s.idx.resize((size_t)n);
+    if (!(par && n > 4096)) {  // one sort by py, then a stable counting sort by cell
+        for (int k = 0; k < n; k++) s.idx[(size_t)k] = {(double)py(k), k};
+        radix_sort_keys(s.idx, s.ra, s.rb);
+        std::vector<int> at(s.start.begin(), s.start.end() - 1);
+        for (int k = 0; k < n; k++) {
+            const int i = s.idx[(size_t)k].second, c = s.cell[(size_t)i];
+            const int q = at[(size_t)c]++;
+            order[q] = i;
+            cell[q] = c;
+        }
+        return;
     }
+    {
+        std::vector<int> at(s.start.begin(), s.start.end() - 1);
+        for (int k = 0; k < n; k++) {
+            const int c = s.cell[(size_t)k];
+            s.idx[(size_t)at[(size_t)c]++] = {(double)py(k), k};
+        }
+    }
+    auto sort_cells = [&](long long c0, long long c1, CellOrderScratch &t) {
+        for (long long c = c0; c < c1; c++) {
+            const int b = s.start[(size_t)c], e = s.start[(size_t)c + 1];
+            if (e - b > 1) {
+                t.idx.assign(s.idx.begin() + b, s.idx.begin() + e);
+                radix_sort_keys(t.idx, t.ra, t.rb);  // by py
+                std::copy(t.idx.begin(), t.idx.end(), s.idx.begin() + b);
+            }
+            for (int q = b; q < e; q++) {
+                order[q] = s.idx[(size_t)q].second;
+                cell[q] = (int)c;
+            }
+        }
+    };
+    parallel_ranges(ncell, 1, [&](long long c0, long long c1) {
+        CellOrderScratch t;
+        sort_cells(c0, c1, t);
+    });
 }
 
-// Work items of consecutive chunks of groups, built on several host threads
-// and concatenated in group order (the result is the serial one).
+// Work items of consecutive chunks of units (cells, events), built on
+// several host threads and concatenated in order (the result is the serial
+// one).
 struct ItemChunk {
     std::vector<WorkItem> items;
     std::vector<JWin> wins;
@@ -142,13 +175,29 @@ inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                        std::vector<WorkItem> &items, std::vector<uint64_t> *pairs,
                        uint64_t &enum_same) {
     const int ngroups = (int)grp_ev.size() - 1;
-    items_by_group_chunks(ngroups, items, nullptr, pairs, &enum_same,
-                          [&](int ga, int gb, ItemChunk &out) {
-        for (int g = ga; g < gb; g++) {
-            const long long nk = off[grp_ev[g + 1]] - off[grp_ev[g]];
-            if (nk >= 2) out.enum_same += (uint64_t)(nk * (nk - 1) / 2);
+    for (int g = 0; g < ngroups; g++) {
+        const long long nk = off[grp_ev[g + 1]] - off[grp_ev[g]];
+        if (nk >= 2) enum_same += (uint64_t)(nk * (nk - 1) / 2);
+    }
+    // the work splits by cell: (group, first position of the cell), in order
+    std::vector<std::vector<long long>> starts((size_t)std::max(0, ngroups));
+    parallel_ranges(ngroups, 1, [&](long long ga, long long gb) {
+        for (long long g = ga; g < gb; g++) {
             const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
             if (n < 2) continue;
+            for (long long q = 0; q < n; q++)
+                if (q == 0 || srt_cell[g0 + q] != srt_cell[g0 + q - 1]) starts[(size_t)g].push_back(q);
+        }
+    });
+    std::vector<std::pair<int, long long>> units;
+    for (int g = 0; g < ngroups; g++)
+        for (long long q : starts[(size_t)g]) units.push_back({g, q});
+    items_by_group_chunks((int)units.size(), items, nullptr, pairs, nullptr,
+                          [&](int ua, int ub, ItemChunk &out) {
+        for (int u = ua; u < ub; u++) {
+            const int g = units[(size_t)u].first;
+            const long long cb = units[(size_t)u].second;
+            const long long g0 = foff[grp_ev[g]], n = foff[grp_ev[g + 1]] - g0;
             const int *cell = srt_cell + g0;
             const double *py = srt_py + g0;
             auto emit = [&](long long a, int ni, long long jb, long long je) {
@@ -165,20 +214,17 @@ inline void items_same(const ItemShape &s, const std::vector<long long> &off,
                     if (pairs) out.pairs.push_back(tile_pairs(a, ni, jb, jb + w.nj));
                 }
             };
-            for (long long cb = 0; cb < n;) {
-                const long long ce = std::upper_bound(cell + cb, cell + n, cell[cb]) - cell;
-                const bool next = ce < n && cell[ce] == cell[cb] + 1;
-                const long long ne = next ? std::upper_bound(cell + ce, cell + n, cell[ce]) - cell : ce;
-                for (long long a = cb; a < ce; a += s.TI) {
-                    const int ni = (int)std::min<long long>(s.TI, ce - a);
-                    const double ylo = py[a] - dpx, yhi = py[a + ni - 1] + dpx;
-                    emit(a, ni, a + 1, std::upper_bound(py + a, py + ce, yhi) - py);
-                    if (next) {
-                        const long long lb = std::lower_bound(py + ce, py + ne, ylo) - py;
-                        emit(a, ni, lb, std::upper_bound(py + lb, py + ne, yhi) - py);
-                    }
+            const long long ce = std::upper_bound(cell + cb, cell + n, cell[cb]) - cell;
+            const bool next = ce < n && cell[ce] == cell[cb] + 1;
+            const long long ne = next ? std::upper_bound(cell + ce, cell + n, cell[ce]) - cell : ce;
+            for (long long a = cb; a < ce; a += s.TI) {
+                const int ni = (int)std::min<long long>(s.TI, ce - a);
+                const double ylo = py[a] - dpx, yhi = py[a + ni - 1] + dpx;
+                emit(a, ni, a + 1, std::upper_bound(py + a, py + ce, yhi) - py);
+                if (next) {
+                    const long long lb = std::lower_bound(py + ce, py + ne, ylo) - py;
+                    emit(a, ni, lb, std::upper_bound(py + lb, py + ne, yhi) - py);
                 }
-                cb = ce;
             }
         }
     });
@@ -200,10 +246,16 @@ inline void items_mixed(const ItemShape &s, const std::vector<long long> &foff,
                         std::vector<uint64_t> *pairs) {
     const int ngroups = (int)grp_ev.size() - 1;
     const double *pt = mf_pt;
-    items_by_group_chunks(ngroups, items, &wins, pairs, nullptr,
-                          [&](int ga, int gb, ItemChunk &out) {
-        for (int g = ga; g < gb; g++) {
-            for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) {
+    // the work splits by event
+    const int nev = ngroups > 0 ? grp_ev[ngroups] : 0;
+    std::vector<int> ev_grp((size_t)nev);
+    for (int g = 0; g < ngroups; g++)
+        for (int e = grp_ev[g]; e < grp_ev[g + 1]; e++) ev_grp[(size_t)e] = g;
+    items_by_group_chunks(nev, items, &wins, pairs, nullptr,
+                          [&](int ea, int eb, ItemChunk &out) {
+        {
+            for (int e = ea; e < eb; e++) {
+                const int g = ev_grp[(size_t)e];
                 const long long n1 = foff[e + 1] - foff[e];
                 for (long long a = 0; a < n1; a += s.TI) {
                     const int ni = (int)std::min<long long>(s.TI, n1 - a);

@@ -11,7 +11,8 @@ binned, so the check is on the enumeration itself:
     of 0 to 1500 particles, clusters of equal momenta, coordinates on a grid
     of spacing D (pairs exactly D apart), a px span wide enough to reach
     the 2^16-cell cap, and several tile and chunk shapes;
-  - cell_order: a stable sort by (cell, py) with cells of width >= D;
+  - cell_order: a stable sort by (cell, py) with cells of width >= D,
+    also through its per-cell parallel sort (lone groups of up to 9000);
   - mixed: for each plan row, every (i, j) with ||pT_i| - |pT_j|| <= D
     exactly once (also |pT| on a grid of spacing D/2, so pairs exactly D
     apart), and the window's row is the plan row.
@@ -40,7 +41,7 @@ static int fail(const char *what, long long a, long long b) {
 
 // One trial of same-event items over ngroups groups.
 static int same_trial(std::mt19937_64 &rng, int ngroups, int nmax, double D, int mode,
-                      ItemShape sh, long long &checked) {
+                      ItemShape sh, long long &checked, bool par = false) {
     std::uniform_int_distribution<int> nd(0, nmax);
     std::vector<long long> foff(1, 0);
     std::vector<int> grp_ev(1, 0);
@@ -76,7 +77,7 @@ static int same_trial(std::mt19937_64 &rng, int ngroups, int nmax, double D, int
         const long long g0 = foff[g];
         const int n = (int)(foff[g + 1] - g0);
         cell_order(n, [&](int k) { return px[g0 + k]; }, [&](int k) { return py[g0 + k]; }, D,
-                   sc, order.data() + g0, cell.data() + g0);
+                   sc, order.data() + g0, cell.data() + g0, par);
         std::vector<char> seen((size_t)n, 0);
         for (int p = 0; p < n; p++) {
             const int o = order[g0 + p];
@@ -227,6 +228,10 @@ int main() {
             for (int mode = 0; mode < 4; mode++)
                 for (int t = 0; t < 2; t++)
                     if (same_trial(rng, 3, 1500, D, mode, sh, checked)) return 1;
+    // lone groups large enough for the per-cell parallel sort
+    for (double D : Ds)
+        for (int mode = 0; mode < 4; mode++)
+            if (same_trial(rng, 1, 9000, D, mode, shapes[0], checked, true)) return 1;
     for (const ItemShape &sh : shapes)
         for (double D : Ds)
             for (int grid = 0; grid < 2; grid++)
