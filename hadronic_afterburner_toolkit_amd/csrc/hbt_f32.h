@@ -6,35 +6,33 @@
 // pair's momentum sums/differences and products in FP64 (one instruction
 // each, exact same rounding as the reference), takes reciprocals and
 // reciprocal square roots from the FP32 transcendental unit
-// (v_rcp_f32 / v_rsq_f32, ~1 ulp) refined by two Newton steps in FP64, and
+// (v_rcp_f32 / v_rsq_f32, ~1 ulp) refined by one Newton step in FP64, and
 // evaluates cos on the FP32 unit after an FP64 Cody-Waite reduction to
 // [-pi, pi].  All survivors of the FP32 prefilter come here.
 //
 // Bit-identical counts: every decision value (K_T index, window cuts, bin
-// indices) is within a few ulp of the reference's correctly rounded value;
+// indices) is within 1e-13 relative of the reference's correctly rounded value;
 // a pair whose value lies within a guard band of a cut or bin edge is
 // re-run through the literal FP64 cascade (bin_*_f64), so bins and counts
-// equal the reference's exactly.  The accumulated q values carry ~1e-15
-// relative error, cos ~1e-7 absolute; sums are accumulated in FP64 atomics.
+// equal the reference's exactly.  The accumulated q values carry ~1e-13
+// relative error, cos ~1e-7 absolute (FP32 in slab records); sums are
+// accumulated in FP64 (atomics) or exact fixed point (slab binning).
 #pragma once
 #include "hbt_device.h"
 
 namespace hbt {
 
-// 1/sqrt(a) from v_rsq_f32, two Newton steps in FP64.
+// 1/sqrt(a) from v_rsq_f32 (relative error e <= 2^-22 for a normal FP32
+// input) and one Newton step in FP64: relative error ~1.5 e^2 < 1e-13, far
+// inside the guard bands (1e-9 bin units, 1e-12 relative on the cuts).
 __device__ __forceinline__ double rsqrt_refined(double a) {
-    double r = (double)__builtin_amdgcn_rsqf((float)a);
-    const double h = 0.5 * a;
-    r = r * fma(-h * r, r, 1.5);
-    r = r * fma(-h * r, r, 1.5);
-    return r;
+    const double r = (double)__builtin_amdgcn_rsqf((float)a);
+    return r * fma(-0.5 * a * r, r, 1.5);
 }
-// 1/a from v_rcp_f32, two Newton steps in FP64.
+// 1/a from v_rcp_f32 and one Newton step in FP64: relative error e^2.
 __device__ __forceinline__ double rcp_refined(double a) {
-    double r = (double)__builtin_amdgcn_rcpf((float)a);
-    r = r * fma(-a, r, 2.0);
-    r = r * fma(-a, r, 2.0);
-    return r;
+    const double r = (double)__builtin_amdgcn_rcpf((float)a);
+    return r * fma(-a, r, 2.0);
 }
 
 // cos(x): argument reduced to [-pi, pi] in FP64, then split r = rh + rl
@@ -100,6 +98,7 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
     const double K_perp_sq = K_x * K_x + K_y * K_y;  // same ops as the reference
     if (!(K_perp_sq >= P.KT_min_sq && K_perp_sq <= P.KT_max_sq)) return 0;
     int st = ST_KT;
+    if (!(K_perp_sq >= 1e-30)) return -1;  // below FP32's normal range: the literal cascade
     const double rK = rsqrt_refined(K_perp_sq);
     const double K_perp = K_perp_sq * rK;
     b.kt = guarded_index((K_perp - P.KT_min) * P.inv_dKT, GUARD_IDX);
