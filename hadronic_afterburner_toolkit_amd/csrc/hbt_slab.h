@@ -72,8 +72,16 @@ __device__ __forceinline__ int slab_coord(const SlabArgs &a, double v) {
     return (int)((v - a.q_base) * a.inv_dq);
 }
 
+// An FP32 record value k_slab_bin sums exactly in its fixed point: 0, or
+// 2^-17 <= |v| <= 1 (then v is a multiple of 2^-40 and |v| 2^40 <= 2^40).
+__device__ __forceinline__ bool fix_exact(float v) {
+    const float m = fabsf(v);
+    return m == 0.0f || (m >= 0x1p-17f && m <= 1.0f);
+}
+
 // A binned same-event pair r as a record of this block's region of slab
-// r.s; false: the pair takes the direct path.  fill, cap and base are the
+// r.s; false: the pair takes the direct path (also for a precision-form
+// value the fixed point would not hold exactly).  fill, cap and base are the
 // block's LDS copies of its fill counters and of the regions' capacity and
 // first record (cap and base as in SlabArgs).
 template <typename Prec>
@@ -82,6 +90,9 @@ __device__ __forceinline__ bool slab_put_same(const SlabArgs &a, unsigned *fill,
     constexpr bool f32 = std::is_same<Prec, float>::value;
     if (slab_coord(a, f32 ? (double)(float)r.qs : r.qs) != r.is ||
         slab_coord(a, f32 ? (double)(float)r.ql : r.ql) != r.il)
+        return false;
+    if (f32 && !(fix_exact((float)r.qo) && fix_exact((float)r.qs) && fix_exact((float)r.ql) &&
+                 fix_exact((float)r.c)))
         return false;
     const unsigned pos = atomicAdd(&fill[r.s], 1u);
     const int c = cap[r.s];
@@ -110,10 +121,10 @@ __device__ __forceinline__ void slab_load(const SlabArgs &a, long long e, double
     }
 }
 
-// Fixed-point scale of the precision form's LDS sums: an FP32 record value
-// v with |v| >= 2^-17 is a multiple of 2^-40, so v * 2^40 is an exact
-// integer (smaller values round by at most 2^-41); |v| <= 1 and at most
-// kMaxUnitRecords records per unit keep every sum below 2^63.  Integer sums are exact,
+// Fixed-point scale of the precision form's LDS sums: a record value is 0
+// or 2^-17 <= |v| <= 1 (fix_exact; other pairs take the direct path), so
+// v * 2^40 is an exact integer, and at most kMaxUnitRecords records per unit
+// keep every sum below 2^63.  Integer sums are exact,
 // so a unit's totals do not depend on the order of its LDS adds, and the
 // 64-bit integer LDS add is cheaper than the FP64 one.
 constexpr double kFixScale = 1099511627776.0;  // 2^40

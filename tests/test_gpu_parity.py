@@ -11,7 +11,10 @@ Bars (tolerances written here):
     same metric.
 The golden cases include binding pair caps (3D per K_T and per K_T x K_phi,
 and the q_inv cap), whose admitted pairs depend on the enumeration order;
-those calls run the engine's ordered path.
+those calls run the engine's ordered path.  The fast forms also run with
+slab binning (hbt_binning = 1), buffer after buffer through
+calculate_HBT_correlation_function, each launch's record regions sized from
+the previous buffer's demand (the K_phi and q_inv cases bin with atomics).
 """
 import numpy as np
 import pytest
@@ -23,9 +26,11 @@ from hadronic_afterburner_toolkit_amd import _lib, hbt, samples
 pytestmark = pytest.mark.gpu
 
 MODES = {
-    "plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12),
-    "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12),
-    "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6),
+    "plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12, 0),
+    "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 0),
+    "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 0),
+    "fast_fp64_slab": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 1),
+    "fast_fp32c_slab": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 1),
 }
 SUMS = (("q_out_sum", "abs_q_out"), ("q_side_sum", "abs_q_side"), ("q_long_sum", "abs_q_long"),
         ("num", "abs_num"))
@@ -37,14 +42,15 @@ def rel_err(gpu, ref, floor):
     return np.abs(gpu - ref) / d
 
 
-def run_engine_case(name, tmp_path, prec, kern):
+def run_engine_case(name, tmp_path, prec, kern, binning=0):
     spec, ref, outs, dig = cases.load(name)
     par = spec["params"]
     paths = cases.write_inputs(spec, dig, str(tmp_path))
     ebs = int(par["event_buffer_size"])
     bufs = samples.read_buffers(paths["same"], ebs, 211)
     mixb = samples.read_buffers(paths["mixed"], ebs, 211) if "mixed" in paths else None
-    g = hbt.HBTCorrelation(cases.hbt_params(spec), precision=prec, kernel=kern)
+    g = hbt.HBTCorrelation(dict(cases.hbt_params(spec), hbt_binning=binning), precision=prec,
+                           kernel=kern)
     rng = hbt.Random(int(par["randomSeed"]))
     for k, b in enumerate(bufs):
         g.calculate_HBT_correlation_function(b, rng, mixb[k] if mixb else None)
@@ -62,9 +68,9 @@ def oracle_floors(spec, bufs, mixb):
 @pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("name", cases.CASE_NAMES)
 def test_engine_vs_reference_golden(name, mode, tmp_path):
-    prec, kern, tol = MODES[mode]
+    prec, kern, tol, binning = MODES[mode]
     print(f"case {name} mode {mode}", flush=True)
-    spec, ref, g, bufs, mixb = run_engine_case(name, tmp_path, prec, kern)
+    spec, ref, g, bufs, mixb = run_engine_case(name, tmp_path, prec, kern, binning)
     r = g.results()
     assert np.array_equal(r.num_count, ref["num_count"]), "num_count"
     assert np.array_equal(r.den, ref["den"]), "den"
@@ -90,11 +96,11 @@ def test_config0_scale_vs_oracle(mode):
     """Config-0 source model (Poisson(150) pi+, groups of 10 events, q grid
     41^3 on [-0.15,0.15], K_T edges 0.2..0.7), 6 groups, reference mixing
     rule, against the oracle on the same inputs and random stream."""
-    prec, kern, tol = MODES[mode]
+    prec, kern, tol, binning = MODES[mode]
     params = dict(qnpts=41, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
                   HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18)
     full = samples.generate(20240, 60, 150.0)
-    g = hbt.HBTCorrelation(params, precision=prec, kernel=kern)
+    g = hbt.HBTCorrelation(dict(params, hbt_binning=binning), precision=prec, kernel=kern)
     o = oracle.Oracle(params)
     rg, ro = hbt.Random(77), oracle.OracleRng(77)
     for grp in range(6):
@@ -117,16 +123,16 @@ def test_config0_scale_vs_oracle(mode):
 
 def test_modes_agree_on_uneven_events():
     """Multiplicities log-uniform on [20, 600], one buffer of 12 events:
-    the three modes give identical counts."""
+    every mode (slab binning included) gives identical counts."""
     params = dict(qnpts=31, n_KT=5, KT_min=0.2, KT_max=0.7, HBTrap_min=-1.0, HBTrap_max=1.0,
                   needed_number_of_pairs=1e18)
     b = samples.generate(99, 12, samples.log_uniform_mult(20, 600))
     res = {}
-    for mode, (prec, kern, _) in MODES.items():
-        g = hbt.HBTCorrelation(params, precision=prec, kernel=kern)
+    for mode, (prec, kern, _, binning) in MODES.items():
+        g = hbt.HBTCorrelation(dict(params, hbt_binning=binning), precision=prec, kernel=kern)
         g.calculate_HBT_correlation_function(b, hbt.Random(1))
         res[mode] = g.results()
-    for mode in ("fast_fp64", "fast_fp32c"):
+    for mode in ("fast_fp64", "fast_fp32c", "fast_fp64_slab", "fast_fp32c_slab"):
         assert np.array_equal(res[mode].num_count, res["plain_fp64"].num_count)
         assert np.array_equal(res[mode].den, res["plain_fp64"].den)
 
@@ -136,7 +142,7 @@ def test_batch_equals_sequential_buffers(mode):
     """The batched form (all groups resident, one persistent launch) gives the
     same result as the groups processed one after another, and the hipGraph
     replays add up exactly."""
-    prec, kern, _ = MODES[mode]
+    prec, kern, _, _ = MODES[mode]
     params = dict(qnpts=21, n_KT=4, KT_min=0.2, KT_max=0.8, HBTrap_min=-1.0, HBTrap_max=1.0,
                   needed_number_of_pairs=1e18)
     full = samples.generate(555, 24, samples.log_uniform_mult(20, 400))
