@@ -11,7 +11,7 @@ Each event here holds pairs built from (K_T, q_out, q_side, q_long) at those edg
 K_T within 1e-9 of the K_T window's ends, |q_out| and |q_side| within 1e-9 of the
 window's half-width, q_T along the x axis (|dp_x| = |q_T| = sqrt(2) H at the corner),
 q_long at its edge.  The fast kernel (FP64 and precision form) must give the
-oracle's counts bit for bit, and sums within the usual bars:
+oracle's counts bit for bit (also with slab binning), and sums within the usual bars:
 err = |S - S_ref| / max(|S_ref|, sum|.|_bin) <= 1e-12 (FP64), 1e-6 (precision form).
 
 Some q components here are exactly zero by construction, so a bin can hold only
@@ -30,8 +30,10 @@ from hadronic_afterburner_toolkit_amd import _lib, hbt, samples
 
 pytestmark = pytest.mark.gpu
 
-MODES = {"fast_fp64": (_lib.HBT_PREC_FP64, 1e-12, 0.0),
-         "fast_fp32c": (_lib.HBT_PREC_FP32C, 1e-6, 1e-14)}
+MODES = {"fast_fp64": (_lib.HBT_PREC_FP64, 1e-12, 0.0, 0),
+         "fast_fp32c": (_lib.HBT_PREC_FP32C, 1e-6, 1e-14, 0),
+         "fast_fp64_slab": (_lib.HBT_PREC_FP64, 1e-12, 0.0, 1),
+         "fast_fp32c_slab": (_lib.HBT_PREC_FP32C, 1e-6, 1e-14, 1)}
 SUMS = (("q_out_sum", "abs_q_out"), ("q_side_sum", "abs_q_side"), ("q_long_sum", "abs_q_long"),
         ("num", "abs_num"))
 M_PI = 0.13957
@@ -90,13 +92,14 @@ def rel(a, b, floor, count, tol, abs_per_pair):
 @pytest.mark.parametrize("boost", [0, 1])
 @pytest.mark.parametrize("mode", list(MODES))
 def test_edge_pairs_match_oracle(mode, boost):
-    prec, tol, abs_per_pair = MODES[mode]
+    prec, tol, abs_per_pair, binning = MODES[mode]
     params = dict(qnpts=21, q_min=-0.15, q_max=0.15, n_KT=6, KT_min=0.2, KT_max=0.7,
                   HBTrap_min=-1.0, HBTrap_max=1.0, needed_number_of_pairs=1e18,
                   number_of_mixed_events=0, long_comoving_boost=boost)
     buf = edge_buffer(params, 4, 11 + boost)
     print(f"edges: {buf.nev} events, {buf.n} particles, boost {boost}, {mode}", flush=True)
-    g = hbt.HBTCorrelation(params, precision=prec, kernel=_lib.HBT_KERNEL_FAST)
+    g = hbt.HBTCorrelation(dict(params, hbt_binning=binning), precision=prec,
+                           kernel=_lib.HBT_KERNEL_FAST)
     g.calculate_groups([buf], hbt.Random(5))
     r = g.results()
     g.close()
@@ -113,9 +116,11 @@ def test_edge_pairs_match_oracle(mode, boost):
         assert rel(getattr(r, key), ref[key], ref[fl], cnt, tol, abs_per_pair) <= tol, key
 
 
-EXACT_MODES = {"plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12, 0.0),
-               "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 0.0),
-               "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 1e-14)}
+EXACT_MODES = {"plain_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_PLAIN, 1e-12, 0.0, 0),
+               "fast_fp64": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 0.0, 0),
+               "fast_fp32c": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 1e-14, 0),
+               "fast_fp64_slab": (_lib.HBT_PREC_FP64, _lib.HBT_KERNEL_FAST, 1e-12, 0.0, 1),
+               "fast_fp32c_slab": (_lib.HBT_PREC_FP32C, _lib.HBT_KERNEL_FAST, 1e-6, 1e-14, 1)}
 
 
 @pytest.mark.parametrize("kind", ["3d", "kphi"])
@@ -130,8 +135,8 @@ def test_exact_edges_same_and_mixed(mode, kind):
     comparisons (>= on hi).  Counts bit-exact with the oracle, sums within the
     mode's bar."""
     import edge_inputs as E
-    prec, kern, tol, abs_per_pair = EXACT_MODES[mode]
-    params = dict(E.EDGE_PARAMS if kind == "3d" else E.KPHI_PARAMS)
+    prec, kern, tol, abs_per_pair, binning = EXACT_MODES[mode]
+    params = dict(E.EDGE_PARAMS if kind == "3d" else E.KPHI_PARAMS, hbt_binning=binning)
     buf = E.build(kind)
     print(f"exact edges {kind}: {buf.n} particles, {mode}", flush=True)
     ids = np.array([1, 0], np.int32)
