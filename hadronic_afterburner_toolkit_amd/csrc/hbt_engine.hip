@@ -858,7 +858,7 @@ void build_items(hbt_ctx *c, bool same, bool mixed, ItemList &L, uint64_t &enum_
         if (mixed && c->plan_valid) {
             const Slot &m = mixed_slot(c);
             items_mixed(sh, s.foff, s.grp_ev, c->rowoff, c->plan_gids, m.foff, s.f_pt.data(),
-                        m.f_pt.data(), D, 0, L.items, L.wins, &pairs);
+                        m.f_pt.data(), pt_window(c->D), 0, L.items, L.wins, &pairs);
         }
         const size_t want = kItemsPerWave * waves;
         if (L.items.size() >= want || (sh.JC <= 64 && sh.KC <= 1)) break;

@@ -256,6 +256,15 @@ inline double px_window(const Derived &d) {
     return std::sqrt(2.0) * Hm * (1.0 + 1e-9) + 1e-12;
 }
 
+// Half-width of the mixed partners' |p_T| window: a binnable pair has
+// ||p1T| - |p2T|| <= |q_out| <= H, since (|p1T| - |p2T|)(|p1T| + |p2T|) =
+// (p1T - p2T).(p1T + p2T) = 2 |K_T| q_out and |p1T| + |p2T| >= 2 |K_T|
+// (margins as the prefilter's).  Infinite when the q_inv branch is on.
+inline double pt_window(const Derived &d) {
+    if (d.P.qinv) return INFINITY;
+    return (d.H * (1.0 + 1e-9) + 1e-12) * (1.0 + 1e-9) + 1e-12;
+}
+
 // Host-to-device copies through a pinned staging buffer: a pageable copy
 // runs at a fraction of the link rate and cannot overlap host work.
 struct StagePiece {

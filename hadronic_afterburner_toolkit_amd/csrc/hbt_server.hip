@@ -968,7 +968,8 @@ int hbt_server_submit(hbt_server *s, int32_t nev, const int64_t *ev_off,
             items_same(s->shape, hp.off, hp.foff, p.grp_ev, hp.srt_cell.data(), hp.srt_py.data(),
                        D, 0, p.items, &p.item_pairs, es);
             items_mixed(s->shape, hp.foff, p.grp_ev, p.rowoff, p.plan_ids, hp.foff,
-                        hp.f_pt.data(), hp.f_pt.data(), D, 0, p.items, p.wins, &p.item_pairs);
+                        hp.f_pt.data(), hp.f_pt.data(), pt_window(s->D), 0, p.items, p.wins,
+                        &p.item_pairs);
             for (uint64_t q : p.item_pairs) p.pairs_left += q;
         } else {
             // no work for this part; the request ends failed once its other

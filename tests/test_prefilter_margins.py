@@ -13,11 +13,13 @@ without the LCMS boost; a narrow window with wide K_T edges), at momentum
 scales up to |pT| ~ 10 GeV.  For every pair the cascade bins (same-event,
 and mixed with the partner rotated as the reference does), the prefilter
 on the particles' FP32 records (the precision form) and FP64 records must
-pass, in both orders of the pair.
+pass, in both orders of the pair, and the two |p_T| must differ by at most
+pt_window (the mixed items' partner window, H with margins).
 
 The program is compiled for the host only with hipcc.  Run with the
-data-scaled margins removed (the planted mistake), the FP32 prefilter must
-drop binnable pairs: the check has teeth."""
+data-scaled margins removed and the |p_T| window narrowed by 1e-3 (the
+planted mistakes), the FP32 prefilter must drop binnable pairs and binnable
+pairs must fall outside the window: the checks have teeth."""
 import shutil
 import subprocess
 from pathlib import Path
@@ -59,7 +61,7 @@ int main(int argc, char **argv) {
     const Cfg cfgs[] = {{41, -0.15, 0.15, 0.2, 0.7, 1}, {31, -0.05, 0.2, 0.3, 0.9, 0},
                         {11, -0.02, 0.02, 0.05, 2.0, 1}};
     const double m = 0.13957;
-    long long binned = 0, dropped32 = 0, dropped64 = 0;
+    long long binned = 0, dropped32 = 0, dropped64 = 0, outside_pt = 0;
     for (const Cfg &cf : cfgs) {
         hbt_config k{};
         k.qnpts = cf.qnpts; k.q_min = cf.qmin; k.q_max = cf.qmax;
@@ -70,6 +72,8 @@ int main(int argc, char **argv) {
         Derived d;
         std::string err;
         if (derive(k, d, err)) { std::printf("derive: %s\n", err.c_str()); return 2; }
+        // the planted mistake also narrows the |p_T| window by 1e-3
+        const double ptw = pt_window(d) * (plant ? 1.0 - 1e-3 : 1.0);
         for (double scale : {1.0, 3.0, 10.0}) {
             // pairs at the edges: p1 = K + q/2, p2 = K - q/2 in the transverse
             // plane (q_out along K), K_z and q_z set from rapidity and q_long
@@ -124,9 +128,11 @@ int main(int argc, char **argv) {
                 double qinv;
                 int ok;
                 // same-event
+                const double pta = std::sqrt(a.px * a.px + a.py * a.py);
                 if (cascade_f64<false, false>(P, a.px, a.py, a.pz, a.E, b.px, b.py, b.pz, b.E, 0.0,
                                               bins, qinv, ok) & ST_BIN) {
                     binned++;
+                    outside_pt += std::fabs(pta - std::sqrt(b.px * b.px + b.py * b.py)) > ptw;
                     for (int o = 0; o < 2; o++) {
                         const bool p32 = o ? passes<PfRec32, float, float>(P, rb32, cb32, ra32, ca32, 1, 0, false)
                                            : passes<PfRec32, float, float>(P, ra32, ca32, rb32, cb32, 1, 0, false);
@@ -144,6 +150,7 @@ int main(int argc, char **argv) {
                 if (cascade_f64<true, false>(P, a.px, a.py, a.pz, a.E, rx, ry, b.pz, b.E, 0.0,
                                              bins, qinv, ok) & ST_BIN) {
                     binned++;
+                    outside_pt += std::fabs(pta - std::sqrt(ux * ux + uy * uy)) > ptw;
                     PfRec32 ru32; float cu32;
                     PfRec64 ru64; double cu64;
                     pf_record(d, ux, uy, b.pz, b.E, bm, ru32, cu32);
@@ -154,7 +161,8 @@ int main(int argc, char **argv) {
             }
         }
     }
-    std::printf("binned %lld dropped32 %lld dropped64 %lld\n", binned, dropped32, dropped64);
+    std::printf("binned %lld dropped32 %lld dropped64 %lld outside_pt %lld\n", binned, dropped32,
+                dropped64, outside_pt);
     return 0;
 }
 """
@@ -174,15 +182,17 @@ def run(exe, *args):
     r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     f = r.stdout.split()
-    return int(f[1]), int(f[3]), int(f[5])
+    return int(f[1]), int(f[3]), int(f[5]), int(f[7])
 
 
 @pytest.mark.skipif(not Path(HIPCC).exists(), reason="needs hipcc")
 def test_prefilter_never_drops_a_binnable_pair(tmp_path):
     exe = build(tmp_path)
-    binned, d32, d64 = run(exe)
+    binned, d32, d64, outside_pt = run(exe)
     assert binned > 100000
     assert d32 == 0 and d64 == 0
+    # every binned pair lies inside the mixed items' |p_T| window
+    assert outside_pt == 0
     # the planted mistake: no data-scaled margin
-    binned_p, d32_p, _ = run(exe, "plant")
-    assert binned_p == binned and d32_p > 0
+    binned_p, d32_p, _, outside_p = run(exe, "plant")
+    assert binned_p == binned and d32_p > 0 and outside_p > 0
