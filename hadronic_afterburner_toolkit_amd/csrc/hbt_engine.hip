@@ -586,6 +586,7 @@ View make_view(hbt_ctx *c) {
     v.mwin = c->d_wins;
     v.plan_cs = c->d_cs;
     v.psi = c->d_psi;
+    v.P = c->P;
     return v;
 }
 

@@ -128,6 +128,11 @@ struct View {
     const JWin *mwin;          // partner windows of the mixed items
     const double *plan_cs;     // cos, sin of each plan row's rotation
     const double *psi;         // psi_ref per group
+    // the analysis' grid constants, for the exact stage: read from the view
+    // table (constant address space) where they are used, so the kernel
+    // neither holds them in SGPRs nor reloads spilled copies with VALU
+    // instructions
+    DevParams P;
 };
 
 // Work queues of a launch: at most one per XCD.  Queue q holds items
@@ -489,7 +494,7 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                     if (tail - head >= 64 || (last && t >= cnt && tail != head)) {
                         const int n = (int)min(64u, tail - head);
                         __builtin_amdgcn_wave_barrier();
-                        const int nb = drain<Prec, EXT, QCAP>(a.P, v, q, stg, head, n, w, psi,
+                        const int nb = drain<Prec, EXT, QCAP>(v.P, v, q, stg, head, n, w, psi,
                                                               a.slab, slab_fill, slab_cap,
                                                               slab_base, direct_n);
                         if (mixed)

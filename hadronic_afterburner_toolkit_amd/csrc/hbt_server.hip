@@ -282,6 +282,7 @@ void make_part_view(hbt_server *s, Part &p) {
     v.mwin = (const JWin *)(b + p.o_win);
     v.plan_cs = (const double *)(b + p.o_cs);
     v.psi = (const double *)(b + p.o_psi);
+    v.P = s->D.P;
 }
 
 // Uploads a part into its arena block on the upload stream (clearing the
