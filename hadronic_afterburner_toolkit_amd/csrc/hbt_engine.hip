@@ -583,6 +583,7 @@ View make_view(hbt_ctx *c) {
     v.cwB = cw(s1.ev);
     v.cwS = cw(s0.srt);
     v.origS = s0.d_orig;
+    v.AS = s0.srt.rec;
     v.mwin = c->d_wins;
     v.plan_cs = c->d_cs;
     v.psi = c->d_psi;
@@ -1308,7 +1309,7 @@ int upload_columns(hbt_ctx *c, Slot &s, bool f32, int32_t nev, const HostParticl
     }
     if ((size_t)nf > s.cap_f || !s.d_orig) {
         for (DevRecords *r : {&s.ev, &s.srt}) {
-            if (!rc && r == &s.ev) rc = dev_alloc(c, &r->rec, (size_t)nf);
+            if (!rc) rc = dev_alloc(c, &r->rec, (size_t)nf);
             if (!rc && !f32) rc = dev_alloc(c, &r->pf64, (size_t)nf);
             if (!rc && f32) rc = dev_alloc(c, &r->pf32, (size_t)nf);
             if (!rc && !f32) rc = dev_alloc(c, &r->cw64, (size_t)nf);
@@ -1375,7 +1376,9 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
         for (const auto &x : sets) {
             DevRecords &d = *x.first;
             const RecordSet &r = *x.second;
-            add(d.rec, r.rec.data(), (size_t)nf * sizeof(PartRec));
+            // the srt set holds prefilter records only: its full records are
+            // gathered on the device below
+            if (x.first == &s.ev) add(d.rec, r.rec.data(), (size_t)nf * sizeof(PartRec));
             if (f32) {
                 add(d.pf32, r.pf32.data(), (size_t)nf * sizeof(PfRec32));
                 add(d.cw32, r.cw32.data(), (size_t)nf * 4);
@@ -1388,6 +1391,8 @@ int upload_slot(hbt_ctx *c, int32_t slot, int32_t ngroups, const int32_t *grp_ev
     }
     HIPCHECK(c, ensure_pinned(&c->h_stage2, &c->stage2_cap, stage_bytes(pieces)));
     HIPCHECK(c, stage_pieces(c->h_stage2, pieces, c->stream));
+    // the records in the cell-sorted order, for the exact stage of same-event pairs
+    HIPCHECK(c, gather_srt(s.ev.rec, s.d_orig, s.srt.rec, nf, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     s.nf = nf;
     s.foff = std::move(h.foff);

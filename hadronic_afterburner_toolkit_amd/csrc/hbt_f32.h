@@ -86,11 +86,17 @@ __device__ __forceinline__ int kphi_index_fast(const DevParams &P, double K_x, d
 
 // The K/q cascade of the precision form.  Returns the stage mask as
 // cascade_f64 does, or -1 when a decision is ambiguous.
+// neg: the pair is passed as (particle 2, particle 1) of the reference's
+// orientation.  K is symmetric in the two particles and IEEE subtraction
+// is antisymmetric (a - b == -(b - a) exactly), so q_out, q_side and q_long
+// of the reference's orientation are exactly the negated ones of the pair
+// as passed; b.q_x .. b.q_E keep the orientation as passed (the phase,
+// a sum of products of two differences, is symmetric).
 template <bool MIXED, bool EXT = false>
 __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, double p1y,
                                             double p1z, double p1E, double p2x, double p2y,
                                             double p2z, double p2E, double psi_ref,
-                                            PairBins &b) {
+                                            PairBins &b, bool neg = false) {
     const double K_z = 0.5 * (p1z + p2z);
     const double K_E = 0.5 * (p1E + p2E);
     const double K_x = 0.5 * (p1x + p2x);
@@ -110,7 +116,8 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
     b.q_E = p1E - p2E;
     const double gq = GUARD_REL * (fabs(b.q_x) + fabs(b.q_y) + 1e-3);
     const double cK = K_x * rK, sK = K_y * rK;
-    const double q_out = b.q_x * cK + b.q_y * sK;
+    double q_out = b.q_x * cK + b.q_y * sK;
+    if (neg) q_out = -q_out;
     int w = guarded_window(q_out, P.q_lo_cut, P.q_hi_cut, gq);
     if (w < 0) return -1;
     if (!w) return st;
@@ -118,7 +125,8 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
     if (b.io < -1) return -1;
     if (b.io >= P.qnpts) return st;
     st |= ST_OUT;
-    const double q_side = b.q_y * cK - b.q_x * sK;
+    double q_side = b.q_y * cK - b.q_x * sK;
+    if (neg) q_side = -q_side;
     w = guarded_window(q_side, P.q_lo_cut, P.q_hi_cut, gq);
     if (w < 0) return -1;
     if (!w) return st;
@@ -135,6 +143,7 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
         q_long = gamma * (b.q_z - beta * b.q_E);
         gl = GUARD_REL * gamma * (fabs(b.q_z) + fabs(b.q_E) + 1e-3);
     }
+    if (neg) q_long = -q_long;
     w = guarded_window(q_long, P.q_lo_cut, P.q_hi_cut, gl);
     if (w < 0) return -1;
     if (!w) return st;
@@ -153,21 +162,25 @@ __device__ __forceinline__ int cascade_fast(const DevParams &P, double p1x, doub
 }
 
 // Same-event pair in the precision form (see eval_same_f64 for the
-// contract).  The q_inv variant and ambiguous pairs use the literal FP64
-// cascade.
+// contract).  The pair is (a, b) in the reference's orientation, or (b, a)
+// when neg is set (see cascade_fast).  The q_inv variant and ambiguous
+// pairs use the literal FP64 cascade, in the reference's orientation.
 template <bool EXT = true>
 __device__ __forceinline__ int eval_same_fast(const DevParams &P, const DevHist &H, double psi,
-                                              const PartRec &p1, const PartRec &p2,
+                                              const PartRec &a, const PartRec &b2, bool neg,
                                               SameRec &rec) {
     PairBins b;
     const int st = (EXT && P.qinv)
                        ? -1
-                       : cascade_fast<false, EXT>(P, p1.px, p1.py, p1.pz, p1.E, p2.px, p2.py,
-                                                  p2.pz, p2.E, psi, b);
-    if (st < 0)
+                       : cascade_fast<false, EXT>(P, a.px, a.py, a.pz, a.E, b2.px, b2.py,
+                                                  b2.pz, b2.E, psi, b, neg);
+    if (st < 0) {
+        const PartRec &p1 = neg ? b2 : a, &p2 = neg ? a : b2;
         return eval_same_f64<EXT>(P, H, psi, p1.px, p1.py, p1.pz, p1.E, p1.x, p1.y, p1.z, p1.t,
                                   p2.px, p2.py, p2.pz, p2.E, p2.x, p2.y, p2.z, p2.t, rec);
+    }
     if (!(st & ST_BIN)) return st;
+    const PartRec &p1 = a, &p2 = b2;
     const double t_diff = p1.t - p2.t;
     const double x_diff = p1.x - p2.x;
     const double y_diff = p1.y - p2.y;

@@ -32,12 +32,14 @@
 namespace hbt {
 
 // Per-particle prefilter record: nrm = px^2+py^2 - mid/2 (so that
-// n1 + n2 + 2 p1.p2 = T - mid), beta = pz/E (or pz without the LCMS boost).
+// n1 + n2 + 2 p1.p2 = T - mid), and blo, the lower end of the particle's
+// beta interval [beta - cw, beta + cw] (beta = pz/E, or pz without the LCMS
+// boost); the upper end bhi is kept in a separate array beside the records.
 struct PfRec64 {
-    double px, py, nrm, beta;
+    double px, py, nrm, blo;
 };
 struct PfRec32 {
-    float px, py, nrm, beta;
+    float px, py, nrm, blo;
 };
 
 // Prefilter constants in the records' precision (FP64 or FP32 set of
@@ -62,19 +64,20 @@ __host__ __device__ __forceinline__ PfConst<Prec> pf_const(const DevParams &P) {
 }
 
 // The prefilter's tests of one pair (for a mixed pair, i rotated by
-// -theta), as four comparisons lhs[k] <= rhs[k]: the K_T window |T - mid|
+// -theta), as five comparisons lhs[k] <= rhs[k]: the K_T window |T - mid|
 // <= half; q_out: dn^2 - c <= H^2 T; q_side: (Q - H^2) T - 2c <= dn^2 - c;
-// beta: |beta_i - beta_j| <= cw_i + cw_j.  The kernel and the host's check
-// of the margins evaluate this same function.
+// beta: |beta_i - beta_j| <= cw_i + cw_j as the overlap of the two beta
+// intervals, blo_i <= bhi_j and blo_j <= bhi_i (no arithmetic).  The kernel
+// and the host's check of the margins evaluate this same function.
 template <typename Prec>
 struct PfTerms {
-    Prec lhs[4], rhs[4];
+    Prec lhs[5], rhs[5];
 };
 
 template <typename Prec>
-__host__ __device__ __forceinline__ PfTerms<Prec> pf_terms(Prec ix, Prec iy, Prec in, Prec ib,
-                                                           Prec icw, Prec jx, Prec jy, Prec jn,
-                                                           Prec jb, Prec jcw,
+__host__ __device__ __forceinline__ PfTerms<Prec> pf_terms(Prec ix, Prec iy, Prec in, Prec ilo,
+                                                           Prec ihi, Prec jx, Prec jy, Prec jn,
+                                                           Prec jlo, Prec jhi,
                                                            const PfConst<Prec> &k) {
     PfTerms<Prec> t;
     const Prec d = fma(ix, jx, iy * jy);
@@ -90,15 +93,17 @@ __host__ __device__ __forceinline__ PfTerms<Prec> pf_terms(Prec ix, Prec iy, Pre
     t.rhs[1] = T * k.h2;
     t.lhs[2] = fma(v + k.midh2, T, k.neg2c);
     t.rhs[2] = ddc;
-    t.lhs[3] = fabs(ib - jb);
-    t.rhs[3] = icw + jcw;
+    t.lhs[3] = ilo;
+    t.rhs[3] = jhi;
+    t.lhs[4] = jlo;
+    t.rhs[4] = ihi;
     return t;
 }
 
 template <typename Prec>
 __host__ __device__ __forceinline__ bool pf_pass(const PfTerms<Prec> &t) {
     return t.lhs[0] <= t.rhs[0] && t.lhs[1] <= t.rhs[1] && t.lhs[2] <= t.rhs[2] &&
-           t.lhs[3] <= t.rhs[3];
+           t.lhs[3] <= t.rhs[3] && t.lhs[4] <= t.rhs[4];
 }
 
 // Same-event pair orientation: for a position in the cell-sorted list, the
@@ -121,10 +126,11 @@ struct View {
     const void *pfA;     // PfRec64 (FP64 kernel) or PfRec32 (precision form) of A
     const void *pfB;     // of B
     const void *pfS;     // of the same-event fast list sorted by (px cell, py) per group
-    const void *cwA;     // per-particle c/E bound of A, double or float like pfA
+    const void *cwA;     // per-particle upper end bhi of the beta interval of A, like pfA
     const void *cwB;
     const void *cwS;
     const OrigRank *origS;     // cell-sorted position -> fast-list index and gather rank
+    const PartRec *AS;         // records of the same-event fast list in the cell-sorted order
     const JWin *mwin;          // partner windows of the mixed items
     const double *plan_cs;     // cos, sin of each plan row's rotation
     const double *psi;         // psi_ref per group
@@ -134,6 +140,24 @@ struct View {
     // instructions
     DevParams P;
 };
+
+// The cell-sorted copy of the same-event records, AS[q] = A[origS[q].idx],
+// built on the device once the lists are resident.
+static __global__ __launch_bounds__(256) void k_gather_srt(const PartRec *__restrict__ A,
+                                                           const OrigRank *__restrict__ orr,
+                                                           PartRec *__restrict__ AS,
+                                                           long long nf) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nf) AS[q] = A[orr[q].idx];
+}
+
+static inline hipError_t gather_srt(const PartRec *A, const OrigRank *orr, PartRec *AS,
+                                    long long nf, hipStream_t st) {
+    if (nf <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_srt, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, st, A,
+                       orr, AS, nf);
+    return hipGetLastError();
+}
 
 // Work queues of a launch: at most one per XCD.  Queue q holds items
 // [qoff[q], qoff[q+1]); its counter is counter[kQueueStride * q].
@@ -226,12 +250,12 @@ struct Exact<double, EXT> {
 
 template <bool EXT>
 struct Exact<float, EXT> {
+    // The records come from the cell-sorted copy, so their loads do not
+    // wait for the gather ranks; the ranks only orient the pair.
     static __device__ __forceinline__ int same(const DevParams &P, const View &v, double psi,
                                                long long i, long long j, SameRec &rec) {
-        const OrigRank oi = v.origS[i], oj = v.origS[j];
-        const bool fwd = oi.rank < oj.rank;
-        return eval_same_fast<EXT>(P, v.H, psi, v.A[fwd ? oi.idx : oj.idx],
-                                   v.A[fwd ? oj.idx : oi.idx], rec);
+        const int ri = v.origS[i].rank, rj = v.origS[j].rank;
+        return eval_same_fast<EXT>(P, v.H, psi, v.AS[i], v.AS[j], rj < ri, rec);
     }
     static __device__ __forceinline__ int mixed(const DevParams &P, const View &v, double psi,
                                                 long long i, long long j, double c, double s,
@@ -436,11 +460,11 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
         // same-event tiles run over the cell-sorted list, mixed ones over the
         // |pT|-sorted fast lists
         const Rec *__restrict__ pfi = (const Rec *)(mixed ? v.pfA : v.pfS);
-        const Prec *__restrict__ cwi = (const Prec *)(mixed ? v.cwA : v.cwS);
+        const Prec *__restrict__ hii = (const Prec *)(mixed ? v.cwA : v.cwS);
         const Rec *pfj = (const Rec *)(mixed ? v.pfB : v.pfS);
-        const Prec *cwj = (const Prec *)(mixed ? v.cwB : v.cwS);
+        const Prec *hij = (const Prec *)(mixed ? v.cwB : v.cwS);
         // i particles, R per lane
-        Prec ix[R], iy[R], in[R], ib[R], icw[R], rx[R], ry[R];
+        Prec ix[R], iy[R], in[R], ilo[R], ihi[R], rx[R], ry[R];
         unsigned long long vmask[R];
         unsigned slot[R];
 #pragma unroll
@@ -452,8 +476,8 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
             ix[r] = rx[r] = p.px;
             iy[r] = ry[r] = p.py;
             in[r] = p.nrm;
-            ib[r] = p.beta;
-            icw[r] = cwi[i];
+            ilo[r] = p.blo;
+            ihi[r] = hii[i];
         }
         const bool partial = w.ni < 64 * R;
         const int nseg = mixed ? w.nj : 1;
@@ -488,7 +512,7 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                 int tri = (int)min(jc - w.i0, 1ll << 20);
                 unsigned ebase = segbits | ((unsigned)(jc - pbase) << 8);
                 const auto *cpf = HBT_CONST(Rec, pfj + jc);
-                const auto *ccw = HBT_CONST(Prec, cwj + jc);
+                const auto *chi = HBT_CONST(Prec, hij + jc);
                 int t = 0;
                 for (;;) {
                     if (tail - head >= 64 || (last && t >= cnt && tail != head)) {
@@ -506,16 +530,17 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                     }
                     if (t >= cnt) break;
                     // one j against the wave's 64R i particles
-                    auto pair_step = [&](int u, const Rec &jr, Prec jcw) {
+                    auto pair_step = [&](int u, const Rec &jr, Prec jhi) {
 #pragma unroll
                         for (int r = 0; r < R; r++) {
                             const PfTerms<Prec> pt =
-                                pf_terms(rx[r], ry[r], in[r], ib[r], icw[r], jr.px, jr.py,
-                                         jr.nrm, jr.beta, jcw, pk);
+                                pf_terms(rx[r], ry[r], in[r], ilo[r], ihi[r], jr.px, jr.py,
+                                         jr.nrm, jr.blo, jhi, pk);
                             unsigned long long m = le_mask(pt.lhs[0], pt.rhs[0]) &
                                                    le_mask(pt.lhs[1], pt.rhs[1]) &
                                                    le_mask(pt.lhs[2], pt.rhs[2]) &
-                                                   le_mask(pt.lhs[3], pt.rhs[3]);
+                                                   le_mask(pt.lhs[3], pt.rhs[3]) &
+                                                   le_mask(pt.lhs[4], pt.rhs[4]);
                             if (partial) m &= vmask[r];
                             if (diag) m &= low_lanes32(tri + u - r * 64);
                             if (m) {
@@ -534,14 +559,14 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
 #pragma unroll
                         for (int u = 0; u < JU; u++) {
                             jr[u] = cpf[t + u];
-                            jw[u] = ccw[t + u];
+                            jw[u] = chi[t + u];
                         }
 #pragma unroll
                         for (int u = 0; u < JU; u++) pair_step(u, jr[u], jw[u]);
                     } else {
 #pragma unroll
                         for (int u = 0; u < JU; u++)
-                            if (t + u < cnt) pair_step(u, cpf[t + u], ccw[t + u]);
+                            if (t + u < cnt) pair_step(u, cpf[t + u], chi[t + u]);
                     }
                     t += JU;
                     tri += JU;

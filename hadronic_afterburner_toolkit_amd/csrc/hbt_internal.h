@@ -112,26 +112,35 @@ inline int derive(const hbt_config &k, Derived &d, std::string &err) {
     return HBT_OK;
 }
 
-// Prefilter record of one particle (px, py, nrm = |pT|^2 - mid/2, beta)
-// and its beta bound cw = (H/2) / E (or H/2 without the LCMS boost; +inf
-// with the q_inv branch on).  FP32 records: cw covers the rounding of beta
-// (|beta| <= bm over the list) and of cw itself.
+// Prefilter record of one particle: px, py, nrm = |pT|^2 - mid/2, and its
+// beta interval [blo, bhi] = beta -/+ cw with cw = (H/2) / E (or H/2
+// without the LCMS boost; infinite with the q_inv branch on): the beta test
+// |beta1 - beta2| <= cw1 + cw2 holds exactly when blo1 <= bhi2 and
+// blo2 <= bhi1.  The ends are rounded outward (blo down, bhi up), so the
+// two comparisons pass every pair the real-valued test passes.  FP32
+// records: cw also covers the rounding of beta (|beta| <= bm over the list)
+// and of cw itself.
 inline void pf_record(const Derived &d, double px, double py, double pz, double E, double bm,
-                      PfRec64 &r, double &cw) {
+                      PfRec64 &r, double &bhi) {
     const DevParams &P = d.P;
-    r = PfRec64{px, py, px * px + py * py - 0.5 * d.mid, P.boost ? pz / E : pz};
-    cw = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
+    const double beta = P.boost ? pz / E : pz;
+    const double cw = P.qinv ? INFINITY : (P.boost ? d.c_beta / E : d.c_beta);
+    r = PfRec64{px, py, px * px + py * py - 0.5 * d.mid, std::nextafter(beta - cw, -INFINITY)};
+    bhi = std::nextafter(beta + cw, INFINITY);
 }
 
 inline void pf_record(const Derived &d, double px, double py, double pz, double E, double bm,
-                      PfRec32 &r, float &cw) {
+                      PfRec32 &r, float &bhi) {
     const double eps32 = 5.9604644775390625e-8;
-    PfRec64 r64;
-    double cw64;
-    pf_record(d, px, py, pz, E, bm, r64, cw64);
-    r = PfRec32{(float)r64.px, (float)r64.py, (float)r64.nrm, (float)r64.beta};
-    cw = d.P.qinv ? INFINITY
-                  : (float)(cw64 * (1.0 + 8 * eps32) + 4 * eps32 * std::max(1.0, bm));
+    const DevParams &P = d.P;
+    const double beta = P.boost ? pz / E : pz;
+    const double cw = P.qinv ? INFINITY
+                             : (P.boost ? d.c_beta / E : d.c_beta) * (1.0 + 8 * eps32) +
+                                   4 * eps32 * std::max(1.0, bm);
+    // to FP32 rounded to nearest, then one FP32 ulp outward
+    r = PfRec32{(float)px, (float)py, (float)(px * px + py * py - 0.5 * d.mid),
+                std::nextafter((float)(beta - cw), -INFINITY)};
+    bhi = std::nextafter((float)(beta + cw), INFINITY);
 }
 
 // Prefilter margins that depend on the data's momentum scale (|pT|^2 up to

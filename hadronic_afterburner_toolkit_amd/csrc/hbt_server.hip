@@ -128,7 +128,7 @@ struct Part {
     double pmax2 = 0.0;
     // device residency
     size_t foot = 0, arena_off = 0;
-    size_t o_rec = 0, o_pf = 0, o_cw = 0, o_spf = 0, o_scw = 0, o_orig = 0, o_win = 0,
+    size_t o_rec = 0, o_srec = 0, o_pf = 0, o_cw = 0, o_spf = 0, o_scw = 0, o_orig = 0, o_win = 0,
            o_cs = 0, o_psi = 0;
     int view = -1;
     View v{};
@@ -234,9 +234,11 @@ size_t part_layout(const hbt_server *s, Part *p, long long nf, long long rows, s
     Part tmp;
     Part &r = p ? *p : tmp;
     size_t o = 0;
-    // fast-list records, their prefilter records, and the px-sorted copy
-    r.o_rec = o;
-    o = align256(o + (size_t)nf * sizeof(PartRec));
+    // fast-list records and their cell-sorted copy, the prefilter records of both
+    for (size_t *po : {&r.o_rec, &r.o_srec}) {
+        *po = o;
+        o = align256(o + (size_t)nf * sizeof(PartRec));
+    }
     for (size_t *po : {&r.o_pf, &r.o_spf}) {
         *po = o;
         o = align256(o + (size_t)nf * (s->f32 ? sizeof(PfRec32) : sizeof(PfRec64)));
@@ -279,6 +281,7 @@ void make_part_view(hbt_server *s, Part &p) {
     v.pfS = (const void *)(b + p.o_spf);
     v.cwS = (const void *)(b + p.o_scw);
     v.origS = (const OrigRank *)(b + p.o_orig);
+    v.AS = (const PartRec *)(b + p.o_srec);
     v.mwin = (const JWin *)(b + p.o_win);
     v.plan_cs = (const double *)(b + p.o_cs);
     v.psi = (const double *)(b + p.o_psi);
@@ -319,6 +322,9 @@ hipError_t upload_part(hbt_server *s, Part &p) {
     // (its upload ended with a synchronization of this stream)
     if (e == hipSuccess) e = ensure_pinned(&s->h_stage, &s->stage_cap, stage_bytes(pieces));
     if (e == hipSuccess) e = stage_pieces(s->h_stage, pieces, s->s_upload);
+    if (e == hipSuccess)
+        e = gather_srt((const PartRec *)(b + p.o_rec), (const OrigRank *)(b + p.o_orig),
+                       (PartRec *)(b + p.o_srec), nf, s->s_upload);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s->d_views + p.view, &p.v, sizeof(View), hipMemcpyHostToDevice,
                            s->s_upload);

@@ -19,7 +19,8 @@ pt_window (the mixed items' partner window, H with margins).
 The program is compiled for the host only with hipcc.  Run with the
 data-scaled margins removed and the |p_T| window narrowed by 1e-3 (the
 planted mistakes), the FP32 prefilter must drop binnable pairs and binnable
-pairs must fall outside the window: the checks have teeth."""
+pairs must fall outside the window; run with the beta intervals narrowed
+by 0.2 %, both prefilters must drop binnable pairs: the checks have teeth."""
 import shutil
 import subprocess
 from pathlib import Path
@@ -41,20 +42,28 @@ using namespace hbt;
 
 struct P4 { double px, py, pz, E; };
 
-template <typename Rec, typename CW, typename Prec>
-static bool passes(const DevParams &P, const Rec &a, CW acw, const Rec &b, CW bcw, double c,
+template <typename Rec, typename HI, typename Prec>
+static bool passes(const DevParams &P, const Rec &a, HI ahi, const Rec &b, HI bhi, double c,
                    double s, bool mixed) {
     Prec ix = a.px, iy = a.py;
     if (mixed) {  // the kernel rotates i by -theta
         ix = (Prec)((double)a.px * c + (double)a.py * s);
         iy = (Prec)((double)a.py * c - (double)a.px * s);
     }
-    return pf_pass(pf_terms<Prec>(ix, iy, a.nrm, a.beta, acw, b.px, b.py, b.nrm, b.beta, bcw,
+    return pf_pass(pf_terms<Prec>(ix, iy, a.nrm, a.blo, ahi, b.px, b.py, b.nrm, b.blo, bhi,
                                   pf_const<Prec>(P)));
 }
 
 int main(int argc, char **argv) {
     const bool plant = argc > 1 && std::strcmp(argv[1], "plant") == 0;
+    // the second planted mistake: beta intervals narrowed by 0.2 %
+    const bool plant_beta = argc > 1 && std::strcmp(argv[1], "plantbeta") == 0;
+    auto narrow = [&](auto &r, auto &hi) {
+        if (!plant_beta) return;
+        const auto w = (hi - r.blo) * 1e-3f;
+        r.blo += w;
+        hi -= w;
+    };
     std::mt19937_64 rng(31);
     std::uniform_real_distribution<double> U(0.0, 1.0);
     struct Cfg { int qnpts; double qmin, qmax, ktmin, ktmax; int boost; };
@@ -124,6 +133,10 @@ int main(int argc, char **argv) {
                 pf_record(d, b.px, b.py, b.pz, b.E, bm, rb32, cb32);
                 pf_record(d, a.px, a.py, a.pz, a.E, bm, ra64, ca64);
                 pf_record(d, b.px, b.py, b.pz, b.E, bm, rb64, cb64);
+                narrow(ra32, ca32);
+                narrow(rb32, cb32);
+                narrow(ra64, ca64);
+                narrow(rb64, cb64);
                 PairBins bins;
                 double qinv;
                 int ok;
@@ -155,6 +168,8 @@ int main(int argc, char **argv) {
                     PfRec64 ru64; double cu64;
                     pf_record(d, ux, uy, b.pz, b.E, bm, ru32, cu32);
                     pf_record(d, ux, uy, b.pz, b.E, bm, ru64, cu64);
+                    narrow(ru32, cu32);
+                    narrow(ru64, cu64);
                     dropped32 += !passes<PfRec32, float, float>(P, ra32, ca32, ru32, cu32, c, s, true);
                     dropped64 += !passes<PfRec64, double, double>(P, ra64, ca64, ru64, cu64, c, s, true);
                 }
@@ -196,3 +211,6 @@ def test_prefilter_never_drops_a_binnable_pair(tmp_path):
     # the planted mistake: no data-scaled margin
     binned_p, d32_p, _, outside_p = run(exe, "plant")
     assert binned_p == binned and d32_p > 0 and outside_p > 0
+    # the beta intervals narrowed by 0.2 %: binnable pairs at the q_long edge are dropped
+    binned_b, d32_b, d64_b, _ = run(exe, "plantbeta")
+    assert binned_b == binned and d32_b > 0 and d64_b > 0
