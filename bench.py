@@ -223,18 +223,21 @@ def batch_line(groups, params, seed, prec, steps, tuning, autotune, log, label):
     # binned pairs per launch (= atomic accesses), from the kernel's counters
     sc, mc = g.stage_counts()
     launches = max(1, int(sc[0]) // es if es else int(mc[0]) // max(1, em))
+    slab = g.slab_info()  # slab binning or device-scope atomics (the autotuner's pick)
     g.close()
     t = float(np.mean(ms)) * 1e-3
     log(f"  {label}: {es + em:.3e} pairs in {t * 1e3:.1f} ms -> {(es + em) / t:.3e} pairs/s")
     return dict(pairs=es + em, same_pairs=es, mixed_pairs=em, ms=t * 1e3,
-                pairs_per_s=(es + em) / t, tuning=tune,
+                pairs_per_s=(es + em) / t, tuning=tune, slab=slab,
                 binned_same=int(sc[4]) // launches, binned_mixed=int(mc[4]) // launches), res
 
 
 def add_atomic_floor(line, rates, nbins=None):
     """The launch's atomic floor: binned pairs over the measured rates, those
     of the micro-benchmark's histogram size nearest to nbins (in ratio) when
-    it measured several."""
+    it measured several.  It is the floor of binning every pair with
+    device-scope atomics; a launch with slab binning (line["slab"]["on"])
+    adds only its mixed pairs that way and can run below it."""
     if not rates:
         return
     same, mixed, at = rates["same"], rates["mixed"], BENCH_BINS
@@ -256,7 +259,8 @@ def batched_uneven(args, log):
     (step 0.1), the reference's partner rule (11 per event)."""
     from hadronic_afterburner_toolkit_amd import _lib, samples
     full = samples.generate(args.seed + 20, 512, samples.log_uniform_mult(20, 4000), species=PI_K)
-    params = dict(PARAMS, azimuthal_flag=1, n_Kphi=16, number_of_mixed_events=0)
+    # binning 2: the autotuner times slab binning against atomics
+    params = dict(PARAMS, azimuthal_flag=1, n_Kphi=16, number_of_mixed_events=0, hbt_binning=2)
     out = {"events": 512, "per_group": 20}
     tot_pairs = tot_t = 0.0
     for monval, name in ((211, "pi+"), (321, "K+")):
@@ -269,6 +273,22 @@ def batched_uneven(args, log):
         tot_pairs += line["pairs"]
         tot_t += line["ms"] * 1e-3
     out["pairs_per_s"] = tot_pairs / tot_t
+    return out
+
+
+def kphi_binning(args, log, groups):
+    """The headline workload analysed in 16 K_phi bins as well (6 x 16 x 41
+    = 3936 slabs), the precision form binned with device-scope atomics and
+    with slab binning, at one fixed launch shape (R 1, jchunk 256, kchunk 4,
+    4 blocks per CU) so that the two differ only in the binning."""
+    from hadronic_afterburner_toolkit_amd import _lib
+    out = {}
+    for b, name in ((0, "atomics"), (1, "slab")):
+        params = dict(PARAMS, azimuthal_flag=1, n_Kphi=16, hbt_binning=b)
+        out[name], _ = batch_line(groups, params, args.seed + 3, _lib.HBT_PREC_FP32C,
+                                  max(1, args.steps), (1, 256, 4, 4), False, log,
+                                  f"K_phi headline, {name}")
+    out["slab_speedup"] = out["atomics"]["ms"] / out["slab"]["ms"]
     return out
 
 
@@ -821,9 +841,11 @@ def main():
         out["batched_uneven"] = batched_uneven(args, log)
         out["large_histogram"] = large_histogram(
             args, log, (t["R"], t["jchunk"], t["kchunk"], t["blocks_per_cu"]))
+        out["kphi_binning"] = kphi_binning(args, log, groups)
         for line, nb in ((out["batched_uneven"]["pi+"], BENCH_BINS * 16),
                          (out["batched_uneven"]["K+"], BENCH_BINS * 16),
-                         (out["large_histogram"], out["large_histogram"]["bins"])):
+                         (out["large_histogram"], out["large_histogram"]["bins"]),
+                         (out["kphi_binning"]["atomics"], BENCH_BINS * 16)):
             add_atomic_floor(line, atom_ceil, nb)
     if not args.no_extra:
         out["ingest"] = ingest(args, log)

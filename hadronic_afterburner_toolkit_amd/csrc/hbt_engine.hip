@@ -632,12 +632,14 @@ int slab_read_stats(hbt_ctx *c) {
 // beyond, regions shrink and more pairs take the direct path.
 constexpr double kSlabBudget = 16e9;
 
-// Slab binning needs a 3D histogram (no K_phi, no q_inv) whose slabs fit
-// the fast kernel's LDS counters and k_slab_bin's LDS.
+// Slab binning needs a histogram without the q_inv branch whose slabs (K_T
+// x K_phi x q_out planes) fit the fast kernel's LDS fill counters and whose
+// q_side x q_long planes fit k_slab_bin's LDS.
 bool slab_applies(const hbt_ctx *c) {
     const int q = c->P.qnpts;
-    return c->cfg.kernel == HBT_KERNEL_FAST && !c->P.qinv && !c->P.azimuthal &&
-           c->cfg.n_KT * c->P.nphi * q <= kMaxSlabs && q * q <= kMaxSlabBins;
+    return c->cfg.kernel == HBT_KERNEL_FAST && !c->P.qinv &&
+           c->cfg.n_KT * c->P.nphi * q <= (c->P.azimuthal ? kMaxSlabsKphi : kMaxSlabs) &&
+           q * q <= kMaxSlabBins;
 }
 
 // k_slab_bin's units: about four per resident block (two 72 KB blocks per
@@ -1426,8 +1428,11 @@ namespace hbt {
 hipError_t launch_fast_kernel(const FastArgs &a, bool f32, int R, bool ext, long blocks,
                               hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, a.counter, a.nqueues);
+    // the K_phi instance's slab fill counters live in dynamic LDS
+    const size_t dyn = ext && a.slab.vals ? (size_t)a.slab.S * sizeof(unsigned) : 0;
+    if (dyn > (size_t)kMaxSlabsKphi * sizeof(unsigned)) return hipErrorInvalidValue;
 #define HBT_LAUNCH(T, RR, X) \
-    hipLaunchKernelGGL((k_fast<T, RR, X>), dim3(blocks), dim3(256), 0, st, a)
+    hipLaunchKernelGGL((k_fast<T, RR, X>), dim3(blocks), dim3(256), dyn, st, a)
 #define HBT_LAUNCH_R(T, X)                    \
     do {                                      \
         if (R == 1) HBT_LAUNCH(T, 1, X);      \

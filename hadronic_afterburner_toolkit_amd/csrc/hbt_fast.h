@@ -323,7 +323,8 @@ __device__ __forceinline__ int drain(const DevParams &P, const View &v, const un
     // slab binning: a same-event pair becomes a record of its (block, slab)
     // region; a full region, or stored values that would land in another
     // bin, send it down the direct path.  Mixed pairs are always direct.
-    const bool slab = !EXT && sl.vals != nullptr;
+    // (The host turns slab binning off for q_inv analyses.)
+    const bool slab = sl.vals != nullptr;
     if (w.kind != 0) {
         if (binned) add_mixed_direct(v.H, kmix);
         return binned ? 1 : 0;
@@ -420,20 +421,29 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
     constexpr int QCAP = Ring<R>::CAP;
     __shared__ unsigned ring[4][QCAP];
     __shared__ Stage stage[4];
-    // slab binning: records per slab of this block, and the regions'
-    // capacity and first record per slab
-    __shared__ unsigned slab_fill[EXT ? 1 : kMaxSlabs];
+    // slab binning: records per slab of this block, and (common-case
+    // instances) the regions' capacity and first record per slab; the K_phi
+    // instance keeps its up to kMaxSlabsKphi fill counters in the launch's
+    // dynamic LDS (none when it bins with atomics) and reads capacity and
+    // first record from SlabArgs
+    extern __shared__ unsigned slab_fill_dyn[];
+    __shared__ unsigned slab_fill_st[EXT ? 1 : kMaxSlabs];
+    unsigned *const slab_fill = EXT ? slab_fill_dyn : slab_fill_st;
     __shared__ int slab_cap[EXT ? 1 : kMaxSlabs];
     __shared__ long long slab_base[EXT ? 1 : kMaxSlabs];
-    const bool slab = !EXT && a.slab.vals != nullptr;
+    const bool slab = a.slab.vals != nullptr;
     if (slab) {
         for (int k = threadIdx.x; k < a.slab.S; k += blockDim.x) {
             slab_fill[k] = 0u;
-            slab_cap[k] = a.slab.scap[k];
-            slab_base[k] = a.slab.sbase[k];
+            if (!EXT) {
+                slab_cap[k] = a.slab.scap[k];
+                slab_base[k] = a.slab.sbase[k];
+            }
         }
         __syncthreads();
     }
+    const int *scap = EXT ? a.slab.scap : slab_cap;
+    const long long *sbase = EXT ? a.slab.sbase : slab_base;
     unsigned direct_n = 0;
     const int lane = __lane_id();
     unsigned *q = ring[threadIdx.x >> 6];
@@ -519,8 +529,8 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                         const int n = (int)min(64u, tail - head);
                         __builtin_amdgcn_wave_barrier();
                         const int nb = drain<Prec, EXT, QCAP>(v.P, v, q, stg, head, n, w, psi,
-                                                              a.slab, slab_fill, slab_cap,
-                                                              slab_base, direct_n);
+                                                              a.slab, slab_fill, scap, sbase,
+                                                              direct_n);
                         if (mixed)
                             nb_mixed += nb;
                         else

@@ -10,7 +10,7 @@
 // nothing to combine inside one block.  Instead, for same-event pairs:
 //
 //  1. k_fast writes each binned pair as a record into the region of its
-//     (block, slab): a slab is one (K_T, q_out) plane of qnpts^2 bins; the
+//     (block, slab): a slab is one (K_T, K_phi, q_out) plane of qnpts^2 bins; the
 //     block's LDS fill counter of the slab gives the record's place, so no
 //     global atomic is involved.  A record is the pair's q_out, q_side,
 //     q_long and cos: one 16-byte store (FP32, the precision form) or 32
@@ -42,6 +42,11 @@
 namespace hbt {
 
 constexpr int kMaxSlabs = 512;     // slabs per analysis (LDS fill counters of k_fast)
+// slabs of a K_phi analysis: its instance of k_fast keeps only the fill
+// counters in LDS, in the launch's dynamic LDS (S x 4 bytes, at most 16 KB;
+// capacity and first record come from SlabArgs), enough for 6 K_T x 16 K_phi
+// x 41 q_out planes
+constexpr int kMaxSlabsKphi = 4096;
 constexpr int kMaxSlabBins = 2048;  // bins per slab (LDS of k_slab_bin)
 constexpr int kMaxUnitRecords = (1 << 23) - 1;  // records of one k_slab_bin unit
 
@@ -81,9 +86,9 @@ __device__ __forceinline__ bool fix_exact(float v) {
 
 // A binned same-event pair r as a record of this block's region of slab
 // r.s; false: the pair takes the direct path (also for a precision-form
-// value the fixed point would not hold exactly).  fill, cap and base are the
-// block's LDS copies of its fill counters and of the regions' capacity and
-// first record (cap and base as in SlabArgs).
+// value the fixed point would not hold exactly).  fill: the block's LDS fill
+// counters; cap and base: the regions' capacity and first record (as in
+// SlabArgs), LDS copies or SlabArgs' own arrays.
 template <typename Prec>
 __device__ __forceinline__ bool slab_put_same(const SlabArgs &a, unsigned *fill, const int *cap,
                                               const long long *base, const SameRec &r) {

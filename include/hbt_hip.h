@@ -74,8 +74,9 @@ typedef struct {
     int32_t kphi_guard_prad;
     /* histogram accumulation of the fast kernel: 0 = every binned pair added
      * to the histogram with device-scope atomics; 1 = slab binning where it
-     * applies (no K_phi, no q_inv, qnpts^2 <= 2048, n_KT * qnpts <= 512):
-     * binned same-event pairs written as records per (block, q_out plane)
+     * applies (no q_inv, qnpts^2 <= 2048, and n_KT * qnpts <= 512, or with
+     * K_phi bins n_KT * n_Kphi * qnpts <= 4096): binned same-event pairs
+     * written as records per (block, K_T x K_phi x q_out plane)
      * and summed in LDS, without device-scope atomics (mixed pairs still add
      * their u32 directly); 2 = the one hbt_autotune measures faster on the
      * prepared batch, timing every launch shape both ways (atomics until

@@ -120,18 +120,23 @@ def oracle_groups(groups, params, seed):
     return o.arrays()
 
 
+@pytest.mark.parametrize("binning", [0, 1])
 @pytest.mark.parametrize("prec,tol", [(_lib.HBT_PREC_FP64, 1e-12), (_lib.HBT_PREC_FP32C, 1e-6)])
-def test_wide_guard_batched(prec, tol):
+def test_wide_guard_batched(prec, tol, binning):
+    """Also with slab binning (binning 1): the K_phi instance writes slab
+    records for the pairs it bins while it sends the guard-band pairs to the
+    host, in the same launch."""
     full = samples.generate(71, 40, samples.log_uniform_mult(30, 400), species=PI)
     groups = [full.events(g, 10) for g in range(0, 40, 10)]
-    params = dict(KPHI_PARAMS, hbt_kphi_guard_prad=WIDE)
-    print(f"K_phi guard band, batched launch, precision {prec}", flush=True)
+    params = dict(KPHI_PARAMS, hbt_kphi_guard_prad=WIDE, hbt_binning=binning)
+    print(f"K_phi guard band, batched launch, precision {prec}, binning {binning}", flush=True)
     g = hbt.HBTCorrelation(params, precision=prec, kernel=_lib.HBT_KERNEL_FAST)
     g.calculate_groups(groups, hbt.Random(7))
     r = g.results()
     # the precision form decides K_phi with FP32 atan2 outside its own 4e-6 rad
     # band, so only the FP64 form is sure to send pairs to the host here
     assert prec != _lib.HBT_PREC_FP64 or g.deferred_pairs() > 0
+    assert g.slab_info()["on"] == (binning == 1)
     g.close()
     a = oracle_groups(groups, KPHI_PARAMS, 7)
     assert np.array_equal(r.num_count, a["num_count"].astype(np.uint64))

@@ -14,7 +14,8 @@ and the q_inv cap), whose admitted pairs depend on the enumeration order;
 those calls run the engine's ordered path.  The fast forms also run with
 slab binning (hbt_binning = 1), buffer after buffer through
 calculate_HBT_correlation_function, each launch's record regions sized from
-the previous buffer's demand (the K_phi and q_inv cases bin with atomics).
+the previous buffer's demand (the q_inv cases bin with atomics; the K_phi
+cases write records per K_T x K_phi x q_out plane).
 """
 import numpy as np
 import pytest
