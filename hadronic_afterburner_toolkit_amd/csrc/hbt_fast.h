@@ -188,6 +188,22 @@ struct FastArgs {
 #define HBT_LOAD_VIEW(tab, k) ((tab)[k])
 #endif
 
+// The view table entry k, read anew where this is called: the empty asm
+// hides the table's address from the optimiser, so the loads that follow
+// cannot be hoisted out of the caller's loop.  For the exact stage, whose
+// grid constants and pointers would otherwise be loaded once per work item
+// and held in (spilled) scalar registers across the whole j loop.
+__device__ __forceinline__ View view_fresh(const View *views, int k) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __attribute__((address_space(4))) View *t =
+        (const __attribute__((address_space(4))) View *)views + k;
+    __asm__ volatile("" : "+s"(t));
+    return *t;
+#else
+    return views[k];
+#endif
+}
+
 // ------------------------------------------------------- wave helpers
 
 // Mask of lanes where a <= b (ordered: NaN compares false), as one SGPR pair.
@@ -528,7 +544,8 @@ __attribute__((amdgpu_waves_per_eu(fast_waves_per_eu<Prec, R, EXT>()))) void k_f
                     if (tail - head >= 64 || (last && t >= cnt && tail != head)) {
                         const int n = (int)min(64u, tail - head);
                         __builtin_amdgcn_wave_barrier();
-                        const int nb = drain<Prec, EXT, QCAP>(v.P, v, q, stg, head, n, w, psi,
+                        const View vd = view_fresh(a.views, w.view);
+                        const int nb = drain<Prec, EXT, QCAP>(vd.P, vd, q, stg, head, n, w, psi,
                                                               a.slab, slab_fill, scap, sbase,
                                                               direct_n);
                         if (mixed)
