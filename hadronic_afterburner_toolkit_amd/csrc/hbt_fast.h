@@ -414,10 +414,12 @@ struct Ring {
 // whenever 64 are queued they are binned, one per lane, and the last
 // survivors of the item are flushed at its final step (the exact cascade
 // has a single call site).
-// Occupancy: the exact stage is latency-bound (dependent loads of the two
-// particle records per lane), so the kernels trade a few spilled registers
-// for more waves per SIMD: common case 5 at R=1 (96 VGPRs) and 4 at R=2
-// (128 VGPRs), 5 % faster on the bench workload.  The K_phi / q_inv
+// Occupancy: the exact stage is latency-bound (the particle records' loads
+// and its FP64 chain), so the kernels trade a few spilled registers for
+// more waves per SIMD: common case 6 at R=1 (80 VGPRs; the LDS of six
+// blocks fills the CU's 160 KB) and 4 at R=2 (128 VGPRs).  At R=1, 6 waves
+// ran the precision form's bench step 3 % faster than 5, and 5 ran it 5 %
+// faster than 4.  The K_phi / q_inv
 // instances carry the literal cascade with atan2 and the host-defer path
 // inline (168 VGPRs unconstrained): the precision form's, which reaches it
 // only for guard-band pairs, runs at 4 (128 VGPRs, spills in that rare
@@ -426,7 +428,7 @@ struct Ring {
 template <typename Prec, int R, bool EXT>
 constexpr int fast_waves_per_eu() {
     return R == 4 ? 1
-                  : (EXT ? (std::is_same<Prec, float>::value ? 4 : 3) : (R == 1 ? 5 : 4));
+                  : (EXT ? (std::is_same<Prec, float>::value ? 4 : 3) : (R == 1 ? 6 : 4));
 }
 
 template <typename Prec, int R, bool EXT>
