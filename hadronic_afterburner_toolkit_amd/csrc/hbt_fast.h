@@ -5,9 +5,11 @@
 // Why: the reference's cascade exits early for ~98% of pairs
 // (HBT_correlation.cpp:319-398).  On a 64-wide wave, lanes that exit early
 // idle while any lane continues, so a direct port pays the whole cascade
-// (sqrt, three to five divisions, cos) for almost every pair.  Here every
-// pair costs ~10 VALU operations; only survivors (~2-4%) reach the exact
-// cascade, and they do so 64 at a time with all lanes busy.
+// (sqrt, three to five divisions, cos) for almost every pair.  Here the
+// work items' windows leave out most pairs unevaluated, every evaluated pair
+// costs 16 VALU operations, and only its survivors (~1.4% of the enumerated
+// pairs on the bench workload) reach the exact cascade, 64 at a time with
+// all lanes busy.
 //
 // Prefilter (all conservative: a pair the exact cascade accepts always
 // passes; the margins exceed the rounding error of both computations):
