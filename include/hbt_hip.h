@@ -307,8 +307,10 @@ typedef struct {
 int hbt_get_tuning(hbt_ctx *ctx, hbt_tuning *t);
 int hbt_set_tuning(hbt_ctx *ctx, const hbt_tuning *t);
 
-/* Autotuner: times the prepared batch over a grid of launch shapes and keeps
- * the fastest (written to *best when not NULL).  Clears the histograms. */
+/* Autotuner: times the prepared batch over a grid of launch shapes (one
+ * launch each after a warm-up), times its four fastest again (five launches
+ * each) and keeps the lowest median (written to *best when not NULL, ms =
+ * that median).  Clears the histograms. */
 int hbt_autotune(hbt_ctx *ctx, hbt_tuning *best);
 
 /* ---------------- request server (continuous batching) ----------------
