@@ -13,6 +13,8 @@ inputs and plan:
   - sums: err = |S - S_ref| / max(|S_ref|, sum|.|_bin) within 1e-12 (FP64)
     and 1e-6 (precision form); the max and RMS of err over the bins with
     pairs are printed for every field.
+The same group is also analysed in 16 K_phi bins (atomics and slab binning,
+also the K_T x K_phi counters bit-exact).
 """
 import numpy as np
 import pytest
@@ -82,3 +84,48 @@ def test_headline_group_vs_oracle(headline_group, mode):
         report.append(f"{key} max {mx:.2e} rms {rms:.2e}")
         assert mx <= tol, (key, mx)
     print("  per-bin error: " + "; ".join(report), flush=True)
+
+
+# The same group analysed in 16 K_phi bins about its event plane: slab
+# binning with one slab per (K_T, K_phi, q_out) plane (6 x 16 x 41 = 3936),
+# and device-scope atomics, against the oracle (also the K_T x K_phi
+# counters, bit-exact).
+PARAMS_KPHI = dict(PARAMS, azimuthal_flag=1, n_Kphi=16)
+MODES_KPHI = {k: MODES[k] for k in ("fast_fp32c", "fast_fp64_slab", "fast_fp32c_slab")}
+
+
+@pytest.fixture(scope="module")
+def headline_group_kphi(oracle_mod, headline_group):
+    g0, (M, ids, cs), _ = headline_group
+    print("headline group 0 with 16 K_phi bins; oracle ...", flush=True)
+    o = oracle_mod.Oracle(PARAMS_KPHI)
+    o.set_psi_ref(oracle_mod.psi_ref(g0, 2))  # the group's event plane, as per buffer
+    o.combine_same(g0)
+    o.run_mixed_plan(g0, M, ids, cs)
+    return o.arrays()
+
+
+@pytest.mark.parametrize("mode", list(MODES_KPHI))
+def test_headline_group_kphi_vs_oracle(headline_group, headline_group_kphi, mode):
+    g0, (M, ids, cs), _ = headline_group
+    ref = headline_group_kphi
+    prec, kern, tol, binning = MODES_KPHI[mode]
+    print(f"headline group, 16 K_phi bins, {mode}", flush=True)
+    g = hbt.HBTCorrelation(dict(PARAMS_KPHI, hbt_binning=binning), precision=prec, kernel=kern)
+    g.upload_groups([g0])
+    g.prepare_batch(np.array([M], np.int32), ids, cs)
+    g.launch_batch()
+    g.sync()
+    r = g.results()
+    slab = g.slab_info()
+    g.close()
+    assert slab["on"] == (binning == 1)
+    assert np.array_equal(r.num_count, ref["num_count"].astype(np.uint64))
+    assert np.array_equal(r.den, ref["den"].astype(np.uint64))
+    assert np.array_equal(r.cnt_num_ktphi, ref["cnt_num_ktphi"].astype(np.uint64))
+    for key, fl in SUMS:
+        a, b = getattr(r, key), ref[key]
+        d = np.maximum(np.abs(b), ref[fl])
+        sel = d > 0
+        err = np.abs(a - b)[sel] / d[sel]
+        assert (float(err.max()) if err.size else 0.0) <= tol, key
